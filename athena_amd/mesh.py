@@ -1,0 +1,173 @@
+"""Mesh / MeshBlock description shared by the problem generators, the host driver and
+the tests.
+
+Mirrors the parts of Athena++'s Mesh that the per-MeshBlock MHD update needs on a
+uniform grid:
+
+* block decomposition and logical locations (``mesh.cpp:1984-2060``
+  ``Mesh::SetBlockSizeAndBoundaries``), blocks numbered row-major with x1 fastest;
+* the uniform mesh generator (``mesh.hpp:457-472`` ``ComputeMeshGeneratorX`` with
+  ``sym_interval=true``, ``mesh.hpp:535-547`` ``UniformMeshGeneratorX*``);
+* Cartesian coordinates ``x?f, x?v, dx?f`` (``coordinates.cpp:129-146``,
+  ``cartesian.cpp:23-72``);
+* AthenaArray shapes of every register (``athena_arrays.hpp``: i fastest).
+
+Only periodic boundaries are modelled: every same-level neighbour exists.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NHYDRO = 5
+IDN, IM1, IM2, IM3, IEN = 0, 1, 2, 3, 4
+IVX, IVY, IVZ, IPR = 1, 2, 3, 4
+IB1, IB2, IB3 = 0, 1, 2
+
+RSOLVER = {"hlld": 0, "hlle": 1}
+FLT_MIN = 1.17549435082228750797e-38          # std::numeric_limits<float>::min()
+DEFAULT_FLOOR = math.sqrt(1024 * FLT_MIN)     # adiabatic_mhd.cpp:29-31
+
+
+@dataclass
+class MeshDesc:
+    """Input-file level description of one uniform, periodic mesh.
+
+    Field names follow the athinput keys (``<mesh> nx1..3, x1min..``,
+    ``<meshblock> nx1..3``, ``<hydro> gamma, dfloor, pfloor``, ``<time> cfl_number,
+    xorder``) and the configure switches (``--flux``, ``-s``, ``--nghost``).
+    """
+
+    nx: tuple = (64, 64, 64)
+    bnx: tuple = (64, 64, 64)
+    xmin: tuple = (0.0, 0.0, 0.0)
+    xmax: tuple = (1.0, 1.0, 1.0)
+    ng: int = 2
+    gamma: float = 5.0 / 3.0
+    cfl: float = 0.3
+    xorder: int = 2
+    rsolver: str = "hlld"
+    relativistic: bool = False
+    dfloor: float = -1.0          # <0: reference default sqrt(1024*FLT_MIN)
+    pfloor: float = -1.0
+    sigma_max: float = -1.0       # SR floors, <0: reference defaults (0, 0, 1000)
+    beta_min: float = -1.0
+    gamma_max: float = -1.0
+
+    def __post_init__(self):
+        self.nx = tuple(int(v) for v in self.nx)
+        self.bnx = tuple(int(v) for v in self.bnx)
+        self.xmin = tuple(float(v) for v in self.xmin)
+        self.xmax = tuple(float(v) for v in self.xmax)
+        for d in range(3):
+            if self.nx[d] % self.bnx[d]:
+                raise ValueError("mesh nx%d=%d not divisible by meshblock nx%d=%d"
+                                 % (d + 1, self.nx[d], d + 1, self.bnx[d]))
+        if self.xorder == 3 and self.ng < 3:
+            # reconstruction.cpp:117-127
+            raise ValueError("xorder=3 (PPM) needs nghost>=3, got %d" % self.ng)
+        if self.xorder not in (1, 2, 3):
+            raise ValueError("xorder must be 1, 2 or 3")
+        if self.rsolver not in RSOLVER:
+            raise ValueError("rsolver must be one of %s" % sorted(RSOLVER))
+        if self.relativistic and self.rsolver != "hlle":
+            raise ValueError("the relativistic path implements HLLE only")
+
+    # ---------------------------------------------------------------- geometry
+    @property
+    def nbx(self):
+        return tuple(self.nx[d] // self.bnx[d] for d in range(3))
+
+    @property
+    def nblocks(self):
+        a, b, c = self.nbx
+        return a * b * c
+
+    @property
+    def nc(self):
+        """(nc1, nc2, nc3): allocated cells per block including ghosts."""
+        return tuple(self.bnx[d] + 2 * self.ng for d in range(3))
+
+    def loc(self, gid):
+        n1, n2, _ = self.nbx
+        return (gid % n1, (gid // n1) % n2, gid // (n1 * n2))
+
+    def gid(self, l1, l2, l3):
+        n1, n2, n3 = self.nbx
+        return (l1 % n1) + n1 * ((l2 % n2) + n2 * (l3 % n3))
+
+    @property
+    def floors(self):
+        d = DEFAULT_FLOOR if self.dfloor < 0 else self.dfloor
+        p = DEFAULT_FLOOR if self.pfloor < 0 else self.pfloor
+        return d, p
+
+    # AthenaArray shapes (n, k, j, i)
+    def shape(self, what):
+        c1, c2, c3 = self.nc
+        return {
+            "u": (NHYDRO, c3, c2, c1), "w": (NHYDRO, c3, c2, c1),
+            "w1": (NHYDRO, c3, c2, c1), "bcc": (3, c3, c2, c1),
+            "b1": (c3, c2, c1 + 1), "b2": (c3, c2 + 1, c1), "b3": (c3 + 1, c2, c1),
+        }[what]
+
+
+def _mesh_gen_x(index: int, nrange: int) -> float:
+    """ComputeMeshGeneratorX with sym_interval=true (mesh.hpp:457-472)."""
+    noffset = index - nrange // 2
+    noffset_ceil = index - (nrange + 1) // 2
+    return float(noffset + noffset_ceil) / (2.0 * nrange)
+
+
+def _uniform_gen(x: float, xmin: float, xmax: float) -> float:
+    """UniformMeshGeneratorX? (mesh.hpp:535-547)."""
+    return 0.5 * (xmin + xmax) + (x * xmax - x * xmin)
+
+
+@dataclass
+class Coords:
+    x1f: np.ndarray = field(default=None)
+    x2f: np.ndarray = field(default=None)
+    x3f: np.ndarray = field(default=None)
+    x1v: np.ndarray = field(default=None)
+    x2v: np.ndarray = field(default=None)
+    x3v: np.ndarray = field(default=None)
+    dx1f: np.ndarray = field(default=None)
+    dx2f: np.ndarray = field(default=None)
+    dx3f: np.ndarray = field(default=None)
+
+
+def block_coords(m: MeshDesc, gid: int) -> Coords:
+    """Face/centre coordinates of one block (coordinates.cpp:129-146, cartesian.cpp)."""
+    lx = m.loc(gid)
+    nbx = m.nbx
+    out = Coords()
+    for d in range(3):
+        nx, ng = m.bnx[d], m.ng
+        nc = nx + 2 * ng
+        l = lx[d]
+        bmin = m.xmin[d] if l == 0 else _uniform_gen(_mesh_gen_x(l, nbx[d]), m.xmin[d], m.xmax[d])
+        bmax = (m.xmax[d] if l == nbx[d] - 1
+                else _uniform_gen(_mesh_gen_x(l + 1, nbx[d]), m.xmin[d], m.xmax[d]))
+        il, iu = ng, ng + nx - 1
+        dx = (bmax - bmin) / (iu - il + 1)
+        xf = np.zeros(nc + 1)
+        for i in range(il - ng, iu + ng + 2):
+            noff = i - il + l * nx
+            xf[i] = _uniform_gen(_mesh_gen_x(noff, m.nx[d]), m.xmin[d], m.xmax[d])
+        xf[il] = bmin
+        xf[iu + 1] = bmax
+        dxf = np.full(nc, dx)
+        xv = np.array([0.5 * (xf[i + 1] + xf[i]) for i in range(nc)])
+        setattr(out, "x%df" % (d + 1), xf)
+        setattr(out, "x%dv" % (d + 1), xv)
+        setattr(out, "dx%df" % (d + 1), dxf)
+    return out
+
+
+def new_block_arrays(m: MeshDesc):
+    """Zero-filled host registers for one block (AthenaArray zero-initialises,
+    athena_arrays.hpp:519-530)."""
+    return {k: np.zeros(m.shape(k)) for k in ("u", "w", "w1", "bcc", "b1", "b2", "b3")}

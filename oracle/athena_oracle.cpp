@@ -1,0 +1,1147 @@
+//========================================================================================
+// athena_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain-C++ CPU restatement of Athena++'s per-MeshBlock MHD update, used as the
+// checker ("oracle") for the MI355X library.  Only tests/, __graft_entry__.smoke() and
+// the cpu_baseline leg of bench.py may load it; the product never links or calls it.
+//
+// Each function below follows the cited reference routine operation-for-operation
+// (same association order, same branch structure, no FMA contraction) so that the
+// device kernels can be compared against it bit for bit.
+//
+// Parity pinning of this file (what fixes each function independently of itself):
+//   * the whole Newtonian VL2+PLM+HLLD+CT path: the reference's linear-wave RMS L1
+//     errors (fast/Alfven/slow/entropy at Nx1=32, generated from the compiled
+//     reference and committed under tests/golden/), mhd_linwave.py thresholds,
+//     L/R-going symmetry, conservation and div(B) invariants;
+//   * Riemann solvers: consistency F(U,U)=F(U), HLLE==HLLD on hydrodynamic states
+//     with no transverse field is NOT assumed; region-select and degeneracy branches
+//     are covered by brute-force sampled states with mirror symmetry checks;
+//   * DC/PLM/PPM: exactness on linear data, identity on constants;
+//   * SR con2prim: PrimitiveToConserved o ConservedToPrimitive round trip.
+// Functions with no independent pin are listed as "parity unpinned" in DESIGN.md.
+//========================================================================================
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle_core.hpp"
+
+namespace ora {
+
+//----------------------------------------------------------------------------------------
+// Uniform mesh generator and coordinates
+// (mesh.hpp:457-472 ComputeMeshGeneratorX, :535-547 UniformMeshGeneratorX*,
+//  mesh.cpp:1984-2060 SetBlockSizeAndBoundaries, coordinates.cpp:129-146, cartesian.cpp)
+
+Real MeshGenX(std::int64_t index, std::int64_t nrange) {   // sym_interval = true
+  std::int64_t noffset = index - (nrange) / 2;
+  std::int64_t noffset_ceil = index - (nrange + 1) / 2;
+  return static_cast<Real>(noffset + noffset_ceil) / (2.0 * nrange);
+}
+Real UniformGen(Real x, Real xmin, Real xmax) {
+  return static_cast<Real>(0.5) * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+void SetCoords(const Mesh &m, Block &b) {
+  for (int d = 0; d < 3; ++d) {
+    int nbd = m.nbx[d];
+    std::int64_t l = b.lx[d];
+    if (l == 0) b.xbmin[d] = m.c.xmin[d];
+    else b.xbmin[d] = UniformGen(MeshGenX(l, nbd), m.c.xmin[d], m.c.xmax[d]);
+    if (l == nbd - 1) b.xbmax[d] = m.c.xmax[d];
+    else b.xbmax[d] = UniformGen(MeshGenX(l + 1, nbd), m.c.xmin[d], m.c.xmax[d]);
+  }
+  auto one = [&](int d, int nx, int nc, std::vector<Real> &xf, std::vector<Real> &xv,
+                 std::vector<Real> &dxf, std::vector<Real> &dxv) {
+    xf.assign(nc + 1, 0.0); xv.assign(nc, 0.0); dxf.assign(nc, 0.0); dxv.assign(nc, 0.0);
+    int il = b.ng, iu = b.ng + nx - 1, ng = b.ng;
+    Real dx = (b.xbmax[d] - b.xbmin[d]) / (iu - il + 1);
+    std::int64_t nroot = m.c.nx[d];
+    for (int i = il - ng; i <= iu + ng + 1; ++i) {
+      std::int64_t noff = static_cast<std::int64_t>(i - il + b.lx[d] * nx);
+      Real rx = MeshGenX(noff, nroot);
+      xf[i] = UniformGen(rx, m.c.xmin[d], m.c.xmax[d]);
+    }
+    xf[il] = b.xbmin[d];
+    xf[iu + 1] = b.xbmax[d];
+    for (int i = il - ng; i <= iu + ng; ++i) dxf[i] = dx;
+    for (int i = il - ng; i <= iu + ng; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
+    for (int i = il - ng; i <= iu + ng - 1; ++i) dxv[i] = dxf[i];
+  };
+  one(0, b.nx1, b.nc1, b.x1f, b.x1v, b.dx1f, b.dx1v);
+  one(1, b.nx2, b.nc2, b.x2f, b.x2v, b.dx2f, b.dx2v);
+  one(2, b.nx3, b.nc3, b.x3f, b.x3v, b.dx3f, b.dx3v);
+}
+
+//----------------------------------------------------------------------------------------
+// EOS helpers (adiabatic_mhd.cpp:149-172)
+
+Real FastMagnetosonicSpeed(const Real prim[NWAVE], Real bx, Real gamma) {
+  Real asq = gamma * prim[IPR];
+  Real vaxsq = bx * bx;
+  Real ct2 = (prim[IBY] * prim[IBY] + prim[IBZ] * prim[IBZ]);
+  Real qsq = vaxsq + ct2 + asq;
+  Real tmp = vaxsq + ct2 - asq;
+  return std::sqrt(0.5 * (qsq + std::sqrt(tmp * tmp + 4.0 * asq * ct2)) / prim[IDN]);
+}
+
+// Hydro::GetWeightForCT (hydro.cpp:154-158)
+Real WeightForCT(Real dflx, Real rhol, Real rhor, Real dx, Real dt) {
+  Real v_over_c = (1024.0) * dt * dflx / (dx * (rhol + rhor));
+  Real tmp_min = rmin(static_cast<Real>(0.5), v_over_c);
+  return 0.5 + rmax(static_cast<Real>(-0.5), tmp_min);
+}
+
+//----------------------------------------------------------------------------------------
+// HLLD Riemann solver, one interface (hlld.cpp:36-376).  wli/wri are in the rotated
+// frame (IVX = normal).  Outputs flxi[7] in the rotated frame.
+
+struct C1 { Real d, mx, my, mz, e, by, bz; };
+
+void HLLD1(const Real wli[NWAVE], const Real wri[NWAVE], Real bxi, Real gamma,
+           Real flxi[NWAVE]) {
+  const Real SMALL_NUMBER = 1.0e-4;
+  Real igm1 = 1.0 / (gamma - 1.0);
+  C1 ul, ur, ulst, uldst, urdst, urst, fl, fr;
+  Real spd[5];
+  Real bxsq = bxi * bxi;
+  Real pbl = 0.5 * (bxsq + (SQR(wli[IBY]) + SQR(wli[IBZ])));
+  Real pbr = 0.5 * (bxsq + (SQR(wri[IBY]) + SQR(wri[IBZ])));
+  Real kel = 0.5 * wli[IDN] * (SQR(wli[IVX]) + (SQR(wli[IVY]) + SQR(wli[IVZ])));
+  Real ker = 0.5 * wri[IDN] * (SQR(wri[IVX]) + (SQR(wri[IVY]) + SQR(wri[IVZ])));
+
+  ul.d = wli[IDN];
+  ul.mx = wli[IVX] * ul.d;
+  ul.my = wli[IVY] * ul.d;
+  ul.mz = wli[IVZ] * ul.d;
+  ul.e = wli[IPR] * igm1 + kel + pbl;
+  ul.by = wli[IBY];
+  ul.bz = wli[IBZ];
+
+  ur.d = wri[IDN];
+  ur.mx = wri[IVX] * ur.d;
+  ur.my = wri[IVY] * ur.d;
+  ur.mz = wri[IVZ] * ur.d;
+  ur.e = wri[IPR] * igm1 + ker + pbr;
+  ur.by = wri[IBY];
+  ur.bz = wri[IBZ];
+
+  Real cfl = FastMagnetosonicSpeed(wli, bxi, gamma);
+  Real cfr = FastMagnetosonicSpeed(wri, bxi, gamma);
+  spd[0] = rmin(wli[IVX] - cfl, wri[IVX] - cfr);
+  spd[4] = rmax(wli[IVX] + cfl, wri[IVX] + cfr);
+
+  Real ptl = wli[IPR] + pbl;
+  Real ptr = wri[IPR] + pbr;
+
+  fl.d = ul.mx;
+  fl.mx = ul.mx * wli[IVX] + ptl - bxsq;
+  fl.my = ul.my * wli[IVX] - bxi * ul.by;
+  fl.mz = ul.mz * wli[IVX] - bxi * ul.bz;
+  fl.e = wli[IVX] * (ul.e + ptl - bxsq) - bxi * (wli[IVY] * ul.by + wli[IVZ] * ul.bz);
+  fl.by = ul.by * wli[IVX] - bxi * wli[IVY];
+  fl.bz = ul.bz * wli[IVX] - bxi * wli[IVZ];
+
+  fr.d = ur.mx;
+  fr.mx = ur.mx * wri[IVX] + ptr - bxsq;
+  fr.my = ur.my * wri[IVX] - bxi * ur.by;
+  fr.mz = ur.mz * wri[IVX] - bxi * ur.bz;
+  fr.e = wri[IVX] * (ur.e + ptr - bxsq) - bxi * (wri[IVY] * ur.by + wri[IVZ] * ur.bz);
+  fr.by = ur.by * wri[IVX] - bxi * wri[IVY];
+  fr.bz = ur.bz * wri[IVX] - bxi * wri[IVZ];
+
+  Real sdl = spd[0] - wli[IVX];
+  Real sdr = spd[4] - wri[IVX];
+  spd[2] = (sdr * ur.mx - sdl * ul.mx + (ptl - ptr)) / (sdr * ur.d - sdl * ul.d);
+
+  Real sdml = spd[0] - spd[2];
+  Real sdmr = spd[4] - spd[2];
+  Real sdml_inv = 1.0 / sdml;
+  Real sdmr_inv = 1.0 / sdmr;
+  ulst.d = ul.d * sdl * sdml_inv;
+  urst.d = ur.d * sdr * sdmr_inv;
+  Real ulst_d_inv = 1.0 / ulst.d;
+  Real urst_d_inv = 1.0 / urst.d;
+  Real sqrtdl = std::sqrt(ulst.d);
+  Real sqrtdr = std::sqrt(urst.d);
+
+  spd[1] = spd[2] - std::abs(bxi) / sqrtdl;
+  spd[3] = spd[2] + std::abs(bxi) / sqrtdr;
+
+  Real ptstl = ptl + ul.d * sdl * (spd[2] - wli[IVX]);
+  Real ptstr = ptr + ur.d * sdr * (spd[2] - wri[IVX]);
+  Real ptst = 0.5 * (ptstr + ptstl);
+
+  ulst.mx = ulst.d * spd[2];
+  if (std::abs(ul.d * sdl * sdml - bxsq) < (SMALL_NUMBER) * ptst) {
+    ulst.my = ulst.d * wli[IVY];
+    ulst.mz = ulst.d * wli[IVZ];
+    ulst.by = ul.by;
+    ulst.bz = ul.bz;
+  } else {
+    Real tmp = bxi * (sdl - sdml) / (ul.d * sdl * sdml - bxsq);
+    ulst.my = ulst.d * (wli[IVY] - ul.by * tmp);
+    ulst.mz = ulst.d * (wli[IVZ] - ul.bz * tmp);
+    tmp = (ul.d * SQR(sdl) - bxsq) / (ul.d * sdl * sdml - bxsq);
+    ulst.by = ul.by * tmp;
+    ulst.bz = ul.bz * tmp;
+  }
+  Real vbstl = (ulst.mx * bxi + (ulst.my * ulst.by + ulst.mz * ulst.bz)) * ulst_d_inv;
+  ulst.e = (sdl * ul.e - ptl * wli[IVX] + ptst * spd[2] +
+            bxi * (wli[IVX] * bxi + (wli[IVY] * ul.by + wli[IVZ] * ul.bz) - vbstl)) *
+           sdml_inv;
+
+  urst.mx = urst.d * spd[2];
+  if (std::abs(ur.d * sdr * sdmr - bxsq) < (SMALL_NUMBER) * ptst) {
+    urst.my = urst.d * wri[IVY];
+    urst.mz = urst.d * wri[IVZ];
+    urst.by = ur.by;
+    urst.bz = ur.bz;
+  } else {
+    Real tmp = bxi * (sdr - sdmr) / (ur.d * sdr * sdmr - bxsq);
+    urst.my = urst.d * (wri[IVY] - ur.by * tmp);
+    urst.mz = urst.d * (wri[IVZ] - ur.bz * tmp);
+    tmp = (ur.d * SQR(sdr) - bxsq) / (ur.d * sdr * sdmr - bxsq);
+    urst.by = ur.by * tmp;
+    urst.bz = ur.bz * tmp;
+  }
+  Real vbstr = (urst.mx * bxi + (urst.my * urst.by + urst.mz * urst.bz)) * urst_d_inv;
+  urst.e = (sdr * ur.e - ptr * wri[IVX] + ptst * spd[2] +
+            bxi * (wri[IVX] * bxi + (wri[IVY] * ur.by + wri[IVZ] * ur.bz) - vbstr)) *
+           sdmr_inv;
+
+  Real invsumd = 1.0 / (sqrtdl + sqrtdr);
+  Real bxsig = (bxi > 0.0 ? 1.0 : -1.0);
+
+  uldst.d = ulst.d;
+  urdst.d = urst.d;
+  uldst.mx = ulst.mx;
+  urdst.mx = urst.mx;
+
+  Real tmp = invsumd * (sqrtdl * (ulst.my * ulst_d_inv) + sqrtdr * (urst.my * urst_d_inv) +
+                        bxsig * (urst.by - ulst.by));
+  uldst.my = uldst.d * tmp;
+  urdst.my = urdst.d * tmp;
+
+  tmp = invsumd * (sqrtdl * (ulst.mz * ulst_d_inv) + sqrtdr * (urst.mz * urst_d_inv) +
+                   bxsig * (urst.bz - ulst.bz));
+  uldst.mz = uldst.d * tmp;
+  urdst.mz = urdst.d * tmp;
+
+  tmp = invsumd * (sqrtdl * urst.by + sqrtdr * ulst.by +
+                   bxsig * sqrtdl * sqrtdr * ((urst.my * urst_d_inv) - (ulst.my * ulst_d_inv)));
+  uldst.by = urdst.by = tmp;
+
+  tmp = invsumd * (sqrtdl * urst.bz + sqrtdr * ulst.bz +
+                   bxsig * sqrtdl * sqrtdr * ((urst.mz * urst_d_inv) - (ulst.mz * ulst_d_inv)));
+  uldst.bz = urdst.bz = tmp;
+
+  tmp = spd[2] * bxi + (uldst.my * uldst.by + uldst.mz * uldst.bz) / uldst.d;
+  uldst.e = ulst.e - sqrtdl * bxsig * (vbstl - tmp);
+  urdst.e = urst.e + sqrtdr * bxsig * (vbstr - tmp);
+
+  uldst.d = spd[1] * (uldst.d - ulst.d);
+  uldst.mx = spd[1] * (uldst.mx - ulst.mx);
+  uldst.my = spd[1] * (uldst.my - ulst.my);
+  uldst.mz = spd[1] * (uldst.mz - ulst.mz);
+  uldst.e = spd[1] * (uldst.e - ulst.e);
+  uldst.by = spd[1] * (uldst.by - ulst.by);
+  uldst.bz = spd[1] * (uldst.bz - ulst.bz);
+
+  ulst.d = spd[0] * (ulst.d - ul.d);
+  ulst.mx = spd[0] * (ulst.mx - ul.mx);
+  ulst.my = spd[0] * (ulst.my - ul.my);
+  ulst.mz = spd[0] * (ulst.mz - ul.mz);
+  ulst.e = spd[0] * (ulst.e - ul.e);
+  ulst.by = spd[0] * (ulst.by - ul.by);
+  ulst.bz = spd[0] * (ulst.bz - ul.bz);
+
+  urdst.d = spd[3] * (urdst.d - urst.d);
+  urdst.mx = spd[3] * (urdst.mx - urst.mx);
+  urdst.my = spd[3] * (urdst.my - urst.my);
+  urdst.mz = spd[3] * (urdst.mz - urst.mz);
+  urdst.e = spd[3] * (urdst.e - urst.e);
+  urdst.by = spd[3] * (urdst.by - urst.by);
+  urdst.bz = spd[3] * (urdst.bz - urst.bz);
+
+  urst.d = spd[4] * (urst.d - ur.d);
+  urst.mx = spd[4] * (urst.mx - ur.mx);
+  urst.my = spd[4] * (urst.my - ur.my);
+  urst.mz = spd[4] * (urst.mz - ur.mz);
+  urst.e = spd[4] * (urst.e - ur.e);
+  urst.by = spd[4] * (urst.by - ur.by);
+  urst.bz = spd[4] * (urst.bz - ur.bz);
+
+  const C1 *a = nullptr;
+  if (spd[0] >= 0.0) {
+    flxi[IDN] = fl.d; flxi[IVX] = fl.mx; flxi[IVY] = fl.my; flxi[IVZ] = fl.mz;
+    flxi[IEN] = fl.e; flxi[IBY] = fl.by; flxi[IBZ] = fl.bz;
+  } else if (spd[4] <= 0.0) {
+    flxi[IDN] = fr.d; flxi[IVX] = fr.mx; flxi[IVY] = fr.my; flxi[IVZ] = fr.mz;
+    flxi[IEN] = fr.e; flxi[IBY] = fr.by; flxi[IBZ] = fr.bz;
+  } else if (spd[1] >= 0.0) {
+    flxi[IDN] = fl.d + ulst.d; flxi[IVX] = fl.mx + ulst.mx; flxi[IVY] = fl.my + ulst.my;
+    flxi[IVZ] = fl.mz + ulst.mz; flxi[IEN] = fl.e + ulst.e; flxi[IBY] = fl.by + ulst.by;
+    flxi[IBZ] = fl.bz + ulst.bz;
+  } else if (spd[3] <= 0.0) {
+    flxi[IDN] = fr.d + urst.d; flxi[IVX] = fr.mx + urst.mx; flxi[IVY] = fr.my + urst.my;
+    flxi[IVZ] = fr.mz + urst.mz; flxi[IEN] = fr.e + urst.e; flxi[IBY] = fr.by + urst.by;
+    flxi[IBZ] = fr.bz + urst.bz;
+  } else if (spd[2] >= 0.0) {
+    flxi[IDN] = fl.d + ulst.d + uldst.d;
+    flxi[IVX] = fl.mx + ulst.mx + uldst.mx;
+    flxi[IVY] = fl.my + ulst.my + uldst.my;
+    flxi[IVZ] = fl.mz + ulst.mz + uldst.mz;
+    flxi[IEN] = fl.e + ulst.e + uldst.e;
+    flxi[IBY] = fl.by + ulst.by + uldst.by;
+    flxi[IBZ] = fl.bz + ulst.bz + uldst.bz;
+  } else {
+    flxi[IDN] = fr.d + urst.d + urdst.d;
+    flxi[IVX] = fr.mx + urst.mx + urdst.mx;
+    flxi[IVY] = fr.my + urst.my + urdst.my;
+    flxi[IVZ] = fr.mz + urst.mz + urdst.mz;
+    flxi[IEN] = fr.e + urst.e + urdst.e;
+    flxi[IBY] = fr.by + urst.by + urdst.by;
+    flxi[IBZ] = fr.bz + urst.bz + urdst.bz;
+  }
+  (void)a;
+}
+
+//----------------------------------------------------------------------------------------
+// HLLE Riemann solver for adiabatic MHD, one interface (hlle_mhd.cpp:25-182)
+
+
+//----------------------------------------------------------------------------------------
+// Reconstruction along one pencil, written per direction d (0,1,2) and pencil (k,j).
+// Output arrays follow the reference convention: for d=0, wl(n,i+1)/wr(n,i) from cell
+// i; for d=1,2, wl(n,i)/wr(n,i) at the face at the lower side of cell index (j or k)
+// after the caller's swap (calculate_fluxes.cpp:184-238, 304-357).
+
+// variable source: n<NHYDRO from w, IBY/IBZ from bcc components per direction
+//   X1: (IB2,IB3)  X2: (IB3,IB1)  X3: (IB1,IB2)   (dc.cpp:24-103, plm.cpp)
+inline void TransB(int d, int &cby, int &cbz) {
+  if (d == 0) { cby = IB2; cbz = IB3; }
+  else if (d == 1) { cby = IB3; cbz = IB1; }
+  else { cby = IB1; cbz = IB2; }
+}
+
+// value of variable n (0..6) of cell (k,j,i)
+Real Wval(Block &b, Arr &w, Arr &bcc, int d, int n, int k, int j, int i) {
+  if (n < NHYDRO) return w(n, k, j, i);
+  int cby, cbz;
+  TransB(d, cby, cbz);
+  return bcc(n == IBY ? cby : cbz, k, j, i);
+}
+
+// Donor cell (dc.cpp:24-103)
+void DonorCell(Block &b, int d, int k, int j, int il, int iu, Arr &w, Arr &bcc,
+               Arr &wl, Arr &wr) {
+  for (int n = 0; n < NWAVE; ++n) {
+    for (int i = il; i <= iu; ++i) {
+      Real v = Wval(b, w, bcc, d, n, k, j, i);
+      if (d == 0) { wl(n, 0, 0, i + 1) = v; wr(n, 0, 0, i) = v; }
+      else { wl(n, 0, 0, i) = v; wr(n, 0, 0, i) = v; }
+    }
+  }
+}
+
+// Piecewise linear, uniform Cartesian VL limiter branch (plm.cpp:27-167, 176-319,
+// 328-463); characteristic projection and minmod are not selected by any config.
+void PiecewiseLinear(Block &b, int d, int k, int j, int il, int iu, Arr &w, Arr &bcc,
+                     Arr &wl, Arr &wr) {
+  int sk = (d == 2), sj = (d == 1), si = (d == 0);
+  for (int n = 0; n < NWAVE; ++n) {
+    for (int i = il; i <= iu; ++i) {
+      Real wc = Wval(b, w, bcc, d, n, k, j, i);
+      Real wm = Wval(b, w, bcc, d, n, k - sk, j - sj, i - si);
+      Real wp = Wval(b, w, bcc, d, n, k + sk, j + sj, i + si);
+      Real dwl = (wc - wm);
+      Real dwr = (wp - wc);
+      Real dw2 = dwl * dwr;
+      Real dwm = 2.0 * dw2 / (dwl + dwr);
+      if (dw2 <= 0.0) dwm = 0.0;
+      if (d == 0) {
+        wl(n, 0, 0, i + 1) = wc + ((b.x1f[i + 1] - b.x1v[i]) / b.dx1f[i]) * dwm;
+        wr(n, 0, 0, i) = wc - ((b.x1v[i] - b.x1f[i]) / b.dx1f[i]) * dwm;
+      } else if (d == 1) {
+        Real dxp = (b.x2f[j + 1] - b.x2v[j]) / b.dx2f[j];
+        Real dxm = (b.x2v[j] - b.x2f[j]) / b.dx2f[j];
+        wl(n, 0, 0, i) = wc + dxp * dwm;
+        wr(n, 0, 0, i) = wc - dxm * dwm;
+      } else {
+        Real dxp = (b.x3f[k + 1] - b.x3v[k]) / b.dx3f[k];
+        Real dxm = (b.x3v[k] - b.x3f[k]) / b.dx3f[k];
+        wl(n, 0, 0, i) = wc + dxp * dwm;
+        wr(n, 0, 0, i) = wc - dxm * dwm;
+      }
+    }
+  }
+}
+
+
+//----------------------------------------------------------------------------------------
+// Riemann solve along a pencil: the wrapper of hlld.cpp:36-376 / hlle_mhd.cpp that
+// loads rotated states, calls the single-interface solver and scatters outputs.
+
+
+void RiemannPencil(Mesh &m, Block &b, int d, int k, int j, int il, int iu, Arr &bx,
+                   Arr &wl, Arr &wr, Arr &flx, Arr &ey, Arr &ez, Arr &wct) {
+  int ivx = IVX + d;
+  int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  Real dt = m.dt;
+  for (int i = il; i <= iu; ++i) {
+    Real wli[NWAVE], wri[NWAVE], flxi[NWAVE];
+    wli[IDN] = wl(IDN, 0, 0, i); wli[IVX] = wl(ivx, 0, 0, i); wli[IVY] = wl(ivy, 0, 0, i);
+    wli[IVZ] = wl(ivz, 0, 0, i); wli[IPR] = wl(IPR, 0, 0, i); wli[IBY] = wl(IBY, 0, 0, i);
+    wli[IBZ] = wl(IBZ, 0, 0, i);
+    wri[IDN] = wr(IDN, 0, 0, i); wri[IVX] = wr(ivx, 0, 0, i); wri[IVY] = wr(ivy, 0, 0, i);
+    wri[IVZ] = wr(ivz, 0, 0, i); wri[IPR] = wr(IPR, 0, 0, i); wri[IBY] = wr(IBY, 0, 0, i);
+    wri[IBZ] = wr(IBZ, 0, 0, i);
+    Real bxi = bx(k, j, i);
+    if (m.c.relativistic) {
+      RiemannSR(m, wli, wri, bxi, flxi);
+    } else if (m.c.rsolver == 0) {
+      HLLD1(wli, wri, bxi, m.c.gamma, flxi);
+    } else {
+      HLLE1(wli, wri, bxi, m.c.gamma, flxi);
+    }
+    flx(IDN, k, j, i) = flxi[IDN];
+    flx(ivx, k, j, i) = flxi[IVX];
+    flx(ivy, k, j, i) = flxi[IVY];
+    flx(ivz, k, j, i) = flxi[IVZ];
+    flx(IEN, k, j, i) = flxi[IEN];
+    ey(k, j, i) = -flxi[IBY];
+    ez(k, j, i) = flxi[IBZ];
+    Real dxw = (d == 0) ? b.dx1f[i] : ((d == 1) ? b.dx2f[j] : b.dx3f[k]);
+    if (m.c.relativistic) {
+      // hlle_mhd_rel.cpp:74-77 (weight from the density flux and primitive rho)
+      wct(k, j, i) = WeightForCT(flxi[IDN], wli[IDN], wri[IDN], dxw, dt);
+    } else {
+      wct(k, j, i) = WeightForCT(flxi[IDN], wli[IDN], wri[IDN], dxw, dt);
+    }
+  }
+}
+
+void Reconstruct(Mesh &m, Block &b, int order, int d, int k, int j, int il, int iu,
+                 Arr &w, Arr &bcc, Arr &wl, Arr &wr) {
+  if (order == 1) DonorCell(b, d, k, j, il, iu, w, bcc, wl, wr);
+  else if (order == 2) PiecewiseLinear(b, d, k, j, il, iu, w, bcc, wl, wr);
+  else PiecewiseParabolic(m, b, d, k, j, il, iu, w, bcc, wl, wr);
+}
+
+// Hydro::CalculateFluxes (calculate_fluxes.cpp:38-415), 3D MHD branch
+void CalculateFluxes(Mesh &m, Block &b, int order) {
+  int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+  Arr wl, wr, wlb;
+  wl.alloc(NWAVE, 1, 1, b.nc1 + 1);
+  wr.alloc(NWAVE, 1, 1, b.nc1 + 1);
+  wlb.alloc(NWAVE, 1, 1, b.nc1 + 1);
+  // x1
+  for (int k = ks - 1; k <= ke + 1; ++k) {
+    for (int j = js - 1; j <= je + 1; ++j) {
+      Reconstruct(m, b, order, 0, k, j, is - 1, ie + 1, b.w, b.bcc, wl, wr);
+      RiemannPencil(m, b, 0, k, j, is, ie + 1, b.b1f, wl, wr, b.flux[0], b.e3x1, b.e2x1,
+                    b.wx1);
+    }
+  }
+  // x2
+  {
+    int il = is - 1, iu = ie + 1;
+    for (int k = ks - 1; k <= ke + 1; ++k) {
+      Reconstruct(m, b, order, 1, k, js - 1, il, iu, b.w, b.bcc, wl, wr);
+      for (int j = js; j <= je + 1; ++j) {
+        Reconstruct(m, b, order, 1, k, j, il, iu, b.w, b.bcc, wlb, wr);
+        RiemannPencil(m, b, 1, k, j, il, iu, b.b2f, wl, wr, b.flux[1], b.e1x2, b.e3x2,
+                      b.wx2);
+        wl.swap(wlb);
+      }
+    }
+  }
+  // x3
+  {
+    int il = is - 1, iu = ie + 1;
+    for (int j = js - 1; j <= je + 1; ++j) {
+      Reconstruct(m, b, order, 2, ks - 1, j, il, iu, b.w, b.bcc, wl, wr);
+      for (int k = ks; k <= ke + 1; ++k) {
+        Reconstruct(m, b, order, 2, k, j, il, iu, b.w, b.bcc, wlb, wr);
+        RiemannPencil(m, b, 2, k, j, il, iu, b.b3f, wl, wr, b.flux[2], b.e2x3, b.e1x3,
+                      b.wx3);
+        wl.swap(wlb);
+      }
+    }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// Field::ComputeCornerE, 3D branch (calculate_corner_e.cpp:129-230)
+void ComputeCornerE(Mesh &m, Block &b) {
+  int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+  Arr &w = b.w, &bcc = b.bcc, &cc = b.cce;
+  for (int k = ks - 1; k <= ke + 1; ++k) {
+    for (int j = js - 1; j <= je + 1; ++j) {
+      for (int i = is - 1; i <= ie + 1; ++i) {
+        if (m.c.relativistic) {
+          const Real u1 = w(IVX, k, j, i), u2 = w(IVY, k, j, i), u3 = w(IVZ, k, j, i);
+          Real u0 = std::sqrt(1.0 + SQR(u1) + SQR(u2) + SQR(u3));
+          cc(IB1, k, j, i) = (u3 * bcc(IB2, k, j, i) - u2 * bcc(IB3, k, j, i)) / u0;
+          cc(IB2, k, j, i) = (u1 * bcc(IB3, k, j, i) - u3 * bcc(IB1, k, j, i)) / u0;
+          cc(IB3, k, j, i) = (u2 * bcc(IB1, k, j, i) - u1 * bcc(IB2, k, j, i)) / u0;
+        } else {
+          cc(IB1, k, j, i) = w(IVZ, k, j, i) * bcc(IB2, k, j, i) -
+                             w(IVY, k, j, i) * bcc(IB3, k, j, i);
+          cc(IB2, k, j, i) = w(IVX, k, j, i) * bcc(IB3, k, j, i) -
+                             w(IVZ, k, j, i) * bcc(IB1, k, j, i);
+          cc(IB3, k, j, i) = w(IVY, k, j, i) * bcc(IB1, k, j, i) -
+                             w(IVX, k, j, i) * bcc(IB2, k, j, i);
+        }
+      }
+    }
+  }
+  Arr &w_x1f = b.wx1, &w_x2f = b.wx2, &w_x3f = b.wx3;
+  Arr &e1_x2f = b.e1x2, &e1_x3f = b.e1x3, &e2_x1f = b.e2x1, &e2_x3f = b.e2x3;
+  Arr &e3_x1f = b.e3x1, &e3_x2f = b.e3x2;
+  for (int k = ks; k <= ke + 1; ++k) {
+    for (int j = js; j <= je + 1; ++j) {
+      for (int i = is; i <= ie + 1; ++i) {
+        Real de1_l3 = (1.0 - w_x2f(k - 1, j, i)) * (e1_x3f(k, j, i) - cc(IB1, k - 1, j, i)) +
+                      (w_x2f(k - 1, j, i)) * (e1_x3f(k, j - 1, i) - cc(IB1, k - 1, j - 1, i));
+        Real de1_r3 = (1.0 - w_x2f(k, j, i)) * (e1_x3f(k, j, i) - cc(IB1, k, j, i)) +
+                      (w_x2f(k, j, i)) * (e1_x3f(k, j - 1, i) - cc(IB1, k, j - 1, i));
+        Real de1_l2 = (1.0 - w_x3f(k, j - 1, i)) * (e1_x2f(k, j, i) - cc(IB1, k, j - 1, i)) +
+                      (w_x3f(k, j - 1, i)) * (e1_x2f(k - 1, j, i) - cc(IB1, k - 1, j - 1, i));
+        Real de1_r2 = (1.0 - w_x3f(k, j, i)) * (e1_x2f(k, j, i) - cc(IB1, k, j, i)) +
+                      (w_x3f(k, j, i)) * (e1_x2f(k - 1, j, i) - cc(IB1, k - 1, j, i));
+        b.e1(k, j, i) = 0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + e1_x2f(k - 1, j, i) +
+                                e1_x2f(k, j, i) + e1_x3f(k, j - 1, i) + e1_x3f(k, j, i));
+
+        Real de2_l3 = (1.0 - w_x1f(k - 1, j, i)) * (e2_x3f(k, j, i) - cc(IB2, k - 1, j, i)) +
+                      (w_x1f(k - 1, j, i)) * (e2_x3f(k, j, i - 1) - cc(IB2, k - 1, j, i - 1));
+        Real de2_r3 = (1.0 - w_x1f(k, j, i)) * (e2_x3f(k, j, i) - cc(IB2, k, j, i)) +
+                      (w_x1f(k, j, i)) * (e2_x3f(k, j, i - 1) - cc(IB2, k, j, i - 1));
+        Real de2_l1 = (1.0 - w_x3f(k, j, i - 1)) * (e2_x1f(k, j, i) - cc(IB2, k, j, i - 1)) +
+                      (w_x3f(k, j, i - 1)) * (e2_x1f(k - 1, j, i) - cc(IB2, k - 1, j, i - 1));
+        Real de2_r1 = (1.0 - w_x3f(k, j, i)) * (e2_x1f(k, j, i) - cc(IB2, k, j, i)) +
+                      (w_x3f(k, j, i)) * (e2_x1f(k - 1, j, i) - cc(IB2, k - 1, j, i));
+        b.e2(k, j, i) = 0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + e2_x3f(k, j, i - 1) +
+                                e2_x3f(k, j, i) + e2_x1f(k - 1, j, i) + e2_x1f(k, j, i));
+
+        Real de3_l2 = (1.0 - w_x1f(k, j - 1, i)) * (e3_x2f(k, j, i) - cc(IB3, k, j - 1, i)) +
+                      (w_x1f(k, j - 1, i)) * (e3_x2f(k, j, i - 1) - cc(IB3, k, j - 1, i - 1));
+        Real de3_r2 = (1.0 - w_x1f(k, j, i)) * (e3_x2f(k, j, i) - cc(IB3, k, j, i)) +
+                      (w_x1f(k, j, i)) * (e3_x2f(k, j, i - 1) - cc(IB3, k, j, i - 1));
+        Real de3_l1 = (1.0 - w_x2f(k, j, i - 1)) * (e3_x1f(k, j, i) - cc(IB3, k, j, i - 1)) +
+                      (w_x2f(k, j, i - 1)) * (e3_x1f(k, j - 1, i) - cc(IB3, k, j - 1, i - 1));
+        Real de3_r1 = (1.0 - w_x2f(k, j, i)) * (e3_x1f(k, j, i) - cc(IB3, k, j, i)) +
+                      (w_x2f(k, j, i)) * (e3_x1f(k, j - 1, i) - cc(IB3, k, j - 1, i));
+        b.e3(k, j, i) = 0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + e3_x2f(k, j, i - 1) +
+                                e3_x2f(k, j, i) + e3_x1f(k, j - 1, i) + e3_x1f(k, j, i));
+      }
+    }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// Same-level EMF flux correction for a uniform periodic mesh:
+// SendFluxCorrection/LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313,
+// 623-678), SetFluxBoundarySameLevel (:689-…), AverageFluxBoundary (:1355-1548),
+// neighbour order of BoundaryBase::SearchAndSetNeighbors (bvals_base.cpp:327-700):
+// faces -x1,+x1,-x2,+x2,-x3,+x3, then x1x2 edges, x1x3 edges, x2x3 edges, each edge
+// list ordered (outer offset, inner offset) = (-,-),(+,-),(-,+),(+,+).
+// Every block packs its un-averaged values before receiving (RECV after SEND,
+// time_integrator.cpp:1048-1049), so the additions read a snapshot.
+void AverageEMF(Mesh &m) {
+  const int nbk = static_cast<int>(m.blocks.size());
+  std::vector<Arr> s1(nbk), s2(nbk), s3(nbk);
+  for (int g = 0; g < nbk; ++g) { s1[g] = m.blocks[g].e1; s2[g] = m.blocks[g].e2; s3[g] = m.blocks[g].e3; }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+    int nx1 = b.nx1, nx2 = b.nx2, nx3 = b.nx3;
+    // face neighbours
+    for (int f = 0; f < 6; ++f) {
+      int dir = f / 2, side = f % 2;   // side 0 = inner, 1 = outer
+      int o[3] = {0, 0, 0};
+      o[dir] = side ? 1 : -1;
+      Block &n = m.blocks[b.nb[o[2] + 1][o[1] + 1][o[0] + 1]];
+      Arr &n1 = s1[n.gid], &n2 = s2[n.gid], &n3 = s3[n.gid];
+      if (dir == 0) {
+        int i = side ? ie + 1 : is;
+        int in = side ? n.is : n.ie + 1;
+        for (int k = ks; k <= ke + 1; k++)
+          for (int j = js; j <= je; j++) b.e2(k, j, i) += n2(k, j, in);
+        for (int k = ks; k <= ke; k++)
+          for (int j = js; j <= je + 1; j++) b.e3(k, j, i) += n3(k, j, in);
+      } else if (dir == 1) {
+        int j = side ? je + 1 : js;
+        int jn = side ? n.js : n.je + 1;
+        for (int k = ks; k <= ke + 1; k++)
+          for (int i = is; i <= ie; i++) b.e1(k, j, i) += n1(k, jn, i);
+        for (int k = ks; k <= ke; k++)
+          for (int i = is; i <= ie + 1; i++) b.e3(k, j, i) += n3(k, jn, i);
+      } else {
+        int k = side ? ke + 1 : ks;
+        int kn = side ? n.ks : n.ke + 1;
+        for (int j = js; j <= je + 1; j++)
+          for (int i = is; i <= ie; i++) b.e1(k, j, i) += n1(kn, j, i);
+        for (int j = js; j <= je; j++)
+          for (int i = is; i <= ie + 1; i++) b.e2(k, j, i) += n2(kn, j, i);
+      }
+    }
+    // edge neighbours
+    for (int eid = 0; eid < 12; ++eid) {
+      int lo = (eid & 1) ? 1 : -1, hi = (eid & 2) ? 1 : -1;
+      if (eid < 4) {          // x1x2 edge, e3
+        Block &n = m.blocks[b.nb[1][hi + 1][lo + 1]];
+        int i = (lo > 0) ? ie + 1 : is, j = (hi > 0) ? je + 1 : js;
+        int in = (lo > 0) ? n.is : n.ie + 1, jn = (hi > 0) ? n.js : n.je + 1;
+        for (int k = ks; k <= ke; k++) b.e3(k, j, i) += s3[n.gid](k, jn, in);
+      } else if (eid < 8) {   // x1x3 edge, e2
+        Block &n = m.blocks[b.nb[hi + 1][1][lo + 1]];
+        int i = (lo > 0) ? ie + 1 : is, k = (hi > 0) ? ke + 1 : ks;
+        int in = (lo > 0) ? n.is : n.ie + 1, kn = (hi > 0) ? n.ks : n.ke + 1;
+        for (int j = js; j <= je; j++) b.e2(k, j, i) += s2[n.gid](kn, j, in);
+      } else {                // x2x3 edge, e1
+        Block &n = m.blocks[b.nb[hi + 1][lo + 1][1]];
+        int j = (lo > 0) ? je + 1 : js, k = (hi > 0) ? ke + 1 : ks;
+        int jn = (lo > 0) ? n.js : n.je + 1, kn = (hi > 0) ? n.ks : n.ke + 1;
+        for (int i = is; i <= ie; i++) b.e1(k, j, i) += s1[n.gid](kn, jn, i);
+      }
+    }
+    (void)nx1; (void)nx2; (void)nx3;
+    // AverageFluxBoundary: faces x0.5 (interior of each face), edges x1/nedge_fine=0.25
+    for (int f = 0; f < 6; ++f) {
+      int dir = f / 2, side = f % 2;
+      if (dir == 0) {
+        int i = side ? ie + 1 : is;
+        for (int k = ks + 1; k <= ke; k++)
+          for (int j = js; j <= je; j++) b.e2(k, j, i) *= 0.5;
+        for (int k = ks; k <= ke; k++)
+          for (int j = js + 1; j <= je; j++) b.e3(k, j, i) *= 0.5;
+      } else if (dir == 1) {
+        int j = side ? je + 1 : js;
+        for (int k = ks + 1; k <= ke; k++)
+          for (int i = is; i <= ie; i++) b.e1(k, j, i) *= 0.5;
+        for (int k = ks; k <= ke; k++)
+          for (int i = is + 1; i <= ie; i++) b.e3(k, j, i) *= 0.5;
+      } else {
+        int k = side ? ke + 1 : ks;
+        for (int j = js + 1; j <= je; j++)
+          for (int i = is; i <= ie; i++) b.e1(k, j, i) *= 0.5;
+        for (int j = js; j <= je; j++)
+          for (int i = is + 1; i <= ie; i++) b.e2(k, j, i) *= 0.5;
+      }
+    }
+    const Real div = 1.0 / static_cast<Real>(4);
+    for (int eid = 0; eid < 12; ++eid) {
+      int lo = (eid & 1), hi = (eid & 2);
+      if (eid < 4) {
+        int i = lo ? ie + 1 : is, j = hi ? je + 1 : js;
+        for (int k = ks; k <= ke; k++) b.e3(k, j, i) *= div;
+      } else if (eid < 8) {
+        int i = lo ? ie + 1 : is, k = hi ? ke + 1 : ks;
+        for (int j = js; j <= je; j++) b.e2(k, j, i) *= div;
+      } else {
+        int j = lo ? je + 1 : js, k = hi ? ke + 1 : ks;
+        for (int i = is; i <= ie; i++) b.e1(k, j, i) *= div;
+      }
+    }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// Hydro::AddFluxDivergence (add_flux_divergence.cpp:39-96), Cartesian geometry
+void AddFluxDivergence(Block &b, Real wght, Arr &u_out) {
+  std::vector<Real> dflx(NHYDRO * b.nc1);
+  for (int k = b.ks; k <= b.ke; ++k) {
+    for (int j = b.js; j <= b.je; ++j) {
+      for (int n = 0; n < NHYDRO; ++n) {
+        for (int i = b.is; i <= b.ie; ++i) {
+          Real a_ip1 = b.dx2f[j] * b.dx3f[k];
+          Real a_i = b.dx2f[j] * b.dx3f[k];
+          dflx[n * b.nc1 + i] = (a_ip1 * b.flux[0](n, k, j, i + 1) - a_i * b.flux[0](n, k, j, i));
+        }
+      }
+      for (int n = 0; n < NHYDRO; ++n) {
+        for (int i = b.is; i <= b.ie; ++i) {
+          Real a2 = b.dx1f[i] * b.dx3f[k];
+          Real a2p = b.dx1f[i] * b.dx3f[k];
+          dflx[n * b.nc1 + i] += (a2p * b.flux[1](n, k, j + 1, i) - a2 * b.flux[1](n, k, j, i));
+        }
+      }
+      for (int n = 0; n < NHYDRO; ++n) {
+        for (int i = b.is; i <= b.ie; ++i) {
+          Real a3 = b.dx1f[i] * b.dx2f[j];
+          Real a3p = b.dx1f[i] * b.dx2f[j];
+          dflx[n * b.nc1 + i] += (a3p * b.flux[2](n, k + 1, j, i) - a3 * b.flux[2](n, k, j, i));
+        }
+      }
+      for (int n = 0; n < NHYDRO; ++n) {
+        for (int i = b.is; i <= b.ie; ++i) {
+          Real vol = b.dx1f[i] * b.dx2f[j] * b.dx3f[k];
+          u_out(n, k, j, i) -= wght * dflx[n * b.nc1 + i] / vol;
+        }
+      }
+    }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// Field::CT (ct.cpp:31-116), Cartesian geometry, no polar boundaries
+void CT(Block &b, Real wght) {
+  int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+  for (int k = ks; k <= ke; ++k) {
+    for (int j = js; j <= je; ++j) {
+      for (int i = is; i <= ie + 1; ++i) {
+        Real area = b.dx2f[j] * b.dx3f[k];
+        Real len = b.dx3f[k], len_p1 = b.dx3f[k];
+        b.b1f(k, j, i) -= (wght / area) * (len_p1 * b.e3(k, j + 1, i) - len * b.e3(k, j, i));
+      }
+      for (int i = is; i <= ie + 1; ++i) {
+        Real area = b.dx2f[j] * b.dx3f[k];
+        Real len = b.dx2f[j], len_p1 = b.dx2f[j];
+        b.b1f(k, j, i) += (wght / area) * (len_p1 * b.e2(k + 1, j, i) - len * b.e2(k, j, i));
+      }
+    }
+  }
+  for (int k = ks; k <= ke; ++k) {
+    for (int j = js; j <= je + 1; ++j) {
+      for (int i = is; i <= ie; ++i) {
+        Real area = b.dx1f[i] * b.dx3f[k];
+        Real len_i = b.dx3f[k], len_ip1 = b.dx3f[k];
+        b.b2f(k, j, i) += (wght / area) * (len_ip1 * b.e3(k, j, i + 1) - len_i * b.e3(k, j, i));
+      }
+      for (int i = is; i <= ie; ++i) {
+        Real area = b.dx1f[i] * b.dx3f[k];
+        Real len = b.dx1f[i], len_p1 = b.dx1f[i];
+        b.b2f(k, j, i) -= (wght / area) * (len_p1 * b.e1(k + 1, j, i) - len * b.e1(k, j, i));
+      }
+    }
+  }
+  for (int k = ks; k <= ke + 1; ++k) {
+    for (int j = js; j <= je; ++j) {
+      for (int i = is; i <= ie; ++i) {
+        Real area = b.dx1f[i] * b.dx2f[j];
+        Real len_i = b.dx2f[j], len_ip1 = b.dx2f[j];
+        b.b3f(k, j, i) -= (wght / area) * (len_ip1 * b.e2(k, j, i + 1) - len_i * b.e2(k, j, i));
+      }
+      for (int i = is; i <= ie; ++i) {
+        Real area = b.dx1f[i] * b.dx2f[j];
+        Real len = b.dx1f[i], len_p1 = b.dx1f[i];
+        b.b3f(k, j, i) += (wght / area) * (len_p1 * b.e1(k, j + 1, i) - len * b.e1(k, j, i));
+      }
+    }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// Field::CalculateCellCenteredField, uniform branch (field.cpp:112-180)
+void CellCenteredField(Block &b, int il, int iu, int jl, int ju, int kl, int ku) {
+  for (int k = kl; k <= ku; ++k)
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        Real lw = 0.5, rw = 0.5;
+        b.bcc(IB1, k, j, i) = lw * b.b1f(k, j, i) + rw * b.b1f(k, j, i + 1);
+        b.bcc(IB2, k, j, i) = lw * b.b2f(k, j, i) + rw * b.b2f(k, j + 1, i);
+        b.bcc(IB3, k, j, i) = lw * b.b3f(k, j, i) + rw * b.b3f(k + 1, j, i);
+      }
+}
+
+// EquationOfState::ConservedToPrimitive, adiabatic MHD (adiabatic_mhd.cpp:41-90)
+void Cons2PrimNewtonian(Mesh &m, Block &b, Arr &cons, Arr &prim, int il, int iu, int jl,
+                        int ju, int kl, int ku) {
+  Real gm1 = m.c.gamma - 1.0;
+  CellCenteredField(b, il, iu, jl, ju, kl, ku);
+  for (int k = kl; k <= ku; ++k)
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        Real &u_d = cons(IDN, k, j, i), &u_m1 = cons(IM1, k, j, i);
+        Real &u_m2 = cons(IM2, k, j, i), &u_m3 = cons(IM3, k, j, i);
+        Real &u_e = cons(IEN, k, j, i);
+        Real &w_d = prim(IDN, k, j, i), &w_vx = prim(IVX, k, j, i);
+        Real &w_vy = prim(IVY, k, j, i), &w_vz = prim(IVZ, k, j, i);
+        Real &w_p = prim(IPR, k, j, i);
+        u_d = (u_d > m.dfloor) ? u_d : m.dfloor;
+        w_d = u_d;
+        Real di = 1.0 / u_d;
+        w_vx = u_m1 * di;
+        w_vy = u_m2 * di;
+        w_vz = u_m3 * di;
+        const Real bcc1 = b.bcc(IB1, k, j, i), bcc2 = b.bcc(IB2, k, j, i);
+        const Real bcc3 = b.bcc(IB3, k, j, i);
+        Real pb = 0.5 * (SQR(bcc1) + SQR(bcc2) + SQR(bcc3));
+        Real e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
+        w_p = gm1 * (u_e - e_k - pb);
+        u_e = (w_p > m.pfloor) ? u_e : ((m.pfloor / gm1) + e_k + pb);
+        w_p = (w_p > m.pfloor) ? w_p : m.pfloor;
+      }
+}
+
+
+void ConservedToPrimitive(Mesh &m, Block &b, Arr &cons, Arr &prim_old, Arr &prim) {
+  int il = b.is - b.ng, iu = b.ie + b.ng, jl = b.js - b.ng, ju = b.je + b.ng;
+  int kl = b.ks - b.ng, ku = b.ke + b.ng;
+  if (m.c.relativistic) Cons2PrimSR(m, b, cons, prim_old, prim, il, iu, jl, ju, kl, ku);
+  else Cons2PrimNewtonian(m, b, cons, prim, il, iu, jl, ju, kl, ku);
+}
+
+//----------------------------------------------------------------------------------------
+// Hydro::NewBlockTimeStep (new_blockdt.cpp:45-230), ideal MHD, no diffusion
+void NewBlockTimeStep(Mesh &m, Block &b) {
+  Real min_dt = std::numeric_limits<Real>::max();
+  Real wi[NWAVE];
+  for (int k = b.ks; k <= b.ke; ++k) {
+    for (int j = b.js; j <= b.je; ++j) {
+      std::vector<Real> dt1(b.nc1), dt2(b.nc1), dt3(b.nc1);
+      for (int i = b.is; i <= b.ie; ++i) {
+        dt1[i] = b.dx1f[i]; dt2[i] = b.dx2f[j]; dt3[i] = b.dx3f[k];
+      }
+      if (!m.c.relativistic) {
+        for (int i = b.is; i <= b.ie; ++i) {
+          wi[IDN] = b.w(IDN, k, j, i); wi[IVX] = b.w(IVX, k, j, i);
+          wi[IVY] = b.w(IVY, k, j, i); wi[IVZ] = b.w(IVZ, k, j, i);
+          wi[IPR] = b.w(IPR, k, j, i);
+          Real bx = b.bcc(IB1, k, j, i) + std::abs(b.b1f(k, j, i) - b.bcc(IB1, k, j, i));
+          wi[IBY] = b.bcc(IB2, k, j, i);
+          wi[IBZ] = b.bcc(IB3, k, j, i);
+          Real cf = FastMagnetosonicSpeed(wi, bx, m.c.gamma);
+          Real speed = rmax(0.0, (std::abs(wi[IVX]) + cf));
+          dt1[i] /= (speed);
+
+          wi[IBY] = b.bcc(IB3, k, j, i);
+          wi[IBZ] = b.bcc(IB1, k, j, i);
+          bx = b.bcc(IB2, k, j, i) + std::abs(b.b2f(k, j, i) - b.bcc(IB2, k, j, i));
+          cf = FastMagnetosonicSpeed(wi, bx, m.c.gamma);
+          speed = rmax(0.0, (std::abs(wi[IVY]) + cf));
+          dt2[i] /= (speed);
+
+          wi[IBY] = b.bcc(IB1, k, j, i);
+          wi[IBZ] = b.bcc(IB2, k, j, i);
+          bx = b.bcc(IB3, k, j, i) + std::abs(b.b3f(k, j, i) - b.bcc(IB3, k, j, i));
+          cf = FastMagnetosonicSpeed(wi, bx, m.c.gamma);
+          speed = rmax(0.0, (std::abs(wi[IVZ]) + cf));
+          dt3[i] /= (speed);
+        }
+      }
+      for (int i = b.is; i <= b.ie; ++i) min_dt = rmin(min_dt, dt1[i]);
+      for (int i = b.is; i <= b.ie; ++i) min_dt = rmin(min_dt, dt2[i]);
+      for (int i = b.is; i <= b.ie; ++i) min_dt = rmin(min_dt, dt3[i]);
+    }
+  }
+  min_dt *= m.c.cfl;
+  b.new_dt = min_dt;
+}
+
+// Mesh::NewTimeStep (mesh.cpp:1236-1282)
+void NewTimeStep(Mesh &m, Real tlim) {
+  m.dt = static_cast<Real>(2.0) * m.dt;
+  m.dt = rmin(m.dt, m.blocks[0].new_dt);
+  for (auto &b : m.blocks) m.dt = rmin(m.dt, b.new_dt);
+  if (m.time < tlim && (tlim - m.time) < m.dt) m.dt = tlim - m.time;
+}
+
+//----------------------------------------------------------------------------------------
+// Same-level ghost exchange for a uniform periodic mesh:
+// cell-centred LoadBoundaryBufferSameLevel/SetBoundarySameLevel
+// (cc/bvals_cc.cpp:287-307,386-427) and face-centred (fc/bvals_fc.cpp:344-401,583-688).
+// Packed ranges are always interior values, so the exchange is a copy from the owner.
+void ExchangeCC(Mesh &m, Arr Block::*field, int nvar) {
+  const int nbk = static_cast<int>(m.blocks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    for (int o3 = -1; o3 <= 1; ++o3)
+      for (int o2 = -1; o2 <= 1; ++o2)
+        for (int o1 = -1; o1 <= 1; ++o1) {
+          if (!o1 && !o2 && !o3) continue;
+          Block &n = m.blocks[b.nb[o3 + 1][o2 + 1][o1 + 1]];
+          int s[3], e[3], nxd[3] = {b.nx1, b.nx2, b.nx3}, o[3] = {o1, o2, o3};
+          int lo[3] = {b.is, b.js, b.ks}, hi[3] = {b.ie, b.je, b.ke};
+          for (int d = 0; d < 3; ++d) {
+            if (o[d] == 0) { s[d] = lo[d]; e[d] = hi[d]; }
+            else if (o[d] > 0) { s[d] = hi[d] + 1; e[d] = hi[d] + b.ng; }
+            else { s[d] = lo[d] - b.ng; e[d] = lo[d] - 1; }
+          }
+          Arr &dst = b.*field, &src = n.*field;
+          for (int v = 0; v < nvar; ++v)
+            for (int k = s[2]; k <= e[2]; ++k)
+              for (int j = s[1]; j <= e[1]; ++j)
+                for (int i = s[0]; i <= e[0]; ++i)
+                  dst(v, k, j, i) = src(v, k - o3 * nxd[2], j - o2 * nxd[1], i - o1 * nxd[0]);
+        }
+  }
+}
+
+void ExchangeFC(Mesh &m) {
+  const int nbk = static_cast<int>(m.blocks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    int nxd[3] = {b.nx1, b.nx2, b.nx3};
+    int lo[3] = {b.is, b.js, b.ks}, hi[3] = {b.ie, b.je, b.ke};
+    for (int o3 = -1; o3 <= 1; ++o3)
+      for (int o2 = -1; o2 <= 1; ++o2)
+        for (int o1 = -1; o1 <= 1; ++o1) {
+          if (!o1 && !o2 && !o3) continue;
+          Block &n = m.blocks[b.nb[o3 + 1][o2 + 1][o1 + 1]];
+          int o[3] = {o1, o2, o3};
+          for (int comp = 0; comp < 3; ++comp) {
+            int s[3], e[3];
+            for (int d = 0; d < 3; ++d) {
+              if (d == comp) {   // face-normal direction of this component
+                if (o[d] == 0) { s[d] = lo[d]; e[d] = hi[d] + 1; }
+                else if (o[d] > 0) { s[d] = hi[d] + 2; e[d] = hi[d] + b.ng + 1; }
+                else { s[d] = lo[d] - b.ng; e[d] = lo[d] - 1; }
+              } else {
+                if (o[d] == 0) { s[d] = lo[d]; e[d] = hi[d]; }
+                else if (o[d] > 0) { s[d] = hi[d] + 1; e[d] = hi[d] + b.ng; }
+                else { s[d] = lo[d] - b.ng; e[d] = lo[d] - 1; }
+              }
+            }
+            Arr &dst = (comp == 0) ? b.b1f : ((comp == 1) ? b.b2f : b.b3f);
+            Arr &src = (comp == 0) ? n.b1f : ((comp == 1) ? n.b2f : n.b3f);
+            for (int k = s[2]; k <= e[2]; ++k)
+              for (int j = s[1]; j <= e[1]; ++j)
+                for (int i = s[0]; i <= e[0]; ++i)
+                  dst(k, j, i) = src(k - o3 * nxd[2], j - o2 * nxd[1], i - o1 * nxd[0]);
+          }
+        }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// VL2 time integrator (time_integrator.cpp:111-242 weights; tasks :1606-1885, 2200-2318)
+void Stage(Mesh &m, int stage, Real tlim) {
+  const Real beta = (stage == 1) ? 0.5 : 1.0;
+  const Real delta = (stage == 1) ? 1.0 : 0.0;
+  const int nbk = static_cast<int>(m.blocks.size());
+  const int order = (stage == 1) ? 1 : m.c.xorder;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    if (stage == 1) { b.u1.zero(); b.bb1f.zero(); b.bb2f.zero(); b.bb3f.zero(); }
+    CalculateFluxes(m, b, order);
+    ComputeCornerE(m, b);
+  }
+  AverageEMF(m);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    // WeightedAve(u1, u, ..., {1, delta, 0, 0, 0}) (weighted_ave.cpp:35-…)
+    if (delta != 0.0) {
+      for (int n = 0; n < NHYDRO; ++n)
+        for (int k = b.ks; k <= b.ke; ++k)
+          for (int j = b.js; j <= b.je; ++j)
+            for (int i = b.is; i <= b.ie; ++i) b.u1(n, k, j, i) += delta * b.u(n, k, j, i);
+    }
+    b.u.swap(b.u1);
+    AddFluxDivergence(b, beta * m.dt, b.u);
+    if (delta != 0.0) {
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je; ++j)
+          for (int i = b.is; i <= b.ie + 1; ++i) b.bb1f(k, j, i) += delta * b.b1f(k, j, i);
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je + 1; ++j)
+          for (int i = b.is; i <= b.ie; ++i) b.bb2f(k, j, i) += delta * b.b2f(k, j, i);
+      for (int k = b.ks; k <= b.ke + 1; ++k)
+        for (int j = b.js; j <= b.je; ++j)
+          for (int i = b.is; i <= b.ie; ++i) b.bb3f(k, j, i) += delta * b.b3f(k, j, i);
+    }
+    b.b1f.swap(b.bb1f); b.b2f.swap(b.bb2f); b.b3f.swap(b.bb3f);
+    CT(b, beta * m.dt);
+  }
+  ExchangeCC(m, &Block::u, NHYDRO);
+  ExchangeFC(m);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    ConservedToPrimitive(m, b, b.u, b.w, b.w1);
+    b.w.swap(b.w1);
+    if (stage == 2) NewBlockTimeStep(m, b);
+  }
+  (void)tlim;
+}
+
+void Initialize(Mesh &m, Real tlim) {
+  ExchangeCC(m, &Block::u, NHYDRO);
+  ExchangeFC(m);
+  const int nbk = static_cast<int>(m.blocks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    ConservedToPrimitive(m, b, b.u, b.w1, b.w);
+    NewBlockTimeStep(m, b);
+  }
+  NewTimeStep(m, tlim);
+}
+
+void Cycle(Mesh &m, Real tlim) {
+  Stage(m, 1, tlim);
+  Stage(m, 2, tlim);
+  m.ncycle++;
+  m.time += m.dt;
+  NewTimeStep(m, tlim);
+}
+
+}  // namespace ora
+
+//========================================================================================
+// C ABI used by the test harness (ctypes)
+
+using namespace ora;
+
+extern "C" {
+
+void *ora_new(const ora_cfg *cfg) {
+  Mesh *m = new Mesh;
+  m->c = *cfg;
+  Real fmin = std::numeric_limits<float>::min();
+  m->dfloor = (cfg->dfloor < 0.0) ? std::sqrt(1024 * fmin) : cfg->dfloor;
+  m->pfloor = (cfg->pfloor < 0.0) ? std::sqrt(1024 * fmin) : cfg->pfloor;
+  m->sigma_max = (cfg->sigma_max < 0.0) ? 0.0 : cfg->sigma_max;
+  m->beta_min = (cfg->beta_min < 0.0) ? 0.0 : cfg->beta_min;
+  m->gamma_max = (cfg->gamma_max < 0.0) ? 1000.0 : cfg->gamma_max;
+  m->gm1 = cfg->gamma - 1.0;
+  for (int d = 0; d < 3; ++d) m->nbx[d] = cfg->nx[d] / cfg->bnx[d];
+  int nb = m->nbx[0] * m->nbx[1] * m->nbx[2];
+  m->blocks.resize(nb);
+  for (int g = 0; g < nb; ++g) {
+    Block &b = m->blocks[g];
+    b.gid = g;
+    b.lx[0] = g % m->nbx[0];
+    b.lx[1] = (g / m->nbx[0]) % m->nbx[1];
+    b.lx[2] = g / (m->nbx[0] * m->nbx[1]);
+    b.nx1 = cfg->bnx[0]; b.nx2 = cfg->bnx[1]; b.nx3 = cfg->bnx[2]; b.ng = cfg->ng;
+    b.is = b.ng; b.ie = b.is + b.nx1 - 1;
+    b.js = b.ng; b.je = b.js + b.nx2 - 1;
+    b.ks = b.ng; b.ke = b.ks + b.nx3 - 1;
+    b.nc1 = b.nx1 + 2 * b.ng; b.nc2 = b.nx2 + 2 * b.ng; b.nc3 = b.nx3 + 2 * b.ng;
+    int c1 = b.nc1, c2 = b.nc2, c3 = b.nc3;
+    b.u.alloc(NHYDRO, c3, c2, c1); b.u1.alloc(NHYDRO, c3, c2, c1);
+    b.w.alloc(NHYDRO, c3, c2, c1); b.w1.alloc(NHYDRO, c3, c2, c1);
+    b.bcc.alloc(3, c3, c2, c1); b.cce.alloc(3, c3, c2, c1);
+    b.b1f.alloc(1, c3, c2, c1 + 1); b.b2f.alloc(1, c3, c2 + 1, c1); b.b3f.alloc(1, c3 + 1, c2, c1);
+    b.bb1f.alloc(1, c3, c2, c1 + 1); b.bb2f.alloc(1, c3, c2 + 1, c1); b.bb3f.alloc(1, c3 + 1, c2, c1);
+    b.flux[0].alloc(NHYDRO, c3, c2, c1 + 1);
+    b.flux[1].alloc(NHYDRO, c3, c2 + 1, c1);
+    b.flux[2].alloc(NHYDRO, c3 + 1, c2, c1);
+    b.e2x1.alloc(1, c3, c2, c1 + 1); b.e3x1.alloc(1, c3, c2, c1 + 1); b.wx1.alloc(1, c3, c2, c1 + 1);
+    b.e1x2.alloc(1, c3, c2 + 1, c1); b.e3x2.alloc(1, c3, c2 + 1, c1); b.wx2.alloc(1, c3, c2 + 1, c1);
+    b.e1x3.alloc(1, c3 + 1, c2, c1); b.e2x3.alloc(1, c3 + 1, c2, c1); b.wx3.alloc(1, c3 + 1, c2, c1);
+    b.e1.alloc(1, c3 + 1, c2 + 1, c1); b.e2.alloc(1, c3 + 1, c2, c1 + 1); b.e3.alloc(1, c3, c2 + 1, c1 + 1);
+    SetCoords(*m, b);
+  }
+  for (int g = 0; g < nb; ++g) {
+    Block &b = m->blocks[g];
+    for (int o3 = -1; o3 <= 1; ++o3)
+      for (int o2 = -1; o2 <= 1; ++o2)
+        for (int o1 = -1; o1 <= 1; ++o1) {
+          int l1 = (b.lx[0] + o1 + m->nbx[0]) % m->nbx[0];
+          int l2 = (b.lx[1] + o2 + m->nbx[1]) % m->nbx[1];
+          int l3 = (b.lx[2] + o3 + m->nbx[2]) % m->nbx[2];
+          b.nb[o3 + 1][o2 + 1][o1 + 1] = l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
+        }
+  }
+  return m;
+}
+
+void ora_free(void *h) { delete static_cast<Mesh *>(h); }
+
+int ora_nblocks(void *h) { return static_cast<int>(static_cast<Mesh *>(h)->blocks.size()); }
+
+// what: 0=u 1=w 2=bcc 3=b.x1f 4=b.x2f 5=b.x3f 6=w1 7..9 = x1f,x2f,x3f coords
+//       10..12 = x1v,x2v,x3v 13..15 = dx1f..dx3f
+static Arr *pick(Block &b, int what) {
+  switch (what) {
+    case 0: return &b.u;
+    case 1: return &b.w;
+    case 2: return &b.bcc;
+    case 3: return &b.b1f;
+    case 4: return &b.b2f;
+    case 5: return &b.b3f;
+    case 6: return &b.w1;
+    default: return nullptr;
+  }
+}
+static std::vector<Real> *pickc(Block &b, int what) {
+  switch (what) {
+    case 7: return &b.x1f; case 8: return &b.x2f; case 9: return &b.x3f;
+    case 10: return &b.x1v; case 11: return &b.x2v; case 12: return &b.x3v;
+    case 13: return &b.dx1f; case 14: return &b.dx2f; case 15: return &b.dx3f;
+    default: return nullptr;
+  }
+}
+
+long ora_size(void *h, int gid, int what) {
+  Block &b = static_cast<Mesh *>(h)->blocks[gid];
+  if (Arr *a = pick(b, what)) return static_cast<long>(a->v.size());
+  if (auto *c = pickc(b, what)) return static_cast<long>(c->size());
+  return -1;
+}
+
+int ora_set(void *h, int gid, int what, const double *src) {
+  Block &b = static_cast<Mesh *>(h)->blocks[gid];
+  if (Arr *a = pick(b, what)) { std::memcpy(a->v.data(), src, a->v.size() * 8); return 0; }
+  return -1;
+}
+
+int ora_get(void *h, int gid, int what, double *dst) {
+  Block &b = static_cast<Mesh *>(h)->blocks[gid];
+  if (Arr *a = pick(b, what)) { std::memcpy(dst, a->v.data(), a->v.size() * 8); return 0; }
+  if (auto *c = pickc(b, what)) { std::memcpy(dst, c->data(), c->size() * 8); return 0; }
+  return -1;
+}
+
+void ora_initialize(void *h, double tlim) { Initialize(*static_cast<Mesh *>(h), tlim); }
+
+// Runs up to ncycles cycles while time < tlim; returns the number of cycles done.
+int ora_run(void *h, int ncycles, double tlim) {
+  Mesh &m = *static_cast<Mesh *>(h);
+  int done = 0;
+  while (m.time < tlim && done < ncycles) { Cycle(m, tlim); ++done; }
+  return done;
+}
+
+void ora_state(void *h, double *time, double *dt, int *ncycle) {
+  Mesh &m = *static_cast<Mesh *>(h);
+  *time = m.time; *dt = m.dt; *ncycle = m.ncycle;
+}
+
+void ora_set_dt(void *h, double dt) { static_cast<Mesh *>(h)->dt = dt; }
+
+int ora_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Single-interface Riemann solve for known-answer vectors.  w* are rotated-frame
+// primitive states (7 each), out = 7 fluxes (rotated frame).
+void ora_riemann(int solver, int relativistic, double gamma, int n, const double *wl,
+                 const double *wr, const double *bx, double *out) {
+  Mesh m;
+  std::memset(&m.c, 0, sizeof(m.c));
+  m.c.gamma = gamma; m.c.relativistic = relativistic; m.c.rsolver = solver;
+  Real fmin = std::numeric_limits<float>::min();
+  m.dfloor = std::sqrt(1024 * fmin); m.pfloor = m.dfloor;
+  m.gm1 = gamma - 1.0;
+  for (int q = 0; q < n; ++q) {
+    if (relativistic) RiemannSR(m, wl + 7 * q, wr + 7 * q, bx[q], out + 7 * q);
+    else if (solver == 0) HLLD1(wl + 7 * q, wr + 7 * q, bx[q], gamma, out + 7 * q);
+    else HLLE1(wl + 7 * q, wr + 7 * q, bx[q], gamma, out + 7 * q);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+"""ctypes binding of the CPU restatement in oracle/ (test infrastructure only).
+
+The oracle is the checker: tests feed it the same initial registers as the device
+library and compare the results.  It is built by ``make -C oracle``."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from athena_amd.mesh import MeshDesc, RSOLVER
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+WHAT = {"u": 0, "w": 1, "bcc": 2, "b1": 3, "b2": 4, "b3": 5, "w1": 6,
+        "x1f": 7, "x2f": 8, "x3f": 9, "x1v": 10, "x2v": 11, "x3v": 12,
+        "dx1f": 13, "dx2f": 14, "dx3f": 15}
+
+
+class OraCfg(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int * 3), ("bnx", ctypes.c_int * 3), ("ng", ctypes.c_int),
+                ("xorder", ctypes.c_int), ("rsolver", ctypes.c_int),
+                ("relativistic", ctypes.c_int),
+                ("xmin", ctypes.c_double * 3), ("xmax", ctypes.c_double * 3),
+                ("gamma", ctypes.c_double), ("dfloor", ctypes.c_double),
+                ("pfloor", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("sigma_max", ctypes.c_double), ("beta_min", ctypes.c_double),
+                ("gamma_max", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+        L = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        D = ctypes.POINTER(ctypes.c_double)
+        L.ora_new.restype = P
+        L.ora_new.argtypes = [ctypes.POINTER(OraCfg)]
+        L.ora_free.argtypes = [P]
+        L.ora_nblocks.argtypes = [P]
+        L.ora_size.restype = ctypes.c_long
+        L.ora_size.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.ora_set.argtypes = [P, ctypes.c_int, ctypes.c_int, D]
+        L.ora_get.argtypes = [P, ctypes.c_int, ctypes.c_int, D]
+        L.ora_initialize.argtypes = [P, ctypes.c_double]
+        L.ora_run.argtypes = [P, ctypes.c_int, ctypes.c_double]
+        L.ora_run.restype = ctypes.c_int
+        L.ora_state.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int)]
+        L.ora_set_dt.argtypes = [P, ctypes.c_double]
+        L.ora_threads.restype = ctypes.c_int
+        L.ora_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  D, D, D, D]
+        L.ora_sr_con2prim.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + \
+            [D, D, D, ctypes.POINTER(ctypes.c_int)]
+        L.ora_sr_prim2cons.argtypes = [ctypes.c_int, ctypes.c_double, D, D, D]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def cfg_of(m: MeshDesc) -> OraCfg:
+    c = OraCfg()
+    c.nx[:] = list(m.nx)
+    c.bnx[:] = list(m.bnx)
+    c.ng = m.ng
+    c.xorder = m.xorder
+    c.rsolver = RSOLVER[m.rsolver]
+    c.relativistic = int(m.relativistic)
+    c.xmin[:] = list(m.xmin)
+    c.xmax[:] = list(m.xmax)
+    c.gamma = m.gamma
+    c.dfloor, c.pfloor = m.dfloor, m.pfloor
+    c.cfl = m.cfl
+    c.sigma_max, c.beta_min, c.gamma_max = m.sigma_max, m.beta_min, m.gamma_max
+    return c
+
+
+class OracleMesh:
+    """CPU restatement of one uniform periodic mesh."""
+
+    def __init__(self, m: MeshDesc):
+        self.m = m
+        self._cfg = cfg_of(m)
+        self.h = lib().ora_new(ctypes.byref(self._cfg))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_free(self.h)
+            self.h = None
+
+    def set(self, gid, what, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        n = lib().ora_size(self.h, gid, WHAT[what])
+        assert a.size == n, (what, a.size, n)
+        assert lib().ora_set(self.h, gid, WHAT[what], _dp(a)) == 0
+
+    def get(self, gid, what):
+        n = lib().ora_size(self.h, gid, WHAT[what])
+        a = np.empty(n)
+        assert lib().ora_get(self.h, gid, WHAT[what], _dp(a)) == 0
+        if what in self.m.shape.__code__.co_consts or what in ("u", "w", "w1", "bcc", "b1", "b2", "b3"):
+            return a.reshape(self.m.shape(what))
+        return a
+
+    def load(self, blocks):
+        for gid, b in enumerate(blocks):
+            for k in ("u", "b1", "b2", "b3", "w", "w1"):
+                if k in b:
+                    self.set(gid, k, b[k])
+
+    def initialize(self, tlim=1e300):
+        lib().ora_initialize(self.h, tlim)
+
+    def run(self, ncycles, tlim=1e300):
+        return lib().ora_run(self.h, ncycles, tlim)
+
+    def state(self):
+        t, dt, nc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        lib().ora_state(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(nc))
+        return t.value, dt.value, nc.value
+
+    def block(self, gid):
+        return {k: self.get(gid, k) for k in ("u", "w", "bcc", "b1", "b2", "b3")}
+
+
+def riemann(solver, relativistic, gamma, wl, wr, bx):
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    bx = np.ascontiguousarray(bx, dtype=np.float64)
+    out = np.empty_like(wl)
+    lib().ora_riemann(solver, int(relativistic), gamma, wl.shape[0], _dp(wl), _dp(wr),
+                      _dp(bx), _dp(out))
+    return out
+
+
+def sr_con2prim(cons, prim_seed, bcc, gamma, dfloor, pfloor, sigma_max=0.0, beta_min=0.0,
+                gamma_max=1000.0):
+    cons = np.array(cons, dtype=np.float64, order="C")
+    prim = np.array(prim_seed, dtype=np.float64, order="C")
+    bcc = np.ascontiguousarray(bcc, dtype=np.float64)
+    iters = np.empty(cons.shape[0], dtype=np.int32)
+    lib().ora_sr_con2prim(cons.shape[0], gamma, dfloor, pfloor, sigma_max, beta_min,
+                          gamma_max, _dp(cons), _dp(prim), _dp(bcc),
+                          iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return cons, prim, iters
+
+
+def sr_prim2cons(prim, bcc, gamma):
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    bcc = np.ascontiguousarray(bcc, dtype=np.float64)
+    cons = np.empty_like(prim)
+    lib().ora_sr_prim2cons(prim.shape[0], gamma, _dp(prim), _dp(bcc), _dp(cons))
+    return cons
