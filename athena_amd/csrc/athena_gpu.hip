@@ -1,0 +1,1459 @@
+// athena_gpu.hip -- MI355X (gfx950) implementation of Athena++'s per-MeshBlock MHD
+// update behind the C ABI declared in include/athena_gpu.h.
+//
+// Data layout in HBM: one pool per register, blocks back to back, each block an
+// AthenaArray ((n,k,j,i), i fastest).  Kernels take the whole mesh in one launch
+// (grid.y = block id), so a VL2 stage is five launches regardless of block count:
+//   flux x1/x2/x3 (reconstruct + Riemann) -> corner EMF -> update (u and b, with the
+//   same-level EMF averaging folded in and the ghost exchange done as a push from the
+//   owning cell) -> primitive recovery (+ CFL reduction on the last stage).
+// The mesh clock (time, dt) lives in device memory, so cycles run without host
+// synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/athena_gpu.h"
+#include "physics.hpp"
+
+using namespace athgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(std::string(#expr) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+constexpr int WG = 256;
+
+// Geometry and constants shared by all kernels of one mesh (passed by value).
+struct MeshK {
+  int nx1, nx2, nx3, ng;
+  int nc1, nc2, nc3;
+  int is, ie, js, je, ks, ke;
+  int nblocks;
+  long ncell;                 // nc1*nc2*nc3
+  long nf1, nf2, nf3;         // x1f, x2f, x3f face-array sizes per block
+  long ne1, ne2, ne3;         // x1e, x2e, x3e edge-array sizes per block
+  int cstride;                // doubles of coordinate table per block
+  int coff[3];                // offset of direction d inside the table
+  const int *nbr;             // [nblocks][27] same-level neighbour gid
+  const double *coord;        // [nblocks][cstride]: per d: dxf[nc], pf[nc], mf[nc]
+  EosK eos;
+  double cfl;
+};
+
+struct Clock {
+  double time, dt, tlim;
+  double ncycle;
+};
+
+__device__ __forceinline__ long cidx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * M.nc2 + j) * M.nc1 + i;
+}
+__device__ __forceinline__ long f1idx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * M.nc2 + j) * (M.nc1 + 1) + i;
+}
+__device__ __forceinline__ long f2idx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * (M.nc2 + 1) + j) * M.nc1 + i;
+}
+__device__ __forceinline__ long f3idx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * M.nc2 + j) * M.nc1 + i;
+}
+// edge arrays: x1e (nc3+1, nc2+1, nc1), x2e (nc3+1, nc2, nc1+1), x3e (nc3, nc2+1, nc1+1)
+__device__ __forceinline__ long e1idx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * (M.nc2 + 1) + j) * M.nc1 + i;
+}
+__device__ __forceinline__ long e2idx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * M.nc2 + j) * (M.nc1 + 1) + i;
+}
+__device__ __forceinline__ long e3idx(const MeshK &M, int k, int j, int i) {
+  return (static_cast<long>(k) * (M.nc2 + 1) + j) * (M.nc1 + 1) + i;
+}
+__device__ __forceinline__ int nbr(const MeshK &M, int g, int o1, int o2, int o3) {
+  return M.nbr[g * 27 + (o3 + 1) * 9 + (o2 + 1) * 3 + (o1 + 1)];
+}
+__device__ __forceinline__ const double *ctab(const MeshK &M, int g, int d) {
+  return M.coord + static_cast<long>(g) * M.cstride + M.coff[d];
+}
+
+//========================================================================================
+// Reconstruction of one variable at one interface along DIR.
+//   q[] holds the variable at cells c-3 .. c+2 (q[3] is the cell on the right of the
+//   face, q[2] the cell on the left).  Produces the left state (right edge of cell c-1)
+//   and right state (left edge of cell c).
+//   ORDER 1: donor cell (dc.cpp:24-103).
+//   ORDER 2: PLM, uniform VL limiter (plm.cpp:105-112, faces :149-155 / :300-308).
+//   ORDER 3: PPM + Colella-Sekora limiter (ppm.cpp, uniform Cartesian branch).
+
+__device__ __forceinline__ double vl_slope(double qm, double q0, double qp) {
+  double dwl = (q0 - qm);
+  double dwr = (qp - q0);
+  double dw2 = dwl * dwr;
+  double dwm = 2.0 * dw2 / (dwl + dwr);
+  if (dw2 <= 0.0) dwm = 0.0;
+  return dwm;
+}
+
+// PPM edge values of the cell q[2] given q[0..4] = cells (c-2..c+2):
+// returns (qminus, qplus) = (left-edge, right-edge) values after limiting.
+__device__ __forceinline__ void ppm_cell(const double *q, double &qminus, double &qplus) {
+  const double C2 = 1.25;
+  const double c1 = 0.5, c2 = 0.5, c3 = 0.5, c4 = 0.5, c5 = 1.0 / 6.0, c6 = -1.0 / 6.0;
+  const double q_im2 = q[0], q_im1 = q[1], q0 = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  double qa = (q0 - q_im1);
+  double qb = (q_ip1 - q0);
+  double dd_im1 = c1 * qa + c2 * (q_im1 - q_im2);
+  double dd = c1 * qb + c2 * qa;
+  double dd_ip1 = c1 * (q_ip2 - q_ip1) + c2 * qb;
+  double dph = (c3 * q_im1 + c4 * q0) + (c5 * dd_im1 + c6 * dd);
+  double dph_ip1 = (c3 * q0 + c4 * q_ip1) + (c5 * dd + c6 * dd_ip1);
+  double d2qc_im1 = q_im2 + q0 - 2.0 * q_im1;
+  double d2qc = q_im1 + q_ip1 - 2.0 * q0;
+  double d2qc_ip1 = q0 + q_ip2 - 2.0 * q_ip1;
+  {
+    double qa_tmp = dph - q_im1, qb_tmp = q0 - dph;
+    double a = 3.0 * (q_im1 + q0 - 2.0 * dph), b = d2qc_im1, c = d2qc, qd = 0.0;
+    if (sgn(a) == sgn(b) && sgn(a) == sgn(c))
+      qd = sgn(a) * dmin(C2 * fabs(b), dmin(C2 * fabs(c), fabs(a)));
+    double t = 0.5 * (q_im1 + q0) - qd / 6.0;
+    if (qa_tmp * qb_tmp < 0.0) dph = t;
+  }
+  {
+    double qa_tmp = dph_ip1 - q0, qb_tmp = q_ip1 - dph_ip1;
+    double a = 3.0 * (q0 + q_ip1 - 2.0 * dph_ip1), b = d2qc, c = d2qc_ip1, qd = 0.0;
+    if (sgn(a) == sgn(b) && sgn(a) == sgn(c))
+      qd = sgn(a) * dmin(C2 * fabs(b), dmin(C2 * fabs(c), fabs(a)));
+    double t = 0.5 * (q0 + q_ip1) - qd / 6.0;
+    if (qa_tmp * qb_tmp < 0.0) dph_ip1 = t;
+  }
+  double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * q0);
+  qminus = dph;
+  qplus = dph_ip1;
+  double dqf_minus = q0 - qminus;
+  double dqf_plus = qplus - q0;
+  double qa_tmp = dqf_minus * dqf_plus;
+  double qb_tmp = (q_ip1 - q0) * (q0 - q_im1);
+  double a = d2qc_im1, b = d2qc, c = d2qc_ip1, qd = d2qf, qe = 0.0;
+  if (sgn(a) == sgn(b) && sgn(a) == sgn(c) && sgn(a) == sgn(qd))
+    qe = sgn(qd) * dmin(dmin(C2 * fabs(a), C2 * fabs(b)), dmin(C2 * fabs(c), fabs(qd)));
+  a = dmax(fabs(q_im1), fabs(q_im2));
+  b = dmax(dmax(fabs(q0), fabs(q_ip1)), fabs(q_ip2));
+  double rho = 0.0;
+  if (fabs(qd) > (1.0e-12) * dmax(a, b)) rho = qe / qd;
+  double tmp_m = q0 - rho * dqf_minus;
+  double tmp_p = q0 + rho * dqf_plus;
+  double tmp2_m = q0 - 2.0 * dqf_plus;
+  double tmp2_p = q0 + 2.0 * dqf_minus;
+  if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
+    if (rho <= (1.0 - (1.0e-12))) {
+      qminus = tmp_m;
+      qplus = tmp_p;
+    }
+  } else {
+    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qminus = tmp2_m;
+    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) qplus = tmp2_p;
+  }
+}
+
+//========================================================================================
+// Kernel 1: interface fluxes along DIR for every block (Hydro::CalculateFluxes one
+// direction + the Riemann solver + GetWeightForCT).  One thread per interface.
+template <int DIR, int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__ w,
+                                              const double *__restrict__ bcc,
+                                              const double *__restrict__ bn,
+                                              double *__restrict__ flx,
+                                              double *__restrict__ ey,
+                                              double *__restrict__ ez,
+                                              double *__restrict__ wct,
+                                              const Clock *__restrict__ clk) {
+  const int g = blockIdx.y;
+  int ni, nj, nk, i0, j0, k0;
+  if constexpr (DIR == 0) { ni = M.nx1 + 1; nj = M.nx2 + 2; nk = M.nx3 + 2; i0 = M.is; j0 = M.js - 1; k0 = M.ks - 1; }
+  else if constexpr (DIR == 1) { ni = M.nx1 + 2; nj = M.nx2 + 1; nk = M.nx3 + 2; i0 = M.is - 1; j0 = M.js; k0 = M.ks - 1; }
+  else { ni = M.nx1 + 2; nj = M.nx2 + 2; nk = M.nx3 + 1; i0 = M.is - 1; j0 = M.js - 1; k0 = M.ks; }
+  const int t = blockIdx.x * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = i0 + t % ni;
+  const int r = t / ni;
+  const int j = j0 + r % nj;
+  const int k = k0 + r / nj;
+
+  const long s = (DIR == 0) ? 1 : ((DIR == 1) ? M.nc1 : static_cast<long>(M.nc1) * M.nc2);
+  const long c = cidx(M, k, j, i);
+  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+  constexpr int ivx = IVX + DIR;
+  constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  constexpr int cby = (DIR == 0) ? IB2 : ((DIR == 1) ? IB3 : IB1);
+  constexpr int cbz = (DIR == 0) ? IB3 : ((DIR == 1) ? IB1 : IB2);
+  const int fc = (DIR == 0) ? i : ((DIR == 1) ? j : k);    // face/cell index along DIR
+  const double *ct = ctab(M, g, DIR);
+  const int ncd = (DIR == 0) ? M.nc1 : ((DIR == 1) ? M.nc2 : M.nc3);
+
+  double wl[NWAVE], wr[NWAVE];
+#pragma unroll
+  for (int v = 0; v < NWAVE; ++v) {
+    const double *src;
+    if (v == IDN) src = wb;
+    else if (v == IVX) src = wb + ivx * M.ncell;
+    else if (v == IVY) src = wb + ivy * M.ncell;
+    else if (v == IVZ) src = wb + ivz * M.ncell;
+    else if (v == IPR) src = wb + IPR * M.ncell;
+    else if (v == IBY) src = bb + cby * M.ncell;
+    else src = bb + cbz * M.ncell;
+    if (ORDER == 1) {
+      wl[v] = src[c - s];
+      wr[v] = src[c];
+    } else if (ORDER == 2) {
+      double qm2 = src[c - 2 * s], qm1 = src[c - s], q0 = src[c], qp1 = src[c + s];
+      double dl = vl_slope(qm2, qm1, q0);
+      double dr = vl_slope(qm1, q0, qp1);
+      wl[v] = qm1 + ct[ncd + (fc - 1)] * dl;        // pf of cell fc-1
+      wr[v] = q0 - ct[2 * ncd + fc] * dr;           // mf of cell fc
+    } else {
+      double q[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) q[m] = src[c + (m - 3) * s];
+      double qmL, qpL, qmR, qpR;
+      ppm_cell(q, qmL, qpL);        // cell c-1: q[0..4]
+      ppm_cell(q + 1, qmR, qpR);    // cell c:   q[1..5]
+      wl[v] = qpL;
+      wr[v] = qmR;
+    }
+  }
+  if (ORDER == 3) {   // ApplyPrimitiveFloors on both reconstructed states
+    wl[IDN] = (wl[IDN] > M.eos.dfloor) ? wl[IDN] : M.eos.dfloor;
+    wl[IPR] = (wl[IPR] > M.eos.pfloor) ? wl[IPR] : M.eos.pfloor;
+    wr[IDN] = (wr[IDN] > M.eos.dfloor) ? wr[IDN] : M.eos.dfloor;
+    wr[IPR] = (wr[IPR] > M.eos.pfloor) ? wr[IPR] : M.eos.pfloor;
+  }
+
+  long fi;
+  if (DIR == 0) fi = f1idx(M, k, j, i);
+  else if (DIR == 1) fi = f2idx(M, k, j, i);
+  else fi = f3idx(M, k, j, i);
+  const long nf = (DIR == 0) ? M.nf1 : ((DIR == 1) ? M.nf2 : M.nf3);
+  const double bx = bn[static_cast<long>(g) * nf + fi];
+
+  double f[NWAVE];
+  if (RS == 0) hlld(wl, wr, bx, M.eos, f);
+  else if (RS == 1) hlle(wl, wr, bx, M.eos, f);
+  else sr_hlle(wl, wr, bx, M.eos, f);
+
+  double *fb = flx + static_cast<long>(g) * NHYDRO * nf;
+  fb[IDN * nf + fi] = f[IDN];
+  fb[ivx * nf + fi] = f[IVX];
+  fb[ivy * nf + fi] = f[IVY];
+  fb[ivz * nf + fi] = f[IVZ];
+  fb[IEN * nf + fi] = f[IEN];
+  ey[static_cast<long>(g) * nf + fi] = -f[IBY];
+  ez[static_cast<long>(g) * nf + fi] = f[IBZ];
+  const double dxw = ct[fc];
+  wct[static_cast<long>(g) * nf + fi] = ct_weight(f[IDN], wl[IDN], wr[IDN], dxw, clk->dt);
+  (void)ncd;
+}
+
+//========================================================================================
+// Kernel 2: corner EMFs, Field::ComputeCornerE 3-D branch (calculate_corner_e.cpp:129-230)
+// One thread per edge position (k,j,i) in [ks..ke+1]x[js..je+1]x[is..ie+1]; the
+// cell-centred E (=-v x B, SR: divided by u0) is recomputed on the fly.
+template <bool SR>
+__device__ __forceinline__ double cc_e(const MeshK &M, const double *wb, const double *bb,
+                                       int comp, long c) {
+  const long n = M.ncell;
+  double vx = wb[IVX * n + c], vy = wb[IVY * n + c], vz = wb[IVZ * n + c];
+  double b1 = bb[IB1 * n + c], b2 = bb[IB2 * n + c], b3 = bb[IB3 * n + c];
+  if (!SR) {
+    if (comp == 0) return vz * b2 - vy * b3;
+    if (comp == 1) return vx * b3 - vz * b1;
+    return vy * b1 - vx * b2;
+  } else {
+    double u0 = sqrt(1.0 + sq(vx) + sq(vy) + sq(vz));
+    if (comp == 0) return (vz * b2 - vy * b3) / u0;
+    if (comp == 1) return (vx * b3 - vz * b1) / u0;
+    return (vy * b1 - vx * b2) / u0;
+  }
+}
+
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_corner(
+    MeshK M, const double *__restrict__ w, const double *__restrict__ bcc,
+    const double *__restrict__ e2x1, const double *__restrict__ e3x1,
+    const double *__restrict__ e1x2, const double *__restrict__ e3x2,
+    const double *__restrict__ e1x3, const double *__restrict__ e2x3,
+    const double *__restrict__ wx1, const double *__restrict__ wx2,
+    const double *__restrict__ wx3, double *__restrict__ e1, double *__restrict__ e2,
+    double *__restrict__ e3) {
+  const int g = blockIdx.y;
+  const int ni = M.nx1 + 1, nj = M.nx2 + 1, nk = M.nx3 + 1;
+  const int t = blockIdx.x * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = M.is + t % ni;
+  const int r = t / ni;
+  const int j = M.js + r % nj;
+  const int k = M.ks + r / nj;
+  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+  const long o1 = static_cast<long>(g) * M.nf1, o2 = static_cast<long>(g) * M.nf2;
+  const long o3 = static_cast<long>(g) * M.nf3;
+#define W1F(kk, jj, ii) wx1[o1 + f1idx(M, kk, jj, ii)]
+#define W2F(kk, jj, ii) wx2[o2 + f2idx(M, kk, jj, ii)]
+#define W3F(kk, jj, ii) wx3[o3 + f3idx(M, kk, jj, ii)]
+#define E2X1(kk, jj, ii) e2x1[o1 + f1idx(M, kk, jj, ii)]
+#define E3X1(kk, jj, ii) e3x1[o1 + f1idx(M, kk, jj, ii)]
+#define E1X2(kk, jj, ii) e1x2[o2 + f2idx(M, kk, jj, ii)]
+#define E3X2(kk, jj, ii) e3x2[o2 + f2idx(M, kk, jj, ii)]
+#define E1X3(kk, jj, ii) e1x3[o3 + f3idx(M, kk, jj, ii)]
+#define E2X3(kk, jj, ii) e2x3[o3 + f3idx(M, kk, jj, ii)]
+#define CC(comp, kk, jj, ii) cc_e<SR>(M, wb, bb, comp, cidx(M, kk, jj, ii))
+  {
+    double de1_l3 = (1.0 - W2F(k - 1, j, i)) * (E1X3(k, j, i) - CC(0, k - 1, j, i)) +
+                    (W2F(k - 1, j, i)) * (E1X3(k, j - 1, i) - CC(0, k - 1, j - 1, i));
+    double de1_r3 = (1.0 - W2F(k, j, i)) * (E1X3(k, j, i) - CC(0, k, j, i)) +
+                    (W2F(k, j, i)) * (E1X3(k, j - 1, i) - CC(0, k, j - 1, i));
+    double de1_l2 = (1.0 - W3F(k, j - 1, i)) * (E1X2(k, j, i) - CC(0, k, j - 1, i)) +
+                    (W3F(k, j - 1, i)) * (E1X2(k - 1, j, i) - CC(0, k - 1, j - 1, i));
+    double de1_r2 = (1.0 - W3F(k, j, i)) * (E1X2(k, j, i) - CC(0, k, j, i)) +
+                    (W3F(k, j, i)) * (E1X2(k - 1, j, i) - CC(0, k - 1, j, i));
+    e1[static_cast<long>(g) * M.ne1 + e1idx(M, k, j, i)] =
+        0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + E1X2(k - 1, j, i) + E1X2(k, j, i) +
+                E1X3(k, j - 1, i) + E1X3(k, j, i));
+  }
+  {
+    double de2_l3 = (1.0 - W1F(k - 1, j, i)) * (E2X3(k, j, i) - CC(1, k - 1, j, i)) +
+                    (W1F(k - 1, j, i)) * (E2X3(k, j, i - 1) - CC(1, k - 1, j, i - 1));
+    double de2_r3 = (1.0 - W1F(k, j, i)) * (E2X3(k, j, i) - CC(1, k, j, i)) +
+                    (W1F(k, j, i)) * (E2X3(k, j, i - 1) - CC(1, k, j, i - 1));
+    double de2_l1 = (1.0 - W3F(k, j, i - 1)) * (E2X1(k, j, i) - CC(1, k, j, i - 1)) +
+                    (W3F(k, j, i - 1)) * (E2X1(k - 1, j, i) - CC(1, k - 1, j, i - 1));
+    double de2_r1 = (1.0 - W3F(k, j, i)) * (E2X1(k, j, i) - CC(1, k, j, i)) +
+                    (W3F(k, j, i)) * (E2X1(k - 1, j, i) - CC(1, k - 1, j, i));
+    e2[static_cast<long>(g) * M.ne2 + e2idx(M, k, j, i)] =
+        0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + E2X3(k, j, i - 1) + E2X3(k, j, i) +
+                E2X1(k - 1, j, i) + E2X1(k, j, i));
+  }
+  {
+    double de3_l2 = (1.0 - W1F(k, j - 1, i)) * (E3X2(k, j, i) - CC(2, k, j - 1, i)) +
+                    (W1F(k, j - 1, i)) * (E3X2(k, j, i - 1) - CC(2, k, j - 1, i - 1));
+    double de3_r2 = (1.0 - W1F(k, j, i)) * (E3X2(k, j, i) - CC(2, k, j, i)) +
+                    (W1F(k, j, i)) * (E3X2(k, j, i - 1) - CC(2, k, j, i - 1));
+    double de3_l1 = (1.0 - W2F(k, j, i - 1)) * (E3X1(k, j, i) - CC(2, k, j, i - 1)) +
+                    (W2F(k, j, i - 1)) * (E3X1(k, j - 1, i) - CC(2, k, j - 1, i - 1));
+    double de3_r1 = (1.0 - W2F(k, j, i)) * (E3X1(k, j, i) - CC(2, k, j, i)) +
+                    (W2F(k, j, i)) * (E3X1(k, j - 1, i) - CC(2, k, j - 1, i));
+    e3[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)] =
+        0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + E3X2(k, j, i - 1) + E3X2(k, j, i) +
+                E3X1(k, j - 1, i) + E3X1(k, j, i));
+  }
+#undef W1F
+#undef W2F
+#undef W3F
+#undef E2X1
+#undef E3X1
+#undef E1X2
+#undef E3X2
+#undef E1X3
+#undef E2X3
+#undef CC
+}
+
+//========================================================================================
+// Same-level EMF flux correction on a uniform periodic mesh, evaluated where CT reads
+// the EMF.  A surface EMF is own + face neighbour(s) + edge neighbour, summed in the
+// reference's neighbour order (x1 faces, x2 faces, x3 faces, then the edge) and scaled
+// by 1/2 (face) or 1/nedge_fine = 1/4 (edge) (flux_correction_fc.cpp:51-…, 689-…,
+// 1355-1548; bvals_base.cpp:327-700).  Raw EMFs are never modified, so every thread
+// reads the un-averaged neighbour values exactly like the packed send buffers.
+__device__ __forceinline__ double emf3(const MeshK &M, const double *e, int g, int k,
+                                       int j, int i) {
+  double v = e[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)];
+  const int o1 = (i == M.is) ? -1 : ((i == M.ie + 1) ? 1 : 0);
+  const int o2 = (j == M.js) ? -1 : ((j == M.je + 1) ? 1 : 0);
+  if (!o1 && !o2) return v;
+  if (o1) v += e[static_cast<long>(nbr(M, g, o1, 0, 0)) * M.ne3 + e3idx(M, k, j, i - o1 * M.nx1)];
+  if (o2) v += e[static_cast<long>(nbr(M, g, 0, o2, 0)) * M.ne3 + e3idx(M, k, j - o2 * M.nx2, i)];
+  if (o1 && o2) {
+    v += e[static_cast<long>(nbr(M, g, o1, o2, 0)) * M.ne3 +
+           e3idx(M, k, j - o2 * M.nx2, i - o1 * M.nx1)];
+    return v * 0.25;
+  }
+  return v * 0.5;
+}
+__device__ __forceinline__ double emf2(const MeshK &M, const double *e, int g, int k,
+                                       int j, int i) {
+  double v = e[static_cast<long>(g) * M.ne2 + e2idx(M, k, j, i)];
+  const int o1 = (i == M.is) ? -1 : ((i == M.ie + 1) ? 1 : 0);
+  const int o3 = (k == M.ks) ? -1 : ((k == M.ke + 1) ? 1 : 0);
+  if (!o1 && !o3) return v;
+  if (o1) v += e[static_cast<long>(nbr(M, g, o1, 0, 0)) * M.ne2 + e2idx(M, k, j, i - o1 * M.nx1)];
+  if (o3) v += e[static_cast<long>(nbr(M, g, 0, 0, o3)) * M.ne2 + e2idx(M, k - o3 * M.nx3, j, i)];
+  if (o1 && o3) {
+    v += e[static_cast<long>(nbr(M, g, o1, 0, o3)) * M.ne2 +
+           e2idx(M, k - o3 * M.nx3, j, i - o1 * M.nx1)];
+    return v * 0.25;
+  }
+  return v * 0.5;
+}
+__device__ __forceinline__ double emf1(const MeshK &M, const double *e, int g, int k,
+                                       int j, int i) {
+  double v = e[static_cast<long>(g) * M.ne1 + e1idx(M, k, j, i)];
+  const int o2 = (j == M.js) ? -1 : ((j == M.je + 1) ? 1 : 0);
+  const int o3 = (k == M.ks) ? -1 : ((k == M.ke + 1) ? 1 : 0);
+  if (!o2 && !o3) return v;
+  if (o2) v += e[static_cast<long>(nbr(M, g, 0, o2, 0)) * M.ne1 + e1idx(M, k, j - o2 * M.nx2, i)];
+  if (o3) v += e[static_cast<long>(nbr(M, g, 0, 0, o3)) * M.ne1 + e1idx(M, k - o3 * M.nx3, j, i)];
+  if (o2 && o3) {
+    v += e[static_cast<long>(nbr(M, g, 0, o2, o3)) * M.ne1 +
+           e1idx(M, k - o3 * M.nx3, j - o2 * M.nx2, i)];
+    return v * 0.25;
+  }
+  return v * 0.5;
+}
+
+//========================================================================================
+// Kernel 3a: conserved update, MeshBlock::WeightedAve + Hydro::AddFluxDivergence
+// (weighted_ave.cpp:35-…, add_flux_divergence.cpp:39-96) for active cells, followed by
+// the same-level ghost exchange done as a push: each updated cell is also written to
+// every neighbour ghost cell that is a copy of it (cc/bvals_cc.cpp:287-427).
+// first_stage: out = (0 + 1*base) - wght*div F   (u1 zeroed, delta = 1, then swapped)
+// otherwise:   out = base - wght*div F           (in place allowed: out == base)
+__global__ void __launch_bounds__(WG) k_update_u(MeshK M, const double *base, double *out,
+                                                  const double *__restrict__ f1,
+                                                  const double *__restrict__ f2,
+                                                  const double *__restrict__ f3,
+                                                  double beta, int first_stage,
+                                                  const Clock *__restrict__ clk) {
+  const int g = blockIdx.y;
+  const int ni = M.nx1, nj = M.nx2, nk = M.nx3;
+  const int t = blockIdx.x * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = M.is + t % ni;
+  const int r = t / ni;
+  const int j = M.js + r % nj;
+  const int k = M.ks + r / nj;
+  const double wght = beta * clk->dt;
+  const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
+  const double a1 = dx2[j] * dx3[k];
+  const double a2 = dx1[i] * dx3[k];
+  const double a3 = dx1[i] * dx2[j];
+  const double vol = dx1[i] * dx2[j] * dx3[k];
+  const long c = cidx(M, k, j, i);
+  const long b0 = static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *F1 = f1 + static_cast<long>(g) * NHYDRO * M.nf1;
+  const double *F2 = f2 + static_cast<long>(g) * NHYDRO * M.nf2;
+  const double *F3 = f3 + static_cast<long>(g) * NHYDRO * M.nf3;
+  const long x1 = f1idx(M, k, j, i), x2 = f2idx(M, k, j, i), x3 = f3idx(M, k, j, i);
+  const long sx2 = M.nc1, sx3 = static_cast<long>(M.nc2) * M.nc1;
+
+  // ghost targets of this cell
+  int lo1 = (i - M.is < M.ng) ? -1 : 0, hi1 = (M.ie - i < M.ng) ? 1 : 0;
+  int lo2 = (j - M.js < M.ng) ? -1 : 0, hi2 = (M.je - j < M.ng) ? 1 : 0;
+  int lo3 = (k - M.ks < M.ng) ? -1 : 0, hi3 = (M.ke - k < M.ng) ? 1 : 0;
+
+#pragma unroll
+  for (int n = 0; n < NHYDRO; ++n) {
+    double dflx = (a1 * F1[n * M.nf1 + x1 + 1] - a1 * F1[n * M.nf1 + x1]);
+    dflx += (a2 * F2[n * M.nf2 + x2 + sx2] - a2 * F2[n * M.nf2 + x2]);
+    dflx += (a3 * F3[n * M.nf3 + x3 + sx3] - a3 * F3[n * M.nf3 + x3]);
+    double u0 = base[b0 + n * M.ncell + c];
+    if (first_stage) u0 = 0.0 + 1.0 * u0;
+    double v = u0 - wght * dflx / vol;
+    out[b0 + n * M.ncell + c] = v;
+    for (int p3 = lo3; p3 <= hi3; ++p3)
+      for (int p2 = lo2; p2 <= hi2; ++p2)
+        for (int p1 = lo1; p1 <= hi1; ++p1) {
+          if (!p1 && !p2 && !p3) continue;
+          const int tg = nbr(M, g, p1, p2, p3);
+          out[static_cast<long>(tg) * NHYDRO * M.ncell + n * M.ncell +
+              cidx(M, k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1)] = v;
+        }
+  }
+}
+
+// Kernel 3b: face-field update, WeightedAve(FaceField) + Field::CT (ct.cpp:31-116) with
+// averaged surface EMFs, then the face-centred ghost push (fc/bvals_fc.cpp:344-688).
+// grid.z selects the component.
+__global__ void __launch_bounds__(WG) k_update_b(MeshK M, const double *b1in,
+                                                  const double *b2in, const double *b3in,
+                                                  double *b1out, double *b2out,
+                                                  double *b3out, const double *__restrict__ e1,
+                                                  const double *__restrict__ e2,
+                                                  const double *__restrict__ e3, double beta,
+                                                  int first_stage,
+                                                  const Clock *__restrict__ clk) {
+  const int g = blockIdx.y;
+  const int comp = blockIdx.z;
+  const int ni = M.nx1 + (comp == 0), nj = M.nx2 + (comp == 1), nk = M.nx3 + (comp == 2);
+  const int t = blockIdx.x * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = M.is + t % ni;
+  const int r = t / ni;
+  const int j = M.js + r % nj;
+  const int k = M.ks + r / nj;
+  const double wght = beta * clk->dt;
+  const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
+  double v;
+  long nf;
+  double *outp;
+  if (comp == 0) {
+    nf = M.nf1;
+    const long x = f1idx(M, k, j, i);
+    v = b1in[static_cast<long>(g) * nf + x];
+    if (first_stage) v = 0.0 + 1.0 * v;
+    const double area = dx2[j] * dx3[k];
+    const double l3 = dx3[k], l2 = dx2[j];
+    v -= (wght / area) * (l3 * emf3(M, e3, g, k, j + 1, i) - l3 * emf3(M, e3, g, k, j, i));
+    v += (wght / area) * (l2 * emf2(M, e2, g, k + 1, j, i) - l2 * emf2(M, e2, g, k, j, i));
+    outp = b1out;
+  } else if (comp == 1) {
+    nf = M.nf2;
+    const long x = f2idx(M, k, j, i);
+    v = b2in[static_cast<long>(g) * nf + x];
+    if (first_stage) v = 0.0 + 1.0 * v;
+    const double area = dx1[i] * dx3[k];
+    const double l3 = dx3[k], l1 = dx1[i];
+    v += (wght / area) * (l3 * emf3(M, e3, g, k, j, i + 1) - l3 * emf3(M, e3, g, k, j, i));
+    v -= (wght / area) * (l1 * emf1(M, e1, g, k + 1, j, i) - l1 * emf1(M, e1, g, k, j, i));
+    outp = b2out;
+  } else {
+    nf = M.nf3;
+    const long x = f3idx(M, k, j, i);
+    v = b3in[static_cast<long>(g) * nf + x];
+    if (first_stage) v = 0.0 + 1.0 * v;
+    const double area = dx1[i] * dx2[j];
+    const double l2 = dx2[j], l1 = dx1[i];
+    v -= (wght / area) * (l2 * emf2(M, e2, g, k, j, i + 1) - l2 * emf2(M, e2, g, k, j, i));
+    v += (wght / area) * (l1 * emf1(M, e1, g, k, j + 1, i) - l1 * emf1(M, e1, g, k, j, i));
+    outp = b3out;
+  }
+  auto fidx = [&](int kk, int jj, int ii) -> long {
+    if (comp == 0) return f1idx(M, kk, jj, ii);
+    if (comp == 1) return f2idx(M, kk, jj, ii);
+    return f3idx(M, kk, jj, ii);
+  };
+  outp[static_cast<long>(g) * nf + fidx(k, j, i)] = v;
+  // ghost push: along the face-normal direction the copies come from [s+1, s+ng] and
+  // [e-ng+1, e] (shared faces s and e+1 stay with their owner); transverse directions
+  // behave like cells.
+  int lo[3], hi[3];
+  const int idx[3] = {i, j, k};
+  const int s_[3] = {M.is, M.js, M.ks}, e_[3] = {M.ie, M.je, M.ke};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d == comp) {
+      lo[d] = (idx[d] >= s_[d] + 1 && idx[d] <= s_[d] + M.ng) ? -1 : 0;
+      hi[d] = (idx[d] >= e_[d] - M.ng + 1 && idx[d] <= e_[d]) ? 1 : 0;
+    } else {
+      lo[d] = (idx[d] - s_[d] < M.ng) ? -1 : 0;
+      hi[d] = (e_[d] - idx[d] < M.ng) ? 1 : 0;
+    }
+  }
+  for (int p3 = lo[2]; p3 <= hi[2]; ++p3)
+    for (int p2 = lo[1]; p2 <= hi[1]; ++p2)
+      for (int p1 = lo[0]; p1 <= hi[0]; ++p1) {
+        if (!p1 && !p2 && !p3) continue;
+        const int tg = nbr(M, g, p1, p2, p3);
+        outp[static_cast<long>(tg) * nf + fidx(k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1)] = v;
+      }
+}
+
+//========================================================================================
+// Kernel 4: ConservedToPrimitive over every cell (ghosts included,
+// time_integrator.cpp:2206-2212) with Field::CalculateCellCenteredField
+// (field.cpp:112-180, uniform branch), writing prim (the output register) and bcc.
+// With dt_partial != nullptr the active cells also evaluate Hydro::NewBlockTimeStep
+// (new_blockdt.cpp:45-230) and each work-group writes its minimum.
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ u,
+                                                   double *__restrict__ prim,
+                                                   double *__restrict__ bcc,
+                                                   const double *__restrict__ b1,
+                                                   const double *__restrict__ b2,
+                                                   const double *__restrict__ b3,
+                                                   double *__restrict__ dt_partial) {
+  const int g = blockIdx.y;
+  const int t = blockIdx.x * WG + threadIdx.x;
+  double my_dt = DBL_MAX;
+  if (t < M.ncell) {
+    const int i = t % M.nc1;
+    const int r = t / M.nc1;
+    const int j = r % M.nc2;
+    const int k = r / M.nc2;
+    const long n = M.ncell;
+    const long c = t;
+    const double *B1 = b1 + static_cast<long>(g) * M.nf1;
+    const double *B2 = b2 + static_cast<long>(g) * M.nf2;
+    const double *B3 = b3 + static_cast<long>(g) * M.nf3;
+    const double b1l = B1[f1idx(M, k, j, i)], b1r = B1[f1idx(M, k, j, i + 1)];
+    const double b2l = B2[f2idx(M, k, j, i)], b2r = B2[f2idx(M, k, j + 1, i)];
+    const double b3l = B3[f3idx(M, k, j, i)], b3r = B3[f3idx(M, k + 1, j, i)];
+    double bc[3];
+    bc[IB1] = 0.5 * b1l + 0.5 * b1r;
+    bc[IB2] = 0.5 * b2l + 0.5 * b2r;
+    bc[IB3] = 0.5 * b3l + 0.5 * b3r;
+    double *bb = bcc + static_cast<long>(g) * 3 * n;
+    bb[IB1 * n + c] = bc[IB1];
+    bb[IB2 * n + c] = bc[IB2];
+    bb[IB3 * n + c] = bc[IB3];
+    double *ub = u + static_cast<long>(g) * NHYDRO * n;
+    double *pb = prim + static_cast<long>(g) * NHYDRO * n;
+    double w[NHYDRO];
+    if (!SR) {
+      // adiabatic_mhd.cpp:41-90
+      const double gm1 = M.eos.gamma - 1.0;
+      double u_d = ub[IDN * n + c];
+      const double u_m1 = ub[IM1 * n + c], u_m2 = ub[IM2 * n + c], u_m3 = ub[IM3 * n + c];
+      double u_e = ub[IEN * n + c];
+      const double ud_in = u_d, ue_in = u_e;
+      u_d = (u_d > M.eos.dfloor) ? u_d : M.eos.dfloor;
+      w[IDN] = u_d;
+      double di = 1.0 / u_d;
+      w[IVX] = u_m1 * di;
+      w[IVY] = u_m2 * di;
+      w[IVZ] = u_m3 * di;
+      double pb_ = 0.5 * (sq(bc[IB1]) + sq(bc[IB2]) + sq(bc[IB3]));
+      double e_k = 0.5 * di * (sq(u_m1) + sq(u_m2) + sq(u_m3));
+      double w_p = gm1 * (u_e - e_k - pb_);
+      u_e = (w_p > M.eos.pfloor) ? u_e : ((M.eos.pfloor / gm1) + e_k + pb_);
+      w_p = (w_p > M.eos.pfloor) ? w_p : M.eos.pfloor;
+      w[IPR] = w_p;
+      if (u_d != ud_in) ub[IDN * n + c] = u_d;
+      if (u_e != ue_in) ub[IEN * n + c] = u_e;
+    } else {
+      double cons[NHYDRO];
+#pragma unroll
+      for (int q = 0; q < NHYDRO; ++q) { cons[q] = ub[q * n + c]; w[q] = pb[q * n + c]; }
+      double cin[NHYDRO];
+#pragma unroll
+      for (int q = 0; q < NHYDRO; ++q) cin[q] = cons[q];
+      sr_cons2prim_cell(cons, w, bc, M.eos, 0.0, false);
+#pragma unroll
+      for (int q = 0; q < NHYDRO; ++q)
+        if (cons[q] != cin[q]) ub[q * n + c] = cons[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q) pb[q * n + c] = w[q];
+
+    if (dt_partial && i >= M.is && i <= M.ie && j >= M.js && j <= M.je && k >= M.ks &&
+        k <= M.ke) {
+      const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
+      double dt1 = dx1[i], dt2 = dx2[j], dt3 = dx3[k];
+      if (!SR) {
+        double wi[NWAVE];
+        wi[IDN] = w[IDN]; wi[IVX] = w[IVX]; wi[IVY] = w[IVY]; wi[IVZ] = w[IVZ];
+        wi[IPR] = w[IPR];
+        double bx = bc[IB1] + fabs(b1l - bc[IB1]);
+        wi[IBY] = bc[IB2]; wi[IBZ] = bc[IB3];
+        double cf = fast_speed(wi, bx, M.eos.gamma);
+        dt1 /= dmax(0.0, (fabs(wi[IVX]) + cf));
+        wi[IBY] = bc[IB3]; wi[IBZ] = bc[IB1];
+        bx = bc[IB2] + fabs(b2l - bc[IB2]);
+        cf = fast_speed(wi, bx, M.eos.gamma);
+        dt2 /= dmax(0.0, (fabs(wi[IVY]) + cf));
+        wi[IBY] = bc[IB1]; wi[IBZ] = bc[IB2];
+        bx = bc[IB3] + fabs(b3l - bc[IB3]);
+        cf = fast_speed(wi, bx, M.eos.gamma);
+        dt3 /= dmax(0.0, (fabs(wi[IVZ]) + cf));
+      }
+      my_dt = dmin(dmin(dt1, dt2), dt3);
+    }
+  }
+  if (dt_partial) {
+    // work-group minimum: wavefront (64 lanes) shuffle, then LDS across 4 wavefronts
+    __shared__ double sm[WG / 64];
+    double v = my_dt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_down(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = sm[0];
+      for (int q = 1; q < WG / 64; ++q) m = dmin(m, sm[q]);
+      dt_partial[static_cast<long>(g) * gridDim.x + blockIdx.x] = m;
+    }
+  }
+}
+
+// Final reduction + Mesh::NewTimeStep (mesh.cpp:1236-1282) and, when advance != 0,
+// the end-of-cycle clock update of the main loop (main.cpp:535-541).
+__global__ void __launch_bounds__(1024) k_new_dt(const double *__restrict__ part, long npart,
+                                                  Clock *clk, double cfl, int advance) {
+  __shared__ double sm[1024 / 64];
+  double v = DBL_MAX;
+  for (long q = threadIdx.x; q < npart; q += 1024) v = dmin(v, part[q]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sm[0];
+    for (int q = 1; q < 1024 / 64; ++q) m = dmin(m, sm[q]);
+    m *= cfl;   // per-block min_dt_hyperbolic *= cfl_number; min commutes with it
+    Clock c = *clk;
+    if (advance) {
+      c.ncycle += 1.0;
+      c.time += c.dt;
+    }
+    double dt = 2.0 * c.dt;
+    dt = dmin(dt, m);
+    if (c.time < c.tlim && (c.tlim - c.time) < dt) dt = c.tlim - c.time;
+    c.dt = dt;
+    *clk = c;
+  }
+}
+
+// Pull-based same-level ghost fill for Mesh::Initialize (cell-centred register with
+// nvar components).
+__global__ void __launch_bounds__(WG) k_ghost_cc(MeshK M, double *a, int nvar) {
+  const int g = blockIdx.y;
+  const int t = blockIdx.x * WG + threadIdx.x;
+  if (t >= M.ncell) return;
+  const int i = t % M.nc1;
+  const int r = t / M.nc1;
+  const int j = r % M.nc2;
+  const int k = r / M.nc2;
+  const int p1 = (i < M.is) ? -1 : ((i > M.ie) ? 1 : 0);
+  const int p2 = (j < M.js) ? -1 : ((j > M.je) ? 1 : 0);
+  const int p3 = (k < M.ks) ? -1 : ((k > M.ke) ? 1 : 0);
+  if (!p1 && !p2 && !p3) return;
+  const int src = nbr(M, g, p1, p2, p3);
+  const long cs = cidx(M, k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1);
+  for (int v = 0; v < nvar; ++v)
+    a[(static_cast<long>(g) * nvar + v) * M.ncell + t] =
+        a[(static_cast<long>(src) * nvar + v) * M.ncell + cs];
+}
+
+__global__ void __launch_bounds__(WG) k_ghost_fc(MeshK M, double *b1, double *b2,
+                                                  double *b3) {
+  const int g = blockIdx.y;
+  const int comp = blockIdx.z;
+  const int n1 = M.nc1 + (comp == 0), n2 = M.nc2 + (comp == 1), n3 = M.nc3 + (comp == 2);
+  const int t = blockIdx.x * WG + threadIdx.x;
+  if (t >= n1 * n2 * n3) return;
+  const int i = t % n1;
+  const int r = t / n1;
+  const int j = r % n2;
+  const int k = r / n2;
+  const int idx[3] = {i, j, k};
+  const int s_[3] = {M.is, M.js, M.ks}, e_[3] = {M.ie, M.je, M.ke};
+  int p[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int hiend = (d == comp) ? e_[d] + 1 : e_[d];
+    p[d] = (idx[d] < s_[d]) ? -1 : ((idx[d] > hiend) ? 1 : 0);
+  }
+  if (!p[0] && !p[1] && !p[2]) return;
+  const int src = nbr(M, g, p[0], p[1], p[2]);
+  double *a = (comp == 0) ? b1 : ((comp == 1) ? b2 : b3);
+  const long nf = (comp == 0) ? M.nf1 : ((comp == 1) ? M.nf2 : M.nf3);
+  const long ls = ((static_cast<long>(k - p[2] * M.nx3) * n2) + (j - p[1] * M.nx2)) * n1 +
+                  (i - p[0] * M.nx1);
+  a[static_cast<long>(g) * nf + t] = a[static_cast<long>(src) * nf + ls];
+}
+
+// Batched single-interface Riemann solves (known-answer vectors)
+__global__ void k_riemann_batch(int solver, EosK eos, int n, const double *wl,
+                                const double *wr, const double *bx, double *out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double a[NWAVE], b[NWAVE], f[NWAVE];
+  for (int v = 0; v < NWAVE; ++v) { a[v] = wl[q * NWAVE + v]; b[v] = wr[q * NWAVE + v]; }
+  if (solver == 0) hlld(a, b, bx[q], eos, f);
+  else if (solver == 1) hlle(a, b, bx[q], eos, f);
+  else sr_hlle(a, b, bx[q], eos, f);
+  for (int v = 0; v < NWAVE; ++v) out[q * NWAVE + v] = f[v];
+}
+
+__global__ void k_sr_c2p_batch(EosK eos, int n, double *cons, double *prim,
+                               const double *bcc, int *iters) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double c[5], p[5], b[3];
+  for (int v = 0; v < 5; ++v) { c[v] = cons[q * 5 + v]; p[v] = prim[q * 5 + v]; }
+  for (int v = 0; v < 3; ++v) b[v] = bcc[q * 3 + v];
+  iters[q] = sr_cons2prim_cell(c, p, b, eos, 0.0, false);
+  for (int v = 0; v < 5; ++v) { cons[q * 5 + v] = c[v]; prim[q * 5 + v] = p[v]; }
+}
+
+//========================================================================================
+// Host side
+
+double mesh_gen_x(long index, long nrange) {   // mesh.hpp:457-472, sym_interval = true
+  long noffset = index - nrange / 2;
+  long noffset_ceil = index - (nrange + 1) / 2;
+  return static_cast<double>(noffset + noffset_ceil) / (2.0 * nrange);
+}
+double uniform_gen(double x, double xmin, double xmax) {   // mesh.hpp:535-547
+  return static_cast<double>(0.5) * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+}  // namespace
+
+struct ath_mesh {
+  ath_mesh_desc d;
+  MeshK M;
+  int nbx[3];
+  hipStream_t stream = nullptr;
+  // registers
+  double *u = nullptr, *u1 = nullptr, *w = nullptr, *w1 = nullptr, *bcc = nullptr;
+  double *b[3] = {nullptr, nullptr, nullptr}, *bb[3] = {nullptr, nullptr, nullptr};
+  double *flux[3] = {nullptr, nullptr, nullptr};
+  double *e2x1 = nullptr, *e3x1 = nullptr, *e1x2 = nullptr, *e3x2 = nullptr;
+  double *e1x3 = nullptr, *e2x3 = nullptr, *wx[3] = {nullptr, nullptr, nullptr};
+  double *e[3] = {nullptr, nullptr, nullptr};
+  double *dtpart = nullptr;
+  long ndtpart = 0;
+  int *nbr_d = nullptr;
+  double *coord_d = nullptr;
+  Clock *clk = nullptr;
+  std::vector<void *> allocs;
+  // optional per-kernel event timing (ath_gpu_kernel_timing)
+  bool prof_on = false;
+  struct Rec { int id; hipEvent_t a, b; };
+  std::vector<Rec> prof_rec;
+  double prof_ms[64] = {0};
+  long prof_cnt[64] = {0};
+
+  int alloc(double **p, long n) {
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(p), sizeof(double) * n));
+    HIPCHK(hipMemsetAsync(*p, 0, sizeof(double) * n, stream));
+    allocs.push_back(*p);
+    return 0;
+  }
+};
+
+namespace {
+
+// kernel ids for the per-kernel timer
+enum KernelId {
+  KID_FLUX = 0,          // + 3*dir + (order-1): 9 flux variants
+  KID_CORNER = 9, KID_UPDATE_U = 10, KID_UPDATE_B = 11, KID_CONS2PRIM = 12,
+  KID_NEW_DT = 13, KID_GHOST = 14, KID_COUNT = 15
+};
+const char *kernel_name(int id) {
+  static const char *flux[9] = {
+      "flux_x1_dc", "flux_x1_plm", "flux_x1_ppm", "flux_x2_dc", "flux_x2_plm", "flux_x2_ppm",
+      "flux_x3_dc", "flux_x3_plm", "flux_x3_ppm"};
+  if (id >= 0 && id < 9) return flux[id];
+  switch (id) {
+    case KID_CORNER: return "corner_emf";
+    case KID_UPDATE_U: return "update_u";
+    case KID_UPDATE_B: return "update_b_ct";
+    case KID_CONS2PRIM: return "cons2prim";
+    case KID_NEW_DT: return "new_dt";
+    case KID_GHOST: return "ghost_fill";
+    default: return "unknown";
+  }
+}
+
+struct ProfScope {
+  ath_mesh *m;
+  int id;
+  hipEvent_t b = nullptr;
+  ProfScope(ath_mesh *mm, int kid) : m(mm), id(kid) {
+    if (!m->prof_on) return;
+    hipEvent_t a;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      m->prof_on = false;
+      return;
+    }
+    (void)hipEventRecord(a, m->stream);
+    m->prof_rec.push_back({id, a, b});
+  }
+  ~ProfScope() {
+    if (m->prof_on && b) (void)hipEventRecord(b, m->stream);
+  }
+};
+
+int launch_flux(ath_mesh *m, int order) {
+  const MeshK &M = m->M;
+  const int rs = m->d.relativistic ? 2 : m->d.rsolver;
+  for (int dir = 0; dir < 3; ++dir) {
+    long nface;
+    if (dir == 0) nface = static_cast<long>(M.nx1 + 1) * (M.nx2 + 2) * (M.nx3 + 2);
+    else if (dir == 1) nface = static_cast<long>(M.nx1 + 2) * (M.nx2 + 1) * (M.nx3 + 2);
+    else nface = static_cast<long>(M.nx1 + 2) * (M.nx2 + 2) * (M.nx3 + 1);
+    dim3 grid((nface + WG - 1) / WG, M.nblocks);
+    double *ey, *ez;
+    if (dir == 0) { ey = m->e3x1; ez = m->e2x1; }
+    else if (dir == 1) { ey = m->e1x2; ez = m->e3x2; }
+    else { ey = m->e2x3; ez = m->e1x3; }
+#define LF(D, O, R)                                                                       \
+  hipLaunchKernelGGL((k_flux<D, O, R>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,   \
+                     m->b[D], m->flux[D], ey, ez, m->wx[D], m->clk)
+#define LF_O(D, R)                      \
+  do {                                  \
+    if (order == 1) LF(D, 1, R);        \
+    else if (order == 2) LF(D, 2, R);   \
+    else LF(D, 3, R);                   \
+  } while (0)
+#define LF_R(D)                         \
+  do {                                  \
+    if (rs == 0) LF_O(D, 0);            \
+    else if (rs == 1) LF_O(D, 1);       \
+    else LF_O(D, 2);                    \
+  } while (0)
+    {
+      ProfScope ps(m, KID_FLUX + 3 * dir + (order - 1));
+      if (dir == 0) LF_R(0);
+      else if (dir == 1) LF_R(1);
+      else LF_R(2);
+    }
+#undef LF_R
+#undef LF_O
+#undef LF
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int launch_corner(ath_mesh *m) {
+  const MeshK &M = m->M;
+  long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
+  dim3 grid((n + WG - 1) / WG, M.nblocks);
+  ProfScope ps(m, KID_CORNER);
+  if (m->d.relativistic)
+    hipLaunchKernelGGL((k_corner<true>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,
+                       m->e2x1, m->e3x1, m->e1x2, m->e3x2, m->e1x3, m->e2x3, m->wx[0],
+                       m->wx[1], m->wx[2], m->e[0], m->e[1], m->e[2]);
+  else
+    hipLaunchKernelGGL((k_corner<false>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,
+                       m->e2x1, m->e3x1, m->e1x2, m->e3x2, m->e1x3, m->e2x3, m->wx[0],
+                       m->wx[1], m->wx[2], m->e[0], m->e[1], m->e[2]);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int launch_cons2prim(ath_mesh *m, double *prim_out, bool with_dt) {
+  const MeshK &M = m->M;
+  dim3 grid((M.ncell + WG - 1) / WG, M.nblocks);
+  double *part = with_dt ? m->dtpart : nullptr;
+  ProfScope ps(m, KID_CONS2PRIM);
+  if (m->d.relativistic)
+    hipLaunchKernelGGL((k_cons2prim<true>), grid, dim3(WG), 0, m->stream, M, m->u, prim_out,
+                       m->bcc, m->b[0], m->b[1], m->b[2], part);
+  else
+    hipLaunchKernelGGL((k_cons2prim<false>), grid, dim3(WG), 0, m->stream, M, m->u, prim_out,
+                       m->bcc, m->b[0], m->b[1], m->b[2], part);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int launch_new_dt(ath_mesh *m, int advance) {
+  ProfScope ps(m, KID_NEW_DT);
+  hipLaunchKernelGGL(k_new_dt, dim3(1), dim3(1024), 0, m->stream, m->dtpart, m->ndtpart,
+                     m->clk, m->d.cfl, advance);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int stage(ath_mesh *m, int stg) {
+  const MeshK &M = m->M;
+  const double beta = (stg == 1) ? 0.5 : 1.0;
+  const int order = (stg == 1) ? 1 : m->d.xorder;
+  if (launch_flux(m, order)) return 1;
+  if (launch_corner(m)) return 1;
+  // conserved update: stage 1 reads u and writes u1, then the registers swap
+  // (IntegrateHydro: WeightedAve into zeroed u1, u.SwapAthenaArray(u1)); stage 2 is
+  // in place on the swapped-in U^n.
+  {
+    long n = static_cast<long>(M.nx1) * M.nx2 * M.nx3;
+    dim3 grid((n + WG - 1) / WG, M.nblocks);
+    double *base, *out;
+    if (stg == 1) { base = m->u; out = m->u1; }
+    else { base = m->u1; out = m->u1; }
+    {
+      ProfScope ps(m, KID_UPDATE_U);
+      hipLaunchKernelGGL(k_update_u, grid, dim3(WG), 0, m->stream, M, base, out, m->flux[0],
+                         m->flux[1], m->flux[2], beta, stg == 1 ? 1 : 0, m->clk);
+    }
+    HIPCHK(hipGetLastError());
+    std::swap(m->u, m->u1);
+  }
+  {
+    long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
+    dim3 grid((n + WG - 1) / WG, M.nblocks, 3);
+    double **in = (stg == 1) ? m->b : m->bb;
+    double **out = m->bb;
+    {
+      ProfScope ps(m, KID_UPDATE_B);
+      hipLaunchKernelGGL(k_update_b, grid, dim3(WG), 0, m->stream, M, in[0], in[1], in[2],
+                         out[0], out[1], out[2], m->e[0], m->e[1], m->e[2], beta,
+                         stg == 1 ? 1 : 0, m->clk);
+    }
+    HIPCHK(hipGetLastError());
+    for (int c = 0; c < 3; ++c) std::swap(m->b[c], m->bb[c]);
+  }
+  // Primitives: ConservedToPrimitive(u, w, b, w1, bcc) then w.SwapAthenaArray(w1)
+  if (launch_cons2prim(m, m->w1, stg == 2)) return 1;
+  std::swap(m->w, m->w1);
+  return 0;
+}
+
+int block_count(ath_mesh *m, int what, long *n, double ***pool) {
+  const MeshK &M = m->M;
+  switch (what) {
+    case ATH_U: *n = NHYDRO * M.ncell; *pool = &m->u; return 0;
+    case ATH_W: *n = NHYDRO * M.ncell; *pool = &m->w; return 0;
+    case ATH_W1: *n = NHYDRO * M.ncell; *pool = &m->w1; return 0;
+    case ATH_BCC: *n = 3 * M.ncell; *pool = &m->bcc; return 0;
+    case ATH_B1: *n = M.nf1; *pool = &m->b[0]; return 0;
+    case ATH_B2: *n = M.nf2; *pool = &m->b[1]; return 0;
+    case ATH_B3: *n = M.nf3; *pool = &m->b[2]; return 0;
+    case ATH_FLUX1: *n = NHYDRO * M.nf1; *pool = &m->flux[0]; return 0;
+    case ATH_FLUX2: *n = NHYDRO * M.nf2; *pool = &m->flux[1]; return 0;
+    case ATH_FLUX3: *n = NHYDRO * M.nf3; *pool = &m->flux[2]; return 0;
+    case ATH_E1: *n = M.ne1; *pool = &m->e[0]; return 0;
+    case ATH_E2: *n = M.ne2; *pool = &m->e[1]; return 0;
+    case ATH_E3: *n = M.ne3; *pool = &m->e[2]; return 0;
+    case ATH_WGHT1: *n = M.nf1; *pool = &m->wx[0]; return 0;
+    case ATH_WGHT2: *n = M.nf2; *pool = &m->wx[1]; return 0;
+    case ATH_WGHT3: *n = M.nf3; *pool = &m->wx[2]; return 0;
+    default: return fail("unknown register code " + std::to_string(what));
+  }
+}
+
+}  // namespace
+
+//========================================================================================
+// C ABI
+
+extern "C" {
+
+const char *ath_gpu_last_error(void) { return g_err.c_str(); }
+
+int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
+  if (!desc || !out) return fail("null argument");
+  *out = nullptr;
+  const ath_mesh_desc &d = *desc;
+  for (int q = 0; q < 3; ++q) {
+    if (d.bnx[q] <= 0 || d.nx[q] <= 0 || d.nx[q] % d.bnx[q])
+      return fail("mesh nx must be a positive multiple of meshblock nx in every direction");
+    if (d.bnx[q] < 2) return fail("meshblocks must have at least 2 cells per direction (3-D only)");
+    if (!(d.xmax[q] > d.xmin[q])) return fail("xmax must exceed xmin");
+  }
+  if (d.ng < 2) return fail("nghost must be >= 2");
+  if (d.xorder < 1 || d.xorder > 3) return fail("xorder must be 1, 2 or 3");
+  if (d.xorder == 3 && d.ng < 3)
+    return fail("xorder=3 (PPM) reconstruction selected, but nghost < 3");
+  if (d.rsolver < 0 || d.rsolver > 1) return fail("rsolver must be 0 (HLLD) or 1 (HLLE)");
+  if (d.relativistic && d.rsolver != 1)
+    return fail("the relativistic path implements the HLLE solver only");
+  if (!(d.gamma > 1.0)) return fail("gamma must exceed 1");
+  if (!(d.cfl > 0.0) || d.cfl > 0.5) return fail("cfl_number must be in (0, 0.5] for 3-D VL2");
+  for (int q = 0; q < 3; ++q)
+    if (d.bnx[q] < d.ng) return fail("meshblock nx must be >= nghost");
+
+  ath_mesh *m = new ath_mesh;
+  m->d = d;
+  MeshK &M = m->M;
+  M.nx1 = d.bnx[0]; M.nx2 = d.bnx[1]; M.nx3 = d.bnx[2]; M.ng = d.ng;
+  M.nc1 = M.nx1 + 2 * M.ng; M.nc2 = M.nx2 + 2 * M.ng; M.nc3 = M.nx3 + 2 * M.ng;
+  M.is = M.ng; M.ie = M.ng + M.nx1 - 1;
+  M.js = M.ng; M.je = M.ng + M.nx2 - 1;
+  M.ks = M.ng; M.ke = M.ng + M.nx3 - 1;
+  for (int q = 0; q < 3; ++q) m->nbx[q] = d.nx[q] / d.bnx[q];
+  M.nblocks = m->nbx[0] * m->nbx[1] * m->nbx[2];
+  M.ncell = static_cast<long>(M.nc1) * M.nc2 * M.nc3;
+  M.nf1 = static_cast<long>(M.nc3) * M.nc2 * (M.nc1 + 1);
+  M.nf2 = static_cast<long>(M.nc3) * (M.nc2 + 1) * M.nc1;
+  M.nf3 = static_cast<long>(M.nc3 + 1) * M.nc2 * M.nc1;
+  M.ne1 = static_cast<long>(M.nc3 + 1) * (M.nc2 + 1) * M.nc1;
+  M.ne2 = static_cast<long>(M.nc3 + 1) * M.nc2 * (M.nc1 + 1);
+  M.ne3 = static_cast<long>(M.nc3) * (M.nc2 + 1) * (M.nc1 + 1);
+  M.coff[0] = 0;
+  M.coff[1] = 3 * M.nc1;
+  M.coff[2] = 3 * (M.nc1 + M.nc2);
+  M.cstride = 3 * (M.nc1 + M.nc2 + M.nc3);
+  const double fmin = static_cast<double>(std::numeric_limits<float>::min());
+  M.eos.gamma = d.gamma;
+  M.eos.igm1 = 1.0 / (d.gamma - 1.0);
+  M.eos.dfloor = (d.dfloor < 0.0) ? std::sqrt(1024 * fmin) : d.dfloor;
+  M.eos.pfloor = (d.pfloor < 0.0) ? std::sqrt(1024 * fmin) : d.pfloor;
+  M.eos.sigma_max = (d.sigma_max < 0.0) ? 0.0 : d.sigma_max;
+  M.eos.beta_min = (d.beta_min < 0.0) ? 0.0 : d.beta_min;
+  M.eos.gamma_max = (d.gamma_max < 0.0) ? 1000.0 : d.gamma_max;
+  M.cfl = d.cfl;
+
+  // neighbour table and coordinates (coordinates.cpp:129-146, cartesian.cpp:23-72)
+  const int nb = M.nblocks;
+  std::vector<int> nbrh(static_cast<size_t>(nb) * 27);
+  std::vector<double> coordh(static_cast<size_t>(nb) * M.cstride);
+  for (int g = 0; g < nb; ++g) {
+    int lx[3] = {g % m->nbx[0], (g / m->nbx[0]) % m->nbx[1], g / (m->nbx[0] * m->nbx[1])};
+    for (int o3 = -1; o3 <= 1; ++o3)
+      for (int o2 = -1; o2 <= 1; ++o2)
+        for (int o1 = -1; o1 <= 1; ++o1) {
+          int l1 = (lx[0] + o1 + m->nbx[0]) % m->nbx[0];
+          int l2 = (lx[1] + o2 + m->nbx[1]) % m->nbx[1];
+          int l3 = (lx[2] + o3 + m->nbx[2]) % m->nbx[2];
+          nbrh[g * 27 + (o3 + 1) * 9 + (o2 + 1) * 3 + (o1 + 1)] =
+              l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
+        }
+    for (int q = 0; q < 3; ++q) {
+      const int nx = d.bnx[q], ng = d.ng, nc = nx + 2 * ng, nbq = m->nbx[q];
+      const long l = lx[q];
+      double bmin = (l == 0) ? d.xmin[q] : uniform_gen(mesh_gen_x(l, nbq), d.xmin[q], d.xmax[q]);
+      double bmax = (l == nbq - 1) ? d.xmax[q]
+                                   : uniform_gen(mesh_gen_x(l + 1, nbq), d.xmin[q], d.xmax[q]);
+      const int il = ng, iu = ng + nx - 1;
+      const double dx = (bmax - bmin) / (iu - il + 1);
+      std::vector<double> xf(nc + 1), xv(nc);
+      for (int i = il - ng; i <= iu + ng + 1; ++i)
+        xf[i] = uniform_gen(mesh_gen_x(i - il + l * nx, d.nx[q]), d.xmin[q], d.xmax[q]);
+      xf[il] = bmin;
+      xf[iu + 1] = bmax;
+      for (int i = 0; i < nc; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
+      double *tab = coordh.data() + static_cast<size_t>(g) * M.cstride + M.coff[q];
+      for (int i = 0; i < nc; ++i) {
+        tab[i] = dx;                                   // dxNf
+        tab[nc + i] = (xf[i + 1] - xv[i]) / dx;        // PLM right-face factor
+        tab[2 * nc + i] = (xv[i] - xf[i]) / dx;        // PLM left-face factor
+      }
+    }
+  }
+
+  int rc = 0;
+  auto cleanup = [&]() {
+    for (void *p : m->allocs) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+  };
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete m;
+    return fail("hipStreamCreate failed");
+  }
+  const long nc5 = NHYDRO * M.ncell * nb;
+  rc |= m->alloc(&m->u, nc5);
+  rc |= m->alloc(&m->u1, nc5);
+  rc |= m->alloc(&m->w, nc5);
+  rc |= m->alloc(&m->w1, nc5);
+  rc |= m->alloc(&m->bcc, 3 * M.ncell * nb);
+  const long nf[3] = {M.nf1, M.nf2, M.nf3};
+  for (int q = 0; q < 3 && !rc; ++q) {
+    rc |= m->alloc(&m->b[q], nf[q] * nb);
+    rc |= m->alloc(&m->bb[q], nf[q] * nb);
+    rc |= m->alloc(&m->flux[q], NHYDRO * nf[q] * nb);
+    rc |= m->alloc(&m->wx[q], nf[q] * nb);
+  }
+  if (!rc) {
+    rc |= m->alloc(&m->e2x1, nf[0] * nb);
+    rc |= m->alloc(&m->e3x1, nf[0] * nb);
+    rc |= m->alloc(&m->e1x2, nf[1] * nb);
+    rc |= m->alloc(&m->e3x2, nf[1] * nb);
+    rc |= m->alloc(&m->e1x3, nf[2] * nb);
+    rc |= m->alloc(&m->e2x3, nf[2] * nb);
+    rc |= m->alloc(&m->e[0], M.ne1 * nb);
+    rc |= m->alloc(&m->e[1], M.ne2 * nb);
+    rc |= m->alloc(&m->e[2], M.ne3 * nb);
+    m->ndtpart = ((M.ncell + WG - 1) / WG) * nb;
+    rc |= m->alloc(&m->dtpart, m->ndtpart);
+    rc |= m->alloc(&m->coord_d, static_cast<long>(coordh.size()));
+  }
+  if (!rc) {
+    if (hipMalloc(reinterpret_cast<void **>(&m->nbr_d), sizeof(int) * nbrh.size()) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&m->clk), sizeof(Clock)) != hipSuccess) {
+      rc = fail("hipMalloc failed for tables");
+    } else {
+      m->allocs.push_back(m->nbr_d);
+      m->allocs.push_back(m->clk);
+    }
+  }
+  if (!rc) {
+    Clock c0{0.0, std::numeric_limits<double>::max(), std::numeric_limits<double>::max(), 0.0};
+    if (hipMemcpyAsync(m->nbr_d, nbrh.data(), sizeof(int) * nbrh.size(), hipMemcpyHostToDevice,
+                       m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->coord_d, coordh.data(), sizeof(double) * coordh.size(),
+                       hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->clk, &c0, sizeof(Clock), hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+      rc = fail("upload of mesh tables failed");
+  }
+  if (rc) {
+    std::string keep = g_err;
+    cleanup();
+    g_err = keep;
+    return 1;
+  }
+  M.nbr = m->nbr_d;
+  M.coord = m->coord_d;
+  *out = m;
+  return 0;
+}
+
+int ath_gpu_mesh_destroy(ath_mesh_t m) {
+  if (!m) return 0;
+  (void)hipStreamSynchronize(m->stream);
+  for (auto &r : m->prof_rec) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (void *p : m->allocs) (void)hipFree(p);
+  (void)hipStreamDestroy(m->stream);
+  delete m;
+  return 0;
+}
+
+int ath_gpu_nblocks(ath_mesh_t m, int *nb) {
+  if (!m || !nb) return fail("null argument");
+  *nb = m->M.nblocks;
+  return 0;
+}
+
+int ath_gpu_block_size(ath_mesh_t m, int what, long *n) {
+  if (!m || !n) return fail("null argument");
+  double **pool;
+  return block_count(m, what, n, &pool);
+}
+
+int ath_gpu_set_block(ath_mesh_t m, int gid, int what, const double *host) {
+  if (!m || !host) return fail("null argument");
+  if (gid < 0 || gid >= m->M.nblocks) return fail("block id out of range");
+  long n;
+  double **pool;
+  if (block_count(m, what, &n, &pool)) return 1;
+  HIPCHK(hipMemcpyAsync(*pool + static_cast<long>(gid) * n, host, sizeof(double) * n,
+                        hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int ath_gpu_get_block(ath_mesh_t m, int gid, int what, double *host) {
+  if (!m || !host) return fail("null argument");
+  if (gid < 0 || gid >= m->M.nblocks) return fail("block id out of range");
+  long n;
+  double **pool;
+  if (block_count(m, what, &n, &pool)) return 1;
+  HIPCHK(hipMemcpyAsync(host, *pool + static_cast<long>(gid) * n, sizeof(double) * n,
+                        hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int ath_gpu_initialize(ath_mesh_t m, double tlim) {
+  if (!m) return fail("null argument");
+  const MeshK &M = m->M;
+  Clock c0{0.0, std::numeric_limits<double>::max(), tlim, 0.0};
+  HIPCHK(hipMemcpyAsync(m->clk, &c0, sizeof(Clock), hipMemcpyHostToDevice, m->stream));
+  {
+    dim3 grid((M.ncell + WG - 1) / WG, M.nblocks);
+    hipLaunchKernelGGL(k_ghost_cc, grid, dim3(WG), 0, m->stream, M, m->u, NHYDRO);
+    HIPCHK(hipGetLastError());
+  }
+  {
+    long n = std::max(M.nf1, std::max(M.nf2, M.nf3));
+    dim3 grid((n + WG - 1) / WG, M.nblocks, 3);
+    hipLaunchKernelGGL(k_ghost_fc, grid, dim3(WG), 0, m->stream, M, m->b[0], m->b[1], m->b[2]);
+    HIPCHK(hipGetLastError());
+  }
+  // ConservedToPrimitive(u, w1, b, w, bcc): output register is w
+  if (launch_cons2prim(m, m->w, true)) return 1;
+  if (launch_new_dt(m, 0)) return 1;
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int ath_gpu_vl2_stage(ath_mesh_t m, int stg) {
+  if (!m) return fail("null argument");
+  if (stg != 1 && stg != 2) return fail("stage must be 1 or 2");
+  if (stage(m, stg)) return 1;
+  if (stg == 2 && launch_new_dt(m, 1)) return 1;
+  return 0;
+}
+
+int ath_gpu_cycle(ath_mesh_t m, int ncycles, double tlim, int *done) {
+  if (!m) return fail("null argument");
+  int n = 0;
+  const bool bounded = std::isfinite(tlim);
+  if (bounded) {   // keep the device tlim in step with the caller's
+    HIPCHK(hipMemcpyAsync(&m->clk->tlim, &tlim, sizeof(double), hipMemcpyHostToDevice,
+                          m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+  }
+  for (; n < ncycles; ++n) {
+    if (bounded) {
+      Clock c;
+      HIPCHK(hipMemcpyAsync(&c, m->clk, sizeof(Clock), hipMemcpyDeviceToHost, m->stream));
+      HIPCHK(hipStreamSynchronize(m->stream));
+      if (!(c.time < tlim)) break;
+    }
+    if (stage(m, 1)) return 1;
+    if (stage(m, 2)) return 1;
+    if (launch_new_dt(m, 1)) return 1;
+  }
+  if (done) *done = n;
+  return 0;
+}
+
+int ath_gpu_get_clock(ath_mesh_t m, double *time, double *dt, int *ncycle) {
+  if (!m) return fail("null argument");
+  Clock c;
+  HIPCHK(hipMemcpyAsync(&c, m->clk, sizeof(Clock), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  if (time) *time = c.time;
+  if (dt) *dt = c.dt;
+  if (ncycle) *ncycle = static_cast<int>(c.ncycle);
+  return 0;
+}
+
+int ath_gpu_set_clock(ath_mesh_t m, double time, double dt, int ncycle) {
+  if (!m) return fail("null argument");
+  Clock c;
+  HIPCHK(hipMemcpyAsync(&c, m->clk, sizeof(Clock), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  c.time = time;
+  c.dt = dt;
+  c.ncycle = ncycle;
+  HIPCHK(hipMemcpyAsync(m->clk, &c, sizeof(Clock), hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int ath_gpu_calculate_fluxes(ath_mesh_t m, int order) {
+  if (!m) return fail("null argument");
+  if (order < 1 || order > 3) return fail("order must be 1, 2 or 3");
+  if (order == 3 && m->M.ng < 3) return fail("PPM needs nghost >= 3");
+  return launch_flux(m, order);
+}
+
+int ath_gpu_compute_corner_e(ath_mesh_t m) {
+  if (!m) return fail("null argument");
+  return launch_corner(m);
+}
+
+int ath_gpu_stream(ath_mesh_t m, void **stream) {
+  if (!m || !stream) return fail("null argument");
+  *stream = m->stream;
+  return 0;
+}
+
+int ath_gpu_synchronize(ath_mesh_t m) {
+  if (!m) return fail("null argument");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int ath_gpu_kernel_timing(ath_mesh_t m, int enable) {
+  if (!m) return fail("null argument");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  for (auto &r : m->prof_rec) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  m->prof_rec.clear();
+  for (int q = 0; q < 64; ++q) { m->prof_ms[q] = 0.0; m->prof_cnt[q] = 0; }
+  m->prof_on = enable != 0;
+  return 0;
+}
+
+int ath_gpu_kernel_stats(ath_mesh_t m, int id, char *name, int namelen, double *total_ms,
+                         long *count) {
+  if (!m) return fail("null argument");
+  if (id < 0 || id >= KID_COUNT) return fail("kernel id out of range");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  for (auto &r : m->prof_rec) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    m->prof_ms[r.id] += ms;
+    m->prof_cnt[r.id] += 1;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  m->prof_rec.clear();
+  if (name && namelen > 0) {
+    std::snprintf(name, namelen, "%s", kernel_name(id));
+  }
+  if (total_ms) *total_ms = m->prof_ms[id];
+  if (count) *count = m->prof_cnt[id];
+  return 0;
+}
+
+int ath_gpu_riemann_batch(int solver, double gamma, int n, const double *wl,
+                          const double *wr, const double *bx, double *flux) {
+  if (solver < 0 || solver > 2) return fail("solver must be 0, 1 or 2");
+  if (n <= 0) return 0;
+  if (!wl || !wr || !bx || !flux) return fail("null argument");
+  EosK eos{};
+  eos.gamma = gamma;
+  eos.igm1 = 1.0 / (gamma - 1.0);
+  double *d = nullptr;
+  const size_t n7 = static_cast<size_t>(n) * NWAVE;
+  HIPCHK(hipMalloc(reinterpret_cast<void **>(&d), sizeof(double) * (3 * n7 + n)));
+  double *dwl = d, *dwr = d + n7, *dout = d + 2 * n7, *dbx = d + 3 * n7;
+  int rc = 0;
+  if (hipMemcpy(dwl, wl, sizeof(double) * n7, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dwr, wr, sizeof(double) * n7, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dbx, bx, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail("upload failed");
+  } else {
+    hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, 0, solver, eos,
+                       n, dwl, dwr, dbx, dout);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(flux, dout, sizeof(double) * n7, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail("riemann batch failed");
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
+                              double sigma_max, double beta_min, double gamma_max,
+                              double *cons, double *prim, const double *bcc, int *iters) {
+  if (n <= 0) return 0;
+  if (!cons || !prim || !bcc || !iters) return fail("null argument");
+  EosK eos{};
+  eos.gamma = gamma;
+  eos.igm1 = 1.0 / (gamma - 1.0);
+  eos.dfloor = dfloor;
+  eos.pfloor = pfloor;
+  eos.sigma_max = sigma_max;
+  eos.beta_min = beta_min;
+  eos.gamma_max = gamma_max;
+  double *d = nullptr;
+  int *di = nullptr;
+  const size_t n5 = static_cast<size_t>(n) * 5;
+  HIPCHK(hipMalloc(reinterpret_cast<void **>(&d), sizeof(double) * (2 * n5 + 3 * n)));
+  if (hipMalloc(reinterpret_cast<void **>(&di), sizeof(int) * n) != hipSuccess) {
+    (void)hipFree(d);
+    return fail("hipMalloc failed");
+  }
+  int rc = 0;
+  double *dc = d, *dp = d + n5, *db = d + 2 * n5;
+  if (hipMemcpy(dc, cons, sizeof(double) * n5, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dp, prim, sizeof(double) * n5, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, bcc, sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail("upload failed");
+  } else {
+    hipLaunchKernelGGL(k_sr_c2p_batch, dim3((n + 255) / 256), dim3(256), 0, 0, eos, n, dc, dp,
+                       db, di);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(cons, dc, sizeof(double) * n5, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(prim, dp, sizeof(double) * n5, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(iters, di, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail("sr con2prim batch failed");
+  }
+  (void)hipFree(d);
+  (void)hipFree(di);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+"""Python mirror of the C ABI in include/athena_gpu.h (ctypes).
+
+This is the host-side handle the problem generators, tests and bench use.  It loads
+the in-tree ``athena_amd/libathena_gpu.so`` and raises if it is missing or if a call
+fails: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .mesh import MeshDesc, RSOLVER
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libathena_gpu.so")
+
+ATH = {"u": 0, "w": 1, "bcc": 2, "b1": 3, "b2": 4, "b3": 5, "w1": 6,
+       "flux1": 20, "flux2": 21, "flux3": 22, "e1": 23, "e2": 24, "e3": 25,
+       "wght1": 26, "wght2": 27, "wght3": 28}
+
+EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
+           "ath_gpu_block_size", "ath_gpu_set_block", "ath_gpu_get_block",
+           "ath_gpu_initialize", "ath_gpu_vl2_stage", "ath_gpu_cycle",
+           "ath_gpu_get_clock", "ath_gpu_set_clock", "ath_gpu_calculate_fluxes",
+           "ath_gpu_compute_corner_e", "ath_gpu_stream", "ath_gpu_synchronize",
+           "ath_gpu_riemann_batch", "ath_gpu_sr_con2prim_batch", "ath_gpu_last_error",
+           "ath_gpu_kernel_timing", "ath_gpu_kernel_stats"]
+NKERNELS = 15
+
+
+class AthGpuError(RuntimeError):
+    pass
+
+
+class MeshDescC(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int * 3), ("bnx", ctypes.c_int * 3), ("ng", ctypes.c_int),
+                ("xorder", ctypes.c_int), ("rsolver", ctypes.c_int),
+                ("relativistic", ctypes.c_int),
+                ("xmin", ctypes.c_double * 3), ("xmax", ctypes.c_double * 3),
+                ("gamma", ctypes.c_double), ("dfloor", ctypes.c_double),
+                ("pfloor", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("sigma_max", ctypes.c_double), ("beta_min", ctypes.c_double),
+                ("gamma_max", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load the library (no device call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise AthGpuError("libathena_gpu.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIBPATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.c_int
+    for name in EXPORTS:
+        getattr(L, name).restype = ctypes.c_int
+    L.ath_gpu_last_error.restype = ctypes.c_char_p
+    L.ath_gpu_mesh_create.argtypes = [ctypes.POINTER(MeshDescC), ctypes.POINTER(P)]
+    L.ath_gpu_mesh_destroy.argtypes = [P]
+    L.ath_gpu_nblocks.argtypes = [P, ctypes.POINTER(I)]
+    L.ath_gpu_block_size.argtypes = [P, I, ctypes.POINTER(ctypes.c_long)]
+    L.ath_gpu_set_block.argtypes = [P, I, I, D]
+    L.ath_gpu_get_block.argtypes = [P, I, I, D]
+    L.ath_gpu_initialize.argtypes = [P, ctypes.c_double]
+    L.ath_gpu_vl2_stage.argtypes = [P, I]
+    L.ath_gpu_cycle.argtypes = [P, I, ctypes.c_double, ctypes.POINTER(I)]
+    L.ath_gpu_get_clock.argtypes = [P, D, D, ctypes.POINTER(I)]
+    L.ath_gpu_set_clock.argtypes = [P, ctypes.c_double, ctypes.c_double, I]
+    L.ath_gpu_calculate_fluxes.argtypes = [P, I]
+    L.ath_gpu_compute_corner_e.argtypes = [P]
+    L.ath_gpu_stream.argtypes = [P, ctypes.POINTER(P)]
+    L.ath_gpu_synchronize.argtypes = [P]
+    L.ath_gpu_kernel_timing.argtypes = [P, I]
+    L.ath_gpu_kernel_stats.argtypes = [P, I, ctypes.c_char_p, I, D, ctypes.POINTER(ctypes.c_long)]
+    L.ath_gpu_riemann_batch.argtypes = [I, ctypes.c_double, I, D, D, D, D]
+    L.ath_gpu_sr_con2prim_batch.argtypes = [I] + [ctypes.c_double] * 6 + \
+        [D, D, D, ctypes.POINTER(I)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise AthGpuError(load().ath_gpu_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def desc_of(m: MeshDesc) -> MeshDescC:
+    c = MeshDescC()
+    c.nx[:] = list(m.nx)
+    c.bnx[:] = list(m.bnx)
+    c.ng = m.ng
+    c.xorder = m.xorder
+    c.rsolver = RSOLVER[m.rsolver]
+    c.relativistic = int(m.relativistic)
+    c.xmin[:] = list(m.xmin)
+    c.xmax[:] = list(m.xmax)
+    c.gamma = m.gamma
+    c.dfloor, c.pfloor = m.dfloor, m.pfloor
+    c.cfl = m.cfl
+    c.sigma_max, c.beta_min, c.gamma_max = m.sigma_max, m.beta_min, m.gamma_max
+    return c
+
+
+class GpuMesh:
+    """Device mirror of one uniform periodic mesh (all registers resident in HBM)."""
+
+    def __init__(self, m: MeshDesc):
+        L = load()
+        self.m = m
+        self._desc = desc_of(m)
+        h = ctypes.c_void_p()
+        _check(L.ath_gpu_mesh_create(ctypes.byref(self._desc), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().ath_gpu_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self, what):
+        n = ctypes.c_long()
+        _check(load().ath_gpu_block_size(self.h, ATH[what], ctypes.byref(n)))
+        return n.value
+
+    def set(self, gid, what, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != self.size(what):
+            raise AthGpuError("register %s of block %d: got %d doubles, need %d"
+                              % (what, gid, a.size, self.size(what)))
+        _check(load().ath_gpu_set_block(self.h, gid, ATH[what], _dp(a)))
+
+    def get(self, gid, what):
+        a = np.empty(self.size(what))
+        _check(load().ath_gpu_get_block(self.h, gid, ATH[what], _dp(a)))
+        if what in ("u", "w", "w1", "bcc", "b1", "b2", "b3"):
+            return a.reshape(self.m.shape(what))
+        return a
+
+    def load(self, blocks):
+        for gid, b in enumerate(blocks):
+            for k in ("u", "b1", "b2", "b3", "w", "w1"):
+                if k in b:
+                    self.set(gid, k, b[k])
+
+    def initialize(self, tlim=float("inf")):
+        _check(load().ath_gpu_initialize(self.h, tlim))
+
+    def run(self, ncycles, tlim=float("inf")):
+        done = ctypes.c_int()
+        _check(load().ath_gpu_cycle(self.h, ncycles, tlim, ctypes.byref(done)))
+        return done.value
+
+    def stage(self, s):
+        _check(load().ath_gpu_vl2_stage(self.h, s))
+
+    def state(self):
+        t, dt, nc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(load().ath_gpu_get_clock(self.h, ctypes.byref(t), ctypes.byref(dt),
+                                        ctypes.byref(nc)))
+        return t.value, dt.value, nc.value
+
+    def set_clock(self, time, dt, ncycle):
+        _check(load().ath_gpu_set_clock(self.h, time, dt, ncycle))
+
+    def synchronize(self):
+        _check(load().ath_gpu_synchronize(self.h))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _check(load().ath_gpu_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    def block(self, gid):
+        return {k: self.get(gid, k) for k in ("u", "w", "bcc", "b1", "b2", "b3")}
+
+    def kernel_timing(self, enable=True):
+        _check(load().ath_gpu_kernel_timing(self.h, int(enable)))
+
+    def kernel_stats(self):
+        """{name: (total_ms, launches)} for kernels launched since timing was enabled."""
+        out = {}
+        buf = ctypes.create_string_buffer(64)
+        for kid in range(NKERNELS):
+            ms, cnt = ctypes.c_double(), ctypes.c_long()
+            _check(load().ath_gpu_kernel_stats(self.h, kid, buf, 64, ctypes.byref(ms),
+                                               ctypes.byref(cnt)))
+            if cnt.value:
+                out[buf.value.decode()] = (ms.value, cnt.value)
+        return out
+
+
+def riemann_batch(solver, gamma, wl, wr, bx):
+    """solver: 0 HLLD, 1 HLLE, 2 SR HLLE; rotated-frame states (n,7)."""
+    wl = np.ascontiguousarray(wl, dtype=np.float64)
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    bx = np.ascontiguousarray(bx, dtype=np.float64)
+    out = np.empty_like(wl)
+    _check(load().ath_gpu_riemann_batch(solver, gamma, wl.shape[0], _dp(wl), _dp(wr), _dp(bx),
+                                        _dp(out)))
+    return out
+
+
+def sr_con2prim_batch(cons, prim_seed, bcc, gamma, dfloor, pfloor, sigma_max=0.0,
+                      beta_min=0.0, gamma_max=1000.0):
+    cons = np.array(cons, dtype=np.float64, order="C")
+    prim = np.array(prim_seed, dtype=np.float64, order="C")
+    bcc = np.ascontiguousarray(bcc, dtype=np.float64)
+    iters = np.empty(cons.shape[0], dtype=np.int32)
+    _check(load().ath_gpu_sr_con2prim_batch(
+        cons.shape[0], gamma, dfloor, pfloor, sigma_max, beta_min, gamma_max, _dp(cons),
+        _dp(prim), _dp(bcc), iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return cons, prim, iters

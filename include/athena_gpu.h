@@ -1,0 +1,129 @@
+/*
+ * athena_gpu.h -- C ABI of the MI355X per-MeshBlock MHD update.
+ *
+ * The library keeps every MeshBlock register (u, u1, w, w1, bcc, b, b1, fluxes, face
+ * and corner EMFs, CT weights) resident in HBM in the reference's AthenaArray layout
+ * (athena_arrays.hpp: (n,k,j,i), i fastest, one contiguous array per block), so a host
+ * AthenaArray can be copied in or out with one memcpy per register.  Blocks of one
+ * mesh are numbered row-major, x1 fastest.  All boundaries are periodic.
+ *
+ * Every function returns 0 on success and a non-zero status otherwise; the message is
+ * available from ath_gpu_last_error().  A non-zero status must be mapped to the
+ * reference's fatal ATHENA_ERROR (defs.hpp.in:155-159), never to TaskStatus::fail,
+ * which means "not ready, retry" (task_list.cpp:43-52).
+ *
+ * Calls on one mesh are ordered on that mesh's HIP stream; distinct meshes may be
+ * driven from distinct host threads.
+ */
+#ifndef ATHENA_GPU_H_
+#define ATHENA_GPU_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Input-file level description of one uniform periodic mesh.  Mirrors the athinput
+ * keys read by Mesh (mesh.cpp:72-…), Reconstruction (reconstruction.cpp:39-…:
+ * time/xorder), EquationOfState (adiabatic_mhd.cpp:26-31, adiabatic_mhd_sr.cpp:36-48)
+ * and the configure switches --flux, -s, --nghost (configure.py:80-353). */
+typedef struct ath_mesh_desc {
+  int nx[3];          /* <mesh> nx1..nx3 */
+  int bnx[3];         /* <meshblock> nx1..nx3 */
+  int ng;             /* NGHOST (>= 3 for xorder 3) */
+  int xorder;         /* 1 donor cell, 2 PLM, 3 PPM (second VL2 stage; first is DC) */
+  int rsolver;        /* 0 HLLD, 1 HLLE (relativistic: HLLE only) */
+  int relativistic;   /* 0 Newtonian, 1 special relativistic */
+  double xmin[3], xmax[3];
+  double gamma;       /* <hydro> gamma */
+  double dfloor, pfloor;   /* < 0 selects the reference default sqrt(1024*FLT_MIN) */
+  double cfl;         /* <time> cfl_number */
+  double sigma_max, beta_min, gamma_max;   /* SR floors; < 0 selects 0, 0, 1000 */
+} ath_mesh_desc;
+
+typedef struct ath_mesh *ath_mesh_t;
+
+/* Register codes for ath_gpu_set_block / ath_gpu_get_block. */
+enum {
+  ATH_U = 0,     /* Hydro::u   (NHYDRO, nc3, nc2, nc1)   */
+  ATH_W = 1,     /* Hydro::w   (NHYDRO, nc3, nc2, nc1)   */
+  ATH_BCC = 2,   /* Field::bcc (3, nc3, nc2, nc1)        */
+  ATH_B1 = 3,    /* Field::b.x1f (nc3, nc2, nc1+1)       */
+  ATH_B2 = 4,    /* Field::b.x2f (nc3, nc2+1, nc1)       */
+  ATH_B3 = 5,    /* Field::b.x3f (nc3+1, nc2, nc1)       */
+  ATH_W1 = 6,    /* Hydro::w1  (NHYDRO, nc3, nc2, nc1)   */
+  ATH_FLUX1 = 20, ATH_FLUX2 = 21, ATH_FLUX3 = 22,   /* Hydro::flux[X1DIR..X3DIR] */
+  ATH_E1 = 23, ATH_E2 = 24, ATH_E3 = 25,            /* Field::e.x1e .. x3e       */
+  ATH_WGHT1 = 26, ATH_WGHT2 = 27, ATH_WGHT3 = 28    /* Field::wght.x1f .. x3f    */
+};
+
+/* Creates the device mirror of a mesh (replaces the per-block allocations of the
+ * Hydro/Field constructors, hydro.cpp:31-60, field.cpp:27-105). */
+int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out);
+int ath_gpu_mesh_destroy(ath_mesh_t mesh);
+int ath_gpu_nblocks(ath_mesh_t mesh, int *nblocks);
+/* Number of doubles of one block's register `what`. */
+int ath_gpu_block_size(ath_mesh_t mesh, int what, long *ndoubles);
+
+/* Copy one block's register between a host AthenaArray buffer and HBM. */
+int ath_gpu_set_block(ath_mesh_t mesh, int gid, int what, const double *host);
+int ath_gpu_get_block(ath_mesh_t mesh, int gid, int what, double *host);
+
+/* Mesh::Initialize after the problem generator (mesh.cpp:1686-1966): same-level ghost
+ * exchange of u and b, ConservedToPrimitive(u, w1, b, w, bcc) over every cell,
+ * NewBlockTimeStep on every block and Mesh::NewTimeStep. */
+int ath_gpu_initialize(ath_mesh_t mesh, double tlim);
+
+/* One VL2 stage of the TimeIntegratorTaskList for all blocks (stage = 1 or 2):
+ * CalculateFluxes (DC on stage 1, xorder on stage 2), ComputeCornerE, the same-level
+ * EMF flux correction, WeightedAve + AddFluxDivergence, WeightedAve + CT, ghost
+ * exchange, ConservedToPrimitive and, on stage 2, NewBlockTimeStep
+ * (time_integrator.cpp:1606-2318). */
+int ath_gpu_vl2_stage(ath_mesh_t mesh, int stage);
+
+/* Main-loop cycles (main.cpp:457-560): both stages, ncycle++, time += dt,
+ * Mesh::NewTimeStep.  Runs at most ncycles cycles while time < tlim; *done receives
+ * the number of cycles run.  With tlim = +inf no host synchronisation happens. */
+int ath_gpu_cycle(ath_mesh_t mesh, int ncycles, double tlim, int *done);
+
+/* Mesh clock (Mesh::time, Mesh::dt, Mesh::ncycle). */
+int ath_gpu_get_clock(ath_mesh_t mesh, double *time, double *dt, int *ncycle);
+int ath_gpu_set_clock(ath_mesh_t mesh, double time, double dt, int ncycle);
+
+/* Per-function entry points over all blocks, for function-level parity:
+ * Hydro::CalculateFluxes(w, b, bcc, order)  (calculate_fluxes.cpp:38-415)
+ * Field::ComputeCornerE(w, bcc)              (calculate_corner_e.cpp:28-236)
+ * EMF same-level flux correction is applied inside ath_gpu_ct. */
+int ath_gpu_calculate_fluxes(ath_mesh_t mesh, int order);
+int ath_gpu_compute_corner_e(ath_mesh_t mesh);
+
+/* Per-kernel timing with HIP events recorded around every launch on the mesh stream
+ * (no counterpart in the reference, which only prints zone-cycles/s,
+ * main.cpp:634-651).  enable != 0 clears and starts collection; stats return the
+ * kernel name, accumulated milliseconds and launch count for kernel id 0..14. */
+int ath_gpu_kernel_timing(ath_mesh_t mesh, int enable);
+int ath_gpu_kernel_stats(ath_mesh_t mesh, int id, char *name, int namelen,
+                         double *total_ms, long *count);
+
+/* Hydro stream the library launches on (a hipStream_t), for event timing. */
+int ath_gpu_stream(ath_mesh_t mesh, void **stream);
+int ath_gpu_synchronize(ath_mesh_t mesh);
+
+/* Batched single-interface Riemann solves (rotated frame, 7 primitives per side):
+ * solver 0 HLLD (hlld.cpp), 1 HLLE (hlle_mhd.cpp), 2 SR HLLE (hlle_mhd_rel.cpp).
+ * Host arrays of n*7, n and n*7 doubles. */
+int ath_gpu_riemann_batch(int solver, double gamma, int n, const double *wl,
+                          const double *wr, const double *bx, double *flux);
+
+/* Batched SR primitive recovery, one cell per entry (adiabatic_mhd_sr.cpp:72-272):
+ * cons (n,5) in/out, prim (n,5) seed in/result out, bcc (n,3); iters (n) receives the
+ * first-solve Newman-Hamlin iteration count or -1.  Host arrays. */
+int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
+                              double sigma_max, double beta_min, double gamma_max,
+                              double *cons, double *prim, const double *bcc, int *iters);
+
+const char *ath_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATHENA_GPU_H_ */
