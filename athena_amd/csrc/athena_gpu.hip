@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -57,6 +58,7 @@ struct MeshK {
   const double *coord;        // [nblocks][cstride]: per d: dxf[nc], pf[nc], mf[nc]
   EosK eos;
   double cfl;
+  int xcd_remap;              // 1: XCD-contiguous work-group numbering (ATH_XCD_REMAP=1)
 };
 
 struct Clock {
@@ -91,6 +93,19 @@ __device__ __forceinline__ int nbr(const MeshK &M, int g, int o1, int o2, int o3
 }
 __device__ __forceinline__ const double *ctab(const MeshK &M, int g, int d) {
   return M.coord + static_cast<long>(g) * M.cstride + M.coff[d];
+}
+
+// XCD-aware work-group numbering.  The hardware hands work-groups to the 8 XCDs
+// round-robin by id, and each XCD has its own L2; renumbering so that the ids one XCD
+// receives form one contiguous logical range keeps a MeshBlock (and its stencil
+// neighbours) inside one L2.  Bijective for any grid size.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_logical_wg(int remap) {
+  const int w = blockIdx.x, n = gridDim.x;
+  if (!remap) return w;
+  const int q = n / NXCD, r = n % NXCD;
+  const int x = w % NXCD, l = w / NXCD;
+  return x * q + (x < r ? x : r) + l;
 }
 
 //========================================================================================
@@ -184,12 +199,14 @@ __global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__
                                               double *__restrict__ ez,
                                               double *__restrict__ wct,
                                               const Clock *__restrict__ clk) {
-  const int g = blockIdx.y;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
   int ni, nj, nk, i0, j0, k0;
   if constexpr (DIR == 0) { ni = M.nx1 + 1; nj = M.nx2 + 2; nk = M.nx3 + 2; i0 = M.is; j0 = M.js - 1; k0 = M.ks - 1; }
   else if constexpr (DIR == 1) { ni = M.nx1 + 2; nj = M.nx2 + 1; nk = M.nx3 + 2; i0 = M.is - 1; j0 = M.js; k0 = M.ks - 1; }
   else { ni = M.nx1 + 2; nj = M.nx2 + 2; nk = M.nx3 + 1; i0 = M.is - 1; j0 = M.js - 1; k0 = M.ks; }
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int t = (lw % nch) * WG + threadIdx.x;
   if (t >= ni * nj * nk) return;
   const int i = i0 + t % ni;
   const int r = t / ni;
@@ -303,9 +320,11 @@ __global__ void __launch_bounds__(WG) k_corner(
     const double *__restrict__ wx1, const double *__restrict__ wx2,
     const double *__restrict__ wx3, double *__restrict__ e1, double *__restrict__ e2,
     double *__restrict__ e3) {
-  const int g = blockIdx.y;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
   const int ni = M.nx1 + 1, nj = M.nx2 + 1, nk = M.nx3 + 1;
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int t = (lw % nch) * WG + threadIdx.x;
   if (t >= ni * nj * nk) return;
   const int i = M.is + t % ni;
   const int r = t / ni;
@@ -442,9 +461,11 @@ __global__ void __launch_bounds__(WG) k_update_u(MeshK M, const double *base, do
                                                   const double *__restrict__ f3,
                                                   double beta, int first_stage,
                                                   const Clock *__restrict__ clk) {
-  const int g = blockIdx.y;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
   const int ni = M.nx1, nj = M.nx2, nk = M.nx3;
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int t = (lw % nch) * WG + threadIdx.x;
   if (t >= ni * nj * nk) return;
   const int i = M.is + t % ni;
   const int r = t / ni;
@@ -500,10 +521,12 @@ __global__ void __launch_bounds__(WG) k_update_b(MeshK M, const double *b1in,
                                                   const double *__restrict__ e3, double beta,
                                                   int first_stage,
                                                   const Clock *__restrict__ clk) {
-  const int g = blockIdx.y;
-  const int comp = blockIdx.z;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / (3 * M.nblocks);
+  const int g = lw / (3 * nch);
+  const int comp = (lw / nch) % 3;
   const int ni = M.nx1 + (comp == 0), nj = M.nx2 + (comp == 1), nk = M.nx3 + (comp == 2);
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int t = (lw % nch) * WG + threadIdx.x;
   if (t >= ni * nj * nk) return;
   const int i = M.is + t % ni;
   const int r = t / ni;
@@ -590,8 +613,10 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
                                                    const double *__restrict__ b2,
                                                    const double *__restrict__ b3,
                                                    double *__restrict__ dt_partial) {
-  const int g = blockIdx.y;
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
+  const int t = (lw % nch) * WG + threadIdx.x;
   double my_dt = DBL_MAX;
   if (t < M.ncell) {
     const int i = t % M.nc1;
@@ -688,7 +713,7 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
     if (threadIdx.x == 0) {
       double m = sm[0];
       for (int q = 1; q < WG / 64; ++q) m = dmin(m, sm[q]);
-      dt_partial[static_cast<long>(g) * gridDim.x + blockIdx.x] = m;
+      dt_partial[blockIdx.x] = m;
     }
   }
 }
@@ -724,8 +749,10 @@ __global__ void __launch_bounds__(1024) k_new_dt(const double *__restrict__ part
 // Pull-based same-level ghost fill for Mesh::Initialize (cell-centred register with
 // nvar components).
 __global__ void __launch_bounds__(WG) k_ghost_cc(MeshK M, double *a, int nvar) {
-  const int g = blockIdx.y;
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
+  const int t = (lw % nch) * WG + threadIdx.x;
   if (t >= M.ncell) return;
   const int i = t % M.nc1;
   const int r = t / M.nc1;
@@ -744,10 +771,12 @@ __global__ void __launch_bounds__(WG) k_ghost_cc(MeshK M, double *a, int nvar) {
 
 __global__ void __launch_bounds__(WG) k_ghost_fc(MeshK M, double *b1, double *b2,
                                                   double *b3) {
-  const int g = blockIdx.y;
-  const int comp = blockIdx.z;
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / (3 * M.nblocks);
+  const int g = lw / (3 * nch);
+  const int comp = (lw / nch) % 3;
   const int n1 = M.nc1 + (comp == 0), n2 = M.nc2 + (comp == 1), n3 = M.nc3 + (comp == 2);
-  const int t = blockIdx.x * WG + threadIdx.x;
+  const int t = (lw % nch) * WG + threadIdx.x;
   if (t >= n1 * n2 * n3) return;
   const int i = t % n1;
   const int r = t / n1;
@@ -892,7 +921,7 @@ int launch_flux(ath_mesh *m, int order) {
     if (dir == 0) nface = static_cast<long>(M.nx1 + 1) * (M.nx2 + 2) * (M.nx3 + 2);
     else if (dir == 1) nface = static_cast<long>(M.nx1 + 2) * (M.nx2 + 1) * (M.nx3 + 2);
     else nface = static_cast<long>(M.nx1 + 2) * (M.nx2 + 2) * (M.nx3 + 1);
-    dim3 grid((nface + WG - 1) / WG, M.nblocks);
+    dim3 grid(((nface + WG - 1) / WG) * M.nblocks);
     double *ey, *ez;
     if (dir == 0) { ey = m->e3x1; ez = m->e2x1; }
     else if (dir == 1) { ey = m->e1x2; ez = m->e3x2; }
@@ -929,7 +958,7 @@ int launch_flux(ath_mesh *m, int order) {
 int launch_corner(ath_mesh *m) {
   const MeshK &M = m->M;
   long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
-  dim3 grid((n + WG - 1) / WG, M.nblocks);
+  dim3 grid(((n + WG - 1) / WG) * M.nblocks);
   ProfScope ps(m, KID_CORNER);
   if (m->d.relativistic)
     hipLaunchKernelGGL((k_corner<true>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,
@@ -945,7 +974,7 @@ int launch_corner(ath_mesh *m) {
 
 int launch_cons2prim(ath_mesh *m, double *prim_out, bool with_dt) {
   const MeshK &M = m->M;
-  dim3 grid((M.ncell + WG - 1) / WG, M.nblocks);
+  dim3 grid(((M.ncell + WG - 1) / WG) * M.nblocks);
   double *part = with_dt ? m->dtpart : nullptr;
   ProfScope ps(m, KID_CONS2PRIM);
   if (m->d.relativistic)
@@ -977,7 +1006,7 @@ int stage(ath_mesh *m, int stg) {
   // in place on the swapped-in U^n.
   {
     long n = static_cast<long>(M.nx1) * M.nx2 * M.nx3;
-    dim3 grid((n + WG - 1) / WG, M.nblocks);
+    dim3 grid(((n + WG - 1) / WG) * M.nblocks);
     double *base, *out;
     if (stg == 1) { base = m->u; out = m->u1; }
     else { base = m->u1; out = m->u1; }
@@ -991,7 +1020,7 @@ int stage(ath_mesh *m, int stg) {
   }
   {
     long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
-    dim3 grid((n + WG - 1) / WG, M.nblocks, 3);
+    dim3 grid(((n + WG - 1) / WG) * M.nblocks * 3);
     double **in = (stg == 1) ? m->b : m->bb;
     double **out = m->bb;
     {
@@ -1093,6 +1122,10 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   M.eos.beta_min = (d.beta_min < 0.0) ? 0.0 : d.beta_min;
   M.eos.gamma_max = (d.gamma_max < 0.0) ? 1000.0 : d.gamma_max;
   M.cfl = d.cfl;
+  {
+    const char *env = std::getenv("ATH_XCD_REMAP");
+    M.xcd_remap = (env && env[0] == '1') ? 1 : 0;
+  }
 
   // neighbour table and coordinates (coordinates.cpp:129-146, cartesian.cpp:23-72)
   const int nb = M.nblocks;
@@ -1255,13 +1288,13 @@ int ath_gpu_initialize(ath_mesh_t m, double tlim) {
   Clock c0{0.0, std::numeric_limits<double>::max(), tlim, 0.0};
   HIPCHK(hipMemcpyAsync(m->clk, &c0, sizeof(Clock), hipMemcpyHostToDevice, m->stream));
   {
-    dim3 grid((M.ncell + WG - 1) / WG, M.nblocks);
+    dim3 grid(((M.ncell + WG - 1) / WG) * M.nblocks);
     hipLaunchKernelGGL(k_ghost_cc, grid, dim3(WG), 0, m->stream, M, m->u, NHYDRO);
     HIPCHK(hipGetLastError());
   }
   {
     long n = std::max(M.nf1, std::max(M.nf2, M.nf3));
-    dim3 grid((n + WG - 1) / WG, M.nblocks, 3);
+    dim3 grid(((n + WG - 1) / WG) * M.nblocks * 3);
     hipLaunchKernelGGL(k_ghost_fc, grid, dim3(WG), 0, m->stream, M, m->b[0], m->b[1], m->b[2]);
     HIPCHK(hipGetLastError());
   }
