@@ -188,35 +188,25 @@ __device__ __forceinline__ void ppm_cell(const double *q, double &qminus, double
 }
 
 //========================================================================================
-// Kernel 1: interface fluxes along DIR for every block (Hydro::CalculateFluxes one
-// direction + the Riemann solver + GetWeightForCT).  One thread per interface.
-template <int DIR, int ORDER, int RS>
-__global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__ w,
-                                              const double *__restrict__ bcc,
-                                              const double *__restrict__ bn,
-                                              double *__restrict__ flx,
-                                              double *__restrict__ ey,
-                                              double *__restrict__ ez,
-                                              double *__restrict__ wct,
-                                              const Clock *__restrict__ clk) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  int ni, nj, nk, i0, j0, k0;
-  if constexpr (DIR == 0) { ni = M.nx1 + 1; nj = M.nx2 + 2; nk = M.nx3 + 2; i0 = M.is; j0 = M.js - 1; k0 = M.ks - 1; }
-  else if constexpr (DIR == 1) { ni = M.nx1 + 2; nj = M.nx2 + 1; nk = M.nx3 + 2; i0 = M.is - 1; j0 = M.js; k0 = M.ks - 1; }
-  else { ni = M.nx1 + 2; nj = M.nx2 + 2; nk = M.nx3 + 1; i0 = M.is - 1; j0 = M.js - 1; k0 = M.ks; }
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  const int i = i0 + t % ni;
-  const int r = t / ni;
-  const int j = j0 + r % nj;
-  const int k = k0 + r / nj;
+// Kernel 1: interface fluxes (Hydro::CalculateFluxes + the Riemann solver +
+// GetWeightForCT).  flux_face computes one interface along DIR; k_flux3 runs one
+// thread per cell of the union range and computes the three faces on the cell's low
+// sides, so w and bcc are fetched from HBM once per stage instead of once per
+// direction.  k_flux keeps the one-direction form for the per-function entry.
 
+// Per-direction output registers.
+struct FaceOut {
+  const double *bn;   // normal face field b.xNf
+  double *flx, *ey, *ez, *wct;
+};
+
+template <int DIR, int ORDER, int RS>
+__device__ __forceinline__ void flux_face(const MeshK &M, int g, int k, int j, int i,
+                                          const double *__restrict__ wb,
+                                          const double *__restrict__ bb, const FaceOut &o,
+                                          double dt) {
   const long s = (DIR == 0) ? 1 : ((DIR == 1) ? M.nc1 : static_cast<long>(M.nc1) * M.nc2);
   const long c = cidx(M, k, j, i);
-  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
-  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
   constexpr int ivx = IVX + DIR;
   constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
   constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
@@ -269,24 +259,70 @@ __global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__
   else if (DIR == 1) fi = f2idx(M, k, j, i);
   else fi = f3idx(M, k, j, i);
   const long nf = (DIR == 0) ? M.nf1 : ((DIR == 1) ? M.nf2 : M.nf3);
-  const double bx = bn[static_cast<long>(g) * nf + fi];
+  const double bx = o.bn[static_cast<long>(g) * nf + fi];
 
   double f[NWAVE];
   if (RS == 0) hlld(wl, wr, bx, M.eos, f);
   else if (RS == 1) hlle(wl, wr, bx, M.eos, f);
   else sr_hlle(wl, wr, bx, M.eos, f);
 
-  double *fb = flx + static_cast<long>(g) * NHYDRO * nf;
+  double *fb = o.flx + static_cast<long>(g) * NHYDRO * nf;
   fb[IDN * nf + fi] = f[IDN];
   fb[ivx * nf + fi] = f[IVX];
   fb[ivy * nf + fi] = f[IVY];
   fb[ivz * nf + fi] = f[IVZ];
   fb[IEN * nf + fi] = f[IEN];
-  ey[static_cast<long>(g) * nf + fi] = -f[IBY];
-  ez[static_cast<long>(g) * nf + fi] = f[IBZ];
+  o.ey[static_cast<long>(g) * nf + fi] = -f[IBY];
+  o.ez[static_cast<long>(g) * nf + fi] = f[IBZ];
   const double dxw = ct[fc];
-  wct[static_cast<long>(g) * nf + fi] = ct_weight(f[IDN], wl[IDN], wr[IDN], dxw, clk->dt);
-  (void)ncd;
+  o.wct[static_cast<long>(g) * nf + fi] = ct_weight(f[IDN], wl[IDN], wr[IDN], dxw, dt);
+}
+
+template <int DIR, int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__ w,
+                                              const double *__restrict__ bcc, FaceOut o,
+                                              const Clock *__restrict__ clk) {
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
+  int ni, nj, nk, i0, j0, k0;
+  if constexpr (DIR == 0) { ni = M.nx1 + 1; nj = M.nx2 + 2; nk = M.nx3 + 2; i0 = M.is; j0 = M.js - 1; k0 = M.ks - 1; }
+  else if constexpr (DIR == 1) { ni = M.nx1 + 2; nj = M.nx2 + 1; nk = M.nx3 + 2; i0 = M.is - 1; j0 = M.js; k0 = M.ks - 1; }
+  else { ni = M.nx1 + 2; nj = M.nx2 + 2; nk = M.nx3 + 1; i0 = M.is - 1; j0 = M.js - 1; k0 = M.ks; }
+  const int t = (lw % nch) * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = i0 + t % ni;
+  const int r = t / ni;
+  const int j = j0 + r % nj;
+  const int k = k0 + r / nj;
+  flux_face<DIR, ORDER, RS>(M, g, k, j, i, w + static_cast<long>(g) * NHYDRO * M.ncell,
+                            bcc + static_cast<long>(g) * 3 * M.ncell, o, clk->dt);
+}
+
+// One thread per cell of [is-1,ie+1]x[js-1,je+1]x[ks-1,ke+1]; each computes the faces on
+// its low sides that lie in the reference's loop ranges (calculate_fluxes.cpp:68-76,
+// 176-182, 299-302).
+template <int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_flux3(MeshK M, const double *__restrict__ w,
+                                               const double *__restrict__ bcc, FaceOut o1,
+                                               FaceOut o2, FaceOut o3,
+                                               const Clock *__restrict__ clk) {
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
+  const int ni = M.nx1 + 3, nj = M.nx2 + 3, nk = M.nx3 + 3;
+  const int t = (lw % nch) * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = M.is - 1 + t % ni;
+  const int r = t / ni;
+  const int j = M.js - 1 + r % nj;
+  const int k = M.ks - 1 + r / nj;
+  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+  const double dt = clk->dt;
+  if (i >= M.is) flux_face<0, ORDER, RS>(M, g, k, j, i, wb, bb, o1, dt);
+  if (j >= M.js) flux_face<1, ORDER, RS>(M, g, k, j, i, wb, bb, o2, dt);
+  if (k >= M.ks) flux_face<2, ORDER, RS>(M, g, k, j, i, wb, bb, o3, dt);
 }
 
 //========================================================================================
@@ -455,23 +491,12 @@ __device__ __forceinline__ double emf1(const MeshK &M, const double *e, int g, i
 // every neighbour ghost cell that is a copy of it (cc/bvals_cc.cpp:287-427).
 // first_stage: out = (0 + 1*base) - wght*div F   (u1 zeroed, delta = 1, then swapped)
 // otherwise:   out = base - wght*div F           (in place allowed: out == base)
-__global__ void __launch_bounds__(WG) k_update_u(MeshK M, const double *base, double *out,
-                                                  const double *__restrict__ f1,
-                                                  const double *__restrict__ f2,
-                                                  const double *__restrict__ f3,
-                                                  double beta, int first_stage,
-                                                  const Clock *__restrict__ clk) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  const int ni = M.nx1, nj = M.nx2, nk = M.nx3;
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  const int i = M.is + t % ni;
-  const int r = t / ni;
-  const int j = M.js + r % nj;
-  const int k = M.ks + r / nj;
-  const double wght = beta * clk->dt;
+__device__ __forceinline__ void update_cell_u(const MeshK &M, int g, int k, int j, int i,
+                                              const double *base, double *out,
+                                              const double *__restrict__ f1,
+                                              const double *__restrict__ f2,
+                                              const double *__restrict__ f3, double wght,
+                                              int first_stage) {
   const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
   const double a1 = dx2[j] * dx3[k];
   const double a2 = dx1[i] * dx3[k];
@@ -510,63 +535,42 @@ __global__ void __launch_bounds__(WG) k_update_u(MeshK M, const double *base, do
   }
 }
 
-// Kernel 3b: face-field update, WeightedAve(FaceField) + Field::CT (ct.cpp:31-116) with
-// averaged surface EMFs, then the face-centred ghost push (fc/bvals_fc.cpp:344-688).
-// grid.z selects the component.
-__global__ void __launch_bounds__(WG) k_update_b(MeshK M, const double *b1in,
-                                                  const double *b2in, const double *b3in,
-                                                  double *b1out, double *b2out,
-                                                  double *b3out, const double *__restrict__ e1,
-                                                  const double *__restrict__ e2,
-                                                  const double *__restrict__ e3, double beta,
-                                                  int first_stage,
-                                                  const Clock *__restrict__ clk) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / (3 * M.nblocks);
-  const int g = lw / (3 * nch);
-  const int comp = (lw / nch) % 3;
-  const int ni = M.nx1 + (comp == 0), nj = M.nx2 + (comp == 1), nk = M.nx3 + (comp == 2);
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  const int i = M.is + t % ni;
-  const int r = t / ni;
-  const int j = M.js + r % nj;
-  const int k = M.ks + r / nj;
-  const double wght = beta * clk->dt;
+// Face-field update, WeightedAve(FaceField) + Field::CT (ct.cpp:31-116) with averaged
+// surface EMFs, then the face-centred ghost push (fc/bvals_fc.cpp:344-688).
+template <int comp>
+__device__ __forceinline__ void update_face_b(const MeshK &M, int g, int k, int j, int i,
+                                              const double *bin, double *outp,
+                                              const double *__restrict__ e1,
+                                              const double *__restrict__ e2,
+                                              const double *__restrict__ e3, double wght,
+                                              int first_stage) {
   const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
   double v;
   long nf;
-  double *outp;
   if (comp == 0) {
     nf = M.nf1;
-    const long x = f1idx(M, k, j, i);
-    v = b1in[static_cast<long>(g) * nf + x];
+    v = bin[static_cast<long>(g) * nf + f1idx(M, k, j, i)];
     if (first_stage) v = 0.0 + 1.0 * v;
     const double area = dx2[j] * dx3[k];
     const double l3 = dx3[k], l2 = dx2[j];
     v -= (wght / area) * (l3 * emf3(M, e3, g, k, j + 1, i) - l3 * emf3(M, e3, g, k, j, i));
     v += (wght / area) * (l2 * emf2(M, e2, g, k + 1, j, i) - l2 * emf2(M, e2, g, k, j, i));
-    outp = b1out;
   } else if (comp == 1) {
     nf = M.nf2;
-    const long x = f2idx(M, k, j, i);
-    v = b2in[static_cast<long>(g) * nf + x];
+    v = bin[static_cast<long>(g) * nf + f2idx(M, k, j, i)];
     if (first_stage) v = 0.0 + 1.0 * v;
     const double area = dx1[i] * dx3[k];
     const double l3 = dx3[k], l1 = dx1[i];
     v += (wght / area) * (l3 * emf3(M, e3, g, k, j, i + 1) - l3 * emf3(M, e3, g, k, j, i));
     v -= (wght / area) * (l1 * emf1(M, e1, g, k + 1, j, i) - l1 * emf1(M, e1, g, k, j, i));
-    outp = b2out;
   } else {
     nf = M.nf3;
-    const long x = f3idx(M, k, j, i);
-    v = b3in[static_cast<long>(g) * nf + x];
+    v = bin[static_cast<long>(g) * nf + f3idx(M, k, j, i)];
     if (first_stage) v = 0.0 + 1.0 * v;
     const double area = dx1[i] * dx2[j];
     const double l2 = dx2[j], l1 = dx1[i];
     v -= (wght / area) * (l2 * emf2(M, e2, g, k, j, i + 1) - l2 * emf2(M, e2, g, k, j, i));
     v += (wght / area) * (l1 * emf1(M, e1, g, k, j + 1, i) - l1 * emf1(M, e1, g, k, j, i));
-    outp = b3out;
   }
   auto fidx = [&](int kk, int jj, int ii) -> long {
     if (comp == 0) return f1idx(M, kk, jj, ii);
@@ -597,6 +601,42 @@ __global__ void __launch_bounds__(WG) k_update_b(MeshK M, const double *b1in,
         const int tg = nbr(M, g, p1, p2, p3);
         outp[static_cast<long>(tg) * nf + fidx(k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1)] = v;
       }
+}
+
+// One thread per (k,j,i) in [ks,ke+1]x[js,je+1]x[is,ie+1]: the active cell's conserved
+// update and the three faces on its low sides that are active faces.
+struct UpdateArgs {
+  const double *ubase;
+  double *uout;
+  const double *f1, *f2, *f3;
+  const double *bin[3];
+  double *bout[3];
+  const double *e1, *e2, *e3;
+};
+
+__global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double beta,
+                                                int first_stage,
+                                                const Clock *__restrict__ clk) {
+  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int nch = gridDim.x / M.nblocks;
+  const int g = lw / nch;
+  const int ni = M.nx1 + 1, nj = M.nx2 + 1, nk = M.nx3 + 1;
+  const int t = (lw % nch) * WG + threadIdx.x;
+  if (t >= ni * nj * nk) return;
+  const int i = M.is + t % ni;
+  const int r = t / ni;
+  const int j = M.js + r % nj;
+  const int k = M.ks + r / nj;
+  const double wght = beta * clk->dt;
+  const bool ci = i <= M.ie, cj = j <= M.je, ck = k <= M.ke;
+  if (ci && cj && ck)
+    update_cell_u(M, g, k, j, i, a.ubase, a.uout, a.f1, a.f2, a.f3, wght, first_stage);
+  if (cj && ck)
+    update_face_b<0>(M, g, k, j, i, a.bin[0], a.bout[0], a.e1, a.e2, a.e3, wght, first_stage);
+  if (ci && ck)
+    update_face_b<1>(M, g, k, j, i, a.bin[1], a.bout[1], a.e1, a.e2, a.e3, wght, first_stage);
+  if (ci && cj)
+    update_face_b<2>(M, g, k, j, i, a.bin[2], a.bout[2], a.e1, a.e2, a.e3, wght, first_stage);
 }
 
 //========================================================================================
@@ -876,7 +916,8 @@ namespace {
 enum KernelId {
   KID_FLUX = 0,          // + 3*dir + (order-1): 9 flux variants
   KID_CORNER = 9, KID_UPDATE_U = 10, KID_UPDATE_B = 11, KID_CONS2PRIM = 12,
-  KID_NEW_DT = 13, KID_GHOST = 14, KID_COUNT = 15
+  KID_NEW_DT = 13, KID_GHOST = 14, KID_FLUX3 = 15,   // + (order-1): 3 fused variants
+  KID_COUNT = 18
 };
 const char *kernel_name(int id) {
   static const char *flux[9] = {
@@ -885,11 +926,14 @@ const char *kernel_name(int id) {
   if (id >= 0 && id < 9) return flux[id];
   switch (id) {
     case KID_CORNER: return "corner_emf";
-    case KID_UPDATE_U: return "update_u";
+    case KID_UPDATE_U: return "update_u_b_ct";
     case KID_UPDATE_B: return "update_b_ct";
     case KID_CONS2PRIM: return "cons2prim";
     case KID_NEW_DT: return "new_dt";
     case KID_GHOST: return "ghost_fill";
+    case KID_FLUX3: return "flux_xyz_dc";
+    case KID_FLUX3 + 1: return "flux_xyz_plm";
+    case KID_FLUX3 + 2: return "flux_xyz_ppm";
     default: return "unknown";
   }
 }
@@ -913,7 +957,45 @@ struct ProfScope {
   }
 };
 
+FaceOut face_out(ath_mesh *m, int dir) {
+  FaceOut o;
+  o.bn = m->b[dir];
+  o.flx = m->flux[dir];
+  o.wct = m->wx[dir];
+  if (dir == 0) { o.ey = m->e3x1; o.ez = m->e2x1; }
+  else if (dir == 1) { o.ey = m->e1x2; o.ez = m->e3x2; }
+  else { o.ey = m->e2x3; o.ez = m->e1x3; }
+  return o;
+}
+
+// All three directions in one launch (the production path).
 int launch_flux(ath_mesh *m, int order) {
+  const MeshK &M = m->M;
+  const int rs = m->d.relativistic ? 2 : m->d.rsolver;
+  const long n = static_cast<long>(M.nx1 + 3) * (M.nx2 + 3) * (M.nx3 + 3);
+  dim3 grid(((n + WG - 1) / WG) * M.nblocks);
+  FaceOut o1 = face_out(m, 0), o2 = face_out(m, 1), o3 = face_out(m, 2);
+  ProfScope ps(m, KID_FLUX3 + (order - 1));
+#define LF(O, R)                                                                        \
+  hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc, o1, \
+                     o2, o3, m->clk)
+#define LF_O(R)                         \
+  do {                                  \
+    if (order == 1) LF(1, R);           \
+    else if (order == 2) LF(2, R);      \
+    else LF(3, R);                      \
+  } while (0)
+  if (rs == 0) LF_O(0);
+  else if (rs == 1) LF_O(1);
+  else LF_O(2);
+#undef LF_O
+#undef LF
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// One direction per launch (Hydro::CalculateFluxes direction by direction).
+int launch_flux_dirs(ath_mesh *m, int order) {
   const MeshK &M = m->M;
   const int rs = m->d.relativistic ? 2 : m->d.rsolver;
   for (int dir = 0; dir < 3; ++dir) {
@@ -922,13 +1004,10 @@ int launch_flux(ath_mesh *m, int order) {
     else if (dir == 1) nface = static_cast<long>(M.nx1 + 2) * (M.nx2 + 1) * (M.nx3 + 2);
     else nface = static_cast<long>(M.nx1 + 2) * (M.nx2 + 2) * (M.nx3 + 1);
     dim3 grid(((nface + WG - 1) / WG) * M.nblocks);
-    double *ey, *ez;
-    if (dir == 0) { ey = m->e3x1; ez = m->e2x1; }
-    else if (dir == 1) { ey = m->e1x2; ez = m->e3x2; }
-    else { ey = m->e2x3; ez = m->e1x3; }
+    FaceOut o = face_out(m, dir);
 #define LF(D, O, R)                                                                       \
-  hipLaunchKernelGGL((k_flux<D, O, R>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,   \
-                     m->b[D], m->flux[D], ey, ez, m->wx[D], m->clk)
+  hipLaunchKernelGGL((k_flux<D, O, R>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc, o, \
+                     m->clk)
 #define LF_O(D, R)                      \
   do {                                  \
     if (order == 1) LF(D, 1, R);        \
@@ -1001,35 +1080,28 @@ int stage(ath_mesh *m, int stg) {
   const int order = (stg == 1) ? 1 : m->d.xorder;
   if (launch_flux(m, order)) return 1;
   if (launch_corner(m)) return 1;
-  // conserved update: stage 1 reads u and writes u1, then the registers swap
-  // (IntegrateHydro: WeightedAve into zeroed u1, u.SwapAthenaArray(u1)); stage 2 is
-  // in place on the swapped-in U^n.
-  {
-    long n = static_cast<long>(M.nx1) * M.nx2 * M.nx3;
-    dim3 grid(((n + WG - 1) / WG) * M.nblocks);
-    double *base, *out;
-    if (stg == 1) { base = m->u; out = m->u1; }
-    else { base = m->u1; out = m->u1; }
-    {
-      ProfScope ps(m, KID_UPDATE_U);
-      hipLaunchKernelGGL(k_update_u, grid, dim3(WG), 0, m->stream, M, base, out, m->flux[0],
-                         m->flux[1], m->flux[2], beta, stg == 1 ? 1 : 0, m->clk);
-    }
-    HIPCHK(hipGetLastError());
-    std::swap(m->u, m->u1);
-  }
+  // conserved and face-field update.  Stage 1 reads u, b and writes u1, b1, then the
+  // registers swap (IntegrateHydro/IntegrateField: WeightedAve into zeroed u1/b1 and
+  // SwapAthenaArray); stage 2 is in place on the swapped-in U^n, B^n.
   {
     long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
-    dim3 grid(((n + WG - 1) / WG) * M.nblocks * 3);
-    double **in = (stg == 1) ? m->b : m->bb;
-    double **out = m->bb;
+    dim3 grid(((n + WG - 1) / WG) * M.nblocks);
+    UpdateArgs a;
+    a.ubase = (stg == 1) ? m->u : m->u1;
+    a.uout = m->u1;
+    a.f1 = m->flux[0]; a.f2 = m->flux[1]; a.f3 = m->flux[2];
+    for (int c = 0; c < 3; ++c) {
+      a.bin[c] = (stg == 1) ? m->b[c] : m->bb[c];
+      a.bout[c] = m->bb[c];
+    }
+    a.e1 = m->e[0]; a.e2 = m->e[1]; a.e3 = m->e[2];
     {
-      ProfScope ps(m, KID_UPDATE_B);
-      hipLaunchKernelGGL(k_update_b, grid, dim3(WG), 0, m->stream, M, in[0], in[1], in[2],
-                         out[0], out[1], out[2], m->e[0], m->e[1], m->e[2], beta,
+      ProfScope ps(m, KID_UPDATE_U);
+      hipLaunchKernelGGL(k_update, grid, dim3(WG), 0, m->stream, M, a, beta,
                          stg == 1 ? 1 : 0, m->clk);
     }
     HIPCHK(hipGetLastError());
+    std::swap(m->u, m->u1);
     for (int c = 0; c < 3; ++c) std::swap(m->b[c], m->bb[c]);
   }
   // Primitives: ConservedToPrimitive(u, w, b, w1, bcc) then w.SwapAthenaArray(w1)
@@ -1365,7 +1437,7 @@ int ath_gpu_calculate_fluxes(ath_mesh_t m, int order) {
   if (!m) return fail("null argument");
   if (order < 1 || order > 3) return fail("order must be 1, 2 or 3");
   if (order == 3 && m->M.ng < 3) return fail("PPM needs nghost >= 3");
-  return launch_flux(m, order);
+  return launch_flux_dirs(m, order);
 }
 
 int ath_gpu_compute_corner_e(ath_mesh_t m) {

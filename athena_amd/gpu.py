@@ -27,7 +27,7 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_compute_corner_e", "ath_gpu_stream", "ath_gpu_synchronize",
            "ath_gpu_riemann_batch", "ath_gpu_sr_con2prim_batch", "ath_gpu_last_error",
            "ath_gpu_kernel_timing", "ath_gpu_kernel_stats"]
-NKERNELS = 15
+NKERNELS = 18
 
 
 class AthGpuError(RuntimeError):

@@ -99,7 +99,7 @@ int ath_gpu_compute_corner_e(ath_mesh_t mesh);
 /* Per-kernel timing with HIP events recorded around every launch on the mesh stream
  * (no counterpart in the reference, which only prints zone-cycles/s,
  * main.cpp:634-651).  enable != 0 clears and starts collection; stats return the
- * kernel name, accumulated milliseconds and launch count for kernel id 0..14. */
+ * kernel name, accumulated milliseconds and launch count for kernel id 0..17. */
 int ath_gpu_kernel_timing(ath_mesh_t mesh, int enable);
 int ath_gpu_kernel_stats(ath_mesh_t mesh, int id, char *name, int namelen,
                          double *total_ms, long *count);
