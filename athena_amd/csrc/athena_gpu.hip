@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/athena_gpu.h"
+#include "mlp.hpp"
 #include "physics.hpp"
 
 using namespace athgpu;
@@ -59,6 +60,9 @@ struct MeshK {
   EosK eos;
   double cfl;
   int xcd_remap;              // 1: XCD-contiguous work-group numbering (ATH_XCD_REMAP=1)
+  int c2p_mode;               // SR primitive recovery: C2PMode
+  MlpK mlp;                   // network of C2P_LEARNED
+  unsigned long long *c2p_stats;   // null, or [evals, fallbacks, cells] accumulators
 };
 
 struct Clock {
@@ -645,7 +649,7 @@ __global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double bet
 // (field.cpp:112-180, uniform branch), writing prim (the output register) and bcc.
 // With dt_partial != nullptr the active cells also evaluate Hydro::NewBlockTimeStep
 // (new_blockdt.cpp:45-230) and each work-group writes its minimum.
-template <bool SR>
+template <bool SR, bool LEARNED>
 __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ u,
                                                    double *__restrict__ prim,
                                                    double *__restrict__ bcc,
@@ -658,6 +662,27 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
   const int g = lw / nch;
   const int t = (lw % nch) * WG + threadIdx.x;
   double my_dt = DBL_MAX;
+  double seed = 0.0;
+  C2PStats st{0, 0};
+  if (SR && LEARNED) {
+    // wavefront-collective inference of the gas-pressure seed; dead lanes carry a dummy
+    double cs[NHYDRO] = {1.0, 0.0, 0.0, 0.0, 2.0}, bs[3] = {0.0, 0.0, 0.0};
+    if (t < M.ncell) {
+      const int i = t % M.nc1, j = (t / M.nc1) % M.nc2, k = (t / M.nc1) / M.nc2;
+      const double *B1 = b1 + static_cast<long>(g) * M.nf1;
+      const double *B2 = b2 + static_cast<long>(g) * M.nf2;
+      const double *B3 = b3 + static_cast<long>(g) * M.nf3;
+      bs[IB1] = 0.5 * B1[f1idx(M, k, j, i)] + 0.5 * B1[f1idx(M, k, j, i + 1)];
+      bs[IB2] = 0.5 * B2[f2idx(M, k, j, i)] + 0.5 * B2[f2idx(M, k, j + 1, i)];
+      bs[IB3] = 0.5 * B3[f3idx(M, k, j, i)] + 0.5 * B3[f3idx(M, k + 1, j, i)];
+      const double *ub = u + static_cast<long>(g) * NHYDRO * M.ncell;
+#pragma unroll
+      for (int q = 0; q < NHYDRO; ++q) cs[q] = ub[q * static_cast<long>(M.ncell) + t];
+    }
+    float x[MLP_IN];
+    mlp_features(cs, bs, M.mlp.P, x);
+    seed = mlp_seed(cs, mlp_wave_infer_bf16(M.mlp, x));
+  }
   if (t < M.ncell) {
     const int i = t % M.nc1;
     const int r = t / M.nc1;
@@ -710,7 +735,7 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
       double cin[NHYDRO];
 #pragma unroll
       for (int q = 0; q < NHYDRO; ++q) cin[q] = cons[q];
-      sr_cons2prim_cell(cons, w, bc, M.eos, 0.0, false);
+      sr_cons2prim_cell(cons, w, bc, M.eos, LEARNED ? C2P_LEARNED : M.c2p_mode, seed, &st);
 #pragma unroll
       for (int q = 0; q < NHYDRO; ++q)
         if (cons[q] != cin[q]) ub[q * n + c] = cons[q];
@@ -740,6 +765,20 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
         dt3 /= dmax(0.0, (fabs(wi[IVZ]) + cf));
       }
       my_dt = dmin(dmin(dt1, dt2), dt3);
+    }
+  }
+  if (SR && M.c2p_stats) {   // wavefront sums, one atomic per wavefront
+    unsigned long long ev = st.evals, fb = st.fallback, nc = (t < M.ncell) ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ev += __shfl_down(ev, off, 64);
+      fb += __shfl_down(fb, off, 64);
+      nc += __shfl_down(nc, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&M.c2p_stats[0], ev);
+      atomicAdd(&M.c2p_stats[1], fb);
+      atomicAdd(&M.c2p_stats[2], nc);
     }
   }
   if (dt_partial) {
@@ -859,7 +898,7 @@ __global__ void k_sr_c2p_batch(EosK eos, int n, double *cons, double *prim,
   double c[5], p[5], b[3];
   for (int v = 0; v < 5; ++v) { c[v] = cons[q * 5 + v]; p[v] = prim[q * 5 + v]; }
   for (int v = 0; v < 3; ++v) b[v] = bcc[q * 3 + v];
-  iters[q] = sr_cons2prim_cell(c, p, b, eos, 0.0, false);
+  iters[q] = sr_cons2prim_cell(c, p, b, eos, C2P_REFERENCE, 0.0, nullptr);
   for (int v = 0; v < 5; ++v) { cons[q * 5 + v] = c[v]; prim[q * 5 + v] = p[v]; }
 }
 
@@ -876,6 +915,9 @@ double uniform_gen(double x, double xmin, double xmax) {   // mesh.hpp:535-547
 }
 
 }  // namespace
+
+// Device view of a network created by ath_ml_create (mlp.hip).
+MlpK ath_ml_device_view(ath_mlp_t m);
 
 struct ath_mesh {
   ath_mesh_desc d;
@@ -894,6 +936,7 @@ struct ath_mesh {
   int *nbr_d = nullptr;
   double *coord_d = nullptr;
   Clock *clk = nullptr;
+  unsigned long long *c2p_stats_d = nullptr;   // SR recovery counters, allocated on demand
   std::vector<void *> allocs;
   // optional per-kernel event timing (ath_gpu_kernel_timing)
   bool prof_on = false;
@@ -1056,12 +1099,15 @@ int launch_cons2prim(ath_mesh *m, double *prim_out, bool with_dt) {
   dim3 grid(((M.ncell + WG - 1) / WG) * M.nblocks);
   double *part = with_dt ? m->dtpart : nullptr;
   ProfScope ps(m, KID_CONS2PRIM);
-  if (m->d.relativistic)
-    hipLaunchKernelGGL((k_cons2prim<true>), grid, dim3(WG), 0, m->stream, M, m->u, prim_out,
-                       m->bcc, m->b[0], m->b[1], m->b[2], part);
+  if (m->d.relativistic && M.c2p_mode == C2P_LEARNED)
+    hipLaunchKernelGGL((k_cons2prim<true, true>), grid, dim3(WG), 0, m->stream, M, m->u,
+                       prim_out, m->bcc, m->b[0], m->b[1], m->b[2], part);
+  else if (m->d.relativistic)
+    hipLaunchKernelGGL((k_cons2prim<true, false>), grid, dim3(WG), 0, m->stream, M, m->u,
+                       prim_out, m->bcc, m->b[0], m->b[1], m->b[2], part);
   else
-    hipLaunchKernelGGL((k_cons2prim<false>), grid, dim3(WG), 0, m->stream, M, m->u, prim_out,
-                       m->bcc, m->b[0], m->b[1], m->b[2], part);
+    hipLaunchKernelGGL((k_cons2prim<false, false>), grid, dim3(WG), 0, m->stream, M, m->u,
+                       prim_out, m->bcc, m->b[0], m->b[1], m->b[2], part);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1198,6 +1244,9 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
     const char *env = std::getenv("ATH_XCD_REMAP");
     M.xcd_remap = (env && env[0] == '1') ? 1 : 0;
   }
+  M.c2p_mode = C2P_REFERENCE;
+  M.mlp = MlpK{};
+  M.c2p_stats = nullptr;
 
   // neighbour table and coordinates (coordinates.cpp:129-146, cartesian.cpp:23-72)
   const int nb = M.nblocks;
@@ -1454,6 +1503,34 @@ int ath_gpu_stream(ath_mesh_t m, void **stream) {
 int ath_gpu_synchronize(ath_mesh_t m) {
   if (!m) return fail("null argument");
   HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int ath_gpu_mesh_set_c2p(ath_mesh_t m, ath_mlp_t mlp, int mode) {
+  if (!m) return fail("null argument");
+  if (mode < C2P_REFERENCE || mode > C2P_LEARNED) return fail("mode must be 0, 1 or 2");
+  if (mode != C2P_REFERENCE && !m->d.relativistic)
+    return fail("solver modes other than 0 apply to relativistic meshes only");
+  if (mode == C2P_LEARNED && !mlp) return fail("the learned mode needs an MLP");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  m->M.c2p_mode = mode;
+  m->M.mlp = ath_ml_device_view(mlp);
+  return 0;
+}
+
+int ath_gpu_mesh_c2p_stats(ath_mesh_t m, int enable, unsigned long long *totals) {
+  if (!m) return fail("null argument");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  if (!m->c2p_stats_d) {
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&m->c2p_stats_d), 3 * sizeof(unsigned long long)));
+    m->allocs.push_back(m->c2p_stats_d);
+    HIPCHK(hipMemset(m->c2p_stats_d, 0, 3 * sizeof(unsigned long long)));
+  }
+  if (totals)
+    HIPCHK(hipMemcpy(totals, m->c2p_stats_d, 3 * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+  if (enable > 0) HIPCHK(hipMemset(m->c2p_stats_d, 0, 3 * sizeof(unsigned long long)));
+  if (enable >= 0) m->M.c2p_stats = enable ? m->c2p_stats_d : nullptr;
   return 0;
 }
 

@@ -367,91 +367,224 @@ __device__ __forceinline__ void sr_hlle(const double *wl, const double *wr, doub
 
 //----------------------------------------------------------------------------------------
 // SR primitive recovery (Newman & Hamlin 2014), adiabatic_mhd_sr.cpp:404-522.
-// prim[0..4] out.  Returns true on success; *iters gets the exit iteration count.
-__device__ inline bool nh_solve(double dd, double ee, double mm_sq, double mm1, double mm2,
-                                double mm3, double bb_sq, double bb1, double bb2,
-                                double bb3, double tt, double gamma_adi, double pgas_old,
-                                double *prim, double *p_gamma_lor, double *p_pmag,
-                                int *iters) {
-  const int max_iterations = 15;
-  const double tol = 1.0e-12;
-  const double pgas_uniform_min = 1.0e-12;
-  const double a_min = 1.0e-12;
-  const double v_sq_max = 1.0 - 1.0e-12;
-  const double rr_max = 1.0 - 1.0e-12;
-  const double PI = 3.1415926535897932;
+// NHC holds the per-cell invariants the iteration works on.
+struct NHC {
+  double dd, ee, mm_sq, mm1, mm2, mm3, bb_sq, bb1, bb2, bb3, tt;
+  double d, pgas_min;
+};
 
-  double d = 0.5 * (mm_sq * bb_sq - sq(tt));
+constexpr double NH_TOL = 1.0e-12;
+constexpr double NH_A_MIN = 1.0e-12;
+constexpr double NH_V_SQ_MAX = 1.0 - 1.0e-12;
+constexpr double NH_PI = 3.1415926535897932;
+
+__device__ __forceinline__ void nh_prepare(NHC &c) {
+  const double pgas_uniform_min = 1.0e-12;
+  double d = 0.5 * (c.mm_sq * c.bb_sq - sq(c.tt));   // (NH 5.7)
   d = dmax(d, 0.0);
-  double pgas_min = cbrt(27.0 / 4.0 * d) - ee - 0.5 * bb_sq;
-  pgas_min = dmax(pgas_min, pgas_uniform_min);
+  double pgas_min = cbrt(27.0 / 4.0 * d) - c.ee - 0.5 * c.bb_sq;
+  c.d = d;
+  c.pgas_min = dmax(pgas_min, pgas_uniform_min);
+}
+
+// Primitives from a converged gas pressure (adiabatic_mhd_sr.cpp:486-521).
+__device__ inline bool nh_finish(const NHC &c, double pgas, double gamma_adi, double *prim,
+                                 double *p_gamma_lor, double *p_pmag) {
+  prim[IPR] = pgas;
+  if (!isfinite(pgas)) return false;
+  double a = c.ee + pgas + 0.5 * c.bb_sq;
+  a = dmax(a, NH_A_MIN);
+  double phi = acos(1.0 / a * sqrt(27.0 * c.d / (4.0 * a)));
+  double eee = a / 3.0 - 2.0 / 3.0 * a * cos(2.0 / 3.0 * (phi + NH_PI));
+  double ll = eee - c.bb_sq;
+  double v_sq = (c.mm_sq * sq(ll) + sq(c.tt) * (c.bb_sq + 2.0 * ll)) / sq(ll * (c.bb_sq + ll));
+  v_sq = dmin(dmax(v_sq, 0.0), NH_V_SQ_MAX);
+  double gamma_sq = 1.0 / (1.0 - v_sq);
+  double gamma = sqrt(gamma_sq);
+  prim[IDN] = c.dd / gamma;
+  if (!isfinite(prim[IDN])) return false;
+  double ss = c.tt / ll;
+  double v1 = (c.mm1 + ss * c.bb1) / (ll + c.bb_sq);
+  double v2 = (c.mm2 + ss * c.bb2) / (ll + c.bb_sq);
+  double v3 = (c.mm3 + ss * c.bb3) / (ll + c.bb_sq);
+  prim[IVX] = gamma * v1;
+  prim[IVY] = gamma * v2;
+  prim[IVZ] = gamma * v3;
+  if (!isfinite(prim[IVX]) || !isfinite(prim[IVY]) || !isfinite(prim[IVZ])) return false;
+  *p_gamma_lor = gamma;
+  *p_pmag = 0.5 * (c.bb_sq / gamma_sq + sq(ss));
+  return true;
+}
+
+// Intermediates of one evaluation of the fixed-point map, enough to finish the
+// primitives at the evaluated pressure without solving the cubic again.
+struct NHI {
+  double ll, v_sq, gamma_sq, gamma;
+};
+
+// Primitives at pgas from the intermediates of nh_g(pgas): the tail of nh_finish.
+__device__ inline bool nh_finish_i(const NHC &c, double pgas, const NHI &im, double *prim,
+                                   double *p_gamma_lor, double *p_pmag) {
+  prim[IPR] = pgas;
+  if (!isfinite(pgas)) return false;
+  const double ll = im.ll, gamma = im.gamma;
+  prim[IDN] = c.dd / gamma;
+  if (!isfinite(prim[IDN])) return false;
+  double ss = c.tt / ll;
+  double v1 = (c.mm1 + ss * c.bb1) / (ll + c.bb_sq);
+  double v2 = (c.mm2 + ss * c.bb2) / (ll + c.bb_sq);
+  double v3 = (c.mm3 + ss * c.bb3) / (ll + c.bb_sq);
+  prim[IVX] = gamma * v1;
+  prim[IVY] = gamma * v2;
+  prim[IVZ] = gamma * v3;
+  if (!isfinite(prim[IVX]) || !isfinite(prim[IVY]) || !isfinite(prim[IVZ])) return false;
+  *p_gamma_lor = gamma;
+  *p_pmag = 0.5 * (c.bb_sq / im.gamma_sq + sq(ss));
+  return true;
+}
+
+// The fixed-point map of the NH iteration, pgas -> pgas' (NH 5.7, 5.10, 5.11, 5.5, 5.2,
+// 3.1, 5.1, 4.5, 4.1), with its derivative when dg != nullptr and its intermediates when
+// im != nullptr.
+__device__ inline double nh_g(const NHC &c, double p, double gamma_adi, double *dg,
+                              NHI *im = nullptr) {
+  double a = c.ee + p + 0.5 * c.bb_sq;
+  const bool aclamp = !(a >= NH_A_MIN);
+  a = dmax(a, NH_A_MIN);
+  const double s = 1.0 / a * sqrt(27.0 * c.d / (4.0 * a));
+  const double phi = acos(s);
+  const double th = 2.0 / 3.0 * (phi + NH_PI);
+  const double cth = cos(th);
+  const double eee = a / 3.0 - 2.0 / 3.0 * a * cth;
+  const double ll = eee - c.bb_sq;
+  const double N = c.mm_sq * sq(ll) + sq(c.tt) * (c.bb_sq + 2.0 * ll);
+  const double Q = sq(ll * (c.bb_sq + ll));
+  double v_sq = N / Q;
+  const bool vclamp = !(v_sq >= 0.0 && v_sq <= NH_V_SQ_MAX);
+  v_sq = dmin(dmax(v_sq, 0.0), NH_V_SQ_MAX);
+  const double gamma_sq = 1.0 / (1.0 - v_sq);
+  const double gamma = sqrt(gamma_sq);
+  if (im) *im = NHI{ll, v_sq, gamma_sq, gamma};
+  const double wgas = ll / gamma_sq;
+  const double rho = c.dd / gamma;
+  const double k = (gamma_adi - 1.0) / gamma_adi;
+  const double g = k * (wgas - rho);
+  if (dg) {
+    double deee = 0.0;
+    if (!aclamp) {
+      const double dsda = -1.5 * s / a;
+      const double dphida = -dsda / sqrt(dmax(1.0 - s * s, 1.0e-300));
+      deee = 1.0 / 3.0 - 2.0 / 3.0 * cth + 2.0 / 3.0 * a * sin(th) * (2.0 / 3.0) * dphida;
+    }
+    double dvsq = 0.0;
+    if (!vclamp) {
+      const double dN = 2.0 * c.mm_sq * ll + 2.0 * sq(c.tt);
+      const double dQ = 2.0 * ll * (c.bb_sq + ll) * (c.bb_sq + 2.0 * ll);
+      dvsq = (dN * Q - N * dQ) / (Q * Q);
+    }
+    const double dW = (1.0 - v_sq) - ll * dvsq;
+    const double drho = -c.dd / (2.0 * sqrt(1.0 - v_sq)) * dvsq;
+    *dg = k * (dW - drho) * deee;
+  }
+  return g;
+}
+
+// The reference iteration: fixed point with Aitken extrapolation every third iterate.
+// Same operation order as adiabatic_mhd_sr.cpp:437-483.
+__device__ inline bool nh_solve_c(const NHC &c, double gamma_adi, double pgas_old,
+                                  double *prim, double *p_gamma_lor, double *p_pmag,
+                                  int *iters, int *evals) {
+  const int max_iterations = 15;
+  const double rr_max = 1.0 - 1.0e-12;
+  const double pgas_min = c.pgas_min;
   double p0 = dmax(pgas_old, pgas_min), p1 = 0.0, p2 = 0.0;   // pgas[0..2]
   int n;
   for (n = 0; n < max_iterations; ++n) {
     const int r = n % 3;
     if (r != 2) {
       double pc = (r == 0) ? p0 : p1;
-      double a = ee + pc + 0.5 * bb_sq;
-      a = dmax(a, a_min);
-      double phi = acos(1.0 / a * sqrt(27.0 * d / (4.0 * a)));
-      double eee = a / 3.0 - 2.0 / 3.0 * a * cos(2.0 / 3.0 * (phi + PI));
-      double ll = eee - bb_sq;
-      double v_sq = (mm_sq * sq(ll) + sq(tt) * (bb_sq + 2.0 * ll)) / sq(ll * (bb_sq + ll));
-      v_sq = dmin(dmax(v_sq, 0.0), v_sq_max);
-      double gamma_sq = 1.0 / (1.0 - v_sq);
-      double gamma = sqrt(gamma_sq);
-      double wgas = ll / gamma_sq;
-      double rho = dd / gamma;
-      double pn = (gamma_adi - 1.0) / gamma_adi * (wgas - rho);
-      pn = dmax(pn, pgas_min);
+      double pn = dmax(nh_g(c, pc, gamma_adi, nullptr), pgas_min);
+      *evals += 1;
       if (r == 0) p1 = pn; else p2 = pn;
-      if (pn > pgas_min && fabs(pn - pc) < tol) break;
+      if (pn > pgas_min && fabs(pn - pc) < NH_TOL) break;
     } else {
       double rr = (p2 - p1) / (p1 - p0);
       if (!isfinite(rr) || fabs(rr) > rr_max) continue;
       p0 = p1 + (p2 - p1) / (1.0 - rr);
       p0 = dmax(p0, pgas_min);
-      if (p0 > pgas_min && fabs(p0 - p2) < tol) break;
+      if (p0 > pgas_min && fabs(p0 - p2) < NH_TOL) break;
     }
   }
   *iters = n;
   if (n == max_iterations) return false;
   const int rn = (n + 1) % 3;
   double pgas = (rn == 0) ? p0 : ((rn == 1) ? p1 : p2);
-  prim[IPR] = pgas;
-  if (!isfinite(pgas)) return false;
-  double a = ee + pgas + 0.5 * bb_sq;
-  a = dmax(a, a_min);
-  double phi = acos(1.0 / a * sqrt(27.0 * d / (4.0 * a)));
-  double eee = a / 3.0 - 2.0 / 3.0 * a * cos(2.0 / 3.0 * (phi + PI));
-  double ll = eee - bb_sq;
-  double v_sq = (mm_sq * sq(ll) + sq(tt) * (bb_sq + 2.0 * ll)) / sq(ll * (bb_sq + ll));
-  v_sq = dmin(dmax(v_sq, 0.0), v_sq_max);
-  double gamma_sq = 1.0 / (1.0 - v_sq);
-  double gamma = sqrt(gamma_sq);
-  prim[IDN] = dd / gamma;
-  if (!isfinite(prim[IDN])) return false;
-  double ss = tt / ll;
-  double v1 = (mm1 + ss * bb1) / (ll + bb_sq);
-  double v2 = (mm2 + ss * bb2) / (ll + bb_sq);
-  double v3 = (mm3 + ss * bb3) / (ll + bb_sq);
-  prim[IVX] = gamma * v1;
-  prim[IVY] = gamma * v2;
-  prim[IVZ] = gamma * v3;
-  if (!isfinite(prim[IVX]) || !isfinite(prim[IVY]) || !isfinite(prim[IVZ])) return false;
-  *p_gamma_lor = gamma;
-  *p_pmag = 0.5 * (bb_sq / gamma_sq + sq(ss));
-  return true;
+  return nh_finish(c, pgas, gamma_adi, prim, p_gamma_lor, p_pmag);
 }
+
+// Newton iteration on the same fixed-point residual r(p) = p - g(p) with the analytic
+// derivative of g; no counterpart in the reference.  Converges at the first iterate p
+// with |g(p) - p| < 1e-12, the reference's tolerance on successive fixed-point iterates,
+// and returns the primitives at that p from the intermediates of its evaluation (one
+// cubic solve fewer than finishing at g(p)).  Takes a fixed-point step where 1 - g' is
+// not positive.
+__device__ inline bool newton_solve_c(const NHC &c, double gamma_adi, double seed,
+                                      double *prim, double *p_gamma_lor, double *p_pmag,
+                                      int *iters, int *evals) {
+  const int max_iterations = 15;
+  const double pgas_min = c.pgas_min;
+  double p = dmax(seed, pgas_min);
+  NHI im{0.0, 0.0, 1.0, 1.0};
+  int n;
+  bool conv = false;
+  for (n = 0; n < max_iterations; ++n) {
+    double dg;
+    const double g = dmax(nh_g(c, p, gamma_adi, &dg, &im), pgas_min);
+    *evals += 1;
+    if (g > pgas_min && fabs(g - p) < NH_TOL) { conv = true; break; }
+    const double den = 1.0 - dg;
+    double pn = (den > 0.0 && isfinite(den)) ? p - (p - g) / den : g;
+    if (!isfinite(pn)) pn = g;
+    p = dmax(pn, pgas_min);
+  }
+  *iters = n + (conv ? 1 : 0);
+  if (!conv) return false;
+  return nh_finish_i(c, p, im, prim, p_gamma_lor, p_pmag);
+}
+
+__device__ inline bool nh_solve(double dd, double ee, double mm_sq, double mm1, double mm2,
+                                double mm3, double bb_sq, double bb1, double bb2,
+                                double bb3, double tt, double gamma_adi, double pgas_old,
+                                double *prim, double *p_gamma_lor, double *p_pmag,
+                                int *iters, int *evals) {
+  NHC c{dd, ee, mm_sq, mm1, mm2, mm3, bb_sq, bb1, bb2, bb3, tt, 0.0, 0.0};
+  nh_prepare(c);
+  return nh_solve_c(c, gamma_adi, pgas_old, prim, p_gamma_lor, p_pmag, iters, evals);
+}
+
+// Solver selection for the first solve of sr_cons2prim_cell.
+enum C2PMode {
+  C2P_REFERENCE = 0,   // NH seeded from the output register (the reference)
+  C2P_NEWTON = 1,      // Newton seeded from the output register
+  C2P_LEARNED = 2      // Newton seeded by the learned guess; reference NH on failure
+};
+
+// Cost counters of one cell's recovery: evaluations of the fixed-point map over all
+// solves of the cell, and whether the learned-seed Newton solve fell back to NH.
+struct C2PStats {
+  int evals;
+  int fallback;
+};
 
 // One cell of the SR ConservedToPrimitive wrapper (adiabatic_mhd_sr.cpp:72-272):
 // floors, NH solve seeded from the old content of the output array, fallbacks,
 // gamma_max ceiling and the conserved rewrite when anything was fixed.
 // cons[5] in/out, prim[5] seed in / result out.  Returns first-solve NH count, or -1
-// when the first solve failed.
+// when the first solve failed.  mode selects the first solver (C2PMode); learned_seed is
+// the gas-pressure seed of C2P_LEARNED; st (may be null) receives the cost counters.
 __device__ inline int sr_cons2prim_cell(double *cons, double *prim, const double *bcc,
-                                        const EosK &eos, double pgas_seed_override,
-                                        bool use_override) {
+                                        const EosK &eos, int mode, double learned_seed,
+                                        C2PStats *st) {
   const double mm_sq_ee_sq_max = 1.0 - 1.0e-12;
   const double gamma_adi = eos.gamma;
   double dd = cons[IDN];
@@ -477,12 +610,23 @@ __device__ inline int sr_cons2prim_cell(double *cons, double *prim, const double
   double rho_old = dmax(prim[IDN], dfl);
   double pgas_old = dmax(prim[IPR], pfl);
   double u1_old = prim[IVX], u2_old = prim[IVY], u3_old = prim[IVZ];
-  double seed = use_override ? pgas_seed_override : pgas_old;
-
+  NHC nc{dd, ee, mm0, mm1, mm2, mm3, bb0, bb1, bb2, bb3, tt, 0.0, 0.0};
+  nh_prepare(nc);
   double gamma = 0.0, pmag = 0.0;
-  int iters = 0;
-  bool success = nh_solve(dd, ee, mm0, mm1, mm2, mm3, bb0, bb1, bb2, bb3, tt, gamma_adi,
-                          seed, prim, &gamma, &pmag, &iters);
+  int iters = 0, evals = 0, fb = 0;
+  bool success;
+  if (mode == C2P_REFERENCE) {
+    success = nh_solve_c(nc, gamma_adi, pgas_old, prim, &gamma, &pmag, &iters, &evals);
+  } else {
+    const double seed = (mode == C2P_LEARNED) ? learned_seed : pgas_old;
+    success = newton_solve_c(nc, gamma_adi, seed, prim, &gamma, &pmag, &iters, &evals);
+    if (!success && mode == C2P_LEARNED) {   // the plain solver is the fallback
+      fb = 1;
+      int it2 = 0;
+      success = nh_solve_c(nc, gamma_adi, pgas_old, prim, &gamma, &pmag, &it2, &evals);
+      iters += it2;
+    }
+  }
   int first = success ? iters : -1;
   if (!success) {
     prim[IDN] = rho_old; prim[IPR] = pgas_old;
@@ -498,7 +642,7 @@ __device__ inline int sr_cons2prim_cell(double *cons, double *prim, const double
     dd += rho_add * gamma;
     ee += wgas_add * sq(gamma) + pgas_add;
     success = nh_solve(dd, ee, mm0, mm1, mm2, mm3, bb0, bb1, bb2, bb3, tt, gamma_adi,
-                       pgas_old, prim, &gamma, &pmag, &iters);
+                       pgas_old, prim, &gamma, &pmag, &iters, &evals);
     if (!success) {
       prim[IDN] = rho_old; prim[IPR] = pgas_old;
       prim[IVX] = u1_old; prim[IVY] = u2_old; prim[IVZ] = u3_old;
@@ -545,6 +689,7 @@ __device__ inline int sr_cons2prim_cell(double *cons, double *prim, const double
     cons[IM2] = wtot * gamma * u2 - b0 * b2;
     cons[IM3] = wtot * gamma * u3 - b0 * b3;
   }
+  if (st) { st->evals = evals; st->fallback = fb; }
   return first;
 }
 

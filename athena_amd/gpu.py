@@ -26,7 +26,11 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_get_clock", "ath_gpu_set_clock", "ath_gpu_calculate_fluxes",
            "ath_gpu_compute_corner_e", "ath_gpu_stream", "ath_gpu_synchronize",
            "ath_gpu_riemann_batch", "ath_gpu_sr_con2prim_batch", "ath_gpu_last_error",
-           "ath_gpu_kernel_timing", "ath_gpu_kernel_stats"]
+           "ath_gpu_kernel_timing", "ath_gpu_kernel_stats",
+           "ath_ml_param_count", "ath_ml_create", "ath_ml_destroy", "ath_ml_set_params",
+           "ath_ml_get_params", "ath_ml_grad", "ath_ml_train", "ath_ml_infer",
+           "ath_ml_features", "ath_ml_last_error", "ath_gpu_sr_con2prim_modes",
+           "ath_gpu_sr_con2prim_device", "ath_gpu_mesh_set_c2p", "ath_gpu_mesh_c2p_stats"]
 NKERNELS = 18
 
 
@@ -82,6 +86,24 @@ def load():
     L.ath_gpu_riemann_batch.argtypes = [I, ctypes.c_double, I, D, D, D, D]
     L.ath_gpu_sr_con2prim_batch.argtypes = [I] + [ctypes.c_double] * 6 + \
         [D, D, D, ctypes.POINTER(I)]
+    F = ctypes.POINTER(ctypes.c_float)
+    IP = ctypes.POINTER(I)
+    L.ath_ml_last_error.restype = ctypes.c_char_p
+    L.ath_ml_param_count.argtypes = []
+    L.ath_ml_create.argtypes = [ctypes.POINTER(P)]
+    L.ath_ml_destroy.argtypes = [P]
+    L.ath_ml_set_params.argtypes = [P, F]
+    L.ath_ml_get_params.argtypes = [P, F]
+    L.ath_ml_grad.argtypes = [P, I, F, F, F, F]
+    L.ath_ml_train.argtypes = [P, I, F, F, I, I, IP, ctypes.c_float, F]
+    L.ath_ml_infer.argtypes = [P, I, F, F, I]
+    L.ath_ml_features.argtypes = [I, D, D, F]
+    L.ath_gpu_sr_con2prim_modes.argtypes = [P, I, I] + [ctypes.c_double] * 6 + \
+        [D, D, D, IP, IP, IP]
+    L.ath_gpu_sr_con2prim_device.argtypes = [P, I, I] + [ctypes.c_double] * 6 + \
+        [P, P, P, P, P, P, P]
+    L.ath_gpu_mesh_set_c2p.argtypes = [P, P, I]
+    L.ath_gpu_mesh_c2p_stats.argtypes = [P, I, ctypes.POINTER(ctypes.c_ulonglong)]
     _lib = L
     return L
 
@@ -91,8 +113,21 @@ def _check(rc):
         raise AthGpuError(load().ath_gpu_last_error().decode())
 
 
+def _check_ml(rc):
+    if rc != 0:
+        raise AthGpuError(load().ath_ml_last_error().decode())
+
+
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
 def desc_of(m: MeshDesc) -> MeshDescC:
@@ -190,6 +225,20 @@ class GpuMesh:
     def block(self, gid):
         return {k: self.get(gid, k) for k in ("u", "w", "bcc", "b1", "b2", "b3")}
 
+    def set_c2p(self, mode, mlp=None):
+        """First solver of the SR ConservedToPrimitive: 0 reference iteration, 1 Newton
+        from the previous pressure, 2 Newton from the learned guess of `mlp`."""
+        _check(load().ath_gpu_mesh_set_c2p(self.h, mlp.h if mlp is not None else None,
+                                           int(mode)))
+        self._mlp = mlp   # the mesh keeps a device reference to the network
+
+    def c2p_stats(self, enable=-1):
+        """(evaluations, fallbacks, cells) counted before this call; enable 1 clears and
+        starts counting, 0 stops, -1 only reads."""
+        t = (ctypes.c_ulonglong * 3)()
+        _check(load().ath_gpu_mesh_c2p_stats(self.h, int(enable), t))
+        return int(t[0]), int(t[1]), int(t[2])
+
     def kernel_timing(self, enable=True):
         _check(load().ath_gpu_kernel_timing(self.h, int(enable)))
 
@@ -227,3 +276,104 @@ def sr_con2prim_batch(cons, prim_seed, bcc, gamma, dfloor, pfloor, sigma_max=0.0
         cons.shape[0], gamma, dfloor, pfloor, sigma_max, beta_min, gamma_max, _dp(cons),
         _dp(prim), _dp(bcc), iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
     return cons, prim, iters
+
+
+C2P_MODES = {"reference": 0, "newton": 1, "learned": 2}
+
+
+class Mlp:
+    """Device copy of the learned-guess network (ath_ml_* entry points)."""
+
+    def __init__(self, params=None):
+        L = load()
+        self.np = L.ath_ml_param_count()
+        h = ctypes.c_void_p()
+        _check_ml(L.ath_ml_create(ctypes.byref(h)))
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().ath_ml_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != self.np:
+            raise AthGpuError("parameter vector has %d floats, need %d" % (p.size, self.np))
+        _check_ml(load().ath_ml_set_params(self.h, _fp(p)))
+
+    def params(self):
+        p = np.empty(self.np, dtype=np.float32)
+        _check_ml(load().ath_ml_get_params(self.h, _fp(p)))
+        return p
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(NW, dtype=np.float32)
+        loss = np.empty(1, dtype=np.float32)
+        _check_ml(load().ath_ml_grad(self.h, t.size, _fp(x), _fp(t), _fp(g), _fp(loss)))
+        return g, float(loss[0])
+
+    def train(self, x, t, order, batch, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        steps = order.size // batch
+        hist = np.empty(steps, dtype=np.float32)
+        _check_ml(load().ath_ml_train(self.h, t.size, _fp(x), _fp(t), steps, batch,
+                                      _ip(order), lr, _fp(hist)))
+        return hist
+
+    def infer(self, x, bf16=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        _check_ml(load().ath_ml_infer(self.h, x.shape[0], _fp(x), _fp(y), int(bool(bf16))))
+        return y
+
+
+NW = 4673          # trained parameters W1..b3 (the rest is normalisation)
+
+
+def ml_features(cons, bcc):
+    """The solver's raw inputs (n,6) float32 of cons (n,5) and bcc (n,3)."""
+    cons = np.ascontiguousarray(cons, dtype=np.float64)
+    bcc = np.ascontiguousarray(bcc, dtype=np.float64)
+    x = np.empty((cons.shape[0], 6), dtype=np.float32)
+    _check_ml(load().ath_ml_features(cons.shape[0], _dp(cons), _dp(bcc), _fp(x)))
+    return x
+
+
+def sr_con2prim_modes(mode, cons, prim_seed, bcc, gamma, dfloor, pfloor, mlp=None,
+                      sigma_max=0.0, beta_min=0.0, gamma_max=1000.0):
+    """Batched SR recovery with the first solver `mode` (0/1/2 or a C2P_MODES key).
+    Returns cons, prim, iters, evals, fallback."""
+    mode = C2P_MODES.get(mode, mode)
+    cons = np.array(cons, dtype=np.float64, order="C")
+    prim = np.array(prim_seed, dtype=np.float64, order="C")
+    bcc = np.ascontiguousarray(bcc, dtype=np.float64)
+    n = cons.shape[0]
+    it, ev, fb = (np.empty(n, dtype=np.int32) for _ in range(3))
+    _check_ml(load().ath_gpu_sr_con2prim_modes(
+        mlp.h if mlp is not None else None, int(mode), n, gamma, dfloor, pfloor, sigma_max,
+        beta_min, gamma_max, _dp(cons), _dp(prim), _dp(bcc), _ip(it), _ip(ev), _ip(fb)))
+    return cons, prim, it, ev, fb
+
+
+def sr_con2prim_device(mode, n, cons, prim, bcc, gamma, dfloor, pfloor, mlp=None,
+                       iters=None, evals=None, fallback=None, stream=None, sigma_max=0.0,
+                       beta_min=0.0, gamma_max=1000.0):
+    """Asynchronous form on device addresses (ints, e.g. a torch tensor's data_ptr()):
+    cons (n,5), prim (n,5), bcc (n,3) float64; iters/evals/fallback int32 or None."""
+    mode = C2P_MODES.get(mode, mode)
+    _check_ml(load().ath_gpu_sr_con2prim_device(
+        mlp.h if mlp is not None else None, int(mode), int(n), gamma, dfloor, pfloor,
+        sigma_max, beta_min, gamma_max, cons, prim, bcc, iters, evals, fallback, stream))

@@ -123,6 +123,65 @@ int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
 
 const char *ath_gpu_last_error(void);
 
+/* ---- Learned initial guess for SR primitive recovery --------------------------------
+ * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds
+ * the Newman-Hamlin iteration with the previous gas pressure.  Here a 6 -> 64 -> 64 -> 1
+ * tanh MLP predicts log10(p_gas / (E - D)) from six rotation- and scale-invariant inputs
+ * (E/D, |M|/E, B^2/E, sqrt(E^2-M^2)/D, the M-B angle); the prediction seeds an FP64 Newton
+ * iteration on the reference's fixed-point residual, converged to the reference's 1e-12,
+ * and the reference iteration is the fallback.  Parameter vector (floats):
+ *   W1[64][6] b1[64] W2[64][64] b2[64] W3[64] b3 | x_mean[6] x_scale[6] y_mean y_scale
+ * Training is FP32 on the GPU (forward/backward/Adam kernels), inference BF16 MFMA.
+ * Errors of the ath_ml_* calls are reported by ath_ml_last_error(). */
+typedef struct ath_mlp *ath_mlp_t;
+
+int ath_ml_param_count(void);                         /* floats in the parameter vector */
+int ath_ml_create(ath_mlp_t *out);                    /* all parameters zero */
+int ath_ml_destroy(ath_mlp_t mlp);
+int ath_ml_set_params(ath_mlp_t mlp, const float *params);   /* resets Adam moments */
+int ath_ml_get_params(ath_mlp_t mlp, float *params);
+/* Gradient of mean((y(x_b) - t_b)^2) over n normalised samples x (n,6), t (n) with
+ * respect to the first 4673 parameters (W1..b3), and the loss.  Host arrays. */
+int ath_ml_grad(ath_mlp_t mlp, int n, const float *x, const float *t, float *grad,
+                float *loss);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8, learning rate lr) on minibatches of
+ * `batch` samples; order (steps*batch) lists the sample of each minibatch slot;
+ * loss_hist (steps, may be null) receives the minibatch losses.  Host arrays. */
+int ath_ml_train(ath_mlp_t mlp, int n, const float *x, const float *t, int steps,
+                 int batch, const int *order, float lr, float *loss_hist);
+/* De-normalised outputs y (n) for normalised features x (n,6): bf16 != 0 runs the
+ * MFMA path used by the solver, 0 the FP32 path.  Host arrays. */
+int ath_ml_infer(ath_mlp_t mlp, int n, const float *x, float *y, int bf16);
+/* The solver's raw (un-normalised) features x (n,6) of cons (n,5), bcc (n,3). */
+int ath_ml_features(int n, const double *cons, const double *bcc, float *x);
+const char *ath_ml_last_error(void);
+
+/* Batched SR primitive recovery with a choice of first solver: mode 0 the reference
+ * iteration (as ath_gpu_sr_con2prim_batch), 1 Newton from the prim seed, 2 Newton from
+ * the learned seed with the reference iteration as fallback (mlp required).  iters
+ * (n): as ath_gpu_sr_con2prim_batch; evals (n): evaluations of the fixed-point map;
+ * fallback (n): 1 where the learned-seed solve fell back.  Output pointers may be null.
+ * _modes takes host arrays and is synchronous; _device takes device arrays and is
+ * asynchronous on `stream` (a hipStream_t, null for the default stream). */
+int ath_gpu_sr_con2prim_modes(ath_mlp_t mlp, int mode, int n, double gamma, double dfloor,
+                              double pfloor, double sigma_max, double beta_min,
+                              double gamma_max, double *cons, double *prim,
+                              const double *bcc, int *iters, int *evals, int *fallback);
+int ath_gpu_sr_con2prim_device(ath_mlp_t mlp, int mode, int n, double gamma,
+                               double dfloor, double pfloor, double sigma_max,
+                               double beta_min, double gamma_max, double *cons,
+                               double *prim, const double *bcc, int *iters, int *evals,
+                               int *fallback, void *stream);
+
+/* Selects the first solver of a relativistic mesh's ConservedToPrimitive (modes as
+ * above; the mesh keeps a reference to mlp, which must outlive it or be replaced). */
+int ath_gpu_mesh_set_c2p(ath_mesh_t mesh, ath_mlp_t mlp, int mode);
+/* Recovery counters summed over every cell solve of the mesh: totals[0] evaluations
+ * of the fixed-point map, [1] learned-seed fallbacks, [2] cells.  enable 1 clears and
+ * starts counting, 0 stops, -1 only reads; totals (may be null) receives the counts
+ * before the call's change. */
+int ath_gpu_mesh_c2p_stats(ath_mesh_t mesh, int enable, unsigned long long *totals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,10 @@ def lib():
         L.ora_sr_con2prim.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + \
             [D, D, D, ctypes.POINTER(ctypes.c_int)]
         L.ora_sr_prim2cons.argtypes = [ctypes.c_int, ctypes.c_double, D, D, D]
+        L.ora_mlp_nw.restype = ctypes.c_int
+        L.ora_mlp_forward.argtypes = [D, ctypes.c_int, D, D]
+        L.ora_mlp_grad.argtypes = [D, ctypes.c_int, D, D, D, D]
+        L.ora_mlp_features.argtypes = [ctypes.c_int, D, D, D]
         _lib = L
     return _lib
 
@@ -163,3 +167,31 @@ def sr_prim2cons(prim, bcc, gamma):
     cons = np.empty_like(prim)
     lib().ora_sr_prim2cons(prim.shape[0], gamma, _dp(prim), _dp(bcc), _dp(cons))
     return cons
+
+
+def mlp_forward(params, x):
+    """De-normalised network outputs for normalised inputs x (n,6), in double."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(x.shape[0])
+    lib().ora_mlp_forward(_dp(P), x.shape[0], _dp(x), _dp(y))
+    return y
+
+
+def mlp_grad(params, x, t):
+    """(gradient (4673,), loss) of the mean squared error in normalised units."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.empty(lib().ora_mlp_nw())
+    loss = np.empty(1)
+    lib().ora_mlp_grad(_dp(P), t.size, _dp(x), _dp(t), _dp(g), _dp(loss))
+    return g, float(loss[0])
+
+
+def mlp_features(cons, bcc):
+    cons = np.ascontiguousarray(cons, dtype=np.float64)
+    bcc = np.ascontiguousarray(bcc, dtype=np.float64)
+    x = np.empty((cons.shape[0], 6))
+    lib().ora_mlp_features(cons.shape[0], _dp(cons), _dp(bcc), _dp(x))
+    return x

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "athena_gpu.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(ath_gpu_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(ath_(?:gpu|ml)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
