@@ -112,6 +112,35 @@ __device__ __forceinline__ int xcd_logical_wg(int remap) {
   return x * q + (x < r ? x : r) + l;
 }
 
+// Work decomposition of one launch over a box of ni x nj x nk positions per MeshBlock:
+// nch work-groups per block (nwg = nblocks * nch), each either a linear chunk of WG
+// positions, i fastest (tx == 0), or a tx x ty x tz tile (tx*ty*tz == WG).  Only the
+// thread-to-position mapping changes; every position is computed by the same arithmetic.
+struct TileK {
+  int tx, ty, tz;
+};
+
+__device__ __forceinline__ bool map_work(int lw, int nwg, int nblocks, int ni, int nj, int nk,
+                                         TileK T, int &g, int &i, int &j, int &k) {
+  const int nch = nwg / nblocks;
+  g = lw / nch;
+  const int c = lw - g * nch;
+  if (T.tx == 0) {
+    const int t = c * WG + threadIdx.x;
+    i = t % ni;
+    const int r = t / ni;
+    j = r % nj;
+    k = r / nj;
+    return t < ni * nj * nk;
+  }
+  const int nti = (ni + T.tx - 1) / T.tx, ntj = (nj + T.ty - 1) / T.ty;
+  const int ti = c % nti, r = c / nti;
+  i = ti * T.tx + static_cast<int>(threadIdx.x) % T.tx;
+  j = (r % ntj) * T.ty + (static_cast<int>(threadIdx.x) / T.tx) % T.ty;
+  k = (r / ntj) * T.tz + static_cast<int>(threadIdx.x) / (T.tx * T.ty);
+  return i < ni && j < nj && k < nk;
+}
+
 //========================================================================================
 // Reconstruction of one variable at one interface along DIR.
 //   q[] holds the variable at cells c-3 .. c+2 (q[3] is the cell on the right of the
@@ -307,26 +336,29 @@ __global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__
 // its low sides that lie in the reference's loop ranges (calculate_fluxes.cpp:68-76,
 // 176-182, 299-302).
 template <int ORDER, int RS>
-__global__ void __launch_bounds__(WG) k_flux3(MeshK M, const double *__restrict__ w,
-                                               const double *__restrict__ bcc, FaceOut o1,
-                                               FaceOut o2, FaceOut o3,
-                                               const Clock *__restrict__ clk) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  const int ni = M.nx1 + 3, nj = M.nx2 + 3, nk = M.nx3 + 3;
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  const int i = M.is - 1 + t % ni;
-  const int r = t / ni;
-  const int j = M.js - 1 + r % nj;
-  const int k = M.ks - 1 + r / nj;
+__device__ __forceinline__ void flux3_body(const MeshK &M, const double *__restrict__ w,
+                                           const double *__restrict__ bcc, const FaceOut &o1,
+                                           const FaceOut &o2, const FaceOut &o3, double dt,
+                                           int lw, int nwg, TileK T) {
+  int g, i, j, k;
+  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 3, M.nx2 + 3, M.nx3 + 3, T, g, i, j, k)) return;
+  i += M.is - 1;
+  j += M.js - 1;
+  k += M.ks - 1;
   const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
   const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
-  const double dt = clk->dt;
   if (i >= M.is) flux_face<0, ORDER, RS>(M, g, k, j, i, wb, bb, o1, dt);
   if (j >= M.js) flux_face<1, ORDER, RS>(M, g, k, j, i, wb, bb, o2, dt);
   if (k >= M.ks) flux_face<2, ORDER, RS>(M, g, k, j, i, wb, bb, o3, dt);
+}
+
+template <int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_flux3(MeshK M, const double *__restrict__ w,
+                                               const double *__restrict__ bcc, FaceOut o1,
+                                               FaceOut o2, FaceOut o3,
+                                               const Clock *__restrict__ clk, TileK T) {
+  flux3_body<ORDER, RS>(M, w, bcc, o1, o2, o3, clk->dt, xcd_logical_wg(M.xcd_remap),
+                        gridDim.x, T);
 }
 
 //========================================================================================
@@ -351,25 +383,30 @@ __device__ __forceinline__ double cc_e(const MeshK &M, const double *wb, const d
   }
 }
 
+// Inputs and outputs of the corner-EMF kernel.
+struct CornerArgs {
+  const double *w, *bcc;
+  const double *e2x1, *e3x1, *e1x2, *e3x2, *e1x3, *e2x3;
+  const double *wx1, *wx2, *wx3;
+  double *e1, *e2, *e3;
+};
+
 template <bool SR>
-__global__ void __launch_bounds__(WG) k_corner(
-    MeshK M, const double *__restrict__ w, const double *__restrict__ bcc,
-    const double *__restrict__ e2x1, const double *__restrict__ e3x1,
-    const double *__restrict__ e1x2, const double *__restrict__ e3x2,
-    const double *__restrict__ e1x3, const double *__restrict__ e2x3,
-    const double *__restrict__ wx1, const double *__restrict__ wx2,
-    const double *__restrict__ wx3, double *__restrict__ e1, double *__restrict__ e2,
-    double *__restrict__ e3) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  const int ni = M.nx1 + 1, nj = M.nx2 + 1, nk = M.nx3 + 1;
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  const int i = M.is + t % ni;
-  const int r = t / ni;
-  const int j = M.js + r % nj;
-  const int k = M.ks + r / nj;
+__device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca, int lw,
+                                            int nwg, TileK T) {
+  int g, i, j, k;
+  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 1, M.nx2 + 1, M.nx3 + 1, T, g, i, j, k)) return;
+  i += M.is;
+  j += M.js;
+  k += M.ks;
+  const double *__restrict__ w = ca.w;
+  const double *__restrict__ bcc = ca.bcc;
+  const double *__restrict__ e2x1 = ca.e2x1, *__restrict__ e3x1 = ca.e3x1;
+  const double *__restrict__ e1x2 = ca.e1x2, *__restrict__ e3x2 = ca.e3x2;
+  const double *__restrict__ e1x3 = ca.e1x3, *__restrict__ e2x3 = ca.e2x3;
+  const double *__restrict__ wx1 = ca.wx1, *__restrict__ wx2 = ca.wx2;
+  const double *__restrict__ wx3 = ca.wx3;
+  double *__restrict__ e1 = ca.e1, *__restrict__ e2 = ca.e2, *__restrict__ e3 = ca.e3;
   const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
   const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
   const long o1 = static_cast<long>(g) * M.nf1, o2 = static_cast<long>(g) * M.nf2;
@@ -433,6 +470,11 @@ __global__ void __launch_bounds__(WG) k_corner(
 #undef E1X3
 #undef E2X3
 #undef CC
+}
+
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_corner(MeshK M, CornerArgs ca, TileK T) {
+  corner_body<SR>(M, ca, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
 }
 
 //========================================================================================
@@ -618,20 +660,15 @@ struct UpdateArgs {
   const double *e1, *e2, *e3;
 };
 
-__global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double beta,
-                                                int first_stage,
-                                                const Clock *__restrict__ clk) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  const int ni = M.nx1 + 1, nj = M.nx2 + 1, nk = M.nx3 + 1;
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= ni * nj * nk) return;
-  const int i = M.is + t % ni;
-  const int r = t / ni;
-  const int j = M.js + r % nj;
-  const int k = M.ks + r / nj;
-  const double wght = beta * clk->dt;
+__device__ __forceinline__ void update_body(const MeshK &M, const UpdateArgs &a, double beta,
+                                            int first_stage, double dt, int lw, int nwg,
+                                            TileK T) {
+  int g, i, j, k;
+  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 1, M.nx2 + 1, M.nx3 + 1, T, g, i, j, k)) return;
+  i += M.is;
+  j += M.js;
+  k += M.ks;
+  const double wght = beta * dt;
   const bool ci = i <= M.ie, cj = j <= M.je, ck = k <= M.ke;
   if (ci && cj && ck)
     update_cell_u(M, g, k, j, i, a.ubase, a.uout, a.f1, a.f2, a.f3, wght, first_stage);
@@ -643,53 +680,47 @@ __global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double bet
     update_face_b<2>(M, g, k, j, i, a.bin[2], a.bout[2], a.e1, a.e2, a.e3, wght, first_stage);
 }
 
+__global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double beta,
+                                                int first_stage,
+                                                const Clock *__restrict__ clk, TileK T) {
+  update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+}
+
 //========================================================================================
 // Kernel 4: ConservedToPrimitive over every cell (ghosts included,
 // time_integrator.cpp:2206-2212) with Field::CalculateCellCenteredField
 // (field.cpp:112-180, uniform branch), writing prim (the output register) and bcc.
 // With dt_partial != nullptr the active cells also evaluate Hydro::NewBlockTimeStep
 // (new_blockdt.cpp:45-230) and each work-group writes its minimum.
+// Inputs and outputs of the ConservedToPrimitive kernel.
+struct C2PArgs {
+  double *u, *prim, *bcc;
+  const double *b1, *b2, *b3;
+  double *dt_partial;         // null: no NewBlockTimeStep
+  const double *lseed;        // learned pressure seeds (C2P_LEARNED)
+};
+
+// Every thread of the work-group runs the whole body (no early return): the dt
+// reduction at its end synchronises the work-group.  `slot` is the work-group's entry
+// in dt_partial.
 template <bool SR, bool LEARNED>
-__global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ u,
-                                                   double *__restrict__ prim,
-                                                   double *__restrict__ bcc,
-                                                   const double *__restrict__ b1,
-                                                   const double *__restrict__ b2,
-                                                   const double *__restrict__ b3,
-                                                   double *__restrict__ dt_partial) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  const int t = (lw % nch) * WG + threadIdx.x;
+__device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int lw, int nwg,
+                                         int slot, TileK T) {
+  double *__restrict__ u = a.u;
+  double *__restrict__ prim = a.prim;
+  double *__restrict__ bcc = a.bcc;
+  const double *__restrict__ b1 = a.b1;
+  const double *__restrict__ b2 = a.b2;
+  const double *__restrict__ b3 = a.b3;
+  double *__restrict__ dt_partial = a.dt_partial;
+  const double *__restrict__ lseed = a.lseed;
+  int g, i, j, k;
+  const bool live = map_work(lw, nwg, M.nblocks, M.nc1, M.nc2, M.nc3, T, g, i, j, k);
   double my_dt = DBL_MAX;
-  double seed = 0.0;
   C2PStats st{0, 0};
-  if (SR && LEARNED) {
-    // wavefront-collective inference of the gas-pressure seed; dead lanes carry a dummy
-    double cs[NHYDRO] = {1.0, 0.0, 0.0, 0.0, 2.0}, bs[3] = {0.0, 0.0, 0.0};
-    if (t < M.ncell) {
-      const int i = t % M.nc1, j = (t / M.nc1) % M.nc2, k = (t / M.nc1) / M.nc2;
-      const double *B1 = b1 + static_cast<long>(g) * M.nf1;
-      const double *B2 = b2 + static_cast<long>(g) * M.nf2;
-      const double *B3 = b3 + static_cast<long>(g) * M.nf3;
-      bs[IB1] = 0.5 * B1[f1idx(M, k, j, i)] + 0.5 * B1[f1idx(M, k, j, i + 1)];
-      bs[IB2] = 0.5 * B2[f2idx(M, k, j, i)] + 0.5 * B2[f2idx(M, k, j + 1, i)];
-      bs[IB3] = 0.5 * B3[f3idx(M, k, j, i)] + 0.5 * B3[f3idx(M, k + 1, j, i)];
-      const double *ub = u + static_cast<long>(g) * NHYDRO * M.ncell;
-#pragma unroll
-      for (int q = 0; q < NHYDRO; ++q) cs[q] = ub[q * static_cast<long>(M.ncell) + t];
-    }
-    float x[MLP_IN];
-    mlp_features(cs, bs, M.mlp.P, x);
-    seed = mlp_seed(cs, mlp_wave_infer_bf16(M.mlp, x));
-  }
-  if (t < M.ncell) {
-    const int i = t % M.nc1;
-    const int r = t / M.nc1;
-    const int j = r % M.nc2;
-    const int k = r / M.nc2;
+  if (live) {
     const long n = M.ncell;
-    const long c = t;
+    const long c = cidx(M, k, j, i);
     const double *B1 = b1 + static_cast<long>(g) * M.nf1;
     const double *B2 = b2 + static_cast<long>(g) * M.nf2;
     const double *B3 = b3 + static_cast<long>(g) * M.nf3;
@@ -735,7 +766,8 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
       double cin[NHYDRO];
 #pragma unroll
       for (int q = 0; q < NHYDRO; ++q) cin[q] = cons[q];
-      sr_cons2prim_cell(cons, w, bc, M.eos, LEARNED ? C2P_LEARNED : M.c2p_mode, seed, &st);
+      sr_cons2prim_cell(cons, w, bc, M.eos, LEARNED ? C2P_LEARNED : M.c2p_mode,
+                        LEARNED ? lseed[static_cast<long>(g) * n + c] : 0.0, &st);
 #pragma unroll
       for (int q = 0; q < NHYDRO; ++q)
         if (cons[q] != cin[q]) ub[q * n + c] = cons[q];
@@ -768,7 +800,7 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
     }
   }
   if (SR && M.c2p_stats) {   // wavefront sums, one atomic per wavefront
-    unsigned long long ev = st.evals, fb = st.fallback, nc = (t < M.ncell) ? 1 : 0;
+    unsigned long long ev = st.evals, fb = st.fallback, nc = live ? 1 : 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       ev += __shfl_down(ev, off, 64);
@@ -792,15 +824,53 @@ __global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, double *__restrict__ 
     if (threadIdx.x == 0) {
       double m = sm[0];
       for (int q = 1; q < WG / 64; ++q) m = dmin(m, sm[q]);
-      dt_partial[blockIdx.x] = m;
+      dt_partial[slot] = m;
     }
+    __syncthreads();   // sm is reused by the next work item of a persistent launch
   }
+}
+
+template <bool SR, bool LEARNED>
+__global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, C2PArgs a, TileK T) {
+  c2p_body<SR, LEARNED>(M, a, xcd_logical_wg(M.xcd_remap), gridDim.x, blockIdx.x, T);
+}
+
+
+// Learned gas-pressure seeds for C2P_LEARNED, cell for cell as k_cons2prim, from the
+// conserved state and the cell-centred field it is about to see.  Wavefront-collective
+// BF16 MFMA inference; a kernel of its own so that the FP64 recovery keeps its register
+// budget.  Dead lanes carry a dummy state.
+__global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restrict__ u,
+                                                  const double *__restrict__ b1,
+                                                  const double *__restrict__ b2,
+                                                  const double *__restrict__ b3,
+                                                  double *__restrict__ lseed, TileK T) {
+  int g, i, j, k;
+  const bool live = map_work(xcd_logical_wg(M.xcd_remap), gridDim.x, M.nblocks, M.nc1, M.nc2,
+                             M.nc3, T, g, i, j, k);
+  double cs[NHYDRO] = {1.0, 0.0, 0.0, 0.0, 2.0}, bs[3] = {0.0, 0.0, 0.0};
+  const long c = live ? cidx(M, k, j, i) : 0;
+  if (live) {
+    const double *B1 = b1 + static_cast<long>(g) * M.nf1;
+    const double *B2 = b2 + static_cast<long>(g) * M.nf2;
+    const double *B3 = b3 + static_cast<long>(g) * M.nf3;
+    bs[IB1] = 0.5 * B1[f1idx(M, k, j, i)] + 0.5 * B1[f1idx(M, k, j, i + 1)];
+    bs[IB2] = 0.5 * B2[f2idx(M, k, j, i)] + 0.5 * B2[f2idx(M, k, j + 1, i)];
+    bs[IB3] = 0.5 * B3[f3idx(M, k, j, i)] + 0.5 * B3[f3idx(M, k + 1, j, i)];
+    const double *ub = u + static_cast<long>(g) * NHYDRO * M.ncell;
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q) cs[q] = ub[q * static_cast<long>(M.ncell) + c];
+  }
+  float x[MLP_IN];
+  mlp_features(cs, bs, M.mlp.P, x);
+  const double sd = mlp_seed(cs, mlp_wave_infer_bf16(M.mlp, x));
+  if (live) lseed[static_cast<long>(g) * M.ncell + c] = sd;
 }
 
 // Final reduction + Mesh::NewTimeStep (mesh.cpp:1236-1282) and, when advance != 0,
 // the end-of-cycle clock update of the main loop (main.cpp:535-541).
-__global__ void __launch_bounds__(1024) k_new_dt(const double *__restrict__ part, long npart,
-                                                  Clock *clk, double cfl, int advance) {
+__device__ __forceinline__ void new_dt_body(const double *__restrict__ part, long npart,
+                                            Clock *clk, double cfl, int advance) {
   __shared__ double sm[1024 / 64];
   double v = DBL_MAX;
   for (long q = threadIdx.x; q < npart; q += 1024) v = dmin(v, part[q]);
@@ -823,6 +893,82 @@ __global__ void __launch_bounds__(1024) k_new_dt(const double *__restrict__ part
     c.dt = dt;
     *clk = c;
   }
+}
+
+__global__ void __launch_bounds__(1024) k_new_dt(const double *__restrict__ part, long npart,
+                                                  Clock *clk, double cfl, int advance) {
+  new_dt_body(part, npart, clk, cfl, advance);
+}
+
+//========================================================================================
+// Ensembles: independent meshes (uneven sizes) advanced together.  Each stage kernel is
+// launched once for the whole ensemble; work item v of a launch is (member, work-group
+// of that member) from a precomputed map, and a launch smaller than the map loops over
+// it (persistent work-groups).  The member table holds every kernel argument of one
+// stage, so a whole cycle is a fixed sequence of launches that a HIP graph can replay.
+struct EnsMember {
+  MeshK M;
+  const double *w, *bcc;
+  FaceOut fo[3];
+  CornerArgs ca;
+  UpdateArgs ua;
+  C2PArgs pa;
+  Clock *clk;
+  int nwg[4];       // work-groups of each work kind
+  TileK tile[4];
+};
+
+__device__ __forceinline__ int2 ens_item(const int2 *__restrict__ map, int v) {
+  const int2 mw = map[v];   // the same for every lane: keep it in scalar registers
+  return make_int2(__builtin_amdgcn_readfirstlane(mw.x), __builtin_amdgcn_readfirstlane(mw.y));
+}
+
+template <int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_flux3_ens(const EnsMember *__restrict__ E,
+                                                   const int2 *__restrict__ map, int nwork) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    flux3_body<ORDER, RS>(e.M, e.w, e.bcc, e.fo[0], e.fo[1], e.fo[2], e.clk->dt, mw.y,
+                          e.nwg[0], e.tile[0]);
+  }
+}
+
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_corner_ens(const EnsMember *__restrict__ E,
+                                                    const int2 *__restrict__ map, int nwork) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    corner_body<SR>(e.M, e.ca, mw.y, e.nwg[1], e.tile[1]);
+  }
+}
+
+__global__ void __launch_bounds__(WG) k_update_ens(const EnsMember *__restrict__ E,
+                                                    const int2 *__restrict__ map, int nwork,
+                                                    double beta, int first_stage) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    update_body(e.M, e.ua, beta, first_stage, e.clk->dt, mw.y, e.nwg[2], e.tile[2]);
+  }
+}
+
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_c2p_ens(const EnsMember *__restrict__ E,
+                                                 const int2 *__restrict__ map, int nwork) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    c2p_body<SR, false>(e.M, e.pa, mw.y, e.nwg[3], mw.y, e.tile[3]);
+  }
+}
+
+// One work-group per member: its dt reduction and clock update.
+__global__ void __launch_bounds__(1024) k_new_dt_ens(const EnsMember *__restrict__ E,
+                                                      int advance) {
+  const EnsMember &e = E[blockIdx.x];
+  new_dt_body(e.pa.dt_partial, e.nwg[3], e.clk, e.M.cfl, advance);
 }
 
 // Pull-based same-level ghost fill for Mesh::Initialize (cell-centred register with
@@ -932,11 +1078,15 @@ struct ath_mesh {
   double *e1x3 = nullptr, *e2x3 = nullptr, *wx[3] = {nullptr, nullptr, nullptr};
   double *e[3] = {nullptr, nullptr, nullptr};
   double *dtpart = nullptr;
-  long ndtpart = 0;
+  long ndtpart = 0;           // capacity of dtpart
+  long ndt_used = 0;          // entries written by the last cons2prim with dt
+  TileK tile[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // per work kind
+  long nstage = 0;            // stages run (the register configuration's parity)
   int *nbr_d = nullptr;
   double *coord_d = nullptr;
   Clock *clk = nullptr;
   unsigned long long *c2p_stats_d = nullptr;   // SR recovery counters, allocated on demand
+  double *lseed = nullptr;   // learned pressure seeds (one scalar register), on demand
   std::vector<void *> allocs;
   // optional per-kernel event timing (ath_gpu_kernel_timing)
   bool prof_on = false;
@@ -958,7 +1108,7 @@ namespace {
 // kernel ids for the per-kernel timer
 enum KernelId {
   KID_FLUX = 0,          // + 3*dir + (order-1): 9 flux variants
-  KID_CORNER = 9, KID_UPDATE_U = 10, KID_UPDATE_B = 11, KID_CONS2PRIM = 12,
+  KID_CORNER = 9, KID_UPDATE_U = 10, KID_SEED = 11, KID_CONS2PRIM = 12,
   KID_NEW_DT = 13, KID_GHOST = 14, KID_FLUX3 = 15,   // + (order-1): 3 fused variants
   KID_COUNT = 18
 };
@@ -970,7 +1120,7 @@ const char *kernel_name(int id) {
   switch (id) {
     case KID_CORNER: return "corner_emf";
     case KID_UPDATE_U: return "update_u_b_ct";
-    case KID_UPDATE_B: return "update_b_ct";
+    case KID_SEED: return "c2p_learned_seed";
     case KID_CONS2PRIM: return "cons2prim";
     case KID_NEW_DT: return "new_dt";
     case KID_GHOST: return "ghost_fill";
@@ -1011,17 +1161,102 @@ FaceOut face_out(ath_mesh *m, int dir) {
   return o;
 }
 
-// All three directions in one launch (the production path).
-int launch_flux(ath_mesh *m, int order) {
+// Work kinds of the stage kernels and their position boxes per MeshBlock.
+enum WorkKind { KK_FLUX = 0, KK_CORNER = 1, KK_UPDATE = 2, KK_C2P = 3, KK_N = 4 };
+
+void kind_box(const MeshK &M, int kind, int *n) {
+  if (kind == KK_FLUX) { n[0] = M.nx1 + 3; n[1] = M.nx2 + 3; n[2] = M.nx3 + 3; }
+  else if (kind == KK_C2P) { n[0] = M.nc1; n[1] = M.nc2; n[2] = M.nc3; }
+  else { n[0] = M.nx1 + 1; n[1] = M.nx2 + 1; n[2] = M.nx3 + 1; }
+}
+
+// Work-groups per MeshBlock of kind `kind` under tile T (map_work's nch).
+long kind_chunks(const MeshK &M, int kind, TileK T) {
+  int n[3];
+  kind_box(M, kind, n);
+  if (T.tx == 0) return (static_cast<long>(n[0]) * n[1] * n[2] + WG - 1) / WG;
+  return static_cast<long>((n[0] + T.tx - 1) / T.tx) * ((n[1] + T.ty - 1) / T.ty) *
+         ((n[2] + T.tz - 1) / T.tz);
+}
+
+// Tile shapes the autotuner chooses from (index 0: linear chunks).
+constexpr int NTILES = 6;
+constexpr TileK TILES[NTILES] = {{0, 0, 0}, {64, 4, 1}, {32, 8, 1}, {16, 16, 1}, {16, 4, 4},
+                                 {8, 8, 4}};
+
+// Register configuration of a mesh (the pointers the VL2 stages swap).
+struct Regs {
+  double *u, *u1, *w, *w1, *b[3], *bb[3];
+};
+Regs regs_of(const ath_mesh *m) {
+  Regs r;
+  r.u = m->u; r.u1 = m->u1; r.w = m->w; r.w1 = m->w1;
+  for (int c = 0; c < 3; ++c) { r.b[c] = m->b[c]; r.bb[c] = m->bb[c]; }
+  return r;
+}
+
+// Every kernel argument of one VL2 stage given the register configuration at its start:
+// flux/corner read w, b; the update writes u1, b1 and then the registers swap
+// (IntegrateHydro/IntegrateField: WeightedAve into zeroed u1/b1 and SwapAthenaArray;
+// stage 2 is in place on the swapped-in U^n, B^n); ConservedToPrimitive(u, w, b, w1,
+// bcc) then writes the output register w1 and w swaps with w1.
+struct StageArgs {
+  int order, first;
+  double beta;
+  const double *w, *bcc;
+  FaceOut fo[3];
+  CornerArgs ca;
+  UpdateArgs ua;
+  C2PArgs pa;
+};
+
+StageArgs stage_args(const ath_mesh *m, const Regs &r, int stg) {
+  StageArgs a;
+  a.order = (stg == 1) ? 1 : m->d.xorder;
+  a.first = (stg == 1) ? 1 : 0;
+  a.beta = (stg == 1) ? 0.5 : 1.0;
+  a.w = r.w;
+  a.bcc = m->bcc;
+  for (int d = 0; d < 3; ++d) {
+    FaceOut &o = a.fo[d];
+    o.bn = r.b[d];
+    o.flx = m->flux[d];
+    o.wct = m->wx[d];
+    if (d == 0) { o.ey = m->e3x1; o.ez = m->e2x1; }
+    else if (d == 1) { o.ey = m->e1x2; o.ez = m->e3x2; }
+    else { o.ey = m->e2x3; o.ez = m->e1x3; }
+  }
+  a.ca = CornerArgs{r.w, m->bcc, m->e2x1, m->e3x1, m->e1x2, m->e3x2, m->e1x3, m->e2x3,
+                    m->wx[0], m->wx[1], m->wx[2], m->e[0], m->e[1], m->e[2]};
+  UpdateArgs &ua = a.ua;
+  ua.ubase = (stg == 1) ? r.u : r.u1;
+  ua.uout = r.u1;
+  ua.f1 = m->flux[0]; ua.f2 = m->flux[1]; ua.f3 = m->flux[2];
+  for (int c = 0; c < 3; ++c) {
+    ua.bin[c] = (stg == 1) ? r.b[c] : r.bb[c];
+    ua.bout[c] = r.bb[c];
+  }
+  ua.e1 = m->e[0]; ua.e2 = m->e[1]; ua.e3 = m->e[2];
+  // after the update's swap, u is the updated state and b the updated faces
+  a.pa = C2PArgs{r.u1, r.w1, m->bcc, r.bb[0], r.bb[1], r.bb[2],
+                 (stg == 2) ? m->dtpart : nullptr, m->lseed};
+  return a;
+}
+
+dim3 kind_grid(const ath_mesh *m, int kind) {
+  return dim3(static_cast<unsigned>(kind_chunks(m->M, kind, m->tile[kind]) * m->M.nblocks));
+}
+
+int launch_flux_args(ath_mesh *m, int order, const double *w, const double *bcc,
+                     const FaceOut *fo) {
   const MeshK &M = m->M;
   const int rs = m->d.relativistic ? 2 : m->d.rsolver;
-  const long n = static_cast<long>(M.nx1 + 3) * (M.nx2 + 3) * (M.nx3 + 3);
-  dim3 grid(((n + WG - 1) / WG) * M.nblocks);
-  FaceOut o1 = face_out(m, 0), o2 = face_out(m, 1), o3 = face_out(m, 2);
+  const dim3 grid = kind_grid(m, KK_FLUX);
+  const TileK T = m->tile[KK_FLUX];
   ProfScope ps(m, KID_FLUX3 + (order - 1));
-#define LF(O, R)                                                                        \
-  hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc, o1, \
-                     o2, o3, m->clk)
+#define LF(O, R)                                                                          \
+  hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], fo[1], \
+                     fo[2], m->clk, T)
 #define LF_O(R)                         \
   do {                                  \
     if (order == 1) LF(1, R);           \
@@ -1035,6 +1270,12 @@ int launch_flux(ath_mesh *m, int order) {
 #undef LF
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// All three directions in one launch (the production path), on the current registers.
+int launch_flux(ath_mesh *m, int order) {
+  const StageArgs a = stage_args(m, regs_of(m), 2);
+  return launch_flux_args(m, order, a.w, a.bcc, a.fo);
 }
 
 // One direction per launch (Hydro::CalculateFluxes direction by direction).
@@ -1077,82 +1318,91 @@ int launch_flux_dirs(ath_mesh *m, int order) {
   return 0;
 }
 
-int launch_corner(ath_mesh *m) {
+int launch_corner_args(ath_mesh *m, const CornerArgs &ca) {
   const MeshK &M = m->M;
-  long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
-  dim3 grid(((n + WG - 1) / WG) * M.nblocks);
+  const dim3 grid = kind_grid(m, KK_CORNER);
   ProfScope ps(m, KID_CORNER);
   if (m->d.relativistic)
-    hipLaunchKernelGGL((k_corner<true>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,
-                       m->e2x1, m->e3x1, m->e1x2, m->e3x2, m->e1x3, m->e2x3, m->wx[0],
-                       m->wx[1], m->wx[2], m->e[0], m->e[1], m->e[2]);
+    hipLaunchKernelGGL((k_corner<true>), grid, dim3(WG), 0, m->stream, M, ca,
+                       m->tile[KK_CORNER]);
   else
-    hipLaunchKernelGGL((k_corner<false>), grid, dim3(WG), 0, m->stream, M, m->w, m->bcc,
-                       m->e2x1, m->e3x1, m->e1x2, m->e3x2, m->e1x3, m->e2x3, m->wx[0],
-                       m->wx[1], m->wx[2], m->e[0], m->e[1], m->e[2]);
+    hipLaunchKernelGGL((k_corner<false>), grid, dim3(WG), 0, m->stream, M, ca,
+                       m->tile[KK_CORNER]);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int launch_cons2prim(ath_mesh *m, double *prim_out, bool with_dt) {
+int launch_corner(ath_mesh *m) {
+  return launch_corner_args(m, stage_args(m, regs_of(m), 2).ca);
+}
+
+int launch_cons2prim_args(ath_mesh *m, const C2PArgs &pa) {
   const MeshK &M = m->M;
-  dim3 grid(((M.ncell + WG - 1) / WG) * M.nblocks);
-  double *part = with_dt ? m->dtpart : nullptr;
+  const dim3 grid = kind_grid(m, KK_C2P);
+  const TileK T = m->tile[KK_C2P];
+  if (pa.dt_partial) m->ndt_used = static_cast<long>(grid.x);
   ProfScope ps(m, KID_CONS2PRIM);
-  if (m->d.relativistic && M.c2p_mode == C2P_LEARNED)
-    hipLaunchKernelGGL((k_cons2prim<true, true>), grid, dim3(WG), 0, m->stream, M, m->u,
-                       prim_out, m->bcc, m->b[0], m->b[1], m->b[2], part);
-  else if (m->d.relativistic)
-    hipLaunchKernelGGL((k_cons2prim<true, false>), grid, dim3(WG), 0, m->stream, M, m->u,
-                       prim_out, m->bcc, m->b[0], m->b[1], m->b[2], part);
-  else
-    hipLaunchKernelGGL((k_cons2prim<false, false>), grid, dim3(WG), 0, m->stream, M, m->u,
-                       prim_out, m->bcc, m->b[0], m->b[1], m->b[2], part);
+  if (m->d.relativistic && M.c2p_mode == C2P_LEARNED) {
+    {
+      ProfScope pss(m, KID_SEED);
+      hipLaunchKernelGGL(k_mesh_seed, grid, dim3(WG), 0, m->stream, M, pa.u, pa.b1, pa.b2,
+                         pa.b3, m->lseed, T);
+    }
+    hipLaunchKernelGGL((k_cons2prim<true, true>), grid, dim3(WG), 0, m->stream, M, pa, T);
+  } else if (m->d.relativistic) {
+    hipLaunchKernelGGL((k_cons2prim<true, false>), grid, dim3(WG), 0, m->stream, M, pa, T);
+  } else {
+    hipLaunchKernelGGL((k_cons2prim<false, false>), grid, dim3(WG), 0, m->stream, M, pa, T);
+  }
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// ConservedToPrimitive(u, ., b, prim_out, bcc) on the current registers.
+int launch_cons2prim(ath_mesh *m, double *prim_out, bool with_dt) {
+  C2PArgs pa{m->u, prim_out, m->bcc, m->b[0], m->b[1], m->b[2],
+             with_dt ? m->dtpart : nullptr, m->lseed};
+  return launch_cons2prim_args(m, pa);
 }
 
 int launch_new_dt(ath_mesh *m, int advance) {
   ProfScope ps(m, KID_NEW_DT);
-  hipLaunchKernelGGL(k_new_dt, dim3(1), dim3(1024), 0, m->stream, m->dtpart, m->ndtpart,
+  hipLaunchKernelGGL(k_new_dt, dim3(1), dim3(1024), 0, m->stream, m->dtpart, m->ndt_used,
                      m->clk, m->d.cfl, advance);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+int launch_update_args(ath_mesh *m, const UpdateArgs &ua, double beta, int first) {
+  const dim3 grid = kind_grid(m, KK_UPDATE);
+  ProfScope ps(m, KID_UPDATE_U);
+  hipLaunchKernelGGL(k_update, grid, dim3(WG), 0, m->stream, m->M, ua, beta, first, m->clk,
+                     m->tile[KK_UPDATE]);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void swap_stage_regs(Regs &r) {   // the two swaps of one stage
+  std::swap(r.u, r.u1);
+  for (int c = 0; c < 3; ++c) std::swap(r.b[c], r.bb[c]);
+  std::swap(r.w, r.w1);
+}
+
+void set_regs(ath_mesh *m, const Regs &r) {
+  m->u = r.u; m->u1 = r.u1; m->w = r.w; m->w1 = r.w1;
+  for (int c = 0; c < 3; ++c) { m->b[c] = r.b[c]; m->bb[c] = r.bb[c]; }
+}
+
 int stage(ath_mesh *m, int stg) {
-  const MeshK &M = m->M;
-  const double beta = (stg == 1) ? 0.5 : 1.0;
-  const int order = (stg == 1) ? 1 : m->d.xorder;
-  if (launch_flux(m, order)) return 1;
-  if (launch_corner(m)) return 1;
-  // conserved and face-field update.  Stage 1 reads u, b and writes u1, b1, then the
-  // registers swap (IntegrateHydro/IntegrateField: WeightedAve into zeroed u1/b1 and
-  // SwapAthenaArray); stage 2 is in place on the swapped-in U^n, B^n.
-  {
-    long n = static_cast<long>(M.nx1 + 1) * (M.nx2 + 1) * (M.nx3 + 1);
-    dim3 grid(((n + WG - 1) / WG) * M.nblocks);
-    UpdateArgs a;
-    a.ubase = (stg == 1) ? m->u : m->u1;
-    a.uout = m->u1;
-    a.f1 = m->flux[0]; a.f2 = m->flux[1]; a.f3 = m->flux[2];
-    for (int c = 0; c < 3; ++c) {
-      a.bin[c] = (stg == 1) ? m->b[c] : m->bb[c];
-      a.bout[c] = m->bb[c];
-    }
-    a.e1 = m->e[0]; a.e2 = m->e[1]; a.e3 = m->e[2];
-    {
-      ProfScope ps(m, KID_UPDATE_U);
-      hipLaunchKernelGGL(k_update, grid, dim3(WG), 0, m->stream, M, a, beta,
-                         stg == 1 ? 1 : 0, m->clk);
-    }
-    HIPCHK(hipGetLastError());
-    std::swap(m->u, m->u1);
-    for (int c = 0; c < 3; ++c) std::swap(m->b[c], m->bb[c]);
-  }
-  // Primitives: ConservedToPrimitive(u, w, b, w1, bcc) then w.SwapAthenaArray(w1)
-  if (launch_cons2prim(m, m->w1, stg == 2)) return 1;
-  std::swap(m->w, m->w1);
+  Regs r = regs_of(m);
+  const StageArgs a = stage_args(m, r, stg);
+  if (launch_flux_args(m, a.order, a.w, a.bcc, a.fo)) return 1;
+  if (launch_corner_args(m, a.ca)) return 1;
+  if (launch_update_args(m, a.ua, a.beta, a.first)) return 1;
+  if (launch_cons2prim_args(m, a.pa)) return 1;
+  swap_stage_regs(r);
+  set_regs(m, r);
+  m->nstage += 1;
   return 0;
 }
 
@@ -1177,6 +1427,152 @@ int block_count(ath_mesh *m, int what, long *n, double ***pool) {
     case ATH_WGHT3: *n = M.nf3; *pool = &m->wx[2]; return 0;
     default: return fail("unknown register code " + std::to_string(what));
   }
+}
+
+}  // namespace
+
+//========================================================================================
+// Ensemble host side
+
+struct ath_ensemble {
+  std::vector<ath_mesh *> mem;
+  int xorder = 2, rs = 0, sr = 0;
+  hipStream_t stream = nullptr;
+  EnsMember *tab_d[2] = {nullptr, nullptr};   // stage-1 and stage-2 member tables
+  int2 *map_d[KK_N] = {nullptr, nullptr, nullptr, nullptr};
+  int nwork[KK_N] = {0, 0, 0, 0};
+  int tile_idx[KK_N] = {0, 0, 0, 0};          // index into TILES
+  int persist[KK_N] = {0, 0, 0, 0};           // 0: one work item per work-group; else grid
+  std::vector<long> parity;                   // members' stage count parity at creation
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~ath_ensemble() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto *p : tab_d) if (p) (void)hipFree(p);
+    for (auto *p : map_d) if (p) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void ens_drop_graph(ath_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+// (Re)builds the member tables and work maps for the current tile choices.
+int ens_build(ath_ensemble *e) {
+  HIPCHK(hipStreamSynchronize(e->stream));
+  ens_drop_graph(e);
+  const int n = static_cast<int>(e->mem.size());
+  std::vector<EnsMember> tab[2];
+  std::vector<int2> map[KK_N];
+  for (int q = 0; q < n; ++q) {
+    ath_mesh *m = e->mem[q];
+    Regs r = regs_of(m);
+    for (int stg = 1; stg <= 2; ++stg) {
+      const StageArgs a = stage_args(m, r, stg);
+      EnsMember em;
+      em.M = m->M;
+      em.w = a.w;
+      em.bcc = a.bcc;
+      for (int d = 0; d < 3; ++d) em.fo[d] = a.fo[d];
+      em.ca = a.ca;
+      em.ua = a.ua;
+      em.pa = a.pa;
+      em.clk = m->clk;
+      for (int k = 0; k < KK_N; ++k) {
+        em.tile[k] = TILES[e->tile_idx[k]];
+        em.nwg[k] = static_cast<int>(kind_chunks(m->M, k, em.tile[k]) * m->M.nblocks);
+      }
+      if (em.nwg[KK_C2P] > m->ndtpart) return fail("dt partial buffer too small");
+      tab[stg - 1].push_back(em);
+      swap_stage_regs(r);
+    }
+    for (int k = 0; k < KK_N; ++k) {
+      const int nw = tab[0].back().nwg[k];
+      for (int w = 0; w < nw; ++w) map[k].push_back(make_int2(q, w));
+    }
+  }
+  for (int t = 0; t < 2; ++t) {
+    if (e->tab_d[t]) HIPCHK(hipFree(e->tab_d[t]));
+    e->tab_d[t] = nullptr;
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&e->tab_d[t]), sizeof(EnsMember) * n));
+    HIPCHK(hipMemcpy(e->tab_d[t], tab[t].data(), sizeof(EnsMember) * n,
+                     hipMemcpyHostToDevice));
+  }
+  for (int k = 0; k < KK_N; ++k) {
+    if (e->map_d[k]) HIPCHK(hipFree(e->map_d[k]));
+    e->map_d[k] = nullptr;
+    e->nwork[k] = static_cast<int>(map[k].size());
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&e->map_d[k]), sizeof(int2) * map[k].size()));
+    HIPCHK(hipMemcpy(e->map_d[k], map[k].data(), sizeof(int2) * map[k].size(),
+                     hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+dim3 ens_grid(const ath_ensemble *e, int kind) {
+  const int p = e->persist[kind];
+  return dim3(static_cast<unsigned>((p > 0 && p < e->nwork[kind]) ? p : e->nwork[kind]));
+}
+
+int ens_launch_kind(ath_ensemble *e, int kind, int stg) {
+  const EnsMember *T = e->tab_d[stg - 1];
+  const dim3 g = ens_grid(e, kind);
+  const int2 *mp = e->map_d[kind];
+  const int nw = e->nwork[kind];
+  if (kind == KK_FLUX) {
+    const int order = (stg == 1) ? 1 : e->xorder;
+#define LE(O, R) hipLaunchKernelGGL((k_flux3_ens<O, R>), g, dim3(WG), 0, e->stream, T, mp, nw)
+#define LE_O(R)                         \
+  do {                                  \
+    if (order == 1) LE(1, R);           \
+    else if (order == 2) LE(2, R);      \
+    else LE(3, R);                      \
+  } while (0)
+    if (e->rs == 0) LE_O(0);
+    else if (e->rs == 1) LE_O(1);
+    else LE_O(2);
+#undef LE_O
+#undef LE
+  } else if (kind == KK_CORNER) {
+    if (e->sr) hipLaunchKernelGGL(k_corner_ens<true>, g, dim3(WG), 0, e->stream, T, mp, nw);
+    else hipLaunchKernelGGL(k_corner_ens<false>, g, dim3(WG), 0, e->stream, T, mp, nw);
+  } else if (kind == KK_UPDATE) {
+    hipLaunchKernelGGL(k_update_ens, g, dim3(WG), 0, e->stream, T, mp, nw,
+                       (stg == 1) ? 0.5 : 1.0, (stg == 1) ? 1 : 0);
+  } else {
+    if (e->sr) hipLaunchKernelGGL(k_c2p_ens<true>, g, dim3(WG), 0, e->stream, T, mp, nw);
+    else hipLaunchKernelGGL(k_c2p_ens<false>, g, dim3(WG), 0, e->stream, T, mp, nw);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// One VL2 cycle of every member: 4 launches per stage and one dt launch.
+int ens_record_cycle(ath_ensemble *e) {
+  for (int stg = 1; stg <= 2; ++stg)
+    for (int k = 0; k < KK_N; ++k)
+      if (ens_launch_kind(e, k, stg)) return 1;
+  hipLaunchKernelGGL(k_new_dt_ens, dim3(static_cast<unsigned>(e->mem.size())), dim3(1024), 0,
+                     e->stream, e->tab_d[1], 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int ens_check(ath_ensemble *e) {
+  for (size_t q = 0; q < e->mem.size(); ++q) {
+    if ((e->mem[q]->nstage & 1) != e->parity[q])
+      return fail("a member ran a single VL2 stage after the ensemble was created; its "
+                  "registers no longer match the ensemble's tables");
+    HIPCHK(hipStreamSynchronize(e->mem[q]->stream));
+  }
+  return 0;
 }
 
 }  // namespace
@@ -1319,7 +1715,9 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
     rc |= m->alloc(&m->e[0], M.ne1 * nb);
     rc |= m->alloc(&m->e[1], M.ne2 * nb);
     rc |= m->alloc(&m->e[2], M.ne3 * nb);
-    m->ndtpart = ((M.ncell + WG - 1) / WG) * nb;
+    m->ndtpart = 0;   // room for any tile shape of the autotuner
+    for (int q = 0; q < NTILES; ++q)
+      m->ndtpart = std::max(m->ndtpart, kind_chunks(M, KK_C2P, TILES[q]) * nb);
     rc |= m->alloc(&m->dtpart, m->ndtpart);
     rc |= m->alloc(&m->coord_d, static_cast<long>(coordh.size()));
   }
@@ -1506,6 +1904,169 @@ int ath_gpu_synchronize(ath_mesh_t m) {
   return 0;
 }
 
+int ath_gpu_ensemble_create(const ath_mesh_t *members, int n, ath_ensemble_t *out) {
+  if (!members || !out || n <= 0) return fail("bad argument");
+  *out = nullptr;
+  ath_ensemble *e = new ath_ensemble;
+  for (int q = 0; q < n; ++q) {
+    ath_mesh *m = members[q];
+    if (!m) { delete e; return fail("null member"); }
+    for (auto *o : e->mem)
+      if (o == m) { delete e; return fail("a mesh appears twice in the ensemble"); }
+    const int rs = m->d.relativistic ? 2 : m->d.rsolver;
+    if (q == 0) {
+      e->xorder = m->d.xorder;
+      e->rs = rs;
+      e->sr = m->d.relativistic;
+    } else if (m->d.xorder != e->xorder || rs != e->rs || m->d.relativistic != e->sr) {
+      delete e;
+      return fail("ensemble members must share xorder, Riemann solver and dynamics");
+    }
+    if (m->M.c2p_mode != C2P_REFERENCE) {
+      delete e;
+      return fail("ensemble members must use the reference primitive recovery");
+    }
+    e->mem.push_back(m);
+    e->parity.push_back(m->nstage & 1);
+  }
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail("hipStreamCreate failed");
+  }
+  if (ens_check(e) || ens_build(e)) {
+    delete e;
+    return 1;
+  }
+  *out = e;
+  return 0;
+}
+
+int ath_gpu_ensemble_destroy(ath_ensemble_t e) {
+  if (!e) return 0;
+  (void)hipStreamSynchronize(e->stream);
+  delete e;
+  return 0;
+}
+
+int ath_gpu_ensemble_size(ath_ensemble_t e, int *nmembers, long *active_cells) {
+  if (!e) return fail("null argument");
+  long cells = 0;
+  for (auto *m : e->mem) cells += static_cast<long>(m->M.nx1) * m->M.nx2 * m->M.nx3 * m->M.nblocks;
+  if (nmembers) *nmembers = static_cast<int>(e->mem.size());
+  if (active_cells) *active_cells = cells;
+  return 0;
+}
+
+int ath_gpu_ensemble_cycle(ath_ensemble_t e, int ncycles, int use_graph) {
+  if (!e) return fail("null argument");
+  if (ens_check(e)) return 1;
+  if (use_graph && !e->exec) {
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = ens_record_cycle(e);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return 1;
+    }
+    HIPCHK(ce);
+    e->graph = g;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  }
+  for (int c = 0; c < ncycles; ++c) {
+    if (use_graph) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+    else if (ens_record_cycle(e)) return 1;
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int ath_gpu_ensemble_get_tiles(ath_ensemble_t e, int *tile_idx, int *persist) {
+  if (!e) return fail("null argument");
+  for (int k = 0; k < KK_N; ++k) {
+    if (tile_idx) tile_idx[k] = e->tile_idx[k];
+    if (persist) persist[k] = e->persist[k];
+  }
+  return 0;
+}
+
+int ath_gpu_ensemble_set_tiles(ath_ensemble_t e, const int *tile_idx, const int *persist) {
+  if (!e) return fail("null argument");
+  for (int k = 0; k < KK_N; ++k) {
+    if (tile_idx && (tile_idx[k] < 0 || tile_idx[k] >= NTILES))
+      return fail("tile index out of range");
+    if (persist && persist[k] < 0) return fail("persistent grid must be >= 0");
+  }
+  for (int k = 0; k < KK_N; ++k) {
+    if (tile_idx) e->tile_idx[k] = tile_idx[k];
+    if (persist) e->persist[k] = persist[k];
+  }
+  return ens_build(e);
+}
+
+// Times every (tile shape, persistent grid) pair of each work kind on the ensemble's own
+// data and keeps the fastest.  The trial launches write only scratch registers (fluxes,
+// EMFs, u1, b1, w1, bcc); the members' W and bcc are then recomputed from U and B by
+// the initialisation path and their clocks restored, so the state after tuning is the
+// state before it.
+int ath_gpu_ensemble_autotune(ath_ensemble_t e, int reps, double *best_ms) {
+  if (!e) return fail("null argument");
+  if (reps <= 0) reps = 3;
+  if (ens_check(e)) return 1;
+  std::vector<Clock> clocks(e->mem.size());
+  for (size_t q = 0; q < e->mem.size(); ++q)
+    HIPCHK(hipMemcpy(&clocks[q], e->mem[q]->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  int ncu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grids[3] = {0, 4 * ncu, 8 * ncu};
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int rc = 0;
+  for (int k = 0; k < KK_N && !rc; ++k) {
+    float best = 1e30f;
+    int bt = e->tile_idx[k], bp = e->persist[k];
+    for (int t = 0; t < NTILES && !rc; ++t)
+      for (int pg = 0; pg < 3 && !rc; ++pg) {
+        e->tile_idx[k] = t;
+        e->persist[k] = grids[pg];
+        if (ens_build(e)) { rc = 1; break; }
+        // stage 1 of the flux/corner/update kinds reads only valid registers; the
+        // recovery reads the u1 that the stage-1 update wrote
+        if (k == KK_C2P && ens_launch_kind(e, KK_UPDATE, 1)) { rc = 1; break; }
+        if (ens_launch_kind(e, k, 1)) { rc = 1; break; }   // warm-up
+        if (hipEventRecord(a, e->stream) != hipSuccess) { rc = fail("hipEventRecord"); break; }
+        for (int r = 0; r < reps && !rc; ++r) rc = ens_launch_kind(e, k, 1);
+        if (rc || hipEventRecord(b, e->stream) != hipSuccess ||
+            hipEventSynchronize(b) != hipSuccess) { rc = rc ? rc : fail("event"); break; }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms < best) { best = ms; bt = t; bp = grids[pg]; }
+      }
+    e->tile_idx[k] = bt;
+    e->persist[k] = bp;
+    if (best_ms) best_ms[k] = best / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) return 1;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  for (size_t q = 0; q < e->mem.size(); ++q) {   // W, bcc from U, B; clock back
+    ath_mesh *m = e->mem[q];
+    if (ath_gpu_initialize(m, clocks[q].tlim)) return 1;
+    HIPCHK(hipMemcpy(m->clk, &clocks[q], sizeof(Clock), hipMemcpyHostToDevice));
+  }
+  return ens_build(e);
+}
+
+int ath_gpu_ensemble_stream(ath_ensemble_t e, void **stream) {
+  if (!e || !stream) return fail("null argument");
+  *stream = e->stream;
+  return 0;
+}
+
 int ath_gpu_mesh_set_c2p(ath_mesh_t m, ath_mlp_t mlp, int mode) {
   if (!m) return fail("null argument");
   if (mode < C2P_REFERENCE || mode > C2P_LEARNED) return fail("mode must be 0, 1 or 2");
@@ -1513,6 +2074,11 @@ int ath_gpu_mesh_set_c2p(ath_mesh_t m, ath_mlp_t mlp, int mode) {
     return fail("solver modes other than 0 apply to relativistic meshes only");
   if (mode == C2P_LEARNED && !mlp) return fail("the learned mode needs an MLP");
   HIPCHK(hipStreamSynchronize(m->stream));
+  if (mode == C2P_LEARNED && !m->lseed) {
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&m->lseed),
+                     sizeof(double) * m->M.ncell * m->M.nblocks));
+    m->allocs.push_back(m->lseed);
+  }
   m->M.c2p_mode = mode;
   m->M.mlp = ath_ml_device_view(mlp);
   return 0;

@@ -239,34 +239,45 @@ __global__ void k_mlp_features(int n, const double *__restrict__ cons,
 }
 
 //----------------------------------------------------------------------------------------
-// Batched SR primitive recovery with the solver mode of physics.hpp (reference NH, Newton
-// from the register seed, or Newton from the learned seed).  The learned mode is
-// wavefront-collective: every lane reaches the inference, dead lanes with a dummy state.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_sr_c2p_modes(MlpK K, EosK eos, int n, double *cons,
-                                                      double *prim, const double *bcc,
-                                                      int *iters, int *evals, int *fallback) {
+// Learned gas-pressure seeds of n cells: wavefront-collective BF16 MFMA inference.  A
+// kernel of its own so that the FP64 solver kernel keeps its own register budget.
+__global__ void __launch_bounds__(256) k_mlp_seed(MlpK K, int n, const double *__restrict__ cons,
+                                                  const double *__restrict__ bcc,
+                                                  double *__restrict__ seed) {
   const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool live = q < n;
-  double c[5] = {1.0, 0.0, 0.0, 0.0, 2.0}, p[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
-  double b[3] = {0.0, 0.0, 0.0};
+  double c[5] = {1.0, 0.0, 0.0, 0.0, 2.0}, b[3] = {0.0, 0.0, 0.0};
   if (live) {
-    for (int v = 0; v < 5; ++v) { c[v] = cons[q * 5 + v]; p[v] = prim[q * 5 + v]; }
+    for (int v = 0; v < 5; ++v) c[v] = cons[q * 5 + v];
     for (int v = 0; v < 3; ++v) b[v] = bcc[q * 3 + v];
   }
-  double seed = 0.0;
-  if (MODE == C2P_LEARNED) {
-    float x[MLP_IN];
-    mlp_features(c, b, K.P, x);
-    seed = mlp_seed(c, mlp_wave_infer_bf16(K, x));
-  }
-  if (!live) return;
+  float x[MLP_IN];
+  mlp_features(c, b, K.P, x);
+  const double s = mlp_seed(c, mlp_wave_infer_bf16(K, x));
+  if (live) seed[q] = s;
+}
+
+// Batched SR primitive recovery with the solver mode of physics.hpp (reference NH, Newton
+// from the register seed, or Newton from the learned seed in `seed`).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_sr_c2p_modes(EosK eos, int n, double *cons,
+                                                      double *prim, const double *bcc,
+                                                      const double *__restrict__ seed,
+                                                      int *iters, int *evals, int *fallback) {
+  const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double c[5], cin[5], p[5], b[3];
+  for (int v = 0; v < 5; ++v) { c[v] = cin[v] = cons[q * 5 + v]; p[v] = prim[q * 5 + v]; }
+  for (int v = 0; v < 3; ++v) b[v] = bcc[q * 3 + v];
   C2PStats st;
-  const int it = sr_cons2prim_cell(c, p, b, eos, MODE, seed, &st);
+  const int it = sr_cons2prim_cell(c, p, b, eos, MODE, MODE == C2P_LEARNED ? seed[q] : 0.0, &st);
   if (iters) iters[q] = it;
   if (evals) evals[q] = st.evals;
   if (fallback) fallback[q] = st.fallback;
-  for (int v = 0; v < 5; ++v) { cons[q * 5 + v] = c[v]; prim[q * 5 + v] = p[v]; }
+  for (int v = 0; v < 5; ++v) {
+    if (c[v] != cin[v]) cons[q * 5 + v] = c[v];   // only where floors changed it
+    prim[q * 5 + v] = p[v];
+  }
 }
 
 }  // namespace
@@ -275,6 +286,8 @@ struct ath_mlp {
   float *P = nullptr;    // parameters + normalisation (MLP_NP floats)
   uint2 *pk = nullptr;   // packed bf16 weights for the MFMA inference
   float *Mo = nullptr, *Vo = nullptr;
+  double *seed = nullptr;   // learned seeds of the batched recovery (grow-only)
+  long nseed = 0;
   long step = 0;
   hipStream_t stream = nullptr;
   MlpK k() const {
@@ -402,6 +415,7 @@ int ath_ml_destroy(ath_mlp_t m) {
   if (m->pk) (void)hipFree(m->pk);
   if (m->Mo) (void)hipFree(m->Mo);
   if (m->Vo) (void)hipFree(m->Vo);
+  if (m->seed) (void)hipFree(m->seed);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -515,20 +529,28 @@ int ath_gpu_sr_con2prim_device(ath_mlp_t mlp, int mode, int n, double gamma, dou
   if (mode == 2 && !mlp) return mfail("the learned mode needs an MLP");
   if (n <= 0) return 0;
   if (!cons || !prim || !bcc) return mfail("null argument");
-  MlpK K;
-  if (mlp) K = mlp->k();
   const EosK eos = make_eos(gamma, dfloor, pfloor, sigma_max, beta_min, gamma_max);
   const dim3 grid((n + 255) / 256), wg(256);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (mode == C2P_REFERENCE)
-    hipLaunchKernelGGL(k_sr_c2p_modes<C2P_REFERENCE>, grid, wg, 0, s, K, eos, n, cons, prim, bcc,
-                       iters, evals, fallback);
-  else if (mode == C2P_NEWTON)
-    hipLaunchKernelGGL(k_sr_c2p_modes<C2P_NEWTON>, grid, wg, 0, s, K, eos, n, cons, prim, bcc,
-                       iters, evals, fallback);
-  else
-    hipLaunchKernelGGL(k_sr_c2p_modes<C2P_LEARNED>, grid, wg, 0, s, K, eos, n, cons, prim, bcc,
-                       iters, evals, fallback);
+  if (mode == C2P_REFERENCE) {
+    hipLaunchKernelGGL(k_sr_c2p_modes<C2P_REFERENCE>, grid, wg, 0, s, eos, n, cons, prim, bcc,
+                       nullptr, iters, evals, fallback);
+  } else if (mode == C2P_NEWTON) {
+    hipLaunchKernelGGL(k_sr_c2p_modes<C2P_NEWTON>, grid, wg, 0, s, eos, n, cons, prim, bcc,
+                       nullptr, iters, evals, fallback);
+  } else {
+    if (mlp->nseed < n) {   // grows once; later calls of this size allocate nothing
+      MCHK(hipStreamSynchronize(s));
+      if (mlp->seed) MCHK(hipFree(mlp->seed));
+      mlp->seed = nullptr;
+      mlp->nseed = 0;
+      MCHK(hipMalloc(reinterpret_cast<void **>(&mlp->seed), sizeof(double) * n));
+      mlp->nseed = n;
+    }
+    hipLaunchKernelGGL(k_mlp_seed, grid, wg, 0, s, mlp->k(), n, cons, bcc, mlp->seed);
+    hipLaunchKernelGGL(k_sr_c2p_modes<C2P_LEARNED>, grid, wg, 0, s, eos, n, cons, prim, bcc,
+                       mlp->seed, iters, evals, fallback);
+  }
   MCHK(hipGetLastError());
   return 0;
 }

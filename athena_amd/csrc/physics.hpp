@@ -474,7 +474,9 @@ __device__ inline double nh_g(const NHC &c, double p, double gamma_adi, double *
     if (!aclamp) {
       const double dsda = -1.5 * s / a;
       const double dphida = -dsda / sqrt(dmax(1.0 - s * s, 1.0e-300));
-      deee = 1.0 / 3.0 - 2.0 / 3.0 * cth + 2.0 / 3.0 * a * sin(th) * (2.0 / 3.0) * dphida;
+      // th = 2/3 (phi + pi) lies in [2pi/3, pi] for phi = acos(s) in [0, pi/2]: sin >= 0
+      const double sth = sqrt(dmax(1.0 - cth * cth, 0.0));
+      deee = 1.0 / 3.0 - 2.0 / 3.0 * cth + 2.0 / 3.0 * a * sth * (2.0 / 3.0) * dphida;
     }
     double dvsq = 0.0;
     if (!vclamp) {

@@ -30,7 +30,11 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_ml_param_count", "ath_ml_create", "ath_ml_destroy", "ath_ml_set_params",
            "ath_ml_get_params", "ath_ml_grad", "ath_ml_train", "ath_ml_infer",
            "ath_ml_features", "ath_ml_last_error", "ath_gpu_sr_con2prim_modes",
-           "ath_gpu_sr_con2prim_device", "ath_gpu_mesh_set_c2p", "ath_gpu_mesh_c2p_stats"]
+           "ath_gpu_sr_con2prim_device", "ath_gpu_mesh_set_c2p", "ath_gpu_mesh_c2p_stats",
+           "ath_gpu_ensemble_create", "ath_gpu_ensemble_destroy", "ath_gpu_ensemble_size",
+           "ath_gpu_ensemble_cycle", "ath_gpu_ensemble_get_tiles",
+           "ath_gpu_ensemble_set_tiles", "ath_gpu_ensemble_autotune",
+           "ath_gpu_ensemble_stream"]
 NKERNELS = 18
 
 
@@ -104,6 +108,14 @@ def load():
         [P, P, P, P, P, P, P]
     L.ath_gpu_mesh_set_c2p.argtypes = [P, P, I]
     L.ath_gpu_mesh_c2p_stats.argtypes = [P, I, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.ath_gpu_ensemble_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
+    L.ath_gpu_ensemble_destroy.argtypes = [P]
+    L.ath_gpu_ensemble_size.argtypes = [P, IP, ctypes.POINTER(ctypes.c_long)]
+    L.ath_gpu_ensemble_cycle.argtypes = [P, I, I]
+    L.ath_gpu_ensemble_get_tiles.argtypes = [P, IP, IP]
+    L.ath_gpu_ensemble_set_tiles.argtypes = [P, IP, IP]
+    L.ath_gpu_ensemble_autotune.argtypes = [P, I, D]
+    L.ath_gpu_ensemble_stream.argtypes = [P, ctypes.POINTER(P)]
     _lib = L
     return L
 
@@ -276,6 +288,62 @@ def sr_con2prim_batch(cons, prim_seed, bcc, gamma, dfloor, pfloor, sigma_max=0.0
         cons.shape[0], gamma, dfloor, pfloor, sigma_max, beta_min, gamma_max, _dp(cons),
         _dp(prim), _dp(bcc), iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
     return cons, prim, iters
+
+
+class Ensemble:
+    """Independent meshes advanced together (ath_gpu_ensemble_*).  The members stay
+    usable for get/set of registers between calls."""
+
+    KINDS = ("flux", "corner", "update", "cons2prim")
+    TILES = ("linear", "64x4x1", "32x8x1", "16x16x1", "16x4x4", "8x8x4")
+
+    def __init__(self, meshes):
+        L = load()
+        self.meshes = list(meshes)
+        arr = (ctypes.c_void_p * len(self.meshes))(*[m.h.value for m in self.meshes])
+        h = ctypes.c_void_p()
+        _check(L.ath_gpu_ensemble_create(arr, len(self.meshes), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().ath_gpu_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self):
+        n, c = ctypes.c_int(), ctypes.c_long()
+        _check(load().ath_gpu_ensemble_size(self.h, ctypes.byref(n), ctypes.byref(c)))
+        return n.value, c.value
+
+    def run(self, ncycles, graph=True):
+        _check(load().ath_gpu_ensemble_cycle(self.h, int(ncycles), int(bool(graph))))
+
+    def tiles(self):
+        t = np.zeros(4, dtype=np.int32)
+        p = np.zeros(4, dtype=np.int32)
+        _check(load().ath_gpu_ensemble_get_tiles(self.h, _ip(t), _ip(p)))
+        return t, p
+
+    def set_tiles(self, tile_idx, persist):
+        t = np.ascontiguousarray(tile_idx, dtype=np.int32)
+        p = np.ascontiguousarray(persist, dtype=np.int32)
+        _check(load().ath_gpu_ensemble_set_tiles(self.h, _ip(t), _ip(p)))
+
+    def autotune(self, reps=3):
+        best = np.zeros(4)
+        _check(load().ath_gpu_ensemble_autotune(self.h, int(reps), _dp(best)))
+        return best
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _check(load().ath_gpu_ensemble_stream(self.h, ctypes.byref(s)))
+        return s.value
 
 
 C2P_MODES = {"reference": 0, "newton": 1, "learned": 2}

@@ -56,24 +56,37 @@ class HipEvents:
 
 
 def kernel_algorithmic_bytes(name, m):
-    """Compulsory HBM bytes of one launch of a kernel over all blocks (doubles touched
-    once: inputs read + outputs written), per the same accounting as the 536 B/zone-
-    cycle figure.  Only the flux kernels are needed for the roofline line."""
+    """Compulsory HBM bytes of one launch of a production kernel over all MeshBlocks of
+    mesh m: every array element the kernel must read or write, counted once (no
+    re-reads, no cache effects).  Per MeshBlock with n active cells per direction and
+    nc = n + 2*NGHOST:
+      flux_xyz_*:  w, bcc (8 doubles) on the stencil box (n+2e)^3 (e = 1 DC, 2 PLM,
+                   3 PPM), normal face fields (3), and per face flux(5)+EMF(2)+weight(1)
+                   on 3 (n+1)(n+2)^2 faces;
+      corner_emf:  v, bcc (6) on (n+2)^3 cells, 6 face EMFs + 3 weights on the faces,
+                   3 edge EMFs written;
+      update_u_b_ct: U^n (5) and 3 face fields on the active region, 15 flux and 3 edge
+                   components, U (5) and B (3) written including the ghost copies;
+      cons2prim:   U (5) and faces (3) read, W (5) and bcc (3) written on nc^3 cells."""
     n1, n2, n3 = m.bnx
+    assert n1 == n2 == n3
+    n = n1
+    nc = n + 2 * m.ng
     nb = m.nblocks
-    if name.startswith("flux_x"):
-        d = int(name[6]) - 1
-        order = name.split("_")[2]
-        ext = {"dc": (1, 0), "plm": (2, 1), "ppm": (3, 2)}[order]   # stencil below/above
-        faces = [n1 + 1, n2 + 2, n3 + 2] if d == 0 else ([n1 + 2, n2 + 1, n3 + 2] if d == 1
-                                                        else [n1 + 2, n2 + 2, n3 + 1])
-        cells = list(faces)
-        cells[d] = [n1, n2, n3][d] + ext[0] + ext[1]
-        nface = faces[0] * faces[1] * faces[2]
-        ncell = cells[0] * cells[1] * cells[2]
-        # reads: w(5) + bcc(3) per cell, bx per face; writes: flux(5) + 2 EMF + weight
-        return 8 * nb * (8 * ncell + 1 * nface + 8 * nface)
-    return None
+    faces = 3 * (n + 1) * (n + 2) ** 2
+    if name.startswith("flux_xyz_"):
+        e = {"dc": 1, "plm": 2, "ppm": 3}[name.split("_")[2]]
+        per = 8 * (n + 2 * e) ** 3 + faces + 8 * faces
+    elif name == "corner_emf":
+        per = 6 * (n + 2) ** 3 + 9 * (n + 1) * (n + 2) ** 2 + 3 * (n + 1) ** 2 * (n + 2)
+    elif name == "update_u_b_ct":
+        per = (5 * n ** 3 + 15 * (n + 1) * n * n + 3 * (n + 1) ** 2 * n + 3 * (n + 1) * n * n
+               + 5 * nc ** 3 + 3 * (nc + 1) * nc * nc)
+    elif name == "cons2prim":
+        per = 5 * nc ** 3 + 3 * (nc + 1) * nc * nc + 8 * nc ** 3
+    else:
+        return None
+    return 8 * nb * per
 
 
 def cpu_baseline(sample_blocks=(4, 4, 4), ncycles=2):
@@ -101,19 +114,303 @@ def cpu_baseline(sample_blocks=(4, 4, 4), ncycles=2):
                       % (n, m.nblocks, ncycles, dt)}
 
 
+C2P_BYTES_PER_SOLVE = 112   # read D, M, E, Bcc and a seed; write 5 primitives
+C2P_METRIC = ("SR con2prim cell-solves/s on 1 MI355X at the 1e-12 tolerance, learned "
+              "guess (BF16 MFMA MLP + FP64 Newton, plain fallback)")
+
+
+def bench_sr_c2p(args):
+    """Batched SR primitive recovery of 128^3 states from the learned-guess distribution,
+    cold (no usable previous pressure), in the three solver modes."""
+    import torch
+    from athena_amd import gpu, ml
+    from athena_amd.mesh import DEFAULT_FLOOR as FL
+    n = args.c2p_n ** 3
+    net = ml.load_mlp()
+    cons, bcc, _ = ml.conserved_batch(n, 20261203)
+    dev = torch.device("cuda:0")
+    C0 = torch.from_numpy(cons).to(dev)
+    B = torch.from_numpy(bcc).to(dev)
+    C, P = torch.empty_like(C0), torch.empty_like(C0)
+    it = torch.empty(n, dtype=torch.int32, device=dev)
+    evs, fbs = torch.empty_like(it), torch.empty_like(it)
+    ev = HipEvents()
+    res = {}
+    for mode, name in ((0, "reference"), (1, "newton"), (2, "learned")):
+        total = 0.0
+        # untimed pass with the per-cell counters, then timed passes without them
+        for r in range(1 + args.warmup + args.steps):
+            C.copy_(C0)
+            P.zero_()                       # cold: the seed register holds nothing
+            torch.cuda.synchronize()
+            e0, e1 = ev.new(), ev.new()
+            ev.record(e0, 0)
+            gpu.sr_con2prim_device(mode, n, C.data_ptr(), P.data_ptr(), B.data_ptr(),
+                                   ml.GAMMA_AD, FL, FL, mlp=net,
+                                   iters=it.data_ptr() if r == 0 else None,
+                                   evals=evs.data_ptr() if r == 0 else None,
+                                   fallback=fbs.data_ptr() if r == 0 else None)
+            ev.record(e1, 0)
+            ms = ev.elapsed_ms(e0, e1)
+            if r > args.warmup:
+                total += ms
+        e = evs.cpu().numpy().astype(np.float64)
+        res[name] = {"cell_solves_per_s": n * args.steps / (total * 1e-3),
+                     "ms_per_call": total / args.steps,
+                     "mean_evaluations": float(e.mean()),
+                     "p99_evaluations": float(np.percentile(e, 99)),
+                     "fallback_rate": float(fbs.cpu().numpy().mean()),
+                     "failed_rate": float((it.cpu().numpy() < 0).mean())}
+        print("  %-9s %.4f ms/call  %.3e solves/s  mean evals %.3f p99 %d"
+              % (name, res[name]["ms_per_call"], res[name]["cell_solves_per_s"],
+                 e.mean(), np.percentile(e, 99)), file=sys.stderr, flush=True)
+    lr = res["learned"]
+    achieved = C2P_BYTES_PER_SOLVE * lr["cell_solves_per_s"]
+    roof = {"kernel": "k_mlp_seed + k_sr_c2p_modes<2> (one call)", "bound": "hbm",
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "avg_launch_ms": lr["ms_per_call"],
+            "algorithmic_bytes_per_launch": C2P_BYTES_PER_SOLVE * n,
+            "note": "FP64-VALU bound (transcendentals and iteration count), not HBM"}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import sr_con2prim as ora_c2p, lib as oralib
+        t0 = time.perf_counter()
+        reps = 0
+        while True:                         # about 10 s of host work
+            ora_c2p(cons, np.zeros((n, 5)), bcc, ml.GAMMA_AD, FL, FL)
+            reps += 1
+            dt = time.perf_counter() - t0
+            if dt > 10.0 or reps >= 200:
+                break
+        cpu = {"value": reps * n / dt, "unit": "cell-solves/s",
+               "cores": int(oralib().ora_threads()), "kind": "port",
+               "sample": "the same %d states x %d passes, reference iteration from a "
+                         "cold seed (%.1f s)" % (n, reps, dt)}
+    out = {"metric": C2P_METRIC, "value": lr["cell_solves_per_s"], "unit": "cell-solves/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": lr["ms_per_call"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (solve), bf16 MFMA (guess)",
+           "data": "synthetic",
+           "config": {"workload": "SR MHD con2prim, %d^3 states: Lorentz factor log-uniform "
+                                  "[1,10], sigma=b^2/rho log-uniform [0.01,10], rho "
+                                  "log-uniform [1e-3,1], p/rho log-uniform [1e-2,10], "
+                                  "gamma 4/3, isotropic directions, cold seed"
+                                  % args.c2p_n, "states": n},
+           "modes": res,
+           "speedup_vs_reference": lr["cell_solves_per_s"] / res["reference"]["cell_solves_per_s"],
+           "speedup_vs_newton": lr["cell_solves_per_s"] / res["newton"]["cell_solves_per_s"],
+           "evaluations_saved_vs_reference": res["reference"]["mean_evaluations"] - lr["mean_evaluations"],
+           "evaluations_saved_vs_newton": res["newton"]["mean_evaluations"] - lr["mean_evaluations"],
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+ENS_METRIC = ("zone-cycles/s on 1 MI355X, ensemble of 48 MHD blast waves of uneven sizes "
+              "(VL2+PLM+HLLD FP64, one persistent launch per stage kernel, HIP graph)")
+KIND_KERNELS = ("flux_xyz_dc", "corner_emf", "update_u_b_ct", "cons2prim")
+
+
+def bench_ensemble(args):
+    from athena_amd.gpu import Ensemble, GpuMesh
+    from athena_amd.pgen.blast import blast_ensemble
+    t0 = time.perf_counter()
+    members = blast_ensemble(args.members)
+    meshes = []
+    for m, ic in members:
+        d = GpuMesh(m)
+        d.load([ic.block(g) for g in range(m.nblocks)])
+        d.initialize()
+        meshes.append(d)
+    ens = Ensemble(meshes)
+    nmem, cells = ens.size()
+    print("setup %.1f s: %d members, %d active cells" % (time.perf_counter() - t0, nmem,
+                                                         cells), file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    best = ens.autotune(reps=3)
+    tiles, persist = ens.tiles()
+    print("autotune %.1f s: tiles %s persist %s" % (time.perf_counter() - t1,
+                                                     [Ensemble.TILES[t] for t in tiles],
+                                                     list(persist)), file=sys.stderr, flush=True)
+    ens.run(args.warmup)
+    ev = HipEvents()
+    stream = ens.stream()
+    e0, e1 = ev.new(), ev.new()
+    ev.record(e0, stream)
+    ens.run(args.steps)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1)
+    zcs = args.steps * cells / (ms * 1e-3)
+    print("timed %d cycles: %.3f ms/cycle, %.3e zone-cycles/s" % (args.steps, ms / args.steps,
+                                                                   zcs), file=sys.stderr,
+          flush=True)
+    # per-kind launch times measured by the autotuner (stage-1 launches over all members)
+    kinds = {}
+    for k, name in enumerate(KIND_KERNELS):
+        ab = sum(kernel_algorithmic_bytes(name, m) for m, _ in members)
+        kinds[name] = {"avg_launch_ms": best[k], "algorithmic_bytes_per_launch": ab,
+                       "achieved_GBps": ab / (best[k] * 1e-3) / 1e9}
+    dom = max(kinds, key=lambda k: kinds[k]["avg_launch_ms"])
+    roof = {"kernel": dom + " (ensemble launch)", "bound": "hbm",
+            "achieved": kinds[dom]["achieved_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": kinds[dom]["achieved_GBps"] * 1e9 / HBM_PEAK, "traffic": None,
+            "avg_launch_ms": kinds[dom]["avg_launch_ms"],
+            "algorithmic_bytes_per_launch": kinds[dom]["algorithmic_bytes_per_launch"]}
+    out = {"metric": ENS_METRIC, "value": zcs, "unit": "zone-cycles/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "%d 3D MHD blast waves, edges from {32,48,64,96,128}, "
+                                  "MeshBlocks 16^3 or 32^3, P_in/P_out log-uniform "
+                                  "[10,1000], |B0| uniform [0.1,1] isotropic, VL2+PLM+HLLD+CT"
+                                  % nmem, "members": nmem, "cells": cells,
+                      "tiles": [Ensemble.TILES[t] for t in tiles],
+                      "persistent_grid": [int(p) for p in persist]},
+           "kernels": kinds, "roofline": roof,
+           "step_roofline": {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE,
+                             "achieved": zcs * BYTES_PER_ZONE_CYCLE / 1e9, "unit": "GB/s",
+                             "frac": zcs * BYTES_PER_ZONE_CYCLE / HBM_PEAK},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
+SHEAR_METRIC = ("zone-cycles/s on 1 MI355X, 3D SR MHD shear layer (VL2+PPM+HLLE+CT FP64, "
+                "learned con2prim guess)")
+
+
+def bench_sr_shear(args):
+    """The SR shear layer at 256^3 in 32^3 MeshBlocks (NGHOST 3, PPM + HLLE + CT), run in
+    each primitive-recovery mode from the same initial state for the same cycles."""
+    from athena_amd.gpu import GpuMesh, C2P_MODES
+    from athena_amd.pgen.sr import ShearLayer, shear_mesh
+    from athena_amd import ml
+    m = shear_mesh(args.n, args.bn)
+    ic = ShearLayer(m)
+    blocks = [ic.block(g) for g in range(m.nblocks)]
+    net = ml.load_mlp()
+    cells = m.nx[0] * m.nx[1] * m.nx[2]
+    ev = HipEvents()
+    runs, finals = {}, {}
+    for name in ("learned", "reference", "newton"):
+        d = GpuMesh(m)
+        d.load(blocks)
+        d.set_c2p(C2P_MODES[name], net if name == "learned" else None)
+        d.initialize()
+        d.run(args.warmup)
+        d.c2p_stats(1)
+        stream = d.stream()
+        e0, e1 = ev.new(), ev.new()
+        d.synchronize()
+        ev.record(e0, stream)
+        d.run(args.steps)
+        ev.record(e1, stream)
+        ms = ev.elapsed_ms(e0, e1)
+        evals, fb, nsolve = d.c2p_stats(0)
+        d.kernel_timing(True)
+        d.run(2)
+        stats = d.kernel_stats()
+        d.kernel_timing(False)
+        zcs = args.steps * cells / (ms * 1e-3)
+        runs[name] = {"zone_cycles_per_s": zcs, "ms_per_step": ms / args.steps,
+                      "evaluations_per_solve": evals / max(nsolve, 1),
+                      "fallback_rate": fb / max(nsolve, 1), "solves": nsolve,
+                      "kernels_ms_per_cycle": {k: v[0] / 2 for k, v in stats.items()},
+                      "kernel_launches": {k: v[1] for k, v in stats.items()}}
+        finals[name] = [d.get(g, "u") for g in range(m.nblocks)]
+        print("  %-9s %.3f ms/cycle %.3e zc/s, %.3f evaluations/solve, fallback %.2e"
+              % (name, ms / args.steps, zcs, evals / max(nsolve, 1), fb / max(nsolve, 1)),
+              file=sys.stderr, flush=True)
+        d.close()
+    ng = m.ng
+    diffs = {}
+    for name in ("learned", "newton"):
+        num = den = 0.0
+        for a, b in zip(finals[name], finals["reference"]):
+            sa = a[:, ng:-ng, ng:-ng, ng:-ng]
+            sb = b[:, ng:-ng, ng:-ng, ng:-ng]
+            num += np.abs(sa - sb).sum(axis=(1, 2, 3))
+            den += np.abs(sb).sum(axis=(1, 2, 3))
+        diffs[name] = [float(x) for x in num / den]
+    lr = runs["learned"]
+    kms = lr["kernels_ms_per_cycle"]
+    dom = max(kms, key=kms.get)
+    ab = kernel_algorithmic_bytes(dom, m)
+    avg = kms[dom] * 2 / lr["kernel_launches"][dom]
+    roof = {"kernel": dom, "bound": "hbm",
+            "achieved": ab / (avg * 1e-3) / 1e9 if ab else None, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": ab / (avg * 1e-3) / HBM_PEAK if ab else None,
+            "traffic": None, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": ab}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_sr()
+    out = {"metric": SHEAR_METRIC, "value": lr["zone_cycles_per_s"], "unit": "zone-cycles/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": lr["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (bf16 MFMA guess)", "data": "synthetic",
+           "config": {"workload": "3D SR MHD shear layer %d^3 as %d MeshBlocks of %d^3, "
+                                  "NGHOST 3, v_x=+-0.5 tanh width 0.01, v_y noise 1e-3, "
+                                  "rho=1, P=1, B_x=0.1, VL2+PPM+HLLE+CT"
+                                  % (args.n, m.nblocks, args.bn), "cells": cells},
+           "modes": runs, "rel_l1_vs_reference_mode": diffs,
+           "speedup_learned_vs_reference": lr["zone_cycles_per_s"]
+           / runs["reference"]["zone_cycles_per_s"],
+           "roofline": roof,
+           "step_roofline": {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE,
+                             "achieved": lr["zone_cycles_per_s"] * BYTES_PER_ZONE_CYCLE / 1e9,
+                             "unit": "GB/s",
+                             "frac": lr["zone_cycles_per_s"] * BYTES_PER_ZONE_CYCLE / HBM_PEAK},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_sr(n=128, ncycles=2):
+    """The oracle on a bounded sample of the SR shear layer: 128^3 as 64 MeshBlocks of
+    32^3, PPM + HLLE, reference primitive recovery."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import OracleMesh, lib as oralib
+    from athena_amd.pgen.sr import ShearLayer, shear_mesh
+    m = shear_mesh(n, 32)
+    ic = ShearLayer(m)
+    o = OracleMesh(m)
+    o.load([ic.block(g) for g in range(m.nblocks)])
+    o.initialize()
+    t0 = time.perf_counter()
+    o.run(ncycles)
+    dt = time.perf_counter() - t0
+    return {"value": ncycles * n ** 3 / dt, "unit": "zone-cycles/s",
+            "cores": int(oralib().ora_threads()), "kind": "port",
+            "sample": "%d^3 cells as %d MeshBlocks of 32^3, %d VL2 cycles (%.2f s)"
+                      % (n, m.nblocks, ncycles, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="turbulence",
+                    choices=["turbulence", "sr_c2p", "ensemble", "sr_shear"],
+                    help="turbulence: the zone-cycles/s throughput case (default); "
+                         "sr_c2p: batched SR primitive recovery with the learned guess; "
+                         "ensemble: 48 blast waves of uneven sizes; sr_shear: the SR shear "
+                         "layer with the learned guess (use --steps 50)")
+    ap.add_argument("--members", type=int, default=48, help="ensemble members")
     ap.add_argument("--n", type=int, default=256, help="mesh cells per direction")
     ap.add_argument("--bn", type=int, default=32, help="MeshBlock cells per direction")
+    ap.add_argument("--c2p-n", type=int, default=128, help="states per direction (sr_c2p)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
               "(run with --gpus 1)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sr_c2p":
+        return bench_sr_c2p(args)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    if args.workload == "sr_shear":
+        return bench_sr_shear(args)
 
     from athena_amd.gpu import GpuMesh
     from athena_amd.pgen.turbulence import Turbulence, turbulence_mesh

@@ -123,6 +123,36 @@ int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
 
 const char *ath_gpu_last_error(void);
 
+/* ---- Ensembles ------------------------------------------------------------------------
+ * Independent meshes of uneven sizes advanced together: each VL2 stage kernel is
+ * launched once for all members (persistent work-groups over a (member, work-group)
+ * map), the cycle can be replayed as one HIP graph, and the thread-to-cell tile shape and
+ * persistent grid of every kernel are chosen by a built-in autotuner.  No counterpart
+ * in the reference, which advances one Mesh per process (main.cpp:457-560); each member
+ * evolves exactly as ath_gpu_cycle would advance it on its own, with its own dt.
+ * Members must share xorder, Riemann solver and dynamics, be initialised
+ * (ath_gpu_initialize) and use the reference primitive recovery; while in an ensemble
+ * they must not run single stages (ath_gpu_vl2_stage), which the ensemble detects. */
+typedef struct ath_ensemble *ath_ensemble_t;
+
+int ath_gpu_ensemble_create(const ath_mesh_t *members, int nmembers, ath_ensemble_t *out);
+int ath_gpu_ensemble_destroy(ath_ensemble_t ens);
+int ath_gpu_ensemble_size(ath_ensemble_t ens, int *nmembers, long *active_cells);
+/* ncycles VL2 cycles of every member (main.cpp:457-560 per member, no tlim check);
+ * use_graph != 0 replays one captured HIP graph per cycle.  Synchronous. */
+int ath_gpu_ensemble_cycle(ath_ensemble_t ens, int ncycles, int use_graph);
+/* Work decomposition per kernel kind (0 fluxes, 1 corner EMFs, 2 update, 3 primitive
+ * recovery): tile_idx picks a thread-to-cell tile shape (0 linear chunks of 256 cells,
+ * 1..5 the tiles 64x4x1, 32x8x1, 16x16x1, 16x4x4, 8x8x4), persist the launch grid of
+ * the persistent kernel (0: one work-group per work item). */
+int ath_gpu_ensemble_get_tiles(ath_ensemble_t ens, int *tile_idx, int *persist);
+int ath_gpu_ensemble_set_tiles(ath_ensemble_t ens, const int *tile_idx, const int *persist);
+/* Times every tile shape and persistent grid per kernel kind on the ensemble's data
+ * (reps launches each), keeps the fastest, and leaves the members' state as it was.
+ * best_ms (4 doubles, may be null) receives the chosen per-launch times. */
+int ath_gpu_ensemble_autotune(ath_ensemble_t ens, int reps, double *best_ms);
+int ath_gpu_ensemble_stream(ath_ensemble_t ens, void **stream);
+
 /* ---- Learned initial guess for SR primitive recovery --------------------------------
  * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds
  * the Newman-Hamlin iteration with the previous gas pressure.  Here a 6 -> 64 -> 64 -> 1

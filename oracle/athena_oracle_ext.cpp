@@ -530,6 +530,7 @@ extern "C" {
 void ora_sr_con2prim(int n, double gamma_adi, double dfloor, double pfloor,
                      double sigma_max, double beta_min, double gamma_max, double *cons,
                      double *prim, const double *bcc, int *iters) {
+#pragma omp parallel for schedule(static)
   for (int q = 0; q < n; ++q)
     iters[q] = ora::Cons2PrimSRCell(cons + 5 * q, prim + 5 * q, bcc + 3 * q, gamma_adi,
                                     dfloor, pfloor, sigma_max, beta_min, gamma_max);
