@@ -297,7 +297,8 @@ __device__ __forceinline__ void flux_face(const MeshK &M, int g, int k, int j, i
   double f[NWAVE];
   if (RS == 0) hlld(wl, wr, bx, M.eos, f);
   else if (RS == 1) hlle(wl, wr, bx, M.eos, f);
-  else sr_hlle(wl, wr, bx, M.eos, f);
+  else if (RS == 2) sr_hlle(wl, wr, bx, M.eos, f);
+  else sr_hlld(wl, wr, bx, M.eos, f);
 
   double *fb = o.flx + static_cast<long>(g) * NHYDRO * nf;
   fb[IDN * nf + fi] = f[IDN];
@@ -341,7 +342,7 @@ __device__ __forceinline__ void flux3_body(const MeshK &M, const double *__restr
                                            const FaceOut &o2, const FaceOut &o3, double dt,
                                            int lw, int nwg, TileK T) {
   int g, i, j, k;
-  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 3, M.nx2 + 3, M.nx3 + 3, T, g, i, j, k)) return;
+  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 2, M.nx2 + 2, M.nx3 + 2, T, g, i, j, k)) return;
   i += M.is - 1;
   j += M.js - 1;
   k += M.ks - 1;
@@ -1033,7 +1034,8 @@ __global__ void k_riemann_batch(int solver, EosK eos, int n, const double *wl,
   for (int v = 0; v < NWAVE; ++v) { a[v] = wl[q * NWAVE + v]; b[v] = wr[q * NWAVE + v]; }
   if (solver == 0) hlld(a, b, bx[q], eos, f);
   else if (solver == 1) hlle(a, b, bx[q], eos, f);
-  else sr_hlle(a, b, bx[q], eos, f);
+  else if (solver == 2) sr_hlle(a, b, bx[q], eos, f);
+  else sr_hlld(a, b, bx[q], eos, f);
   for (int v = 0; v < NWAVE; ++v) out[q * NWAVE + v] = f[v];
 }
 
@@ -1161,11 +1163,17 @@ FaceOut face_out(ath_mesh *m, int dir) {
   return o;
 }
 
+// Riemann solver template index: 0 HLLD, 1 HLLE (Newtonian), 2 SR HLLE, 3 SR HLLD.
+int solver_code(const ath_mesh_desc &d) {
+  if (d.relativistic) return (d.rsolver == 0) ? 3 : 2;
+  return d.rsolver;
+}
+
 // Work kinds of the stage kernels and their position boxes per MeshBlock.
 enum WorkKind { KK_FLUX = 0, KK_CORNER = 1, KK_UPDATE = 2, KK_C2P = 3, KK_N = 4 };
 
 void kind_box(const MeshK &M, int kind, int *n) {
-  if (kind == KK_FLUX) { n[0] = M.nx1 + 3; n[1] = M.nx2 + 3; n[2] = M.nx3 + 3; }
+  if (kind == KK_FLUX) { n[0] = M.nx1 + 2; n[1] = M.nx2 + 2; n[2] = M.nx3 + 2; }
   else if (kind == KK_C2P) { n[0] = M.nc1; n[1] = M.nc2; n[2] = M.nc3; }
   else { n[0] = M.nx1 + 1; n[1] = M.nx2 + 1; n[2] = M.nx3 + 1; }
 }
@@ -1250,7 +1258,7 @@ dim3 kind_grid(const ath_mesh *m, int kind) {
 int launch_flux_args(ath_mesh *m, int order, const double *w, const double *bcc,
                      const FaceOut *fo) {
   const MeshK &M = m->M;
-  const int rs = m->d.relativistic ? 2 : m->d.rsolver;
+  const int rs = solver_code(m->d);
   const dim3 grid = kind_grid(m, KK_FLUX);
   const TileK T = m->tile[KK_FLUX];
   ProfScope ps(m, KID_FLUX3 + (order - 1));
@@ -1265,7 +1273,8 @@ int launch_flux_args(ath_mesh *m, int order, const double *w, const double *bcc,
   } while (0)
   if (rs == 0) LF_O(0);
   else if (rs == 1) LF_O(1);
-  else LF_O(2);
+  else if (rs == 2) LF_O(2);
+  else LF_O(3);
 #undef LF_O
 #undef LF
   HIPCHK(hipGetLastError());
@@ -1281,7 +1290,7 @@ int launch_flux(ath_mesh *m, int order) {
 // One direction per launch (Hydro::CalculateFluxes direction by direction).
 int launch_flux_dirs(ath_mesh *m, int order) {
   const MeshK &M = m->M;
-  const int rs = m->d.relativistic ? 2 : m->d.rsolver;
+  const int rs = solver_code(m->d);
   for (int dir = 0; dir < 3; ++dir) {
     long nface;
     if (dir == 0) nface = static_cast<long>(M.nx1 + 1) * (M.nx2 + 2) * (M.nx3 + 2);
@@ -1302,7 +1311,8 @@ int launch_flux_dirs(ath_mesh *m, int order) {
   do {                                  \
     if (rs == 0) LF_O(D, 0);            \
     else if (rs == 1) LF_O(D, 1);       \
-    else LF_O(D, 2);                    \
+    else if (rs == 2) LF_O(D, 2);       \
+    else LF_O(D, 3);                    \
   } while (0)
     {
       ProfScope ps(m, KID_FLUX + 3 * dir + (order - 1));
@@ -1537,7 +1547,8 @@ int ens_launch_kind(ath_ensemble *e, int kind, int stg) {
   } while (0)
     if (e->rs == 0) LE_O(0);
     else if (e->rs == 1) LE_O(1);
-    else LE_O(2);
+    else if (e->rs == 2) LE_O(2);
+    else LE_O(3);
 #undef LE_O
 #undef LE
   } else if (kind == KK_CORNER) {
@@ -1599,8 +1610,6 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   if (d.xorder == 3 && d.ng < 3)
     return fail("xorder=3 (PPM) reconstruction selected, but nghost < 3");
   if (d.rsolver < 0 || d.rsolver > 1) return fail("rsolver must be 0 (HLLD) or 1 (HLLE)");
-  if (d.relativistic && d.rsolver != 1)
-    return fail("the relativistic path implements the HLLE solver only");
   if (!(d.gamma > 1.0)) return fail("gamma must exceed 1");
   if (!(d.cfl > 0.0) || d.cfl > 0.5) return fail("cfl_number must be in (0, 0.5] for 3-D VL2");
   for (int q = 0; q < 3; ++q)
@@ -1913,7 +1922,7 @@ int ath_gpu_ensemble_create(const ath_mesh_t *members, int n, ath_ensemble_t *ou
     if (!m) { delete e; return fail("null member"); }
     for (auto *o : e->mem)
       if (o == m) { delete e; return fail("a mesh appears twice in the ensemble"); }
-    const int rs = m->d.relativistic ? 2 : m->d.rsolver;
+    const int rs = solver_code(m->d);
     if (q == 0) {
       e->xorder = m->d.xorder;
       e->rs = rs;
@@ -2137,7 +2146,7 @@ int ath_gpu_kernel_stats(ath_mesh_t m, int id, char *name, int namelen, double *
 
 int ath_gpu_riemann_batch(int solver, double gamma, int n, const double *wl,
                           const double *wr, const double *bx, double *flux) {
-  if (solver < 0 || solver > 2) return fail("solver must be 0, 1 or 2");
+  if (solver < 0 || solver > 3) return fail("solver must be 0, 1, 2 or 3");
   if (n <= 0) return 0;
   if (!wl || !wr || !bx || !flux) return fail("null argument");
   EosK eos{};

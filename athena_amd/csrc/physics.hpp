@@ -366,6 +366,263 @@ __device__ __forceinline__ void sr_hlle(const double *wl, const double *wr, doub
 }
 
 //----------------------------------------------------------------------------------------
+// SR HLLD (Mignone, Ugliano & Bodo 2009; hlld_rel.cpp:116-927, special relativity,
+// v_interface = 0), one interface in the rotated frame.  The total pressure of the
+// contact is found by a secant iteration on the jump residual, started from an HLL
+// estimate; any non-finite or non-positive intermediate falls back to the HLL flux.
+
+// Alfven-region state for a trial total pressure (MUB 21-35, 43); side -1 left, +1 right.
+struct SrAlf {
+  double vx, vy, vz, by, bz, eta, kx, ky, kz;
+};
+
+__device__ __forceinline__ SrAlf sr_alfven(const double *r, double lambda, double bbx,
+                                           double ptot, int side) {
+  SrAlf a;
+  const double aa = r[IVX] - lambda * r[IEN] + ptot * (1.0 - sq(lambda));
+  const double gg = sq(r[IBY]) + sq(r[IBZ]);
+  const double cc = r[IVY] * r[IBY] + r[IVZ] * r[IBZ];
+  const double qq = -aa - gg + sq(bbx) * (1.0 - sq(lambda));
+  const double x_inv = 1.0 / (bbx * (aa * lambda * bbx + cc) - (aa + gg) * (lambda * ptot + r[IEN]));
+  a.vx = (bbx * (aa * bbx + lambda * cc) - (aa + gg) * (ptot + r[IVX])) * x_inv;
+  a.vy = (qq * r[IVY] + r[IBY] * (cc + bbx * (lambda * r[IVX] - r[IEN]))) * x_inv;
+  a.vz = (qq * r[IVZ] + r[IBZ] * (cc + bbx * (lambda * r[IVX] - r[IEN]))) * x_inv;
+  a.by = (r[IBY] - bbx * a.vy) / (lambda - a.vx);
+  a.bz = (r[IBZ] - bbx * a.vz) / (lambda - a.vx);
+  const double v_rm = a.vx * r[IVX] + a.vy * r[IVY] + a.vz * r[IVZ];
+  const double wtot = ptot + (r[IEN] - v_rm) / (lambda - a.vx);
+  a.eta = (side < 0) ? -copysign(sqrt(wtot), bbx) : copysign(sqrt(wtot), bbx);
+  const double den_inv = 1.0 / (lambda * ptot + r[IEN] + bbx * a.eta);
+  a.kx = (r[IVX] + ptot + lambda * bbx * a.eta) * den_inv;
+  a.ky = (r[IVY] + r[IBY] * a.eta) * den_inv;
+  a.kz = (r[IVZ] + r[IBZ] * a.eta) * den_inv;
+  return a;
+}
+
+// Jump residual across the contact for a trial pressure (MUB 45, 48, 49).
+__device__ __forceinline__ double sr_contact_res(const SrAlf &l, const SrAlf &rr, double bbx) {
+  const double delta_kx = rr.kx - l.kx + 1.0e-12;
+  const double bbx_c = bbx * delta_kx;
+  const double by_c = rr.by * (rr.kx - rr.vx) - l.by * (l.kx - l.vx) + bbx * (rr.vy - l.vy);
+  const double bz_c = rr.bz * (rr.kx - rr.vx) - l.bz * (l.kx - l.vx) + bbx * (rr.vz - l.vz);
+  const double k_sq_l = sq(l.kx) + sq(l.ky) + sq(l.kz);
+  double kdb = l.kx * bbx_c + l.ky * by_c + l.kz * bz_c;
+  const double y_l = (1.0 - k_sq_l) / (l.eta * delta_kx - kdb);
+  const double k_sq_r = sq(rr.kx) + sq(rr.ky) + sq(rr.kz);
+  kdb = rr.kx * bbx_c + rr.ky * by_c + rr.kz * bz_c;
+  const double y_r = (1.0 - k_sq_r) / (rr.eta * delta_kx - kdb);
+  return delta_kx * (1.0 - bbx * (y_r - y_l));
+}
+
+// Energy residual of a trial total enthalpy and its derivative (Mignone & McKinney 2007
+// A1-A18; hlld_rel.cpp:945-999).
+__device__ __forceinline__ double sr_e_res(double w, double dd, double ee, double m_sq,
+                                           double bb_sq, double ss_sq, double gp) {
+  const double v_sq = (m_sq + ss_sq / sq(w) * (2.0 * w + bb_sq)) / sq(w + bb_sq);
+  const double gamma_sq = 1.0 / (1.0 - v_sq);
+  const double gl = sqrt(gamma_sq);
+  const double chi = (1.0 - v_sq) * (w - gl * dd);
+  const double pgas = chi / gp;
+  const double ee_calc = w - pgas + 0.5 * bb_sq * (1.0 + v_sq) - ss_sq / (2.0 * sq(w));
+  return ee_calc - ee;
+}
+__device__ __forceinline__ double sr_e_res_prime(double w, double dd, double m_sq,
+                                                 double bb_sq, double ss_sq, double gp) {
+  const double v_sq = (m_sq + ss_sq / sq(w) * (2.0 * w + bb_sq)) / sq(w + bb_sq);
+  const double gamma_sq = 1.0 / (1.0 - v_sq);
+  const double gl = sqrt(gamma_sq);
+  const double chi = (1.0 - v_sq) * (w - gl * dd);
+  const double w_cu = sq(w) * w;
+  const double w_b_cu = sq(w + bb_sq) * (w + bb_sq);
+  const double dv_sq_dw = -2.0 / (w_cu * w_b_cu) *
+                          (ss_sq * (3.0 * w * (w + bb_sq) + sq(bb_sq)) + m_sq * w_cu);
+  const double dchi_dw = 1.0 - v_sq - gl / 2.0 * (dd + 2.0 * gl * chi) * dv_sq_dw;
+  const double drho_dw = -gl * dd / 2.0 * dv_sq_dw;
+  const double dpgas_dw = 1.0 / gp * dchi_dw + 0.0 * drho_dw;
+  return 1.0 - dpgas_dw + 0.5 * bb_sq * dv_sq_dw + ss_sq / w_cu;
+}
+
+__device__ inline void sr_hlld(const double *wl, const double *wr, double bbx,
+                               const EosK &eos, double *f) {
+  const double gamma_adi = eos.gamma;
+  const double gp = gamma_adi / (gamma_adi - 1.0);
+  double cons_l[NWAVE], flux_l[NWAVE], cons_r[NWAVE], flux_r[NWAVE];
+  double lpl, lml, lpr, lmr;
+  sr_side(wl, bbx, gamma_adi, cons_l, flux_l, &lpl, &lml);
+  sr_side(wr, bbx, gamma_adi, cons_r, flux_r, &lpr, &lmr);
+  const double lambda_l = dmin(lml, lmr);
+  const double lambda_r = dmax(lpl, lpr);
+  if (lambda_l >= 0.0) {   // the L and R regions need nothing of the fan
+#pragma unroll
+    for (int n = 0; n < NWAVE; ++n) f[n] = flux_l[n];
+    return;
+  }
+  if (lambda_r <= 0.0) {
+#pragma unroll
+    for (int n = 0; n < NWAVE; ++n) f[n] = flux_r[n];
+    return;
+  }
+  double r_l[NWAVE], r_r[NWAVE], cons_hll[NWAVE], flux_hll[NWAVE];
+#pragma unroll
+  for (int n = 0; n < NWAVE; ++n) {
+    r_l[n] = lambda_l * cons_l[n] - flux_l[n];
+    r_r[n] = lambda_r * cons_r[n] - flux_r[n];
+  }
+  const double ldi = 1.0 / (lambda_r - lambda_l);
+#pragma unroll
+  for (int n = 0; n < NWAVE; ++n) {
+    cons_hll[n] = (r_r[n] - r_l[n]) * ldi;
+    flux_hll[n] = (lambda_l * r_r[n] - lambda_r * r_l[n]) * ldi;
+  }
+  double ptot_hll;
+  {
+    const double m_sq = sq(cons_hll[IVX]) + sq(cons_hll[IVY]) + sq(cons_hll[IVZ]);
+    const double bb_sq = sq(bbx) + sq(cons_hll[IBY]) + sq(cons_hll[IBZ]);
+    const double m_dot_bb = cons_hll[IVX] * bbx + cons_hll[IVY] * cons_hll[IBY] +
+                            cons_hll[IVZ] * cons_hll[IBZ];
+    const double ss_sq = sq(m_dot_bb);
+    const double a1 = 4.0 / 3.0 * (bb_sq - cons_hll[IEN]);
+    const double a0 = 0.33333333333333333 * (m_sq + bb_sq * (bb_sq - 2.0 * cons_hll[IEN]));
+    const double s2 = sq(a1) - 4.0 * a0;
+    const double s = (s2 < 0.0) ? 0.0 : sqrt(s2);
+    double w = (s2 >= 0.0 && a1 >= 0.0) ? -2.0 * a0 / (a1 + s) : 0.5 * (-a1 + s);
+    double res = sr_e_res(w, cons_hll[IDN], cons_hll[IEN], m_sq, bb_sq, ss_sq, gp);
+    for (int n = 0; n < 2; ++n) {
+      const double der = sr_e_res_prime(w, cons_hll[IDN], m_sq, bb_sq, ss_sq, gp);
+      const double delta = -res / der;
+      w = w + delta;
+      res = sr_e_res(w, cons_hll[IDN], cons_hll[IEN], m_sq, bb_sq, ss_sq, gp);
+    }
+    const double v_sq = (m_sq + ss_sq / sq(w) * (2.0 * w + bb_sq)) / sq(w + bb_sq);
+    const double gamma_lor_sq = 1.0 / (1.0 - v_sq);
+    const double gamma_lor = sqrt(gamma_lor_sq);
+    const double chi = (1.0 - v_sq) * (w - gamma_lor * cons_hll[IDN]);
+    const double pgas = 1.0 / gp * chi;
+    const double vx = (cons_hll[IVX] + m_dot_bb / w * bbx) / (w + bb_sq);
+    const double vy = (cons_hll[IVY] + m_dot_bb / w * cons_hll[IBY]) / (w + bb_sq);
+    const double vz = (cons_hll[IVZ] + m_dot_bb / w * cons_hll[IBZ]) / (w + bb_sq);
+    const double v_bb = vx * bbx + vy * cons_hll[IBY] + vz * cons_hll[IBZ];
+    const double b_sq = bb_sq / gamma_lor_sq + sq(v_bb);
+    ptot_hll = pgas + 0.5 * b_sq;
+  }
+  double ptot_init;
+  if (sq(bbx) / ptot_hll < 0.01) {   // weak field: quadratic estimate (MUB 55)
+    const double a1 = cons_hll[IEN] - flux_hll[IVX];
+    const double a0 = cons_hll[IVX] * flux_hll[IEN] - flux_hll[IVX] * cons_hll[IEN];
+    const double s2 = sq(a1) - 4.0 * a0;
+    const double s = (s2 < 0.0) ? 0.0 : sqrt(s2);
+    ptot_init = (s2 >= 0.0 && a1 >= 0.0) ? -2.0 * a0 / (a1 + s) : (-a1 + s) / 2.0;
+  } else {
+    ptot_init = ptot_hll;
+  }
+  bool to_hlle = !isfinite(ptot_init) || ptot_init <= 0.0;
+  SrAlf al = sr_alfven(r_l, lambda_l, bbx, ptot_init, -1);
+  SrAlf ar = sr_alfven(r_r, lambda_r, bbx, ptot_init, +1);
+  const double res_0 = sr_contact_res(al, ar, bbx);
+  const double ptot_1 = ptot_init * (1.0 + 1.0e-5);
+  al = sr_alfven(r_l, lambda_l, bbx, ptot_1, -1);
+  ar = sr_alfven(r_r, lambda_r, bbx, ptot_1, +1);
+  double ptot_last = ptot_init, ptot_n = ptot_1, res_last = res_0;
+  double res_n = sr_contact_res(al, ar, bbx);
+  const double tol_ptot = 1.0e-8 * ptot_init;
+  for (int n = 0; n < 4; ++n) {
+    const double ptot_old = ptot_last;
+    ptot_last = ptot_n;
+    const double res_old = res_last;
+    res_last = res_n;
+    const bool iterate = fabs(res_last) > 1.0e-12 && fabs(ptot_last - ptot_old) > tol_ptot;
+    ptot_n = iterate ? (res_last * ptot_old - res_old * ptot_last) / (res_last - res_old)
+                     : ptot_last;
+    al = sr_alfven(r_l, lambda_l, bbx, ptot_n, -1);
+    ar = sr_alfven(r_r, lambda_r, bbx, ptot_n, +1);
+    res_n = sr_contact_res(al, ar, bbx);
+  }
+  const double ptot_c = ptot_n;
+  const double lambda_al = al.kx, lambda_ar = ar.kx;
+  if (!isfinite(lambda_al) || !isfinite(lambda_ar) || !isfinite(ptot_c) || ptot_c <= 0.0)
+    to_hlle = true;
+  const double *fi = flux_hll;
+  double fsel[NWAVE], csel[NWAVE];
+  if (!to_hlle) {
+    // aL / aR states (MUB 32-34) and fluxes (MUB 11-12); contact (MUB 45-52)
+    double cons_al[NWAVE], cons_ar[NWAVE];
+    cons_al[IBY] = al.by; cons_al[IBZ] = al.bz;
+    cons_ar[IBY] = ar.by; cons_ar[IBZ] = ar.bz;
+    const double v_bb_al = al.vx * bbx + al.vy * cons_al[IBY] + al.vz * cons_al[IBZ];
+    cons_al[IDN] = r_l[IDN] / (lambda_l - al.vx);
+    cons_al[IEN] = (r_l[IEN] + ptot_c * al.vx - v_bb_al * bbx) / (lambda_l - al.vx);
+    cons_al[IVX] = (cons_al[IEN] + ptot_c) * al.vx - v_bb_al * bbx;
+    cons_al[IVY] = (cons_al[IEN] + ptot_c) * al.vy - v_bb_al * cons_al[IBY];
+    cons_al[IVZ] = (cons_al[IEN] + ptot_c) * al.vz - v_bb_al * cons_al[IBZ];
+    const double v_bb_ar = ar.vx * bbx + ar.vy * cons_ar[IBY] + ar.vz * cons_ar[IBZ];
+    cons_ar[IDN] = r_r[IDN] / (lambda_r - ar.vx);
+    cons_ar[IEN] = (r_r[IEN] + ptot_c * ar.vx - v_bb_ar * bbx) / (lambda_r - ar.vx);
+    cons_ar[IVX] = (cons_ar[IEN] + ptot_c) * ar.vx - v_bb_ar * bbx;
+    cons_ar[IVY] = (cons_ar[IEN] + ptot_c) * ar.vy - v_bb_ar * cons_ar[IBY];
+    cons_ar[IVZ] = (cons_ar[IEN] + ptot_c) * ar.vz - v_bb_ar * cons_ar[IBZ];
+    if (lambda_al >= 0.0 - 1.0e-6) {           // aL region
+#pragma unroll
+      for (int n = 0; n < NWAVE; ++n) {
+        csel[n] = cons_al[n];
+        fsel[n] = lambda_l * cons_al[n] - r_l[n];
+      }
+    } else if (lambda_ar <= 0.0 + 1.0e-6) {    // aR region
+#pragma unroll
+      for (int n = 0; n < NWAVE; ++n) {
+        csel[n] = cons_ar[n];
+        fsel[n] = lambda_r * cons_ar[n] - r_r[n];
+      }
+    } else {                                   // contact region
+      double cons_c[NWAVE];
+      const double denom_c_inv = 1.0 / (lambda_ar - lambda_al + 1.0e-12);
+      double numer_al = cons_al[IBY] * (lambda_al - al.vx) + bbx * al.vy;
+      double numer_ar = cons_ar[IBY] * (lambda_ar - ar.vx) + bbx * ar.vy;
+      cons_c[IBY] = (numer_ar - numer_al) * denom_c_inv;
+      numer_al = cons_al[IBZ] * (lambda_al - al.vx) + bbx * al.vz;
+      numer_ar = cons_ar[IBZ] * (lambda_ar - ar.vx) + bbx * ar.vz;
+      cons_c[IBZ] = (numer_ar - numer_al) * denom_c_inv;
+      const double k_sq_l = sq(al.kx) + sq(al.ky) + sq(al.kz);
+      const double k_bc_l = al.kx * bbx + al.ky * cons_c[IBY] + al.kz * cons_c[IBZ];
+      const double vx_cl = al.kx - bbx * (1.0 - k_sq_l) / (al.eta - k_bc_l);
+      const double vy_cl = al.ky - cons_c[IBY] * (1.0 - k_sq_l) / (al.eta - k_bc_l);
+      const double vz_cl = al.kz - cons_c[IBZ] * (1.0 - k_sq_l) / (al.eta - k_bc_l);
+      const double k_sq_r = sq(ar.kx) + sq(ar.ky) + sq(ar.kz);
+      const double k_bc_r = ar.kx * bbx + ar.ky * cons_c[IBY] + ar.kz * cons_c[IBZ];
+      const double vx_cr = ar.kx - bbx * (1.0 - k_sq_r) / (ar.eta - k_bc_r);
+      const double vy_cr = ar.ky - cons_c[IBY] * (1.0 - k_sq_r) / (ar.eta - k_bc_r);
+      const double vz_cr = ar.kz - cons_c[IBZ] * (1.0 - k_sq_r) / (ar.eta - k_bc_r);
+      const double vx_c = 0.5 * (vx_cl + vx_cr);
+      const double vy_c = 0.5 * (vy_cl + vy_cr);
+      const double vz_c = 0.5 * (vz_cl + vz_cr);
+      const double v_bb_c = vx_c * bbx + vy_c * cons_c[IBY] + vz_c * cons_c[IBZ];
+      const bool cl = vx_c >= 0.0;
+      const double *ca = cl ? cons_al : cons_ar;
+      const double lam_a = cl ? lambda_al : lambda_ar, vxa = cl ? al.vx : ar.vx;
+      cons_c[IDN] = ca[IDN] * (lam_a - vxa) / (lam_a - vx_c);
+      cons_c[IEN] = (lam_a * ca[IEN] - ca[IVX] + ptot_c * vx_c - v_bb_c * bbx) / (lam_a - vx_c);
+      cons_c[IVX] = (cons_c[IEN] + ptot_c) * vx_c - v_bb_c * bbx;
+      cons_c[IVY] = (cons_c[IEN] + ptot_c) * vy_c - v_bb_c * cons_c[IBY];
+      cons_c[IVZ] = (cons_c[IEN] + ptot_c) * vz_c - v_bb_c * cons_c[IBZ];
+      const double lam_f = cl ? lambda_l : lambda_r;
+      const double *rs = cl ? r_l : r_r;
+#pragma unroll
+      for (int n = 0; n < NWAVE; ++n) {
+        const double fa = lam_f * ca[n] - rs[n];
+        csel[n] = cons_c[n];
+        fsel[n] = fa + lam_a * (cons_c[n] - ca[n]);
+      }
+    }
+    bool bad = false;
+#pragma unroll
+    for (int n = 0; n < NWAVE; ++n)
+      if (!isfinite(csel[n]) || !isfinite(fsel[n])) bad = true;
+    if (!bad) fi = fsel;
+  }
+#pragma unroll
+  for (int n = 0; n < NWAVE; ++n) f[n] = fi[n];
+}
+
+//----------------------------------------------------------------------------------------
 // SR primitive recovery (Newman & Hamlin 2014), adiabatic_mhd_sr.cpp:404-522.
 // NHC holds the per-cell invariants the iteration works on.
 struct NHC {

@@ -268,7 +268,7 @@ class GpuMesh:
 
 
 def riemann_batch(solver, gamma, wl, wr, bx):
-    """solver: 0 HLLD, 1 HLLE, 2 SR HLLE; rotated-frame states (n,7)."""
+    """solver: 0 HLLD, 1 HLLE, 2 SR HLLE, 3 SR HLLD; rotated-frame states (n,7)."""
     wl = np.ascontiguousarray(wl, dtype=np.float64)
     wr = np.ascontiguousarray(wr, dtype=np.float64)
     bx = np.ascontiguousarray(bx, dtype=np.float64)

@@ -72,8 +72,6 @@ class MeshDesc:
             raise ValueError("xorder must be 1, 2 or 3")
         if self.rsolver not in RSOLVER:
             raise ValueError("rsolver must be one of %s" % sorted(RSOLVER))
-        if self.relativistic and self.rsolver != "hlle":
-            raise ValueError("the relativistic path implements HLLE only")
 
     # ---------------------------------------------------------------- geometry
     @property

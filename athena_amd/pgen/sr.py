@@ -44,29 +44,34 @@ def prim2cons(prim, bc, gamma):
 
 
 # problem parameters of inputs/mhd_sr/athinput.mub_{1,2,4} (tlim, nx1 per unit length,
-# gamma, left state, right state: rho, pgas, vx, vy, vz, Bx, By, Bz)
+# gamma, left state, right state: rho, pgas, vx, vy, vz, Bx, By, Bz); tols are the
+# reference tests' relative-L1 tolerances for HLLE (mhd_shocks_hlle.py) and tols_hlld
+# for HLLD (mhd_shocks_hlld.py)
 MUB = {
     1: dict(tlim=0.4, nx1=400, gamma=2.0,
             left=(1.0, 1.0, 0.0, 0.0, 0.0, 0.5, 1.0, 0.0),
             right=(0.125, 0.1, 0.0, 0.0, 0.0, 0.5, -1.0, 0.0),
-            tols=[0.02, 0.02, 0.03, 0.08, 0.0, 0.0, 0.02, 0.0]),
+            tols=[0.02, 0.02, 0.03, 0.08, 0.0, 0.0, 0.02, 0.0],
+            tols_hlld=[0.02, 0.01, 0.02, 0.04, 0.0, 0.0, 0.01, 0.0]),
     2: dict(tlim=0.55, nx1=800, gamma=1.6666666666666667,
             left=(1.08, 0.95, 0.4, 0.3, 0.2, 2.0, 0.3, 0.3),
             right=(1.0, 1.0, -0.45, -0.2, 0.2, 2.0, -0.7, 0.5),
-            tols=[0.003, 0.002, 0.007, 0.01, 0.007, 0.0, 0.005, 0.007]),
+            tols=[0.003, 0.002, 0.007, 0.01, 0.007, 0.0, 0.005, 0.007],
+            tols_hlld=[0.003, 0.002, 0.007, 0.01, 0.005, 0.0, 0.004, 0.007]),
     4: dict(tlim=0.5, nx1=800, gamma=1.6666666666666667,
             left=(1.0, 5.0, 0.0, 0.3, 0.4, 1.0, 6.0, 2.0),
             right=(0.9, 5.3, 0.0, 0.0, 0.0, 1.0, 5.0, 2.0),
-            tols=[0.003, 0.002, 0.03, 0.04, 0.008, 0.0, 0.002, 0.005]),
+            tols=[0.003, 0.002, 0.03, 0.04, 0.008, 0.0, 0.002, 0.005],
+            tols_hlld=[0.002, 0.001, 0.02, 0.03, 0.004, 0.0, 0.001, 0.003]),
 }
 
 
-def shock_tube_mesh(problem, nx1=None, xorder=2, **kw):
+def shock_tube_mesh(problem, nx1=None, xorder=2, rsolver="hlle", **kw):
     p = MUB[problem]
     n = (nx1 or p["nx1"]) * 3
     return MeshDesc(nx=(n, 2, 2), bnx=kw.pop("bnx", (n, 2, 2)), xmin=(-1.5, -0.5, -0.5),
                     xmax=(1.5, 0.5, 0.5), gamma=p["gamma"], cfl=0.4, xorder=xorder,
-                    rsolver="hlle", relativistic=True, **kw)
+                    rsolver=rsolver, relativistic=True, **kw)
 
 
 def _state(s):

@@ -31,7 +31,7 @@ typedef struct ath_mesh_desc {
   int bnx[3];         /* <meshblock> nx1..nx3 */
   int ng;             /* NGHOST (>= 3 for xorder 3) */
   int xorder;         /* 1 donor cell, 2 PLM, 3 PPM (second VL2 stage; first is DC) */
-  int rsolver;        /* 0 HLLD, 1 HLLE (relativistic: HLLE only) */
+  int rsolver;        /* 0 HLLD (hlld.cpp; relativistic: hlld_rel.cpp), 1 HLLE */
   int relativistic;   /* 0 Newtonian, 1 special relativistic */
   double xmin[3], xmax[3];
   double gamma;       /* <hydro> gamma */
@@ -109,7 +109,8 @@ int ath_gpu_stream(ath_mesh_t mesh, void **stream);
 int ath_gpu_synchronize(ath_mesh_t mesh);
 
 /* Batched single-interface Riemann solves (rotated frame, 7 primitives per side):
- * solver 0 HLLD (hlld.cpp), 1 HLLE (hlle_mhd.cpp), 2 SR HLLE (hlle_mhd_rel.cpp).
+ * solver 0 HLLD (hlld.cpp), 1 HLLE (hlle_mhd.cpp), 2 SR HLLE (hlle_mhd_rel.cpp),
+ * 3 SR HLLD (hlld_rel.cpp).
  * Host arrays of n*7, n and n*7 doubles. */
 int ath_gpu_riemann_batch(int solver, double gamma, int n, const double *wl,
                           const double *wr, const double *bx, double *flux);

@@ -408,7 +408,9 @@ void RiemannPencil(Mesh &m, Block &b, int d, int k, int j, int il, int iu, Arr &
     wri[IVZ] = wr(ivz, 0, 0, i); wri[IPR] = wr(IPR, 0, 0, i); wri[IBY] = wr(IBY, 0, 0, i);
     wri[IBZ] = wr(IBZ, 0, 0, i);
     Real bxi = bx(k, j, i);
-    if (m.c.relativistic) {
+    if (m.c.relativistic && m.c.rsolver == 0) {
+      RiemannSRHLLD(m, wli, wri, bxi, flxi);
+    } else if (m.c.relativistic) {
       RiemannSR(m, wli, wri, bxi, flxi);
     } else if (m.c.rsolver == 0) {
       HLLD1(wli, wri, bxi, m.c.gamma, flxi);
@@ -1138,7 +1140,8 @@ void ora_riemann(int solver, int relativistic, double gamma, int n, const double
   m.dfloor = std::sqrt(1024 * fmin); m.pfloor = m.dfloor;
   m.gm1 = gamma - 1.0;
   for (int q = 0; q < n; ++q) {
-    if (relativistic) RiemannSR(m, wl + 7 * q, wr + 7 * q, bx[q], out + 7 * q);
+    if (relativistic && solver == 0) RiemannSRHLLD(m, wl + 7 * q, wr + 7 * q, bx[q], out + 7 * q);
+    else if (relativistic) RiemannSR(m, wl + 7 * q, wr + 7 * q, bx[q], out + 7 * q);
     else if (solver == 0) HLLD1(wl + 7 * q, wr + 7 * q, bx[q], gamma, out + 7 * q);
     else HLLE1(wl + 7 * q, wr + 7 * q, bx[q], gamma, out + 7 * q);
   }

@@ -301,6 +301,300 @@ void RiemannSR(Mesh &m, const Real wl[NWAVE], const Real wr[NWAVE], Real bb1,
 }
 
 //----------------------------------------------------------------------------------------
+// SR HLLD (Mignone, Ugliano & Bodo 2009), one interface in the rotated frame:
+// hlld_rel.cpp:116-927 (HLLDTransforming, special relativity, v_interface = 0).
+namespace {
+
+// State between a fast wave and an Alfven wave for a trial total pressure ptot
+// (MUB 21-35, 43): the Alfven-region velocity, transverse field, eta and K.
+struct SRAlfven {
+  Real vx, vy, vz, by, bz, eta, kx, ky, kz;
+};
+
+// side = -1: left fast wave (eta has the sign of -Bx), +1: right.
+SRAlfven SRAlfvenState(const Real r[NWAVE], Real lambda, Real bbx, Real ptot, int side) {
+  SRAlfven a;
+  Real aa = r[IVX] - lambda * r[IEN] + ptot * (1.0 - SQR(lambda));                 // 26
+  Real gg = SQR(r[IBY]) + SQR(r[IBZ]);                                             // 27
+  Real cc = r[IVY] * r[IBY] + r[IVZ] * r[IBZ];                                     // 28
+  Real qq = -aa - gg + SQR(bbx) * (1.0 - SQR(lambda));                             // 29
+  Real x_inv = 1.0 / (bbx * (aa * lambda * bbx + cc)
+                      - (aa + gg) * (lambda * ptot + r[IEN]));                     // 30
+  a.vx = (bbx * (aa * bbx + lambda * cc) - (aa + gg) * (ptot + r[IVX])) * x_inv;   // 23
+  a.vy = (qq * r[IVY] + r[IBY] * (cc + bbx * (lambda * r[IVX] - r[IEN]))) * x_inv;  // 24
+  a.vz = (qq * r[IVZ] + r[IBZ] * (cc + bbx * (lambda * r[IVX] - r[IEN]))) * x_inv;  // 24
+  a.by = (r[IBY] - bbx * a.vy) / (lambda - a.vx);                                   // 21
+  a.bz = (r[IBZ] - bbx * a.vz) / (lambda - a.vx);
+  Real v_rm = a.vx * r[IVX] + a.vy * r[IVY] + a.vz * r[IVZ];
+  Real wtot = ptot + (r[IEN] - v_rm) / (lambda - a.vx);                            // 31
+  a.eta = (side < 0) ? -std::copysign(std::sqrt(wtot), bbx)
+                     : std::copysign(std::sqrt(wtot), bbx);                         // 35
+  Real den_inv = 1.0 / (lambda * ptot + r[IEN] + bbx * a.eta);                      // 43
+  a.kx = (r[IVX] + ptot + lambda * bbx * a.eta) * den_inv;
+  a.ky = (r[IVY] + r[IBY] * a.eta) * den_inv;
+  a.kz = (r[IVZ] + r[IBZ] * a.eta) * den_inv;
+  return a;
+}
+
+// Contact-jump residual for a trial total pressure (MUB 45, 48, 49).
+Real SRContactResidual(const SRAlfven &l, const SRAlfven &rr, Real bbx) {
+  const Real delta_kx_aug = 1.0e-12;
+  Real delta_kx = rr.kx - l.kx + delta_kx_aug;
+  Real bbx_c_dk = bbx * delta_kx;
+  Real by_c_dk = rr.by * (rr.kx - rr.vx) - l.by * (l.kx - l.vx) + bbx * (rr.vy - l.vy);
+  Real bz_c_dk = rr.bz * (rr.kx - rr.vx) - l.bz * (l.kx - l.vx) + bbx * (rr.vz - l.vz);
+  Real k_sq_l = SQR(l.kx) + SQR(l.ky) + SQR(l.kz);
+  Real kdb = l.kx * bbx_c_dk + l.ky * by_c_dk + l.kz * bz_c_dk;
+  Real y_l = (1.0 - k_sq_l) / (l.eta * delta_kx - kdb);
+  Real k_sq_r = SQR(rr.kx) + SQR(rr.ky) + SQR(rr.kz);
+  kdb = rr.kx * bbx_c_dk + rr.ky * by_c_dk + rr.kz * bz_c_dk;
+  Real y_r = (1.0 - k_sq_r) / (rr.eta * delta_kx - kdb);
+  return delta_kx * (1.0 - bbx * (y_r - y_l));
+}
+
+// hlld_rel.cpp:945-959 and 975-999 (Mignone & McKinney 2007, A1-A18)
+Real SREResidual(Real w, Real dd, Real ee, Real m_sq, Real bb_sq, Real ss_sq, Real gp) {
+  Real v_sq = (m_sq + ss_sq / SQR(w) * (2.0 * w + bb_sq)) / SQR(w + bb_sq);
+  Real gamma_sq = 1.0 / (1.0 - v_sq);
+  Real gamma_lorentz = std::sqrt(gamma_sq);
+  Real chi = (1.0 - v_sq) * (w - gamma_lorentz * dd);
+  Real pgas = chi / gp;
+  Real ee_calc = w - pgas + 0.5 * bb_sq * (1.0 + v_sq) - ss_sq / (2.0 * SQR(w));
+  return ee_calc - ee;
+}
+Real SREResidualPrime(Real w, Real dd, Real m_sq, Real bb_sq, Real ss_sq, Real gp) {
+  Real v_sq = (m_sq + ss_sq / SQR(w) * (2.0 * w + bb_sq)) / SQR(w + bb_sq);
+  Real gamma_sq = 1.0 / (1.0 - v_sq);
+  Real gamma_lorentz = std::sqrt(gamma_sq);
+  Real chi = (1.0 - v_sq) * (w - gamma_lorentz * dd);
+  Real w_cu = SQR(w) * w;
+  Real w_b_cu = SQR(w + bb_sq) * (w + bb_sq);
+  Real dv_sq_dw = -2.0 / (w_cu * w_b_cu) *
+                  (ss_sq * (3.0 * w * (w + bb_sq) + SQR(bb_sq)) + m_sq * w_cu);
+  Real dchi_dw = 1.0 - v_sq - gamma_lorentz / 2.0 * (dd + 2.0 * gamma_lorentz * chi) * dv_sq_dw;
+  Real drho_dw = -gamma_lorentz * dd / 2.0 * dv_sq_dw;
+  Real dpgas_dchi = 1.0 / gp;
+  Real dpgas_drho = 0.0;
+  Real dpgas_dw = dpgas_dchi * dchi_dw + dpgas_drho * drho_dw;
+  return 1.0 - dpgas_dw + 0.5 * bb_sq * dv_sq_dw + ss_sq / w_cu;
+}
+
+// The L/R conserved state and flux of one side (MUB 8, 15).
+void SRSideState(const Real w[NWAVE], Real bbx, Real gamma_adi, Real cons[NWAVE],
+                 Real flux[NWAVE], Real *lp, Real *lm) {
+  const Real gamma_prime = gamma_adi / (gamma_adi - 1.0);
+  Real rho = w[IDN], pgas = w[IPR];
+  Real u[4];
+  u[0] = std::sqrt(1.0 + SQR(w[IVX]) + SQR(w[IVY]) + SQR(w[IVZ]));
+  u[1] = w[IVX]; u[2] = w[IVY]; u[3] = w[IVZ];
+  Real b[4];
+  b[0] = bbx * u[1] + w[IBY] * u[2] + w[IBZ] * u[3];
+  b[1] = (bbx + b[0] * u[1]) / u[0];
+  b[2] = (w[IBY] + b[0] * u[2]) / u[0];
+  b[3] = (w[IBZ] + b[0] * u[3]) / u[0];
+  Real b_sq = -SQR(b[0]) + SQR(b[1]) + SQR(b[2]) + SQR(b[3]);
+  Real wgas = rho + gamma_adi / (gamma_adi - 1.0) * pgas;
+  FastSpeedsSR(gamma_adi, wgas, pgas, u[0], u[1], b_sq, -1.0, 0.0, 1.0, lp, lm);
+  Real wtot = rho + gamma_prime * pgas + b_sq;
+  Real ptot = pgas + 0.5 * b_sq;
+  cons[IDN] = rho * u[0];
+  cons[IEN] = wtot * u[0] * u[0] - b[0] * b[0] - ptot;
+  cons[IVX] = wtot * u[1] * u[0] - b[1] * b[0];
+  cons[IVY] = wtot * u[2] * u[0] - b[2] * b[0];
+  cons[IVZ] = wtot * u[3] * u[0] - b[3] * b[0];
+  cons[IBY] = b[2] * u[0] - b[0] * u[2];
+  cons[IBZ] = b[3] * u[0] - b[0] * u[3];
+  flux[IDN] = rho * u[1];
+  flux[IEN] = wtot * u[0] * u[1] - b[0] * b[1];
+  flux[IVX] = wtot * u[1] * u[1] - b[1] * b[1] + ptot;
+  flux[IVY] = wtot * u[2] * u[1] - b[2] * b[1];
+  flux[IVZ] = wtot * u[3] * u[1] - b[3] * b[1];
+  flux[IBY] = b[2] * u[1] - b[1] * u[2];
+  flux[IBZ] = b[3] * u[1] - b[1] * u[3];
+}
+
+}  // namespace
+
+void RiemannSRHLLD(Mesh &m, const Real wl[NWAVE], const Real wr[NWAVE], Real bbx,
+                   Real flxi[NWAVE]) {
+  const Real p_transition = 0.01;
+  const Real vc_extension = 1.0e-6;
+  const Real delta_kx_aug = 1.0e-12;
+  const int num_secant = 4;
+  const Real initial_offset = 1.0e-5;
+  const Real tol_res = 1.0e-12;
+  const Real gamma_adi = m.c.gamma;
+  const Real gamma_prime = gamma_adi / (gamma_adi - 1.0);
+
+  Real cons_l[NWAVE], flux_l[NWAVE], cons_r[NWAVE], flux_r[NWAVE];
+  Real lp_l, lm_l, lp_r, lm_r;
+  SRSideState(wl, bbx, gamma_adi, cons_l, flux_l, &lp_l, &lm_l);
+  SRSideState(wr, bbx, gamma_adi, cons_r, flux_r, &lp_r, &lm_r);
+  Real lambda_l = std::min(lm_l, lm_r);                                    // MB 55
+  Real lambda_r = std::max(lp_l, lp_r);
+  Real r_l[NWAVE], r_r[NWAVE], cons_hll[NWAVE], flux_hll[NWAVE];
+  for (int n = 0; n < NWAVE; ++n) r_l[n] = lambda_l * cons_l[n] - flux_l[n];   // MUB 12
+  for (int n = 0; n < NWAVE; ++n) r_r[n] = lambda_r * cons_r[n] - flux_r[n];
+  Real lambda_diff_inv = 1.0 / (lambda_r - lambda_l);
+  for (int n = 0; n < NWAVE; ++n) cons_hll[n] = (r_r[n] - r_l[n]) * lambda_diff_inv;
+  for (int n = 0; n < NWAVE; ++n)
+    flux_hll[n] = (lambda_l * r_r[n] - lambda_r * r_l[n]) * lambda_diff_inv;
+
+  // total pressure of the HLL state: two Newton steps on the enthalpy (MM A26-A27)
+  Real ptot_hll;
+  {
+    Real m_sq = SQR(cons_hll[IVX]) + SQR(cons_hll[IVY]) + SQR(cons_hll[IVZ]);
+    Real bb_sq = SQR(bbx) + SQR(cons_hll[IBY]) + SQR(cons_hll[IBZ]);
+    Real m_dot_bb = cons_hll[IVX] * bbx + cons_hll[IVY] * cons_hll[IBY] +
+                    cons_hll[IVZ] * cons_hll[IBZ];
+    Real ss_sq = SQR(m_dot_bb);
+    Real a1 = 4.0 / 3.0 * (bb_sq - cons_hll[IEN]);
+    Real a0 = ONE_3RD * (m_sq + bb_sq * (bb_sq - 2.0 * cons_hll[IEN]));
+    Real s2 = SQR(a1) - 4.0 * a0;
+    Real s = (s2 < 0.0) ? 0.0 : std::sqrt(s2);
+    Real w_new = (s2 >= 0.0 && a1 >= 0.0) ? -2.0 * a0 / (a1 + s) : 0.5 * (-a1 + s);
+    Real res_new = SREResidual(w_new, cons_hll[IDN], cons_hll[IEN], m_sq, bb_sq, ss_sq,
+                               gamma_prime);
+    for (int n = 0; n < 2; ++n) {
+      Real w_old = w_new, res_old = res_new;
+      Real der = SREResidualPrime(w_old, cons_hll[IDN], m_sq, bb_sq, ss_sq, gamma_prime);
+      Real delta = -res_old / der;
+      w_new = w_old + delta;
+      res_new = SREResidual(w_new, cons_hll[IDN], cons_hll[IEN], m_sq, bb_sq, ss_sq,
+                            gamma_prime);
+    }
+    Real w = w_new;
+    Real v_sq = (m_sq + ss_sq / SQR(w) * (2.0 * w + bb_sq)) / SQR(w + bb_sq);
+    Real gamma_lor_sq = 1.0 / (1.0 - v_sq);
+    Real gamma_lor = std::sqrt(gamma_lor_sq);
+    Real chi = (1.0 - v_sq) * (w - gamma_lor * cons_hll[IDN]);
+    Real pgas = 1.0 / gamma_prime * chi;
+    Real vx = (cons_hll[IVX] + m_dot_bb / w * bbx) / (w + bb_sq);
+    Real vy = (cons_hll[IVY] + m_dot_bb / w * cons_hll[IBY]) / (w + bb_sq);
+    Real vz = (cons_hll[IVZ] + m_dot_bb / w * cons_hll[IBZ]) / (w + bb_sq);
+    Real v_bb = vx * bbx + vy * cons_hll[IBY] + vz * cons_hll[IBZ];
+    Real b_sq = bb_sq / gamma_lor_sq + SQR(v_bb);
+    ptot_hll = pgas + 0.5 * b_sq;
+  }
+  Real ptot_init;                                                          // MUB 53, 55
+  if (SQR(bbx) / ptot_hll < p_transition) {
+    Real a1 = cons_hll[IEN] - flux_hll[IVX];
+    Real a0 = cons_hll[IVX] * flux_hll[IEN] - flux_hll[IVX] * cons_hll[IEN];
+    Real s2 = SQR(a1) - 4.0 * a0;
+    Real s = (s2 < 0.0) ? 0.0 : std::sqrt(s2);
+    ptot_init = (s2 >= 0.0 && a1 >= 0.0) ? -2.0 * a0 / (a1 + s) : (-a1 + s) / 2.0;
+  } else {
+    ptot_init = ptot_hll;
+  }
+  bool to_hlle = !std::isfinite(ptot_init) || ptot_init <= 0.0;
+
+  // secant iteration on the contact residual
+  Real ptot_0 = ptot_init;
+  SRAlfven al = SRAlfvenState(r_l, lambda_l, bbx, ptot_0, -1);
+  SRAlfven ar = SRAlfvenState(r_r, lambda_r, bbx, ptot_0, +1);
+  Real res_0 = SRContactResidual(al, ar, bbx);
+  Real ptot_1 = ptot_init * (1.0 + initial_offset);
+  al = SRAlfvenState(r_l, lambda_l, bbx, ptot_1, -1);
+  ar = SRAlfvenState(r_r, lambda_r, bbx, ptot_1, +1);
+  Real res_1 = SRContactResidual(al, ar, bbx);
+  Real ptot_last = ptot_0, ptot_n = ptot_1, res_last = res_0, res_n = res_1;
+  Real tol_ptot = 1.0e-8 * ptot_init;
+  for (int n = 0; n < num_secant; ++n) {
+    Real ptot_old = ptot_last;
+    ptot_last = ptot_n;
+    Real res_old = res_last;
+    res_last = res_n;
+    bool iterate = std::abs(res_last) > tol_res && std::abs(ptot_last - ptot_old) > tol_ptot;
+    ptot_n = iterate ? (res_last * ptot_old - res_old * ptot_last) / (res_last - res_old)
+                     : ptot_last;
+    al = SRAlfvenState(r_l, lambda_l, bbx, ptot_n, -1);
+    ar = SRAlfvenState(r_r, lambda_r, bbx, ptot_n, +1);
+    res_n = SRContactResidual(al, ar, bbx);
+  }
+  Real ptot_c = ptot_n;
+  Real lambda_al = al.kx, lambda_ar = ar.kx;
+  if (!std::isfinite(lambda_al) || !std::isfinite(lambda_ar) || !std::isfinite(ptot_c) ||
+      ptot_c <= 0.0)
+    to_hlle = true;
+
+  // aL and aR regions (MUB 32-34) and their fluxes (MUB 11-12)
+  Real cons_al[NWAVE], cons_ar[NWAVE], flux_al[NWAVE], flux_ar[NWAVE];
+  cons_al[IBY] = al.by; cons_al[IBZ] = al.bz;
+  cons_ar[IBY] = ar.by; cons_ar[IBZ] = ar.bz;
+  Real v_bb_al = al.vx * bbx + al.vy * cons_al[IBY] + al.vz * cons_al[IBZ];
+  cons_al[IDN] = r_l[IDN] / (lambda_l - al.vx);
+  cons_al[IEN] = (r_l[IEN] + ptot_c * al.vx - v_bb_al * bbx) / (lambda_l - al.vx);
+  cons_al[IVX] = (cons_al[IEN] + ptot_c) * al.vx - v_bb_al * bbx;
+  cons_al[IVY] = (cons_al[IEN] + ptot_c) * al.vy - v_bb_al * cons_al[IBY];
+  cons_al[IVZ] = (cons_al[IEN] + ptot_c) * al.vz - v_bb_al * cons_al[IBZ];
+  Real v_bb_ar = ar.vx * bbx + ar.vy * cons_ar[IBY] + ar.vz * cons_ar[IBZ];
+  cons_ar[IDN] = r_r[IDN] / (lambda_r - ar.vx);
+  cons_ar[IEN] = (r_r[IEN] + ptot_c * ar.vx - v_bb_ar * bbx) / (lambda_r - ar.vx);
+  cons_ar[IVX] = (cons_ar[IEN] + ptot_c) * ar.vx - v_bb_ar * bbx;
+  cons_ar[IVY] = (cons_ar[IEN] + ptot_c) * ar.vy - v_bb_ar * cons_ar[IBY];
+  cons_ar[IVZ] = (cons_ar[IEN] + ptot_c) * ar.vz - v_bb_ar * cons_ar[IBZ];
+  for (int n = 0; n < NWAVE; ++n) flux_al[n] = lambda_l * cons_al[n] - r_l[n];
+  for (int n = 0; n < NWAVE; ++n) flux_ar[n] = lambda_r * cons_ar[n] - r_r[n];
+
+  // contact region (MUB 45, 47, 50-52), v_c averaged over the two sides
+  Real cons_c[NWAVE], flux_c[NWAVE];
+  Real denom_c_inv = 1.0 / (lambda_ar - lambda_al + delta_kx_aug);
+  Real numer_al = cons_al[IBY] * (lambda_al - al.vx) + bbx * al.vy;
+  Real numer_ar = cons_ar[IBY] * (lambda_ar - ar.vx) + bbx * ar.vy;
+  cons_c[IBY] = (numer_ar - numer_al) * denom_c_inv;
+  numer_al = cons_al[IBZ] * (lambda_al - al.vx) + bbx * al.vz;
+  numer_ar = cons_ar[IBZ] * (lambda_ar - ar.vx) + bbx * ar.vz;
+  cons_c[IBZ] = (numer_ar - numer_al) * denom_c_inv;
+  Real k_sq_l = SQR(al.kx) + SQR(al.ky) + SQR(al.kz);
+  Real k_bc_l = al.kx * bbx + al.ky * cons_c[IBY] + al.kz * cons_c[IBZ];
+  Real vx_cl = al.kx - bbx * (1.0 - k_sq_l) / (al.eta - k_bc_l);
+  Real vy_cl = al.ky - cons_c[IBY] * (1.0 - k_sq_l) / (al.eta - k_bc_l);
+  Real vz_cl = al.kz - cons_c[IBZ] * (1.0 - k_sq_l) / (al.eta - k_bc_l);
+  Real k_sq_r = SQR(ar.kx) + SQR(ar.ky) + SQR(ar.kz);
+  Real k_bc_r = ar.kx * bbx + ar.ky * cons_c[IBY] + ar.kz * cons_c[IBZ];
+  Real vx_cr = ar.kx - bbx * (1.0 - k_sq_r) / (ar.eta - k_bc_r);
+  Real vy_cr = ar.ky - cons_c[IBY] * (1.0 - k_sq_r) / (ar.eta - k_bc_r);
+  Real vz_cr = ar.kz - cons_c[IBZ] * (1.0 - k_sq_r) / (ar.eta - k_bc_r);
+  Real vx_c = 0.5 * (vx_cl + vx_cr);
+  Real vy_c = 0.5 * (vy_cl + vy_cr);
+  Real vz_c = 0.5 * (vz_cl + vz_cr);
+  Real v_bb_c = vx_c * bbx + vy_c * cons_c[IBY] + vz_c * cons_c[IBZ];
+  const bool cl = vx_c >= 0.0;
+  if (cl) {
+    cons_c[IDN] = cons_al[IDN] * (lambda_al - al.vx) / (lambda_al - vx_c);
+    cons_c[IEN] = (lambda_al * cons_al[IEN] - cons_al[IVX] + ptot_c * vx_c - v_bb_c * bbx) /
+                  (lambda_al - vx_c);
+  } else {
+    cons_c[IDN] = cons_ar[IDN] * (lambda_ar - ar.vx) / (lambda_ar - vx_c);
+    cons_c[IEN] = (lambda_ar * cons_ar[IEN] - cons_ar[IVX] + ptot_c * vx_c - v_bb_c * bbx) /
+                  (lambda_ar - vx_c);
+  }
+  cons_c[IVX] = (cons_c[IEN] + ptot_c) * vx_c - v_bb_c * bbx;
+  cons_c[IVY] = (cons_c[IEN] + ptot_c) * vy_c - v_bb_c * cons_c[IBY];
+  cons_c[IVZ] = (cons_c[IEN] + ptot_c) * vz_c - v_bb_c * cons_c[IBZ];
+  for (int n = 0; n < NWAVE; ++n)
+    flux_c[n] = cl ? flux_al[n] + lambda_al * (cons_c[n] - cons_al[n])
+                   : flux_ar[n] + lambda_ar * (cons_c[n] - cons_ar[n]);
+
+  // region of the wave fan containing the interface (hlld_rel.cpp:829-880)
+  const Real v_interface = 0.0;
+  const Real *ci, *fi;
+  if (lambda_l >= v_interface) { ci = cons_l; fi = flux_l; }
+  else if (lambda_r <= v_interface) { ci = cons_r; fi = flux_r; }
+  else if (to_hlle) { ci = cons_hll; fi = flux_hll; }
+  else if (lambda_al >= v_interface - vc_extension) { ci = cons_al; fi = flux_al; }
+  else if (lambda_ar <= v_interface + vc_extension) { ci = cons_ar; fi = flux_ar; }
+  else { ci = cons_c; fi = flux_c; }
+  if (!to_hlle) {
+    bool bad = false;
+    for (int n = 0; n < NWAVE; ++n)
+      if (!std::isfinite(ci[n]) || !std::isfinite(fi[n])) bad = true;
+    if (bad) fi = flux_hll;
+  }
+  for (int n = 0; n < NWAVE; ++n) flxi[n] = fi[n];
+}
+
+//----------------------------------------------------------------------------------------
 // SR primitive recovery: Newman & Hamlin (2014) iteration
 // (adiabatic_mhd_sr.cpp:404-522 ConservedToPrimitiveNormal)
 // Returns iteration count through *iters (n at exit) for diagnostics.

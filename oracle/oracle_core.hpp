@@ -20,6 +20,7 @@ constexpr int IVX = 1, IVY = 2, IVZ = 3, IPR = 4, IBY = 5, IBZ = 6;
 constexpr int IB1 = 0, IB2 = 1, IB3 = 2;
 constexpr int NHYDRO = 5, NWAVE = 7;
 constexpr Real TINY_NUMBER = 1.0e-20;
+constexpr Real ONE_3RD = 0.33333333333333333;   // defs.hpp.in:144
 
 inline Real SQR(Real x) { return x * x; }
 // std::min / std::max semantics: min(a,b) = (b<a)?b:a ; max(a,b) = (a<b)?b:a
@@ -97,6 +98,8 @@ struct Mesh {
 Real FastMagnetosonicSpeed(const Real prim[NWAVE], Real bx, Real gamma);
 void HLLE1(const Real wli[NWAVE], const Real wri[NWAVE], Real bxi, Real gamma,
            Real flxi[NWAVE]);
+void RiemannSRHLLD(Mesh &m, const Real wl[NWAVE], const Real wr[NWAVE], Real bbx,
+                   Real flxi[NWAVE]);
 void RiemannSR(Mesh &m, const Real wli[NWAVE], const Real wri[NWAVE], Real bxi,
                Real flxi[NWAVE]);
 void PiecewiseParabolic(Mesh &m, Block &b, int d, int k, int j, int il, int iu, Arr &w,

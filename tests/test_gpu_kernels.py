@@ -23,11 +23,11 @@ def random_states(n, seed, relativistic=False):
     return w
 
 
-@pytest.mark.parametrize("solver", [0, 1, 2], ids=["hlld", "hlle", "sr_hlle"])
+@pytest.mark.parametrize("solver", [0, 1, 2, 3], ids=["hlld", "hlle", "sr_hlle", "sr_hlld"])
 def test_riemann_batch_matches_oracle(solver):
     print("case riemann", solver, flush=True)
     n = 20000
-    rel = solver == 2
+    rel = solver >= 2
     wl, wr = random_states(n, 1, rel), random_states(n, 2, rel)
     # include identical states and degenerate (bx=0, bt=0) cases
     wr[:100] = wl[:100]
@@ -36,7 +36,7 @@ def test_riemann_batch_matches_oracle(solver):
     wl[200:300, 5:7] = 0.0
     wr[200:300, 5:7] = 0.0
     gamma = 4.0 / 3.0 if rel else 5.0 / 3.0
-    fo = ora_riemann(1 if solver == 1 else 0, rel, gamma, wl, wr, bx)
+    fo = ora_riemann(1 if solver in (1, 2) else 0, rel, gamma, wl, wr, bx)
     fg = gpu.riemann_batch(solver, gamma, wl, wr, bx)
     assert np.array_equal(np.isfinite(fo), np.isfinite(fg))
     fin = np.isfinite(fo)

@@ -10,16 +10,17 @@ from athena_amd.gpu import GpuMesh
 from athena_amd.pgen.sr import MUB, ShearLayer, ShockTube, shear_mesh, shock_tube_mesh
 from harness import compare, div_b
 from oracle_bind import OracleMesh
-from test_sr_kat import NAMES, assert_tube, ref
+from test_sr_kat import NAMES, ZERO_TOL, assert_tube, ref, tols
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
+@pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
 @pytest.mark.parametrize("problem", [1, 2, 4])
-def test_sr_shock_tube_reference_data(problem):
-    print("case sr shock tube", problem, flush=True)
-    m = shock_tube_mesh(problem)
+def test_sr_shock_tube_reference_data(problem, rsolver):
+    print("case sr shock tube", problem, rsolver, flush=True)
+    m = shock_tube_mesh(problem, rsolver=rsolver)
     st = ShockTube(m, problem)
     d = GpuMesh(m)
     d.load([st.block(0)])
@@ -29,7 +30,7 @@ def test_sr_shock_tube_reference_data(problem):
     res = st.compare(d.block, ref(problem))
     for name, (eps, norm) in zip(NAMES, res):
         print("  %-5s relL1 vs reference %.4e" % (name, eps / norm if norm else eps), flush=True)
-    assert_tube(res, MUB[problem]["tols"])
+    assert_tube(res, tols(problem, rsolver), ZERO_TOL[rsolver])
 
 
 def _parity(m, blocks, ncycles):
@@ -49,9 +50,10 @@ def _parity(m, blocks, ncycles):
     return o, d
 
 
-def test_sr_shock_tube_parity():
-    print("case sr shock tube 1 parity, 100 cycles", flush=True)
-    m = shock_tube_mesh(1, nx1=200)
+@pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
+def test_sr_shock_tube_parity(rsolver):
+    print("case sr shock tube 1 parity, 100 cycles", rsolver, flush=True)
+    m = shock_tube_mesh(1, nx1=200, rsolver=rsolver)
     _parity(m, [ShockTube(m, 1).block(0)], 100)
 
 

@@ -1,8 +1,9 @@
 """SR MHD known answers from the reference's own data: the Mignone-Ugliano-Bodo shock
 tubes (tst/regression/scripts/tests/sr/mhd_shocks_hlle.py:53-85, solutions
 tst/regression/data/sr_mhd_shock{1,2,4}_hlld.vtk committed as tests/golden/*.npz).
-Run with HLLE + PLM + VL2 + CT, compared with the reference test's tolerances
-(relative L1; 0.0 means the component must stay exactly equal)."""
+Run with HLLE or HLLD + PLM + VL2 + CT, compared with the reference tests' tolerances
+(mhd_shocks_hlle.py / mhd_shocks_hlld.py: relative L1; 0.0 means the component must
+stay exactly equal).  The data were made with HLLD, so HLLD meets tighter bounds."""
 import os
 
 import numpy as np
@@ -19,16 +20,29 @@ def ref(problem):
     return dict(np.load(os.path.join(GOLDEN, "sr_mhd_shock%d_hlld.npz" % problem)))
 
 
-def assert_tube(res, tols):
+# The reference runs the tubes in 1-D; here they run in a periodic 3-D box with two
+# transverse cells, so every transverse face solves a Riemann problem with identical
+# left and right states.  HLLE returns exactly F(U) there, but HLLD's wave-fan
+# arithmetic leaves round-off in components that vanish analytically (e.g. the z-face
+# flux of B_x when v_z = B_z = 0), so with HLLD the "exactly zero" components of the
+# reference test are bounded at round-off instead.
+ZERO_TOL = {"hlle": 0.0, "hlld": 1e-14}
+
+
+def assert_tube(res, tols, zero_tol=0.0):
     for name, (eps, norm), tol in zip(NAMES, res, tols):
         if tol == 0.0:
-            assert eps == 0.0, (name, eps)
+            assert eps <= zero_tol, (name, eps)
         else:
             assert eps / norm <= tol, (name, eps / norm, tol)
 
 
-def run_oracle(problem):
-    m = shock_tube_mesh(problem)
+def tols(problem, rsolver):
+    return MUB[problem]["tols_hlld" if rsolver == "hlld" else "tols"]
+
+
+def run_oracle(problem, rsolver="hlle"):
+    m = shock_tube_mesh(problem, rsolver=rsolver)
     st = ShockTube(m, problem)
     o = OracleMesh(m)
     o.load([st.block(0)])
@@ -38,12 +52,14 @@ def run_oracle(problem):
     return st.compare(o.block, ref(problem))
 
 
-def test_oracle_sr_shock_tube_1():
-    assert_tube(run_oracle(1), MUB[1]["tols"])
+@pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
+def test_oracle_sr_shock_tube_1(rsolver):
+    assert_tube(run_oracle(1, rsolver), tols(1, rsolver), ZERO_TOL[rsolver])
 
 
+@pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
 @pytest.mark.parametrize("problem", [2, 4])
-def test_oracle_sr_shock_tubes_slow(problem):
+def test_oracle_sr_shock_tubes_slow(problem, rsolver):
     if not os.environ.get("ATH_SLOW"):
         pytest.skip("set ATH_SLOW=1 (about 30 s each on one core)")
-    assert_tube(run_oracle(problem), MUB[problem]["tols"])
+    assert_tube(run_oracle(problem, rsolver), tols(problem, rsolver), ZERO_TOL[rsolver])
