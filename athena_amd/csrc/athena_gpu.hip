@@ -141,6 +141,39 @@ __device__ __forceinline__ bool map_work(int lw, int nwg, int nblocks, int ni, i
   return i < ni && j < nj && k < nk;
 }
 
+// Position boxes of the stage kernels per MeshBlock (first index lo, extent n):
+//   fluxes:   cells [is-1, ie+1]^3, each computing the faces on its low sides that lie in
+//             the reference's ranges (calculate_fluxes.cpp:68-76, 176-182, 299-302);
+//   corner:   edges [is, ie+1]^3 (calculate_corner_e.cpp:187-229);
+//   update:   [is, ie+1]^3: the active cell and its three low faces (CT ranges);
+//   cons2prim: every cell including ghosts.
+enum WorkKind { KK_FLUX = 0, KK_CORNER = 1, KK_UPDATE = 2, KK_C2P = 3, KK_N = 4 };
+
+__host__ __device__ __forceinline__ void kind_box(const MeshK &M, int kind, int *lo, int *n) {
+  if (kind == KK_FLUX) {
+    lo[0] = M.is - 1; lo[1] = M.js - 1; lo[2] = M.ks - 1;
+    n[0] = M.nx1 + 2; n[1] = M.nx2 + 2; n[2] = M.nx3 + 2;
+  } else if (kind == KK_C2P) {
+    lo[0] = lo[1] = lo[2] = 0;
+    n[0] = M.nc1; n[1] = M.nc2; n[2] = M.nc3;
+  } else {
+    lo[0] = M.is; lo[1] = M.js; lo[2] = M.ks;
+    n[0] = M.nx1 + 1; n[1] = M.nx2 + 1; n[2] = M.nx3 + 1;
+  }
+}
+
+// map_work over kind's box; (i, j, k) are block cell indices.
+__device__ __forceinline__ bool map_kind(const MeshK &M, int kind, int lw, int nwg, TileK T,
+                                         int &g, int &i, int &j, int &k) {
+  int lo[3], n[3];
+  kind_box(M, kind, lo, n);
+  const bool ok = map_work(lw, nwg, M.nblocks, n[0], n[1], n[2], T, g, i, j, k);
+  i += lo[0];
+  j += lo[1];
+  k += lo[2];
+  return ok;
+}
+
 //========================================================================================
 // Reconstruction of one variable at one interface along DIR.
 //   q[] holds the variable at cells c-3 .. c+2 (q[3] is the cell on the right of the
@@ -342,10 +375,7 @@ __device__ __forceinline__ void flux3_body(const MeshK &M, const double *__restr
                                            const FaceOut &o2, const FaceOut &o3, double dt,
                                            int lw, int nwg, TileK T) {
   int g, i, j, k;
-  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 2, M.nx2 + 2, M.nx3 + 2, T, g, i, j, k)) return;
-  i += M.is - 1;
-  j += M.js - 1;
-  k += M.ks - 1;
+  if (!map_kind(M, KK_FLUX, lw, nwg, T, g, i, j, k)) return;
   const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
   const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
   if (i >= M.is) flux_face<0, ORDER, RS>(M, g, k, j, i, wb, bb, o1, dt);
@@ -396,10 +426,7 @@ template <bool SR>
 __device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca, int lw,
                                             int nwg, TileK T) {
   int g, i, j, k;
-  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 1, M.nx2 + 1, M.nx3 + 1, T, g, i, j, k)) return;
-  i += M.is;
-  j += M.js;
-  k += M.ks;
+  if (!map_kind(M, KK_CORNER, lw, nwg, T, g, i, j, k)) return;
   const double *__restrict__ w = ca.w;
   const double *__restrict__ bcc = ca.bcc;
   const double *__restrict__ e2x1 = ca.e2x1, *__restrict__ e3x1 = ca.e3x1;
@@ -665,10 +692,7 @@ __device__ __forceinline__ void update_body(const MeshK &M, const UpdateArgs &a,
                                             int first_stage, double dt, int lw, int nwg,
                                             TileK T) {
   int g, i, j, k;
-  if (!map_work(lw, nwg, M.nblocks, M.nx1 + 1, M.nx2 + 1, M.nx3 + 1, T, g, i, j, k)) return;
-  i += M.is;
-  j += M.js;
-  k += M.ks;
+  if (!map_kind(M, KK_UPDATE, lw, nwg, T, g, i, j, k)) return;
   const double wght = beta * dt;
   const bool ci = i <= M.ie, cj = j <= M.je, ck = k <= M.ke;
   if (ci && cj && ck)
@@ -716,7 +740,7 @@ __device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int l
   double *__restrict__ dt_partial = a.dt_partial;
   const double *__restrict__ lseed = a.lseed;
   int g, i, j, k;
-  const bool live = map_work(lw, nwg, M.nblocks, M.nc1, M.nc2, M.nc3, T, g, i, j, k);
+  const bool live = map_kind(M, KK_C2P, lw, nwg, T, g, i, j, k);
   double my_dt = DBL_MAX;
   C2PStats st{0, 0};
   if (live) {
@@ -847,8 +871,7 @@ __global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restr
                                                   const double *__restrict__ b3,
                                                   double *__restrict__ lseed, TileK T) {
   int g, i, j, k;
-  const bool live = map_work(xcd_logical_wg(M.xcd_remap), gridDim.x, M.nblocks, M.nc1, M.nc2,
-                             M.nc3, T, g, i, j, k);
+  const bool live = map_kind(M, KK_C2P, xcd_logical_wg(M.xcd_remap), gridDim.x, T, g, i, j, k);
   double cs[NHYDRO] = {1.0, 0.0, 0.0, 0.0, 2.0}, bs[3] = {0.0, 0.0, 0.0};
   const long c = live ? cidx(M, k, j, i) : 0;
   if (live) {
@@ -1169,19 +1192,11 @@ int solver_code(const ath_mesh_desc &d) {
   return d.rsolver;
 }
 
-// Work kinds of the stage kernels and their position boxes per MeshBlock.
-enum WorkKind { KK_FLUX = 0, KK_CORNER = 1, KK_UPDATE = 2, KK_C2P = 3, KK_N = 4 };
-
-void kind_box(const MeshK &M, int kind, int *n) {
-  if (kind == KK_FLUX) { n[0] = M.nx1 + 2; n[1] = M.nx2 + 2; n[2] = M.nx3 + 2; }
-  else if (kind == KK_C2P) { n[0] = M.nc1; n[1] = M.nc2; n[2] = M.nc3; }
-  else { n[0] = M.nx1 + 1; n[1] = M.nx2 + 1; n[2] = M.nx3 + 1; }
-}
 
 // Work-groups per MeshBlock of kind `kind` under tile T (map_work's nch).
 long kind_chunks(const MeshK &M, int kind, TileK T) {
-  int n[3];
-  kind_box(M, kind, n);
+  int lo[3], n[3];
+  kind_box(M, kind, lo, n);
   if (T.tx == 0) return (static_cast<long>(n[0]) * n[1] * n[2] + WG - 1) / WG;
   return static_cast<long>((n[0] + T.tx - 1) / T.tx) * ((n[1] + T.ty - 1) / T.ty) *
          ((n[2] + T.tz - 1) / T.tz);
@@ -2068,6 +2083,88 @@ int ath_gpu_ensemble_autotune(ath_ensemble_t e, int reps, double *best_ms) {
     HIPCHK(hipMemcpy(m->clk, &clocks[q], sizeof(Clock), hipMemcpyHostToDevice));
   }
   return ens_build(e);
+}
+
+int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n) {
+  if (kind < 0 || kind >= KK_N || nx <= 0 || ng < 0 || !lo || !n) return fail("bad argument");
+  MeshK M{};
+  M.nx1 = M.nx2 = M.nx3 = nx;
+  M.ng = ng;
+  M.nc1 = M.nc2 = M.nc3 = nx + 2 * ng;
+  M.is = M.js = M.ks = ng;
+  M.ie = M.je = M.ke = ng + nx - 1;
+  kind_box(M, kind, lo, n);
+  return 0;
+}
+
+int ath_gpu_mesh_set_tiles(ath_mesh_t m, const int *tile_idx) {
+  if (!m || !tile_idx) return fail("null argument");
+  for (int k = 0; k < KK_N; ++k)
+    if (tile_idx[k] < 0 || tile_idx[k] >= NTILES) return fail("tile index out of range");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  for (int k = 0; k < KK_N; ++k) m->tile[k] = TILES[tile_idx[k]];
+  return 0;
+}
+
+int ath_gpu_mesh_get_tiles(ath_mesh_t m, int *tile_idx) {
+  if (!m || !tile_idx) return fail("null argument");
+  for (int k = 0; k < KK_N; ++k) {
+    tile_idx[k] = 0;
+    for (int t = 0; t < NTILES; ++t)
+      if (TILES[t].tx == m->tile[k].tx && TILES[t].ty == m->tile[k].ty &&
+          TILES[t].tz == m->tile[k].tz)
+        tile_idx[k] = t;
+  }
+  return 0;
+}
+
+// Single-mesh autotune: a one-member ensemble chooses the tile shape of every kernel
+// kind (persistent grids do not apply to the single-mesh launches); the mesh's state is
+// left as it was.
+int ath_gpu_mesh_autotune(ath_mesh_t m, int reps, double *best_ms) {
+  if (!m) return fail("null argument");
+  ath_mesh_t one[1] = {m};
+  ath_ensemble_t e = nullptr;
+  if (ath_gpu_ensemble_create(one, 1, &e)) return 1;
+  const int nop[KK_N] = {0, 0, 0, 0};
+  int rc = ath_gpu_ensemble_set_tiles(e, nullptr, nop);
+  // only the tile shape is searched: persistent grids stay off
+  if (!rc) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    std::vector<Clock> clk(1);
+    HIPCHK(hipMemcpy(&clk[0], m->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+    for (int k = 0; k < KK_N && !rc; ++k) {
+      float best = 1e30f;
+      int bt = 0;
+      for (int t = 0; t < NTILES && !rc; ++t) {
+        e->tile_idx[k] = t;
+        if (ens_build(e)) { rc = 1; break; }
+        if (k == KK_C2P && ens_launch_kind(e, KK_UPDATE, 1)) { rc = 1; break; }
+        if (ens_launch_kind(e, k, 1)) { rc = 1; break; }
+        if (hipEventRecord(a, e->stream) != hipSuccess) { rc = fail("hipEventRecord"); break; }
+        for (int r = 0; r < (reps > 0 ? reps : 3) && !rc; ++r) rc = ens_launch_kind(e, k, 1);
+        if (rc || hipEventRecord(b, e->stream) != hipSuccess ||
+            hipEventSynchronize(b) != hipSuccess) { rc = rc ? rc : fail("event"); break; }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms < best) { best = ms; bt = t; }
+      }
+      e->tile_idx[k] = bt;
+      m->tile[k] = TILES[bt];
+      if (best_ms) best_ms[k] = best / (reps > 0 ? reps : 3);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!rc) {
+      HIPCHK(hipStreamSynchronize(e->stream));
+      if (ath_gpu_initialize(m, clk[0].tlim)) rc = 1;
+      else HIPCHK(hipMemcpy(m->clk, &clk[0], sizeof(Clock), hipMemcpyHostToDevice));
+    }
+  }
+  ath_gpu_ensemble_destroy(e);
+  return rc;
 }
 
 int ath_gpu_ensemble_stream(ath_ensemble_t e, void **stream) {

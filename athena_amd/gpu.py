@@ -34,7 +34,8 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_ensemble_create", "ath_gpu_ensemble_destroy", "ath_gpu_ensemble_size",
            "ath_gpu_ensemble_cycle", "ath_gpu_ensemble_get_tiles",
            "ath_gpu_ensemble_set_tiles", "ath_gpu_ensemble_autotune",
-           "ath_gpu_ensemble_stream"]
+           "ath_gpu_ensemble_stream", "ath_gpu_mesh_set_tiles", "ath_gpu_mesh_get_tiles",
+           "ath_gpu_mesh_autotune", "ath_gpu_kernel_box"]
 NKERNELS = 18
 
 
@@ -116,6 +117,10 @@ def load():
     L.ath_gpu_ensemble_set_tiles.argtypes = [P, IP, IP]
     L.ath_gpu_ensemble_autotune.argtypes = [P, I, D]
     L.ath_gpu_ensemble_stream.argtypes = [P, ctypes.POINTER(P)]
+    L.ath_gpu_mesh_set_tiles.argtypes = [P, IP]
+    L.ath_gpu_mesh_get_tiles.argtypes = [P, IP]
+    L.ath_gpu_mesh_autotune.argtypes = [P, I, D]
+    L.ath_gpu_kernel_box.argtypes = [I, I, I, IP, IP]
     _lib = L
     return L
 
@@ -250,6 +255,21 @@ class GpuMesh:
         t = (ctypes.c_ulonglong * 3)()
         _check(load().ath_gpu_mesh_c2p_stats(self.h, int(enable), t))
         return int(t[0]), int(t[1]), int(t[2])
+
+    def set_tiles(self, tile_idx):
+        t = np.ascontiguousarray(tile_idx, dtype=np.int32)
+        _check(load().ath_gpu_mesh_set_tiles(self.h, _ip(t)))
+
+    def tiles(self):
+        t = np.zeros(4, dtype=np.int32)
+        _check(load().ath_gpu_mesh_get_tiles(self.h, _ip(t)))
+        return t
+
+    def autotune(self, reps=3):
+        """Choose each kernel kind's tile shape by timing; the state is unchanged."""
+        best = np.zeros(4)
+        _check(load().ath_gpu_mesh_autotune(self.h, int(reps), _dp(best)))
+        return best
 
     def kernel_timing(self, enable=True):
         _check(load().ath_gpu_kernel_timing(self.h, int(enable)))
@@ -445,3 +465,12 @@ def sr_con2prim_device(mode, n, cons, prim, bcc, gamma, dfloor, pfloor, mlp=None
     _check_ml(load().ath_gpu_sr_con2prim_device(
         mlp.h if mlp is not None else None, int(mode), int(n), gamma, dfloor, pfloor,
         sigma_max, beta_min, gamma_max, cons, prim, bcc, iters, evals, fallback, stream))
+
+
+def kernel_box(kind, nx, ng):
+    """(lo, n) of the position box of kernel kind 0 flux, 1 corner, 2 update,
+    3 cons2prim in a MeshBlock of nx^3 cells and ng ghosts (no device call)."""
+    lo = np.zeros(3, dtype=np.int32)
+    n = np.zeros(3, dtype=np.int32)
+    _check(load().ath_gpu_kernel_box(int(kind), int(nx), int(ng), _ip(lo), _ip(n)))
+    return lo, n

@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+TILE_NAMES = ("linear", "64x4x1", "32x8x1", "16x16x1", "16x4x4", "8x8x4")
 BYTES_PER_ZONE_CYCLE = 536  # compulsory FP64 traffic of one VL2 MHD zone-cycle
 METRIC = "zone-cycles/s on 1 MI355X (3D MHD VL2+PLM+HLLD FP64) + % peak HBM GB/s"
 
@@ -400,6 +401,8 @@ def main():
     ap.add_argument("--bn", type=int, default=32, help="MeshBlock cells per direction")
     ap.add_argument("--c2p-n", type=int, default=128, help="states per direction (sr_c2p)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default thread-to-cell mapping (turbulence)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
@@ -427,6 +430,12 @@ def main():
     d.initialize()
     print("setup %.1f s: %d MeshBlocks of %s" % (time.perf_counter() - t0, m.nblocks,
                                                  m.bnx), file=sys.stderr, flush=True)
+    tiles = None
+    if not args.no_autotune:
+        d.autotune(reps=3)
+        tiles = [TILE_NAMES[t] for t in d.tiles()]
+        print("autotuned tiles (flux, corner, update, cons2prim): %s" % tiles,
+              file=sys.stderr, flush=True)
     stream = d.stream()
     ev = HipEvents()
 
@@ -482,7 +491,8 @@ def main():
            "config": {"workload": "3D ideal MHD %d^3 periodic as %d MeshBlocks of %d^3, "
                                   "decaying turbulence (seeded k^-2 velocity, rms Mach 1), "
                                   "VL2+PLM+HLLD+CT" % (args.n, m.nblocks, args.bn),
-                      "cells": cells, "meshblocks": m.nblocks, "nghost": m.ng},
+                      "cells": cells, "meshblocks": m.nblocks, "nghost": m.ng,
+                      "tiles": tiles},
            "roofline": roof, "step_roofline": step_roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

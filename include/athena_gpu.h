@@ -154,6 +154,15 @@ int ath_gpu_ensemble_set_tiles(ath_ensemble_t ens, const int *tile_idx, const in
 int ath_gpu_ensemble_autotune(ath_ensemble_t ens, int reps, double *best_ms);
 int ath_gpu_ensemble_stream(ath_ensemble_t ens, void **stream);
 
+/* Tile shape (index as for ensembles) of each kernel kind of a single mesh, and the
+ * same autotuner for a single mesh (its state is left as it was). */
+int ath_gpu_mesh_set_tiles(ath_mesh_t mesh, const int *tile_idx);
+int ath_gpu_mesh_get_tiles(ath_mesh_t mesh, int *tile_idx);
+int ath_gpu_mesh_autotune(ath_mesh_t mesh, int reps, double *best_ms);
+/* Position box (first index lo[3], extent n[3]) of kernel kind 0..3 in a MeshBlock of
+ * nx^3 active cells and ng ghosts, for host-side bounds checks; no device call. */
+int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n);
+
 /* ---- Learned initial guess for SR primitive recovery --------------------------------
  * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds
  * the Newman-Hamlin iteration with the previous gas pressure.  Here a 6 -> 64 -> 64 -> 1

@@ -1,0 +1,59 @@
+"""CPU bounds checks of the stage kernels' launch geometry (no device calls): the
+position box each kernel covers, combined with the stencil it reads, stays inside the
+MeshBlock's arrays for every reconstruction order and ghost width, and covers exactly
+the reference's loop ranges.  Stencils: donor cell reads cells i-1, i for face i
+(dc.cpp:24-103), PLM i-2..i+1 (plm.cpp:27-167), PPM i-3..i+2 (ppm.cpp:44-334); corner
+EMFs read the four cells around an edge (calculate_corner_e.cpp:187-229); cell-centred
+fields read faces i and i+1 (field.cpp:112-180)."""
+import itertools
+
+import pytest
+
+from athena_amd.gpu import kernel_box
+
+STENCIL = {1: (1, 0), 2: (2, 1), 3: (3, 2)}   # cells below / above a face
+
+
+@pytest.mark.parametrize("nx,ng,order", [(8, 2, 1), (8, 2, 2), (16, 3, 3), (32, 2, 2),
+                                         (32, 3, 3), (12, 4, 3), (16, 2, 1)])
+def test_flux_box_reads_inside_the_block(nx, ng, order):
+    if order == 3 and ng < 3:
+        pytest.skip("PPM needs NGHOST >= 3")
+    lo, n = kernel_box(0, nx, ng)
+    nc = nx + 2 * ng
+    is_, ie = ng, ng + nx - 1
+    below, above = STENCIL[order]
+    faces = set()
+    for d in range(3):
+        for c in range(lo[d], lo[d] + n[d]):
+            if c < is_:                   # the cell computes its low face only from is on
+                continue
+            assert c - below >= 0 and c + above <= nc - 1, (d, c)
+            faces.add(c)
+    # faces is..ie+1 along every direction, cells is-1..ie+1 transverse
+    assert faces == set(range(is_, ie + 2))
+    assert all(lo[d] == is_ - 1 and lo[d] + n[d] - 1 == ie + 1 for d in range(3))
+
+
+@pytest.mark.parametrize("nx,ng", [(8, 2), (32, 2), (16, 3)])
+def test_corner_update_and_recovery_boxes(nx, ng):
+    nc = nx + 2 * ng
+    is_, ie = ng, ng + nx - 1
+    for kind in (1, 2):                   # edges / CT faces is..ie+1
+        lo, n = kernel_box(kind, nx, ng)
+        for d in range(3):
+            assert lo[d] == is_ and lo[d] + n[d] - 1 == ie + 1
+            # corner EMFs read cells c-1 and c; face arrays have nc+1 entries
+            assert lo[d] - 1 >= 0 and lo[d] + n[d] - 1 <= nc
+    lo, n = kernel_box(3, nx, ng)         # every cell; reads faces c and c+1 <= nc
+    assert list(lo) == [0, 0, 0] and list(n) == [nc] * 3
+
+
+def test_boxes_fit_every_tile_shape():
+    # the tile shapes mask positions beyond the box; only the work-group count changes
+    tiles = [(64, 4, 1), (32, 8, 1), (16, 16, 1), (16, 4, 4), (8, 8, 4)]
+    for (tx, ty, tz), kind in itertools.product(tiles, range(4)):
+        assert tx * ty * tz == 256
+        lo, n = kernel_box(kind, 32, 2)
+        chunks = -(-n[0] // tx) * -(-n[1] // ty) * -(-n[2] // tz)
+        assert chunks * 256 >= n[0] * n[1] * n[2]

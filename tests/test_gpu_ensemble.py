@@ -149,3 +149,21 @@ def test_full_ensemble_builds_and_steps():
         assert nc == 2 and t > 0 and np.isfinite(dt)
         u = d.get(0, "u")
         assert np.all(np.isfinite(u))
+
+
+def test_single_mesh_tiles_and_autotune(solo):
+    print("case single-mesh tiles and autotune", flush=True)
+    members, snaps = solo
+    m, ic = members[2]
+    d = fresh([(m, ic)])[0]
+    before = snapshot(d)
+    d.autotune(reps=2)
+    print("  chosen tiles %s" % d.tiles(), flush=True)
+    assert_same(snapshot(d), before)
+    d.run(4)
+    assert_same(snapshot(d), snaps[3][2])
+    d2 = fresh([(m, ic)])[0]
+    d2.set_tiles([4, 5, 3, 2])
+    assert list(d2.tiles()) == [4, 5, 3, 2]
+    d2.run(4)
+    assert_same(snapshot(d2), snaps[3][2])
