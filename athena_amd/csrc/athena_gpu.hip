@@ -891,6 +891,81 @@ __global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restr
   if (live) lseed[static_cast<long>(g) * M.ncell + c] = sd;
 }
 
+//========================================================================================
+// History sums (outputs/history.cpp:84-152, MHD without gravity/scalars): volume-
+// weighted mass, momenta, kinetic energy by direction, total energy and magnetic energy
+// by direction over the active cells.  Per-work-group partial sums, then one work-group
+// adds the partials in a fixed order, so the result does not depend on scheduling.
+constexpr int NHIST = 11;
+
+__global__ void __launch_bounds__(WG) k_history(MeshK M, const double *__restrict__ u,
+                                                 const double *__restrict__ bcc,
+                                                 double *__restrict__ part) {
+  const int nact = M.nx1 * M.nx2 * M.nx3;
+  const int nch = gridDim.x / M.nblocks;
+  const int g = blockIdx.x / nch;
+  const int t = (blockIdx.x % nch) * WG + threadIdx.x;
+  double h[NHIST];
+#pragma unroll
+  for (int q = 0; q < NHIST; ++q) h[q] = 0.0;
+  if (t < nact) {
+    const int i = M.is + t % M.nx1;
+    const int j = M.js + (t / M.nx1) % M.nx2;
+    const int k = M.ks + t / (M.nx1 * M.nx2);
+    const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
+    const double vol = dx1[i] * dx2[j] * dx3[k];
+    const long n = M.ncell, c = cidx(M, k, j, i);
+    const double *ub = u + static_cast<long>(g) * NHYDRO * n;
+    const double *bb = bcc + static_cast<long>(g) * 3 * n;
+    const double u_d = ub[IDN * n + c], u_mx = ub[IM1 * n + c];
+    const double u_my = ub[IM2 * n + c], u_mz = ub[IM3 * n + c], u_e = ub[IEN * n + c];
+    const double b1 = bb[IB1 * n + c], b2 = bb[IB2 * n + c], b3 = bb[IB3 * n + c];
+    h[0] = vol * u_d;
+    h[1] = vol * u_mx;
+    h[2] = vol * u_my;
+    h[3] = vol * u_mz;
+    h[4] = vol * 0.5 * sq(u_mx) / u_d;
+    h[5] = vol * 0.5 * sq(u_my) / u_d;
+    h[6] = vol * 0.5 * sq(u_mz) / u_d;
+    h[7] = vol * u_e;
+    h[8] = vol * 0.5 * b1 * b1;
+    h[9] = vol * 0.5 * b2 * b2;
+    h[10] = vol * 0.5 * b3 * b3;
+  }
+  __shared__ double sm[NHIST][WG / 64];
+#pragma unroll
+  for (int q = 0; q < NHIST; ++q) {
+    double v = h[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sm[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NHIST) {
+    double v = sm[threadIdx.x][0];
+    for (int w = 1; w < WG / 64; ++w) v += sm[threadIdx.x][w];
+    part[static_cast<long>(blockIdx.x) * NHIST + threadIdx.x] = v;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_history_final(const double *__restrict__ part,
+                                                         long npart, double *out) {
+  __shared__ double sm[NHIST][1024 / 64];
+  for (int q = 0; q < NHIST; ++q) {
+    double v = 0.0;
+    for (long p = threadIdx.x; p < npart; p += 1024) v += part[p * NHIST + q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sm[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NHIST) {
+    double v = sm[threadIdx.x][0];
+    for (int w = 1; w < 1024 / 64; ++w) v += sm[threadIdx.x][w];
+    out[threadIdx.x] = v;
+  }
+}
+
 // Final reduction + Mesh::NewTimeStep (mesh.cpp:1236-1282) and, when advance != 0,
 // the end-of-cycle clock update of the main loop (main.cpp:535-541).
 __device__ __forceinline__ void new_dt_body(const double *__restrict__ part, long npart,
@@ -1112,6 +1187,8 @@ struct ath_mesh {
   Clock *clk = nullptr;
   unsigned long long *c2p_stats_d = nullptr;   // SR recovery counters, allocated on demand
   double *lseed = nullptr;   // learned pressure seeds (one scalar register), on demand
+  double *hist_d = nullptr;  // history partial sums, on demand
+  long nhist = 0;
   std::vector<void *> allocs;
   // optional per-kernel event timing (ath_gpu_kernel_timing)
   bool prof_on = false;
@@ -1784,6 +1861,7 @@ int ath_gpu_mesh_destroy(ath_mesh_t m) {
     (void)hipEventDestroy(r.b);
   }
   for (void *p : m->allocs) (void)hipFree(p);
+  if (m->hist_d) (void)hipFree(m->hist_d);
   (void)hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -2083,6 +2161,29 @@ int ath_gpu_ensemble_autotune(ath_ensemble_t e, int reps, double *best_ms) {
     HIPCHK(hipMemcpy(m->clk, &clocks[q], sizeof(Clock), hipMemcpyHostToDevice));
   }
   return ens_build(e);
+}
+
+int ath_gpu_history(ath_mesh_t m, double *sums) {
+  if (!m || !sums) return fail("null argument");
+  const MeshK &M = m->M;
+  const long nact = static_cast<long>(M.nx1) * M.nx2 * M.nx3;
+  const long nch = (nact + WG - 1) / WG;
+  const long nwg = nch * M.nblocks;
+  if (!m->hist_d || m->nhist < nwg) {
+    if (m->hist_d) HIPCHK(hipFree(m->hist_d));
+    m->hist_d = nullptr;
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&m->hist_d),
+                     sizeof(double) * (nwg + 1) * NHIST));
+    m->nhist = nwg;
+  }
+  double *out = m->hist_d + nwg * NHIST;
+  hipLaunchKernelGGL(k_history, dim3(static_cast<unsigned>(nwg)), dim3(WG), 0, m->stream, M,
+                     m->u, m->bcc, m->hist_d);
+  hipLaunchKernelGGL(k_history_final, dim3(1), dim3(1024), 0, m->stream, m->hist_d, nwg, out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(sums, out, sizeof(double) * NHIST, hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
 }
 
 int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n) {

@@ -35,7 +35,7 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_ensemble_cycle", "ath_gpu_ensemble_get_tiles",
            "ath_gpu_ensemble_set_tiles", "ath_gpu_ensemble_autotune",
            "ath_gpu_ensemble_stream", "ath_gpu_mesh_set_tiles", "ath_gpu_mesh_get_tiles",
-           "ath_gpu_mesh_autotune", "ath_gpu_kernel_box"]
+           "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history"]
 NKERNELS = 18
 
 
@@ -121,6 +121,7 @@ def load():
     L.ath_gpu_mesh_get_tiles.argtypes = [P, IP]
     L.ath_gpu_mesh_autotune.argtypes = [P, I, D]
     L.ath_gpu_kernel_box.argtypes = [I, I, I, IP, IP]
+    L.ath_gpu_history.argtypes = [P, D]
     _lib = L
     return L
 
@@ -255,6 +256,12 @@ class GpuMesh:
         t = (ctypes.c_ulonglong * 3)()
         _check(load().ath_gpu_mesh_c2p_stats(self.h, int(enable), t))
         return int(t[0]), int(t[1]), int(t[2])
+
+    def history(self):
+        """The 11 history sums (mass, 1..3-mom, 1..3-KE, tot-E, 1..3-ME)."""
+        h = np.zeros(11)
+        _check(load().ath_gpu_history(self.h, _dp(h)))
+        return h
 
     def set_tiles(self, tile_idx):
         t = np.ascontiguousarray(tile_idx, dtype=np.int32)

@@ -104,6 +104,11 @@ int ath_gpu_kernel_timing(ath_mesh_t mesh, int enable);
 int ath_gpu_kernel_stats(ath_mesh_t mesh, int id, char *name, int namelen,
                          double *total_ms, long *count);
 
+/* History sums (outputs/history.cpp:84-152 for MHD): mass, 1-mom, 2-mom, 3-mom, 1-KE,
+ * 2-KE, 3-KE, tot-E, 1-ME, 2-ME, 3-ME, volume-weighted over the active cells of all
+ * blocks (11 doubles).  Summed in a fixed order; synchronous. */
+int ath_gpu_history(ath_mesh_t mesh, double *sums);
+
 /* Hydro stream the library launches on (a hipStream_t), for event timing. */
 int ath_gpu_stream(ath_mesh_t mesh, void **stream);
 int ath_gpu_synchronize(ath_mesh_t mesh);

@@ -27,17 +27,24 @@ def rel_l1(a, b):
     return num / den if den > 0 else num
 
 
-def compare(m, get_a, get_b, keys=ACTIVE_KEYS):
-    """Per-variable relative L1 and bitwise-equal flags over all blocks."""
+def compare(m, get_a, get_b, keys=ACTIVE_KEYS, floor=0.0):
+    """Per-variable relative L1 and bitwise-equal flags over all blocks.  With floor > 0
+    a component's L1 norm is floored at `floor` times the largest component norm of its
+    register, so a component that vanishes analytically and holds only round-off is
+    judged on its register's scale instead of its own."""
     out = {}
     for k in keys:
         A = [active(m, k, get_a(g, k)) for g in range(m.nblocks)]
         B = [active(m, k, get_b(g, k)) for g in range(m.nblocks)]
         if k in ("u", "w", "bcc"):
-            for n in range(A[0].shape[0]):
-                a = np.concatenate([x[n].ravel() for x in A])
-                b = np.concatenate([x[n].ravel() for x in B])
-                out["%s%d" % (k, n)] = (rel_l1(a, b), bool(np.array_equal(a, b)))
+            nv = A[0].shape[0]
+            comps = [(np.concatenate([x[n].ravel() for x in A]),
+                      np.concatenate([x[n].ravel() for x in B])) for n in range(nv)]
+            big = max(np.sum(np.abs(b)) for _, b in comps)
+            for n, (a, b) in enumerate(comps):
+                den = max(np.sum(np.abs(b)), floor * big)
+                r = np.sum(np.abs(a - b)) / den if den > 0 else np.sum(np.abs(a - b))
+                out["%s%d" % (k, n)] = (r, bool(np.array_equal(a, b)))
         else:
             a = np.concatenate([x.ravel() for x in A])
             b = np.concatenate([x.ravel() for x in B])
