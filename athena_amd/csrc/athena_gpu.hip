@@ -560,9 +560,8 @@ __device__ __forceinline__ double emf1(const MeshK &M, const double *e, int g, i
 
 //========================================================================================
 // Kernel 3a: conserved update, MeshBlock::WeightedAve + Hydro::AddFluxDivergence
-// (weighted_ave.cpp:35-…, add_flux_divergence.cpp:39-96) for active cells, followed by
-// the same-level ghost exchange done as a push: each updated cell is also written to
-// every neighbour ghost cell that is a copy of it (cc/bvals_cc.cpp:287-427).
+// (weighted_ave.cpp:35-…, add_flux_divergence.cpp:39-96) for active cells; the ghost
+// copies are written afterwards by the exchange kernel (k_ghost).
 // first_stage: out = (0 + 1*base) - wght*div F   (u1 zeroed, delta = 1, then swapped)
 // otherwise:   out = base - wght*div F           (in place allowed: out == base)
 __device__ __forceinline__ void update_cell_u(const MeshK &M, int g, int k, int j, int i,
@@ -583,12 +582,6 @@ __device__ __forceinline__ void update_cell_u(const MeshK &M, int g, int k, int 
   const double *F3 = f3 + static_cast<long>(g) * NHYDRO * M.nf3;
   const long x1 = f1idx(M, k, j, i), x2 = f2idx(M, k, j, i), x3 = f3idx(M, k, j, i);
   const long sx2 = M.nc1, sx3 = static_cast<long>(M.nc2) * M.nc1;
-
-  // ghost targets of this cell
-  int lo1 = (i - M.is < M.ng) ? -1 : 0, hi1 = (M.ie - i < M.ng) ? 1 : 0;
-  int lo2 = (j - M.js < M.ng) ? -1 : 0, hi2 = (M.je - j < M.ng) ? 1 : 0;
-  int lo3 = (k - M.ks < M.ng) ? -1 : 0, hi3 = (M.ke - k < M.ng) ? 1 : 0;
-
 #pragma unroll
   for (int n = 0; n < NHYDRO; ++n) {
     double dflx = (a1 * F1[n * M.nf1 + x1 + 1] - a1 * F1[n * M.nf1 + x1]);
@@ -596,21 +589,12 @@ __device__ __forceinline__ void update_cell_u(const MeshK &M, int g, int k, int 
     dflx += (a3 * F3[n * M.nf3 + x3 + sx3] - a3 * F3[n * M.nf3 + x3]);
     double u0 = base[b0 + n * M.ncell + c];
     if (first_stage) u0 = 0.0 + 1.0 * u0;
-    double v = u0 - wght * dflx / vol;
-    out[b0 + n * M.ncell + c] = v;
-    for (int p3 = lo3; p3 <= hi3; ++p3)
-      for (int p2 = lo2; p2 <= hi2; ++p2)
-        for (int p1 = lo1; p1 <= hi1; ++p1) {
-          if (!p1 && !p2 && !p3) continue;
-          const int tg = nbr(M, g, p1, p2, p3);
-          out[static_cast<long>(tg) * NHYDRO * M.ncell + n * M.ncell +
-              cidx(M, k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1)] = v;
-        }
+    out[b0 + n * M.ncell + c] = u0 - wght * dflx / vol;
   }
 }
 
 // Face-field update, WeightedAve(FaceField) + Field::CT (ct.cpp:31-116) with averaged
-// surface EMFs, then the face-centred ghost push (fc/bvals_fc.cpp:344-688).
+// surface EMFs.
 template <int comp>
 __device__ __forceinline__ void update_face_b(const MeshK &M, int g, int k, int j, int i,
                                               const double *bin, double *outp,
@@ -646,35 +630,11 @@ __device__ __forceinline__ void update_face_b(const MeshK &M, int g, int k, int 
     v -= (wght / area) * (l2 * emf2(M, e2, g, k, j, i + 1) - l2 * emf2(M, e2, g, k, j, i));
     v += (wght / area) * (l1 * emf1(M, e1, g, k, j + 1, i) - l1 * emf1(M, e1, g, k, j, i));
   }
-  auto fidx = [&](int kk, int jj, int ii) -> long {
-    if (comp == 0) return f1idx(M, kk, jj, ii);
-    if (comp == 1) return f2idx(M, kk, jj, ii);
-    return f3idx(M, kk, jj, ii);
-  };
-  outp[static_cast<long>(g) * nf + fidx(k, j, i)] = v;
-  // ghost push: along the face-normal direction the copies come from [s+1, s+ng] and
-  // [e-ng+1, e] (shared faces s and e+1 stay with their owner); transverse directions
-  // behave like cells.
-  int lo[3], hi[3];
-  const int idx[3] = {i, j, k};
-  const int s_[3] = {M.is, M.js, M.ks}, e_[3] = {M.ie, M.je, M.ke};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    if (d == comp) {
-      lo[d] = (idx[d] >= s_[d] + 1 && idx[d] <= s_[d] + M.ng) ? -1 : 0;
-      hi[d] = (idx[d] >= e_[d] - M.ng + 1 && idx[d] <= e_[d]) ? 1 : 0;
-    } else {
-      lo[d] = (idx[d] - s_[d] < M.ng) ? -1 : 0;
-      hi[d] = (e_[d] - idx[d] < M.ng) ? 1 : 0;
-    }
-  }
-  for (int p3 = lo[2]; p3 <= hi[2]; ++p3)
-    for (int p2 = lo[1]; p2 <= hi[1]; ++p2)
-      for (int p1 = lo[0]; p1 <= hi[0]; ++p1) {
-        if (!p1 && !p2 && !p3) continue;
-        const int tg = nbr(M, g, p1, p2, p3);
-        outp[static_cast<long>(tg) * nf + fidx(k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1)] = v;
-      }
+  long fi;
+  if (comp == 0) fi = f1idx(M, k, j, i);
+  else if (comp == 1) fi = f2idx(M, k, j, i);
+  else fi = f3idx(M, k, j, i);
+  outp[static_cast<long>(g) * nf + fi] = v;
 }
 
 // One thread per (k,j,i) in [ks,ke+1]x[js,je+1]x[is,ie+1]: the active cell's conserved
@@ -709,6 +669,120 @@ __global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double bet
                                                 int first_stage,
                                                 const Clock *__restrict__ clk, TileK T) {
   update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+}
+
+//========================================================================================
+// Kernel 3b: same-level ghost exchange on a periodic uniform mesh as a gather
+// (cc/bvals_cc.cpp:287-427, fc/bvals_fc.cpp:344-688): every ghost position of the
+// cell-centred register (5 components) and of the three face-field components takes the
+// value of its source, the active cell or face of a neighbour MeshBlock that the
+// reference's send buffers carry there.  Every array of a block has NGHOST ghost layers
+// on each side of each direction, active positions [ng, n-ng-1] (for a face component
+// along its normal that is [s, e+1], the shared faces staying with their owner), and a
+// ghost position idx < ng copies idx + nx of the low neighbour, idx > n-ng-1 copies
+// idx - nx of the high neighbour.  One thread per (component, ghost position); the
+// ghost positions are numbered in memory order, so consecutive threads write
+// consecutive addresses: the ng low ghost planes, per active plane its ng low ghost rows,
+// the 2*ng ends of each active row and its ng high ghost rows, then the ng high planes.
+struct GhostArgs {
+  double *u;
+  double *b[3];
+};
+
+__host__ __device__ __forceinline__ long ghost_count(int n1, int n2, int n3, int ng) {
+  return static_cast<long>(n1) * n2 * n3 -
+         static_cast<long>(n1 - 2 * ng) * (n2 - 2 * ng) * (n3 - 2 * ng);
+}
+
+// ghost number t of an (n3, n2, n1) array -> (k, j, i)
+__host__ __device__ __forceinline__ void ghost_pos(int t, int n1, int n2, int n3, int ng,
+                                                   int &i, int &j, int &k) {
+  const int plane = n1 * n2;
+  const int m2 = n2 - 2 * ng, m3 = n3 - 2 * ng;
+  const int rows = ng * n1, ends = m2 * 2 * ng;
+  const int per = 2 * rows + ends;               // ghost positions of one active plane
+  if (t < ng * plane || t >= ng * plane + m3 * per) {   // a whole ghost plane
+    if (t >= ng * plane) t -= m3 * per - m3 * plane;    // high planes follow the active
+    const int q = t / plane, r = t - q * plane;
+    k = q;
+    j = r / n1;
+    i = r - j * n1;
+    return;
+  }
+  t -= ng * plane;
+  const int q = t / per;
+  int r = t - q * per;
+  k = ng + q;
+  if (r < rows) {
+    j = r / n1;
+    i = r - j * n1;
+  } else if (r < rows + ends) {
+    r -= rows;
+    const int row = r / (2 * ng), x = r - row * 2 * ng;
+    j = ng + row;
+    i = (x < ng) ? x : n1 - 2 * ng + x;
+  } else {
+    r -= rows + ends;
+    const int row = r / n1;
+    j = n2 - ng + row;
+    i = r - row * n1;
+  }
+}
+
+__host__ __device__ __forceinline__ int ghost_off(int idx, int n, int ng) {
+  return (idx < ng) ? -1 : ((idx > n - ng - 1) ? 1 : 0);
+}
+
+// Work items of one block: 5 cell components, then the x1, x2, x3 face components.
+__host__ __device__ __forceinline__ long ghost_items(const MeshK &M) {
+  return NHYDRO * ghost_count(M.nc1, M.nc2, M.nc3, M.ng) +
+         ghost_count(M.nc1 + 1, M.nc2, M.nc3, M.ng) +
+         ghost_count(M.nc1, M.nc2 + 1, M.nc3, M.ng) +
+         ghost_count(M.nc1, M.nc2, M.nc3 + 1, M.ng);
+}
+
+__device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, int lw,
+                                           int nwg) {
+  const int nch = nwg / M.nblocks;
+  const int g = lw / nch;
+  int t = (lw - g * nch) * WG + threadIdx.x;
+  const int ng = M.ng;
+  const int nc_ = static_cast<int>(ghost_count(M.nc1, M.nc2, M.nc3, ng));
+  int n1 = M.nc1, n2 = M.nc2, n3 = M.nc3;
+  double *arr;
+  long stride;                                   // one component of one block
+  if (t < NHYDRO * nc_) {
+    const int v = t / nc_;
+    t -= v * nc_;
+    stride = NHYDRO * M.ncell;
+    arr = a.u + v * M.ncell;
+  } else {
+    t -= NHYDRO * nc_;
+    const int c1 = static_cast<int>(ghost_count(M.nc1 + 1, M.nc2, M.nc3, ng));
+    const int c2 = static_cast<int>(ghost_count(M.nc1, M.nc2 + 1, M.nc3, ng));
+    const int c3 = static_cast<int>(ghost_count(M.nc1, M.nc2, M.nc3 + 1, ng));
+    if (t < c1) {
+      n1 += 1; arr = a.b[0]; stride = M.nf1;
+    } else if (t < c1 + c2) {
+      t -= c1; n2 += 1; arr = a.b[1]; stride = M.nf2;
+    } else if (t < c1 + c2 + c3) {
+      t -= c1 + c2; n3 += 1; arr = a.b[2]; stride = M.nf3;
+    } else {
+      return;
+    }
+  }
+  int i, j, k;
+  ghost_pos(t, n1, n2, n3, ng, i, j, k);
+  const int p1 = ghost_off(i, n1, ng), p2 = ghost_off(j, n2, ng), p3 = ghost_off(k, n3, ng);
+  const int src = nbr(M, g, p1, p2, p3);
+  const long own = (static_cast<long>(k) * n2 + j) * n1 + i;
+  const long from =
+      (static_cast<long>(k - p3 * M.nx3) * n2 + (j - p2 * M.nx2)) * n1 + (i - p1 * M.nx1);
+  arr[static_cast<long>(g) * stride + own] = arr[static_cast<long>(src) * stride + from];
+}
+
+__global__ void __launch_bounds__(WG) k_ghost(MeshK M, GhostArgs a) {
+  ghost_body(M, a, xcd_logical_wg(M.xcd_remap), gridDim.x);
 }
 
 //========================================================================================
@@ -1011,10 +1085,12 @@ struct EnsMember {
   FaceOut fo[3];
   CornerArgs ca;
   UpdateArgs ua;
+  GhostArgs ga;
   C2PArgs pa;
   Clock *clk;
   int nwg[4];       // work-groups of each work kind
   TileK tile[4];
+  int nwg_ghost;    // work-groups of the ghost exchange
 };
 
 __device__ __forceinline__ int2 ens_item(const int2 *__restrict__ map, int v) {
@@ -1043,6 +1119,15 @@ __global__ void __launch_bounds__(WG) k_corner_ens(const EnsMember *__restrict__
   }
 }
 
+__global__ void __launch_bounds__(WG) k_ghost_ens(const EnsMember *__restrict__ E,
+                                                   const int2 *__restrict__ map, int nwork) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    ghost_body(e.M, e.ga, mw.y, e.nwg_ghost);
+  }
+}
+
 __global__ void __launch_bounds__(WG) k_update_ens(const EnsMember *__restrict__ E,
                                                     const int2 *__restrict__ map, int nwork,
                                                     double beta, int first_stage) {
@@ -1068,59 +1153,6 @@ __global__ void __launch_bounds__(1024) k_new_dt_ens(const EnsMember *__restrict
                                                       int advance) {
   const EnsMember &e = E[blockIdx.x];
   new_dt_body(e.pa.dt_partial, e.nwg[3], e.clk, e.M.cfl, advance);
-}
-
-// Pull-based same-level ghost fill for Mesh::Initialize (cell-centred register with
-// nvar components).
-__global__ void __launch_bounds__(WG) k_ghost_cc(MeshK M, double *a, int nvar) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / M.nblocks;
-  const int g = lw / nch;
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= M.ncell) return;
-  const int i = t % M.nc1;
-  const int r = t / M.nc1;
-  const int j = r % M.nc2;
-  const int k = r / M.nc2;
-  const int p1 = (i < M.is) ? -1 : ((i > M.ie) ? 1 : 0);
-  const int p2 = (j < M.js) ? -1 : ((j > M.je) ? 1 : 0);
-  const int p3 = (k < M.ks) ? -1 : ((k > M.ke) ? 1 : 0);
-  if (!p1 && !p2 && !p3) return;
-  const int src = nbr(M, g, p1, p2, p3);
-  const long cs = cidx(M, k - p3 * M.nx3, j - p2 * M.nx2, i - p1 * M.nx1);
-  for (int v = 0; v < nvar; ++v)
-    a[(static_cast<long>(g) * nvar + v) * M.ncell + t] =
-        a[(static_cast<long>(src) * nvar + v) * M.ncell + cs];
-}
-
-__global__ void __launch_bounds__(WG) k_ghost_fc(MeshK M, double *b1, double *b2,
-                                                  double *b3) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
-  const int nch = gridDim.x / (3 * M.nblocks);
-  const int g = lw / (3 * nch);
-  const int comp = (lw / nch) % 3;
-  const int n1 = M.nc1 + (comp == 0), n2 = M.nc2 + (comp == 1), n3 = M.nc3 + (comp == 2);
-  const int t = (lw % nch) * WG + threadIdx.x;
-  if (t >= n1 * n2 * n3) return;
-  const int i = t % n1;
-  const int r = t / n1;
-  const int j = r % n2;
-  const int k = r / n2;
-  const int idx[3] = {i, j, k};
-  const int s_[3] = {M.is, M.js, M.ks}, e_[3] = {M.ie, M.je, M.ke};
-  int p[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int hiend = (d == comp) ? e_[d] + 1 : e_[d];
-    p[d] = (idx[d] < s_[d]) ? -1 : ((idx[d] > hiend) ? 1 : 0);
-  }
-  if (!p[0] && !p[1] && !p[2]) return;
-  const int src = nbr(M, g, p[0], p[1], p[2]);
-  double *a = (comp == 0) ? b1 : ((comp == 1) ? b2 : b3);
-  const long nf = (comp == 0) ? M.nf1 : ((comp == 1) ? M.nf2 : M.nf3);
-  const long ls = ((static_cast<long>(k - p[2] * M.nx3) * n2) + (j - p[1] * M.nx2)) * n1 +
-                  (i - p[0] * M.nx1);
-  a[static_cast<long>(g) * nf + t] = a[static_cast<long>(src) * nf + ls];
 }
 
 // Batched single-interface Riemann solves (known-answer vectors)
@@ -1225,7 +1257,7 @@ const char *kernel_name(int id) {
     case KID_SEED: return "c2p_learned_seed";
     case KID_CONS2PRIM: return "cons2prim";
     case KID_NEW_DT: return "new_dt";
-    case KID_GHOST: return "ghost_fill";
+    case KID_GHOST: return "ghost_exchange";
     case KID_FLUX3: return "flux_xyz_dc";
     case KID_FLUX3 + 1: return "flux_xyz_plm";
     case KID_FLUX3 + 2: return "flux_xyz_ppm";
@@ -1307,6 +1339,7 @@ struct StageArgs {
   FaceOut fo[3];
   CornerArgs ca;
   UpdateArgs ua;
+  GhostArgs ga;
   C2PArgs pa;
 };
 
@@ -1337,6 +1370,7 @@ StageArgs stage_args(const ath_mesh *m, const Regs &r, int stg) {
     ua.bout[c] = r.bb[c];
   }
   ua.e1 = m->e[0]; ua.e2 = m->e[1]; ua.e3 = m->e[2];
+  a.ga = GhostArgs{ua.uout, {ua.bout[0], ua.bout[1], ua.bout[2]}};
   // after the update's swap, u is the updated state and b the updated faces
   a.pa = C2PArgs{r.u1, r.w1, m->bcc, r.bb[0], r.bb[1], r.bb[2],
                  (stg == 2) ? m->dtpart : nullptr, m->lseed};
@@ -1484,6 +1518,17 @@ int launch_update_args(ath_mesh *m, const UpdateArgs &ua, double beta, int first
   return 0;
 }
 
+int ghost_grid(const MeshK &M) {
+  return static_cast<int>((ghost_items(M) + WG - 1) / WG) * M.nblocks;
+}
+
+int launch_ghost_args(ath_mesh *m, const GhostArgs &ga) {
+  ProfScope ps(m, KID_GHOST);
+  hipLaunchKernelGGL(k_ghost, dim3(ghost_grid(m->M)), dim3(WG), 0, m->stream, m->M, ga);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 void swap_stage_regs(Regs &r) {   // the two swaps of one stage
   std::swap(r.u, r.u1);
   for (int c = 0; c < 3; ++c) std::swap(r.b[c], r.bb[c]);
@@ -1501,6 +1546,7 @@ int stage(ath_mesh *m, int stg) {
   if (launch_flux_args(m, a.order, a.w, a.bcc, a.fo)) return 1;
   if (launch_corner_args(m, a.ca)) return 1;
   if (launch_update_args(m, a.ua, a.beta, a.first)) return 1;
+  if (launch_ghost_args(m, a.ga)) return 1;
   if (launch_cons2prim_args(m, a.pa)) return 1;
   swap_stage_regs(r);
   set_regs(m, r);
@@ -1543,6 +1589,8 @@ struct ath_ensemble {
   EnsMember *tab_d[2] = {nullptr, nullptr};   // stage-1 and stage-2 member tables
   int2 *map_d[KK_N] = {nullptr, nullptr, nullptr, nullptr};
   int nwork[KK_N] = {0, 0, 0, 0};
+  int2 *map_ghost_d = nullptr;                // work items of the ghost exchange
+  int nwork_ghost = 0;
   int tile_idx[KK_N] = {0, 0, 0, 0};          // index into TILES
   int persist[KK_N] = {0, 0, 0, 0};           // 0: one work item per work-group; else grid
   std::vector<long> parity;                   // members' stage count parity at creation
@@ -1553,6 +1601,7 @@ struct ath_ensemble {
     if (graph) (void)hipGraphDestroy(graph);
     for (auto *p : tab_d) if (p) (void)hipFree(p);
     for (auto *p : map_d) if (p) (void)hipFree(p);
+    if (map_ghost_d) (void)hipFree(map_ghost_d);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -1572,7 +1621,7 @@ int ens_build(ath_ensemble *e) {
   ens_drop_graph(e);
   const int n = static_cast<int>(e->mem.size());
   std::vector<EnsMember> tab[2];
-  std::vector<int2> map[KK_N];
+  std::vector<int2> map[KK_N], mapg;
   for (int q = 0; q < n; ++q) {
     ath_mesh *m = e->mem[q];
     Regs r = regs_of(m);
@@ -1585,6 +1634,8 @@ int ens_build(ath_ensemble *e) {
       for (int d = 0; d < 3; ++d) em.fo[d] = a.fo[d];
       em.ca = a.ca;
       em.ua = a.ua;
+      em.ga = a.ga;
+      em.nwg_ghost = ghost_grid(m->M);
       em.pa = a.pa;
       em.clk = m->clk;
       for (int k = 0; k < KK_N; ++k) {
@@ -1599,6 +1650,7 @@ int ens_build(ath_ensemble *e) {
       const int nw = tab[0].back().nwg[k];
       for (int w = 0; w < nw; ++w) map[k].push_back(make_int2(q, w));
     }
+    for (int w = 0; w < tab[0].back().nwg_ghost; ++w) mapg.push_back(make_int2(q, w));
   }
   for (int t = 0; t < 2; ++t) {
     if (e->tab_d[t]) HIPCHK(hipFree(e->tab_d[t]));
@@ -1615,6 +1667,12 @@ int ens_build(ath_ensemble *e) {
     HIPCHK(hipMemcpy(e->map_d[k], map[k].data(), sizeof(int2) * map[k].size(),
                      hipMemcpyHostToDevice));
   }
+  if (e->map_ghost_d) HIPCHK(hipFree(e->map_ghost_d));
+  e->map_ghost_d = nullptr;
+  e->nwork_ghost = static_cast<int>(mapg.size());
+  HIPCHK(hipMalloc(reinterpret_cast<void **>(&e->map_ghost_d), sizeof(int2) * mapg.size()));
+  HIPCHK(hipMemcpy(e->map_ghost_d, mapg.data(), sizeof(int2) * mapg.size(),
+                   hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1649,6 +1707,9 @@ int ens_launch_kind(ath_ensemble *e, int kind, int stg) {
   } else if (kind == KK_UPDATE) {
     hipLaunchKernelGGL(k_update_ens, g, dim3(WG), 0, e->stream, T, mp, nw,
                        (stg == 1) ? 0.5 : 1.0, (stg == 1) ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_ghost_ens, dim3(static_cast<unsigned>(e->nwork_ghost)), dim3(WG), 0,
+                       e->stream, T, e->map_ghost_d, e->nwork_ghost);
   } else {
     if (e->sr) hipLaunchKernelGGL(k_c2p_ens<true>, g, dim3(WG), 0, e->stream, T, mp, nw);
     else hipLaunchKernelGGL(k_c2p_ens<false>, g, dim3(WG), 0, e->stream, T, mp, nw);
@@ -1657,7 +1718,8 @@ int ens_launch_kind(ath_ensemble *e, int kind, int stg) {
   return 0;
 }
 
-// One VL2 cycle of every member: 4 launches per stage and one dt launch.
+// One VL2 cycle of every member: 5 launches per stage (the update followed by the ghost
+// exchange) and one dt launch.
 int ens_record_cycle(ath_ensemble *e) {
   for (int stg = 1; stg <= 2; ++stg)
     for (int k = 0; k < KK_N; ++k)
@@ -1908,17 +1970,7 @@ int ath_gpu_initialize(ath_mesh_t m, double tlim) {
   const MeshK &M = m->M;
   Clock c0{0.0, std::numeric_limits<double>::max(), tlim, 0.0};
   HIPCHK(hipMemcpyAsync(m->clk, &c0, sizeof(Clock), hipMemcpyHostToDevice, m->stream));
-  {
-    dim3 grid(((M.ncell + WG - 1) / WG) * M.nblocks);
-    hipLaunchKernelGGL(k_ghost_cc, grid, dim3(WG), 0, m->stream, M, m->u, NHYDRO);
-    HIPCHK(hipGetLastError());
-  }
-  {
-    long n = std::max(M.nf1, std::max(M.nf2, M.nf3));
-    dim3 grid(((n + WG - 1) / WG) * M.nblocks * 3);
-    hipLaunchKernelGGL(k_ghost_fc, grid, dim3(WG), 0, m->stream, M, m->b[0], m->b[1], m->b[2]);
-    HIPCHK(hipGetLastError());
-  }
+  if (launch_ghost_args(m, GhostArgs{m->u, {m->b[0], m->b[1], m->b[2]}})) return 1;
   // ConservedToPrimitive(u, w1, b, w, bcc): output register is w
   if (launch_cons2prim(m, m->w, true)) return 1;
   if (launch_new_dt(m, 0)) return 1;
@@ -2195,6 +2247,20 @@ int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n) {
   M.is = M.js = M.ks = ng;
   M.ie = M.je = M.ke = ng + nx - 1;
   kind_box(M, kind, lo, n);
+  return 0;
+}
+
+int ath_gpu_ghost_position(int nx, int ng, int array, int t, int *pos) {
+  if (!pos) return fail("null argument");
+  if (nx < 1 || ng < 1 || array < 0 || array > 3) return fail("bad argument");
+  const int nc = nx + 2 * ng;
+  const int n1 = nc + (array == 1), n2 = nc + (array == 2), n3 = nc + (array == 3);
+  if (t < 0 || t >= ghost_count(n1, n2, n3, ng)) return fail("ghost number out of range");
+  int i, j, k;
+  ghost_pos(t, n1, n2, n3, ng, i, j, k);
+  const int p1 = ghost_off(i, n1, ng), p2 = ghost_off(j, n2, ng), p3 = ghost_off(k, n3, ng);
+  const int v[9] = {i, j, k, i - p1 * nx, j - p2 * nx, k - p3 * nx, p1, p2, p3};
+  for (int q = 0; q < 9; ++q) pos[q] = v[q];
   return 0;
 }
 

@@ -35,7 +35,8 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_ensemble_cycle", "ath_gpu_ensemble_get_tiles",
            "ath_gpu_ensemble_set_tiles", "ath_gpu_ensemble_autotune",
            "ath_gpu_ensemble_stream", "ath_gpu_mesh_set_tiles", "ath_gpu_mesh_get_tiles",
-           "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history"]
+           "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history",
+           "ath_gpu_ghost_position"]
 NKERNELS = 18
 
 
@@ -121,6 +122,7 @@ def load():
     L.ath_gpu_mesh_get_tiles.argtypes = [P, IP]
     L.ath_gpu_mesh_autotune.argtypes = [P, I, D]
     L.ath_gpu_kernel_box.argtypes = [I, I, I, IP, IP]
+    L.ath_gpu_ghost_position.argtypes = [I, I, I, I, IP]
     L.ath_gpu_history.argtypes = [P, D]
     _lib = L
     return L
@@ -481,3 +483,11 @@ def kernel_box(kind, nx, ng):
     n = np.zeros(3, dtype=np.int32)
     _check(load().ath_gpu_kernel_box(int(kind), int(nx), int(ng), _ip(lo), _ip(n)))
     return lo, n
+
+
+def ghost_position(nx, ng, array, t):
+    """Ghost position number t of the exchange kernel (array 0 cells, 1..3 face
+    components): (i, j, k, source i, source j, source k, p1, p2, p3); no device call."""
+    pos = np.zeros(9, dtype=np.int32)
+    _check(load().ath_gpu_ghost_position(int(nx), int(ng), int(array), int(t), _ip(pos)))
+    return pos

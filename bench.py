@@ -67,7 +67,9 @@ def kernel_algorithmic_bytes(name, m):
       corner_emf:  v, bcc (6) on (n+2)^3 cells, 6 face EMFs + 3 weights on the faces,
                    3 edge EMFs written;
       update_u_b_ct: U^n (5) and 3 face fields on the active region, 15 flux and 3 edge
-                   components, U (5) and B (3) written including the ghost copies;
+                   components read, active U (5) and B (3) written;
+      ghost_exchange: every ghost position of U (5 components) and of the 3 face
+                   components read from its source and written;
       cons2prim:   U (5) and faces (3) read, W (5) and bcc (3) written on nc^3 cells."""
     n1, n2, n3 = m.bnx
     assert n1 == n2 == n3
@@ -82,7 +84,9 @@ def kernel_algorithmic_bytes(name, m):
         per = 6 * (n + 2) ** 3 + 9 * (n + 1) * (n + 2) ** 2 + 3 * (n + 1) ** 2 * (n + 2)
     elif name == "update_u_b_ct":
         per = (5 * n ** 3 + 15 * (n + 1) * n * n + 3 * (n + 1) ** 2 * n + 3 * (n + 1) * n * n
-               + 5 * nc ** 3 + 3 * (nc + 1) * nc * nc)
+               + 5 * n ** 3 + 3 * (n + 1) * n * n)
+    elif name == "ghost_exchange":
+        per = 2 * (5 * (nc ** 3 - n ** 3) + 3 * ((nc + 1) * nc * nc - (n + 1) * n * n))
     elif name == "cons2prim":
         per = 5 * nc ** 3 + 3 * (nc + 1) * nc * nc + 8 * nc ** 3
     else:

@@ -167,6 +167,12 @@ int ath_gpu_mesh_autotune(ath_mesh_t mesh, int reps, double *best_ms);
 /* Position box (first index lo[3], extent n[3]) of kernel kind 0..3 in a MeshBlock of
  * nx^3 active cells and ng ghosts, for host-side bounds checks; no device call. */
 int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n);
+/* Ghost position number t of the same-level exchange kernel in a MeshBlock of nx^3
+ * cells and ng ghosts, for array 0 (cell-centred) or 1..3 (x1f, x2f, x3f face field):
+ * pos = {i, j, k, source i, source j, source k, neighbour offsets p1, p2, p3}, the
+ * source being the neighbour's active position that the reference's same-level buffers
+ * (cc/bvals_cc.cpp:287-427, fc/bvals_fc.cpp:344-688) copy there; no device call. */
+int ath_gpu_ghost_position(int nx, int ng, int array, int t, int *pos);
 
 /* ---- Learned initial guess for SR primitive recovery --------------------------------
  * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds

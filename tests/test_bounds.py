@@ -57,3 +57,41 @@ def test_boxes_fit_every_tile_shape():
         lo, n = kernel_box(kind, 32, 2)
         chunks = -(-n[0] // tx) * -(-n[1] // ty) * -(-n[2] // tz)
         assert chunks * 256 >= n[0] * n[1] * n[2]
+
+
+@pytest.mark.parametrize("nx,ng", [(8, 2), (6, 3), (4, 4), (5, 2)])
+def test_ghost_exchange_covers_every_ghost_once_from_an_active_source(nx, ng):
+    # the exchange kernel's numbering, read back through the library: for the cell array
+    # and each face component, ghost numbers 0..count-1 hit every ghost position exactly
+    # once, in memory order, and read an active position of the neighbour that the
+    # reference's same-level exchange sends there: cells is-ng..is-1 <- ie-ng+1..ie and
+    # ie+1..ie+ng <- is..is+ng-1 (bvals_cc.cpp:287-427); faces along the normal
+    # s-ng..s-1 <- e-ng+1..e and e+2..e+ng+1 <- s+1..s+ng, faces s and e+1 staying with
+    # their owner (bvals_fc.cpp:344-688)
+    from athena_amd.gpu import ghost_position
+    nc = nx + 2 * ng
+    is_, ie = ng, ng + nx - 1
+    for array in range(4):
+        n = [nc + (array == d + 1) for d in range(3)]
+        hi = [ie + (array == d + 1) for d in range(3)]      # last active index
+        ghosts = {(i, j, k) for k in range(n[2]) for j in range(n[1]) for i in range(n[0])
+                  if not all(is_ <= x <= h for x, h in zip((i, j, k), hi))}
+        seen, last = set(), -1
+        for t in range(len(ghosts)):
+            i, j, k, si, sj, sk, p1, p2, p3 = ghost_position(nx, ng, array, t)
+            assert (i, j, k) in ghosts and (i, j, k) not in seen
+            seen.add((i, j, k))
+            lin = (k * n[1] + j) * n[0] + i
+            assert lin > last                                 # memory order
+            last = lin
+            for x, s, p, h in zip((i, j, k), (si, sj, sk), (p1, p2, p3), hi):
+                assert is_ <= s <= h                          # an active source
+                if p == -1:
+                    assert x < is_ and s == x + nx
+                elif p == 1:
+                    assert x > h and s == x - nx
+                else:
+                    assert s == x and is_ <= x <= h
+        assert seen == ghosts
+        with pytest.raises(RuntimeError):
+            ghost_position(nx, ng, array, len(ghosts))
