@@ -125,7 +125,7 @@ __device__ __forceinline__ bool map_work(int lw, int nwg, int nblocks, int ni, i
   const int nch = nwg / nblocks;
   g = lw / nch;
   const int c = lw - g * nch;
-  if (T.tx == 0) {
+  if (T.tx <= 0) {
     const int t = c * WG + threadIdx.x;
     i = t % ni;
     const int r = t / ni;
@@ -422,33 +422,16 @@ struct CornerArgs {
   double *e1, *e2, *e3;
 };
 
-template <bool SR>
-__device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca, int lw,
-                                            int nwg, TileK T) {
-  int g, i, j, k;
-  if (!map_kind(M, KK_CORNER, lw, nwg, T, g, i, j, k)) return;
-  const double *__restrict__ w = ca.w;
-  const double *__restrict__ bcc = ca.bcc;
-  const double *__restrict__ e2x1 = ca.e2x1, *__restrict__ e3x1 = ca.e3x1;
-  const double *__restrict__ e1x2 = ca.e1x2, *__restrict__ e3x2 = ca.e3x2;
-  const double *__restrict__ e1x3 = ca.e1x3, *__restrict__ e2x3 = ca.e2x3;
-  const double *__restrict__ wx1 = ca.wx1, *__restrict__ wx2 = ca.wx2;
-  const double *__restrict__ wx3 = ca.wx3;
-  double *__restrict__ e1 = ca.e1, *__restrict__ e2 = ca.e2, *__restrict__ e3 = ca.e3;
-  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
-  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
-  const long o1 = static_cast<long>(g) * M.nf1, o2 = static_cast<long>(g) * M.nf2;
-  const long o3 = static_cast<long>(g) * M.nf3;
-#define W1F(kk, jj, ii) wx1[o1 + f1idx(M, kk, jj, ii)]
-#define W2F(kk, jj, ii) wx2[o2 + f2idx(M, kk, jj, ii)]
-#define W3F(kk, jj, ii) wx3[o3 + f3idx(M, kk, jj, ii)]
-#define E2X1(kk, jj, ii) e2x1[o1 + f1idx(M, kk, jj, ii)]
-#define E3X1(kk, jj, ii) e3x1[o1 + f1idx(M, kk, jj, ii)]
-#define E1X2(kk, jj, ii) e1x2[o2 + f2idx(M, kk, jj, ii)]
-#define E3X2(kk, jj, ii) e3x2[o2 + f2idx(M, kk, jj, ii)]
-#define E1X3(kk, jj, ii) e1x3[o3 + f3idx(M, kk, jj, ii)]
-#define E2X3(kk, jj, ii) e2x3[o3 + f3idx(M, kk, jj, ii)]
-#define CC(comp, kk, jj, ii) cc_e<SR>(M, wb, bb, comp, cidx(M, kk, jj, ii))
+// The three corner EMFs at edge position (k,j,i) from accessors of the face EMFs and
+// upwind weights (W1F.. E3X1, each called as (k, j, i)) and of the cell-centred E
+// (CC(comp, k, j, i)), handed to out(comp, value) one by one; the same expressions for
+// every thread-to-position mapping.
+template <class FW1, class FW2, class FW3, class FE2X1, class FE3X1, class FE1X2, class FE3X2,
+          class FE1X3, class FE2X3, class FCC, class FOut>
+__device__ __forceinline__ void corner_edges(int k, int j, int i, FW1 W1F, FW2 W2F, FW3 W3F,
+                                             FE2X1 E2X1, FE3X1 E3X1, FE1X2 E1X2,
+                                             FE3X2 E3X2, FE1X3 E1X3, FE2X3 E2X3, FCC CC,
+                                             FOut out) {
   {
     double de1_l3 = (1.0 - W2F(k - 1, j, i)) * (E1X3(k, j, i) - CC(0, k - 1, j, i)) +
                     (W2F(k - 1, j, i)) * (E1X3(k, j - 1, i) - CC(0, k - 1, j - 1, i));
@@ -458,9 +441,8 @@ __device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca
                     (W3F(k, j - 1, i)) * (E1X2(k - 1, j, i) - CC(0, k - 1, j - 1, i));
     double de1_r2 = (1.0 - W3F(k, j, i)) * (E1X2(k, j, i) - CC(0, k, j, i)) +
                     (W3F(k, j, i)) * (E1X2(k - 1, j, i) - CC(0, k - 1, j, i));
-    e1[static_cast<long>(g) * M.ne1 + e1idx(M, k, j, i)] =
-        0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + E1X2(k - 1, j, i) + E1X2(k, j, i) +
-                E1X3(k, j - 1, i) + E1X3(k, j, i));
+    out(0, 0.25 * (de1_l3 + de1_r3 + de1_l2 + de1_r2 + E1X2(k - 1, j, i) + E1X2(k, j, i) +
+                E1X3(k, j - 1, i) + E1X3(k, j, i)));
   }
   {
     double de2_l3 = (1.0 - W1F(k - 1, j, i)) * (E2X3(k, j, i) - CC(1, k - 1, j, i)) +
@@ -471,9 +453,8 @@ __device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca
                     (W3F(k, j, i - 1)) * (E2X1(k - 1, j, i) - CC(1, k - 1, j, i - 1));
     double de2_r1 = (1.0 - W3F(k, j, i)) * (E2X1(k, j, i) - CC(1, k, j, i)) +
                     (W3F(k, j, i)) * (E2X1(k - 1, j, i) - CC(1, k - 1, j, i));
-    e2[static_cast<long>(g) * M.ne2 + e2idx(M, k, j, i)] =
-        0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + E2X3(k, j, i - 1) + E2X3(k, j, i) +
-                E2X1(k - 1, j, i) + E2X1(k, j, i));
+    out(1, 0.25 * (de2_l3 + de2_r3 + de2_l1 + de2_r1 + E2X3(k, j, i - 1) + E2X3(k, j, i) +
+                E2X1(k - 1, j, i) + E2X1(k, j, i)));
   }
   {
     double de3_l2 = (1.0 - W1F(k, j - 1, i)) * (E3X2(k, j, i) - CC(2, k, j - 1, i)) +
@@ -484,25 +465,179 @@ __device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca
                     (W2F(k, j, i - 1)) * (E3X1(k, j - 1, i) - CC(2, k, j - 1, i - 1));
     double de3_r1 = (1.0 - W2F(k, j, i)) * (E3X1(k, j, i) - CC(2, k, j, i)) +
                     (W2F(k, j, i)) * (E3X1(k, j - 1, i) - CC(2, k, j - 1, i));
-    e3[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)] =
-        0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + E3X2(k, j, i - 1) + E3X2(k, j, i) +
-                E3X1(k, j - 1, i) + E3X1(k, j, i));
+    out(2, 0.25 * (de3_l1 + de3_r1 + de3_l2 + de3_r2 + E3X2(k, j, i - 1) + E3X2(k, j, i) +
+                E3X1(k, j - 1, i) + E3X1(k, j, i)));
   }
-#undef W1F
-#undef W2F
-#undef W3F
-#undef E2X1
-#undef E3X1
-#undef E1X2
-#undef E3X2
-#undef E1X3
-#undef E2X3
-#undef CC
+}
+
+template <bool SR>
+__device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca, int lw,
+                                            int nwg, TileK T) {
+  int g, i, j, k;
+  if (!map_kind(M, KK_CORNER, lw, nwg, T, g, i, j, k)) return;
+  const double *__restrict__ w = ca.w;
+  const double *__restrict__ bcc = ca.bcc;
+  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+  const long o1 = static_cast<long>(g) * M.nf1, o2 = static_cast<long>(g) * M.nf2;
+  const long o3 = static_cast<long>(g) * M.nf3;
+  const double *__restrict__ wx1 = ca.wx1 + o1, *__restrict__ wx2 = ca.wx2 + o2;
+  const double *__restrict__ wx3 = ca.wx3 + o3;
+  const double *__restrict__ e2x1 = ca.e2x1 + o1, *__restrict__ e3x1 = ca.e3x1 + o1;
+  const double *__restrict__ e1x2 = ca.e1x2 + o2, *__restrict__ e3x2 = ca.e3x2 + o2;
+  const double *__restrict__ e1x3 = ca.e1x3 + o3, *__restrict__ e2x3 = ca.e2x3 + o3;
+  double *__restrict__ e1 = ca.e1, *__restrict__ e2 = ca.e2, *__restrict__ e3 = ca.e3;
+  auto W1F = [&](int kk, int jj, int ii) { return wx1[f1idx(M, kk, jj, ii)]; };
+  auto W2F = [&](int kk, int jj, int ii) { return wx2[f2idx(M, kk, jj, ii)]; };
+  auto W3F = [&](int kk, int jj, int ii) { return wx3[f3idx(M, kk, jj, ii)]; };
+  auto E2X1 = [&](int kk, int jj, int ii) { return e2x1[f1idx(M, kk, jj, ii)]; };
+  auto E3X1 = [&](int kk, int jj, int ii) { return e3x1[f1idx(M, kk, jj, ii)]; };
+  auto E1X2 = [&](int kk, int jj, int ii) { return e1x2[f2idx(M, kk, jj, ii)]; };
+  auto E3X2 = [&](int kk, int jj, int ii) { return e3x2[f2idx(M, kk, jj, ii)]; };
+  auto E1X3 = [&](int kk, int jj, int ii) { return e1x3[f3idx(M, kk, jj, ii)]; };
+  auto E2X3 = [&](int kk, int jj, int ii) { return e2x3[f3idx(M, kk, jj, ii)]; };
+  auto cc = [&](int comp, int kk, int jj, int ii) {
+    return cc_e<SR>(M, wb, bb, comp, cidx(M, kk, jj, ii));
+  };
+  auto out = [&](int comp, double v) {
+    if (comp == 0) e1[static_cast<long>(g) * M.ne1 + e1idx(M, k, j, i)] = v;
+    else if (comp == 1) e2[static_cast<long>(g) * M.ne2 + e2idx(M, k, j, i)] = v;
+    else e3[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)] = v;
+  };
+  corner_edges(k, j, i, W1F, W2F, W3F, E2X1, E3X1, E1X2, E3X2, E1X3, E2X3, cc, out);
+}
+
+// Z-marching corner EMFs (tile option with tx < 0, tz = planes per segment): a
+// work-group owns a TI x TJ tile of (i, j) edge positions (TI = the box width up to 256,
+// TJ = 256 / TI rows) and walks up a segment of planes.  The cell-centred E of the
+// (TI+1) x (TJ+1) cells of a plane that the tile's edges touch is evaluated once into
+// LDS (planes k-1 and k, double-buffered), and every thread carries the four face values
+// of plane k-1 that its edges need from the previous step, so each array element is
+// read from memory about once instead of once per neighbouring work-group.
+struct MarchGeom {
+  int ti, tj, nti, ntj, ks, nks;
+};
+
+__host__ __device__ __forceinline__ MarchGeom march_geom(const MeshK &M, int ks) {
+  int lo[3], n[3];
+  kind_box(M, KK_CORNER, lo, n);
+  MarchGeom G;
+  G.ti = (n[0] < WG) ? n[0] : WG;
+  G.tj = WG / G.ti;
+  if (G.tj > n[1]) G.tj = n[1];
+  G.nti = (n[0] + G.ti - 1) / G.ti;
+  G.ntj = (n[1] + G.tj - 1) / G.tj;
+  G.ks = ks;
+  G.nks = (n[2] + ks - 1) / ks;
+  return G;
+}
+
+__host__ __device__ __forceinline__ int march_lds_doubles(const MarchGeom &G) {
+  return 2 * 3 * (G.ti + 1) * (G.tj + 1);
+}
+
+template <bool SR>
+__device__ __forceinline__ void corner_march(const MeshK &M, const CornerArgs &ca, int lw,
+                                             int nwg, int ks) {
+  extern __shared__ double lds_cc[];
+  int lo[3], n[3];
+  kind_box(M, KK_CORNER, lo, n);
+  const MarchGeom G = march_geom(M, ks);
+  const int per = nwg / M.nblocks;               // = G.nti * G.ntj * G.nks
+  const int g = lw / per;
+  int r = lw - g * per;
+  const int seg = r % G.nks;
+  r /= G.nks;
+  const int tjx = r % G.ntj, tix = r / G.ntj;
+  const int i0 = lo[0] + tix * G.ti, j0 = lo[1] + tjx * G.tj;
+  const int k0 = lo[2] + seg * G.ks;
+  const int k1 = (k0 + G.ks < lo[2] + n[2]) ? k0 + G.ks : lo[2] + n[2];
+  const int iend = lo[0] + n[0] - 1, jend = lo[1] + n[1] - 1;   // last edge positions
+  const int t = threadIdx.x;
+  const int i = i0 + t % G.ti, j = j0 + t / G.ti;
+  const bool act = (t / G.ti) < G.tj && i <= iend && j <= jend;
+  const int sw = G.ti + 1, ncs = sw * (G.tj + 1);   // staged cells i0-1.., j0-1.. per plane
+  const double *wb = ca.w + static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *bb = ca.bcc + static_cast<long>(g) * 3 * M.ncell;
+  const long o1 = static_cast<long>(g) * M.nf1, o2 = static_cast<long>(g) * M.nf2;
+  const long o3 = static_cast<long>(g) * M.nf3;
+  auto stage = [&](double *dst, int kk) {
+    for (int c = t; c < ncs; c += WG) {
+      const int ci = i0 - 1 + c % sw, cj = j0 - 1 + c / sw;
+      if (ci > iend || cj > jend) continue;      // beyond the box: never read
+      const long q = cidx(M, kk, cj, ci);
+      dst[c] = cc_e<SR>(M, wb, bb, 0, q);
+      dst[ncs + c] = cc_e<SR>(M, wb, bb, 1, q);
+      dst[2 * ncs + c] = cc_e<SR>(M, wb, bb, 2, q);
+    }
+  };
+  // face values of plane k-1 at (j, i): W1F, W2F, E1X2, E2X1
+  double w1p = 0.0, w2p = 0.0, e12p = 0.0, e21p = 0.0;
+  if (act) {
+    w1p = ca.wx1[o1 + f1idx(M, k0 - 1, j, i)];
+    w2p = ca.wx2[o2 + f2idx(M, k0 - 1, j, i)];
+    e12p = ca.e1x2[o2 + f2idx(M, k0 - 1, j, i)];
+    e21p = ca.e2x1[o1 + f1idx(M, k0 - 1, j, i)];
+  }
+  stage(lds_cc, k0 - 1);
+  int cur = 1;
+  for (int k = k0; k < k1; ++k) {
+    double *C = lds_cc + cur * 3 * ncs;
+    const double *P = lds_cc + (cur ^ 1) * 3 * ncs;
+    stage(C, k);
+    __syncthreads();
+    if (act) {
+      const double w1c = ca.wx1[o1 + f1idx(M, k, j, i)];
+      const double w2c = ca.wx2[o2 + f2idx(M, k, j, i)];
+      const double e12c = ca.e1x2[o2 + f2idx(M, k, j, i)];
+      const double e21c = ca.e2x1[o1 + f1idx(M, k, j, i)];
+      auto carried = [&](const double *a, long o, int comp, double prev, double curv) {
+        return [=, &M](int kk, int jj, int ii) {
+          if (kk < k) return prev;
+          if (jj == j && ii == i) return curv;
+          const long x = (comp == 0) ? f1idx(M, kk, jj, ii)
+                                     : ((comp == 1) ? f2idx(M, kk, jj, ii) : f3idx(M, kk, jj, ii));
+          return a[o + x];
+        };
+      };
+      auto plain = [&](const double *a, long o, int comp) {
+        return [=, &M](int kk, int jj, int ii) {
+          const long x = (comp == 0) ? f1idx(M, kk, jj, ii)
+                                     : ((comp == 1) ? f2idx(M, kk, jj, ii) : f3idx(M, kk, jj, ii));
+          return a[o + x];
+        };
+      };
+      auto cc = [&](int comp, int kk, int jj, int ii) {
+        const double *B = (kk < k) ? P : C;
+        return B[comp * ncs + (jj - j0 + 1) * sw + (ii - i0 + 1)];
+      };
+      auto out = [&](int comp, double v) {
+        if (comp == 0) ca.e1[static_cast<long>(g) * M.ne1 + e1idx(M, k, j, i)] = v;
+        else if (comp == 1) ca.e2[static_cast<long>(g) * M.ne2 + e2idx(M, k, j, i)] = v;
+        else ca.e3[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)] = v;
+      };
+      corner_edges(k, j, i, carried(ca.wx1, o1, 0, w1p, w1c), carried(ca.wx2, o2, 1, w2p, w2c),
+                   plain(ca.wx3, o3, 2), carried(ca.e2x1, o1, 0, e21p, e21c),
+                   plain(ca.e3x1, o1, 0), carried(ca.e1x2, o2, 1, e12p, e12c),
+                   plain(ca.e3x2, o2, 1), plain(ca.e1x3, o3, 2), plain(ca.e2x3, o3, 2), cc, out);
+      w1p = w1c;
+      w2p = w2c;
+      e12p = e12c;
+      e21p = e21c;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
 }
 
 template <bool SR>
 __global__ void __launch_bounds__(WG) k_corner(MeshK M, CornerArgs ca, TileK T) {
   corner_body<SR>(M, ca, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+}
+
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_corner_march(MeshK M, CornerArgs ca, TileK T) {
+  corner_march<SR>(M, ca, xcd_logical_wg(M.xcd_remap), gridDim.x, T.tz);
 }
 
 //========================================================================================
@@ -1119,6 +1254,17 @@ __global__ void __launch_bounds__(WG) k_corner_ens(const EnsMember *__restrict__
   }
 }
 
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_corner_march_ens(const EnsMember *__restrict__ E,
+                                                         const int2 *__restrict__ map,
+                                                         int nwork) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    corner_march<SR>(e.M, e.ca, mw.y, e.nwg[1], e.tile[1].tz);
+  }
+}
+
 __global__ void __launch_bounds__(WG) k_ghost_ens(const EnsMember *__restrict__ E,
                                                    const int2 *__restrict__ map, int nwork) {
   for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
@@ -1306,15 +1452,27 @@ int solver_code(const ath_mesh_desc &d) {
 long kind_chunks(const MeshK &M, int kind, TileK T) {
   int lo[3], n[3];
   kind_box(M, kind, lo, n);
-  if (T.tx == 0) return (static_cast<long>(n[0]) * n[1] * n[2] + WG - 1) / WG;
+  if (T.tx < 0 && kind == KK_CORNER) {
+    const MarchGeom G = march_geom(M, T.tz);
+    return static_cast<long>(G.nti) * G.ntj * G.nks;
+  }
+  if (T.tx <= 0) return (static_cast<long>(n[0]) * n[1] * n[2] + WG - 1) / WG;
   return static_cast<long>((n[0] + T.tx - 1) / T.tx) * ((n[1] + T.ty - 1) / T.ty) *
          ((n[2] + T.tz - 1) / T.tz);
 }
 
-// Tile shapes the autotuner chooses from (index 0: linear chunks).
-constexpr int NTILES = 6;
+// Tile shapes the autotuner chooses from (index 0: linear chunks); indices 6 and 7 are
+// the corner kernel's z-march over segments of 8 and 16 planes.
+constexpr int NTILES = 8;
 constexpr TileK TILES[NTILES] = {{0, 0, 0}, {64, 4, 1}, {32, 8, 1}, {16, 16, 1}, {16, 4, 4},
-                                 {8, 8, 4}};
+                                 {8, 8, 4}, {-1, 0, 8}, {-1, 0, 16}};
+
+bool tile_ok(int kind, int t) { return TILES[t].tx >= 0 || kind == KK_CORNER; }
+
+// Dynamic LDS bytes of a corner launch with tile T.
+size_t corner_lds(const MeshK &M, TileK T) {
+  return (T.tx < 0) ? sizeof(double) * march_lds_doubles(march_geom(M, T.tz)) : 0;
+}
 
 // Register configuration of a mesh (the pointers the VL2 stages swap).
 struct Regs {
@@ -1458,12 +1616,16 @@ int launch_corner_args(ath_mesh *m, const CornerArgs &ca) {
   const MeshK &M = m->M;
   const dim3 grid = kind_grid(m, KK_CORNER);
   ProfScope ps(m, KID_CORNER);
-  if (m->d.relativistic)
-    hipLaunchKernelGGL((k_corner<true>), grid, dim3(WG), 0, m->stream, M, ca,
-                       m->tile[KK_CORNER]);
+  const TileK T = m->tile[KK_CORNER];
+  const size_t lds = corner_lds(M, T);
+  if (T.tx < 0 && m->d.relativistic)
+    hipLaunchKernelGGL((k_corner_march<true>), grid, dim3(WG), lds, m->stream, M, ca, T);
+  else if (T.tx < 0)
+    hipLaunchKernelGGL((k_corner_march<false>), grid, dim3(WG), lds, m->stream, M, ca, T);
+  else if (m->d.relativistic)
+    hipLaunchKernelGGL((k_corner<true>), grid, dim3(WG), 0, m->stream, M, ca, T);
   else
-    hipLaunchKernelGGL((k_corner<false>), grid, dim3(WG), 0, m->stream, M, ca,
-                       m->tile[KK_CORNER]);
+    hipLaunchKernelGGL((k_corner<false>), grid, dim3(WG), 0, m->stream, M, ca, T);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1702,8 +1864,17 @@ int ens_launch_kind(ath_ensemble *e, int kind, int stg) {
 #undef LE_O
 #undef LE
   } else if (kind == KK_CORNER) {
-    if (e->sr) hipLaunchKernelGGL(k_corner_ens<true>, g, dim3(WG), 0, e->stream, T, mp, nw);
-    else hipLaunchKernelGGL(k_corner_ens<false>, g, dim3(WG), 0, e->stream, T, mp, nw);
+    const TileK tk = TILES[e->tile_idx[kind]];
+    size_t lds = 0;
+    for (auto *m : e->mem) lds = std::max(lds, corner_lds(m->M, tk));
+    if (tk.tx < 0 && e->sr)
+      hipLaunchKernelGGL(k_corner_march_ens<true>, g, dim3(WG), lds, e->stream, T, mp, nw);
+    else if (tk.tx < 0)
+      hipLaunchKernelGGL(k_corner_march_ens<false>, g, dim3(WG), lds, e->stream, T, mp, nw);
+    else if (e->sr)
+      hipLaunchKernelGGL(k_corner_ens<true>, g, dim3(WG), 0, e->stream, T, mp, nw);
+    else
+      hipLaunchKernelGGL(k_corner_ens<false>, g, dim3(WG), 0, e->stream, T, mp, nw);
   } else if (kind == KK_UPDATE) {
     hipLaunchKernelGGL(k_update_ens, g, dim3(WG), 0, e->stream, T, mp, nw,
                        (stg == 1) ? 0.5 : 1.0, (stg == 1) ? 1 : 0);
@@ -2149,6 +2320,8 @@ int ath_gpu_ensemble_set_tiles(ath_ensemble_t e, const int *tile_idx, const int 
   for (int k = 0; k < KK_N; ++k) {
     if (tile_idx && (tile_idx[k] < 0 || tile_idx[k] >= NTILES))
       return fail("tile index out of range");
+    if (tile_idx && !tile_ok(k, tile_idx[k]))
+      return fail("the z-march tile applies to the corner kernel only");
     if (persist && persist[k] < 0) return fail("persistent grid must be >= 0");
   }
   for (int k = 0; k < KK_N; ++k) {
@@ -2184,6 +2357,7 @@ int ath_gpu_ensemble_autotune(ath_ensemble_t e, int reps, double *best_ms) {
     int bt = e->tile_idx[k], bp = e->persist[k];
     for (int t = 0; t < NTILES && !rc; ++t)
       for (int pg = 0; pg < 3 && !rc; ++pg) {
+        if (!tile_ok(k, t)) continue;
         e->tile_idx[k] = t;
         e->persist[k] = grids[pg];
         if (ens_build(e)) { rc = 1; break; }
@@ -2268,6 +2442,8 @@ int ath_gpu_mesh_set_tiles(ath_mesh_t m, const int *tile_idx) {
   if (!m || !tile_idx) return fail("null argument");
   for (int k = 0; k < KK_N; ++k)
     if (tile_idx[k] < 0 || tile_idx[k] >= NTILES) return fail("tile index out of range");
+  for (int k = 0; k < KK_N; ++k)
+    if (!tile_ok(k, tile_idx[k])) return fail("the z-march tile applies to the corner kernel only");
   HIPCHK(hipStreamSynchronize(m->stream));
   for (int k = 0; k < KK_N; ++k) m->tile[k] = TILES[tile_idx[k]];
   return 0;
@@ -2306,6 +2482,7 @@ int ath_gpu_mesh_autotune(ath_mesh_t m, int reps, double *best_ms) {
       float best = 1e30f;
       int bt = 0;
       for (int t = 0; t < NTILES && !rc; ++t) {
+        if (!tile_ok(k, t)) continue;
         e->tile_idx[k] = t;
         if (ens_build(e)) { rc = 1; break; }
         if (k == KK_C2P && ens_launch_kind(e, KK_UPDATE, 1)) { rc = 1; break; }

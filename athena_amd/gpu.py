@@ -324,7 +324,8 @@ class Ensemble:
     usable for get/set of registers between calls."""
 
     KINDS = ("flux", "corner", "update", "cons2prim")
-    TILES = ("linear", "64x4x1", "32x8x1", "16x16x1", "16x4x4", "8x8x4")
+    TILES = ("linear", "64x4x1", "32x8x1", "16x16x1", "16x4x4", "8x8x4", "zmarch8",
+             "zmarch16")   # zmarch: corner kernel only
 
     def __init__(self, meshes):
         L = load()

@@ -24,7 +24,8 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
-TILE_NAMES = ("linear", "64x4x1", "32x8x1", "16x16x1", "16x4x4", "8x8x4")
+TILE_NAMES = ("linear", "64x4x1", "32x8x1", "16x16x1", "16x4x4", "8x8x4", "zmarch8",
+              "zmarch16")
 BYTES_PER_ZONE_CYCLE = 536  # compulsory FP64 traffic of one VL2 MHD zone-cycle
 METRIC = "zone-cycles/s on 1 MI355X (3D MHD VL2+PLM+HLLD FP64) + % peak HBM GB/s"
 

@@ -149,8 +149,10 @@ int ath_gpu_ensemble_size(ath_ensemble_t ens, int *nmembers, long *active_cells)
 int ath_gpu_ensemble_cycle(ath_ensemble_t ens, int ncycles, int use_graph);
 /* Work decomposition per kernel kind (0 fluxes, 1 corner EMFs, 2 update, 3 primitive
  * recovery): tile_idx picks a thread-to-cell tile shape (0 linear chunks of 256 cells,
- * 1..5 the tiles 64x4x1, 32x8x1, 16x16x1, 16x4x4, 8x8x4), persist the launch grid of
- * the persistent kernel (0: one work-group per work item). */
+ * 1..5 the tiles 64x4x1, 32x8x1, 16x16x1, 16x4x4, 8x8x4; for the corner kernel only,
+ * 6 and 7 a z-march over segments of 8 and 16 planes with the cell-centred E staged in
+ * LDS), persist the launch grid of the persistent kernel (0: one work-group per work
+ * item). */
 int ath_gpu_ensemble_get_tiles(ath_ensemble_t ens, int *tile_idx, int *persist);
 int ath_gpu_ensemble_set_tiles(ath_ensemble_t ens, const int *tile_idx, const int *persist);
 /* Times every tile shape and persistent grid per kernel kind on the ensemble's data

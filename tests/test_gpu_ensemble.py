@@ -80,7 +80,9 @@ def test_ensemble_matches_members_run_alone(solo, graph):
 
 @pytest.mark.parametrize("tiles,persist", [([1, 2, 3, 4], [0, 0, 0, 0]),
                                            ([5, 4, 2, 1], [64, 96, 128, 200]),
-                                           ([3, 5, 1, 2], [1, 1, 1, 1])])
+                                           ([3, 5, 1, 2], [1, 1, 1, 1]),
+                                           ([0, 6, 0, 0], [0, 0, 0, 0]),
+                                           ([2, 7, 4, 0], [0, 77, 0, 0])])
 def test_ensemble_tiles_and_persistent_grids(solo, tiles, persist):
     print("case ensemble tiles=%s persist=%s" % (tiles, persist), flush=True)
     members, snaps = solo
@@ -167,3 +169,11 @@ def test_single_mesh_tiles_and_autotune(solo):
     assert list(d2.tiles()) == [4, 5, 3, 2]
     d2.run(4)
     assert_same(snapshot(d2), snaps[3][2])
+    for corner in (6, 7):      # the corner kernel's z-march, 8 and 16 planes per segment
+        d3 = fresh([(m, ic)])[0]
+        d3.set_tiles([0, corner, 0, 0])
+        assert list(d3.tiles()) == [0, corner, 0, 0]
+        d3.run(4)
+        assert_same(snapshot(d3), snaps[3][2])
+    with pytest.raises(AthGpuError):         # the z-march is for the corner kernel only
+        fresh([(m, ic)])[0].set_tiles([6, 0, 0, 0])
