@@ -70,27 +70,29 @@ struct Clock {
   double ncycle;
 };
 
+// Element indices within one block's array, written as (uniform plane term) + (row
+// term): with k uniform across a work-group the plane term stays in scalar registers.
 __device__ __forceinline__ long cidx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * M.nc2 + j) * M.nc1 + i;
+  return static_cast<long>(k) * (M.nc2 * M.nc1) + (j * M.nc1 + i);
 }
 __device__ __forceinline__ long f1idx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * M.nc2 + j) * (M.nc1 + 1) + i;
+  return static_cast<long>(k) * (M.nc2 * (M.nc1 + 1)) + (j * (M.nc1 + 1) + i);
 }
 __device__ __forceinline__ long f2idx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * (M.nc2 + 1) + j) * M.nc1 + i;
+  return static_cast<long>(k) * ((M.nc2 + 1) * M.nc1) + (j * M.nc1 + i);
 }
 __device__ __forceinline__ long f3idx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * M.nc2 + j) * M.nc1 + i;
+  return static_cast<long>(k) * (M.nc2 * M.nc1) + (j * M.nc1 + i);
 }
 // edge arrays: x1e (nc3+1, nc2+1, nc1), x2e (nc3+1, nc2, nc1+1), x3e (nc3, nc2+1, nc1+1)
 __device__ __forceinline__ long e1idx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * (M.nc2 + 1) + j) * M.nc1 + i;
+  return static_cast<long>(k) * ((M.nc2 + 1) * M.nc1) + (j * M.nc1 + i);
 }
 __device__ __forceinline__ long e2idx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * M.nc2 + j) * (M.nc1 + 1) + i;
+  return static_cast<long>(k) * (M.nc2 * (M.nc1 + 1)) + (j * (M.nc1 + 1) + i);
 }
 __device__ __forceinline__ long e3idx(const MeshK &M, int k, int j, int i) {
-  return (static_cast<long>(k) * (M.nc2 + 1) + j) * (M.nc1 + 1) + i;
+  return static_cast<long>(k) * ((M.nc2 + 1) * (M.nc1 + 1)) + (j * (M.nc1 + 1) + i);
 }
 __device__ __forceinline__ int nbr(const MeshK &M, int g, int o1, int o2, int o3) {
   return M.nbr[g * 27 + (o3 + 1) * 9 + (o2 + 1) * 3 + (o1 + 1)];
@@ -581,7 +583,8 @@ __device__ __forceinline__ void corner_march(const MeshK &M, const CornerArgs &c
   }
   stage(lds_cc, k0 - 1);
   int cur = 1;
-  for (int k = k0; k < k1; ++k) {
+  for (int kq = k0; kq < k1; ++kq) {
+    const int k = __builtin_amdgcn_readfirstlane(kq);   // the same for the whole work-group
     double *C = lds_cc + cur * 3 * ncs;
     const double *P = lds_cc + (cur ^ 1) * 3 * ncs;
     stage(C, k);
@@ -815,10 +818,11 @@ __global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double bet
 // on each side of each direction, active positions [ng, n-ng-1] (for a face component
 // along its normal that is [s, e+1], the shared faces staying with their owner), and a
 // ghost position idx < ng copies idx + nx of the low neighbour, idx > n-ng-1 copies
-// idx - nx of the high neighbour.  One thread per (component, ghost position); the
-// ghost positions are numbered in memory order, so consecutive threads write
-// consecutive addresses: the ng low ghost planes, per active plane its ng low ghost rows,
-// the 2*ng ends of each active row and its ng high ghost rows, then the ng high planes.
+// idx - nx of the high neighbour.  One thread per ghost position (all 5 components of a
+// cell-centred one); the ghost positions are numbered in memory order, so consecutive
+// threads write consecutive addresses: the ng low ghost planes, per active plane its ng
+// low ghost rows, the 2*ng ends of each active row and its ng high ghost rows, then the
+// ng high planes.
 struct GhostArgs {
   double *u;
   double *b[3];
@@ -868,12 +872,11 @@ __host__ __device__ __forceinline__ int ghost_off(int idx, int n, int ng) {
   return (idx < ng) ? -1 : ((idx > n - ng - 1) ? 1 : 0);
 }
 
-// Work items of one block: 5 cell components, then the x1, x2, x3 face components.
+// Work items of one block: the cell ghost positions (each copies the 5 components), then
+// the x1, x2, x3 face ghost positions.
 __host__ __device__ __forceinline__ long ghost_items(const MeshK &M) {
-  return NHYDRO * ghost_count(M.nc1, M.nc2, M.nc3, M.ng) +
-         ghost_count(M.nc1 + 1, M.nc2, M.nc3, M.ng) +
-         ghost_count(M.nc1, M.nc2 + 1, M.nc3, M.ng) +
-         ghost_count(M.nc1, M.nc2, M.nc3 + 1, M.ng);
+  return ghost_count(M.nc1, M.nc2, M.nc3, M.ng) + ghost_count(M.nc1 + 1, M.nc2, M.nc3, M.ng) +
+         ghost_count(M.nc1, M.nc2 + 1, M.nc3, M.ng) + ghost_count(M.nc1, M.nc2, M.nc3 + 1, M.ng);
 }
 
 __device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, int lw,
@@ -882,17 +885,15 @@ __device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, i
   const int g = lw / nch;
   int t = (lw - g * nch) * WG + threadIdx.x;
   const int ng = M.ng;
-  const int nc_ = static_cast<int>(ghost_count(M.nc1, M.nc2, M.nc3, ng));
-  int n1 = M.nc1, n2 = M.nc2, n3 = M.nc3;
+  const int cc = static_cast<int>(ghost_count(M.nc1, M.nc2, M.nc3, ng));
+  int n1 = M.nc1, n2 = M.nc2, n3 = M.nc3, ncomp;
   double *arr;
-  long stride;                                   // one component of one block
-  if (t < NHYDRO * nc_) {
-    const int v = t / nc_;
-    t -= v * nc_;
-    stride = NHYDRO * M.ncell;
-    arr = a.u + v * M.ncell;
+  long stride, cstride;                          // block stride, component stride
+  if (t < cc) {
+    arr = a.u; ncomp = NHYDRO; stride = NHYDRO * M.ncell; cstride = M.ncell;
   } else {
-    t -= NHYDRO * nc_;
+    t -= cc;
+    ncomp = 1; cstride = 0;
     const int c1 = static_cast<int>(ghost_count(M.nc1 + 1, M.nc2, M.nc3, ng));
     const int c2 = static_cast<int>(ghost_count(M.nc1, M.nc2 + 1, M.nc3, ng));
     const int c3 = static_cast<int>(ghost_count(M.nc1, M.nc2, M.nc3 + 1, ng));
@@ -913,7 +914,17 @@ __device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, i
   const long own = (static_cast<long>(k) * n2 + j) * n1 + i;
   const long from =
       (static_cast<long>(k - p3 * M.nx3) * n2 + (j - p2 * M.nx2)) * n1 + (i - p1 * M.nx1);
-  arr[static_cast<long>(g) * stride + own] = arr[static_cast<long>(src) * stride + from];
+  double *dst = arr + static_cast<long>(g) * stride + own;
+  const double *s0 = arr + static_cast<long>(src) * stride + from;
+  if (ncomp == NHYDRO) {
+    double v[NHYDRO];
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q) v[q] = s0[q * cstride];
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q) dst[q * cstride] = v[q];
+  } else {
+    dst[0] = s0[0];
+  }
 }
 
 __global__ void __launch_bounds__(WG) k_ghost(MeshK M, GhostArgs a) {
@@ -1358,7 +1369,8 @@ struct ath_mesh {
   double *dtpart = nullptr;
   long ndtpart = 0;           // capacity of dtpart
   long ndt_used = 0;          // entries written by the last cons2prim with dt
-  TileK tile[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // per work kind
+  // per work kind: linear chunks, the corner kernel's z-march over 8 planes
+  TileK tile[4] = {{0, 0, 0}, {-1, 0, 8}, {0, 0, 0}, {0, 0, 0}};
   long nstage = 0;            // stages run (the register configuration's parity)
   int *nbr_d = nullptr;
   double *coord_d = nullptr;
@@ -1673,9 +1685,10 @@ int launch_new_dt(ath_mesh *m, int advance) {
 
 int launch_update_args(ath_mesh *m, const UpdateArgs &ua, double beta, int first) {
   const dim3 grid = kind_grid(m, KK_UPDATE);
+  const TileK T = m->tile[KK_UPDATE];
   ProfScope ps(m, KID_UPDATE_U);
   hipLaunchKernelGGL(k_update, grid, dim3(WG), 0, m->stream, m->M, ua, beta, first, m->clk,
-                     m->tile[KK_UPDATE]);
+                     T);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1753,7 +1766,7 @@ struct ath_ensemble {
   int nwork[KK_N] = {0, 0, 0, 0};
   int2 *map_ghost_d = nullptr;                // work items of the ghost exchange
   int nwork_ghost = 0;
-  int tile_idx[KK_N] = {0, 0, 0, 0};          // index into TILES
+  int tile_idx[KK_N] = {0, 6, 0, 0};          // index into TILES
   int persist[KK_N] = {0, 0, 0, 0};           // 0: one work item per work-group; else grid
   std::vector<long> parity;                   // members' stage count parity at creation
   hipGraph_t graph = nullptr;

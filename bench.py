@@ -57,6 +57,21 @@ class HipEvents:
         return ms.value
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
+# default workload (256^3 as 512 MeshBlocks of 32^3; tiles linear, zmarch8, linear,
+# linear), fetched KiB x 2.0 + written KiB x 1.0 (the factors a 512 MiB device copy
+# gives): profiles/vl2_hlld_plm_turbulence256_hbm_traffic.md.
+MEASURED_TRAFFIC = {
+    "flux_xyz_plm": 11457524192,
+    "flux_xyz_dc": 8269057360,
+    "update_u_b_ct": 7935430204,
+    "corner_emf": 3592324120,
+    "cons2prim": 3309707257,
+    "ghost_exchange": 2298980494,
+}
+TRAFFIC_TILES = ["linear", "zmarch8", "linear", "linear"]
+
+
 def kernel_algorithmic_bytes(name, m):
     """Compulsory HBM bytes of one launch of a production kernel over all MeshBlocks of
     mesh m: every array element the kernel must read or write, counted once (no
@@ -239,7 +254,7 @@ def bench_ensemble(args):
     print("setup %.1f s: %d members, %d active cells" % (time.perf_counter() - t0, nmem,
                                                          cells), file=sys.stderr, flush=True)
     t1 = time.perf_counter()
-    best = ens.autotune(reps=3)
+    best = None if args.no_autotune else ens.autotune(reps=3)
     tiles, persist = ens.tiles()
     print("autotune %.1f s: tiles %s persist %s" % (time.perf_counter() - t1,
                                                      [Ensemble.TILES[t] for t in tiles],
@@ -256,18 +271,23 @@ def bench_ensemble(args):
     print("timed %d cycles: %.3f ms/cycle, %.3e zone-cycles/s" % (args.steps, ms / args.steps,
                                                                    zcs), file=sys.stderr,
           flush=True)
-    # per-kind launch times measured by the autotuner (stage-1 launches over all members)
-    kinds = {}
-    for k, name in enumerate(KIND_KERNELS):
+    # per-kind launch times measured by the autotuner (stage-1 launches over all members;
+    # the update kind is the update followed by the ghost exchange)
+    kinds, roof = {}, None
+    for k, name in enumerate(KIND_KERNELS if best is not None else ()):
         ab = sum(kernel_algorithmic_bytes(name, m) for m, _ in members)
+        if name == "update_u_b_ct":
+            name = "update_u_b_ct + ghost_exchange"
+            ab += sum(kernel_algorithmic_bytes("ghost_exchange", m) for m, _ in members)
         kinds[name] = {"avg_launch_ms": best[k], "algorithmic_bytes_per_launch": ab,
                        "achieved_GBps": ab / (best[k] * 1e-3) / 1e9}
-    dom = max(kinds, key=lambda k: kinds[k]["avg_launch_ms"])
-    roof = {"kernel": dom + " (ensemble launch)", "bound": "hbm",
-            "achieved": kinds[dom]["achieved_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": kinds[dom]["achieved_GBps"] * 1e9 / HBM_PEAK, "traffic": None,
-            "avg_launch_ms": kinds[dom]["avg_launch_ms"],
-            "algorithmic_bytes_per_launch": kinds[dom]["algorithmic_bytes_per_launch"]}
+    if kinds:
+        dom = max(kinds, key=lambda k: kinds[k]["avg_launch_ms"])
+        roof = {"kernel": dom + " (ensemble launch)", "bound": "hbm",
+                "achieved": kinds[dom]["achieved_GBps"], "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": kinds[dom]["achieved_GBps"] * 1e9 / HBM_PEAK,
+                "traffic": None, "avg_launch_ms": kinds[dom]["avg_launch_ms"],
+                "algorithmic_bytes_per_launch": kinds[dom]["algorithmic_bytes_per_launch"]}
     out = {"metric": ENS_METRIC, "value": zcs, "unit": "zone-cycles/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -400,17 +420,30 @@ def cpu_baseline_sr(n=128, ncycles=2, min_seconds=10.0):
                       % (n, m.nblocks, done, dt)}
 
 
+def copy_calibration():
+    """Three plain device copies of 512 MiB: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    they give the factors that turn those counters into bytes moved."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied %d bytes x 3" % (a.numel() * 8), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="turbulence",
-                    choices=["turbulence", "sr_c2p", "ensemble", "sr_shear"],
+                    choices=["turbulence", "sr_c2p", "ensemble", "sr_shear", "copy"],
                     help="turbulence: the zone-cycles/s throughput case (default); "
                          "sr_c2p: batched SR primitive recovery with the learned guess; "
                          "ensemble: 48 blast waves of uneven sizes; sr_shear: the SR shear "
-                         "layer with the learned guess (use --steps 50)")
+                         "layer with the learned guess (use --steps 50); copy: three "
+                         "512 MiB device copies, the calibration of the HBM counters")
     ap.add_argument("--members", type=int, default=48, help="ensemble members")
     ap.add_argument("--n", type=int, default=256, help="mesh cells per direction")
     ap.add_argument("--bn", type=int, default=32, help="MeshBlock cells per direction")
@@ -429,6 +462,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "sr_shear":
         return bench_sr_shear(args)
+    if args.workload == "copy":
+        return copy_calibration()
 
     from athena_amd.gpu import GpuMesh
     from athena_amd.pgen.turbulence import Turbulence, turbulence_mesh
@@ -445,12 +480,12 @@ def main():
     d.initialize()
     print("setup %.1f s: %d MeshBlocks of %s" % (time.perf_counter() - t0, m.nblocks,
                                                  m.bnx), file=sys.stderr, flush=True)
-    tiles = None
     if not args.no_autotune:
         d.autotune(reps=3)
-        tiles = [TILE_NAMES[t] for t in d.tiles()]
-        print("autotuned tiles (flux, corner, update, cons2prim): %s" % tiles,
-              file=sys.stderr, flush=True)
+    tiles = [TILE_NAMES[t] for t in d.tiles()]
+    print("%s tiles (flux, corner, update, cons2prim): %s"
+          % ("default" if args.no_autotune else "autotuned", tiles), file=sys.stderr,
+          flush=True)
     stream = d.stream()
     ev = HipEvents()
 
@@ -484,13 +519,16 @@ def main():
     for k, (t, c) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
         print("  %-14s %8.3f ms total %5d launches %8.4f ms avg %5.1f%%"
               % (k, t, c, t / c, 100 * t / tot), file=sys.stderr)
+    measured = (args.n, args.bn) == (256, 32) and tiles == TRAFFIC_TILES
     roof = {"kernel": dom, "bound": "hbm",
             "achieved": abytes / dom_avg_s / 1e9 if abytes else None,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": abytes / dom_avg_s / HBM_PEAK if abytes else None,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(dom) if measured else None,
             "avg_launch_ms": dom_ms / dom_n,
             "algorithmic_bytes_per_launch": abytes}
+    if roof["traffic"]:
+        roof["traffic_GBps"] = roof["traffic"] / dom_avg_s / 1e9
     step_roof = {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE,
                  "achieved": zcs * BYTES_PER_ZONE_CYCLE / 1e9, "unit": "GB/s",
                  "frac": zcs * BYTES_PER_ZONE_CYCLE / HBM_PEAK}

@@ -169,11 +169,13 @@ def test_single_mesh_tiles_and_autotune(solo):
     assert list(d2.tiles()) == [4, 5, 3, 2]
     d2.run(4)
     assert_same(snapshot(d2), snaps[3][2])
-    for corner in (6, 7):      # the corner kernel's z-march, 8 and 16 planes per segment
+    for tiles in ([0, 6, 0, 0], [0, 7, 0, 0]):
+        # the corner kernel's z-march, 8 and 16 planes per segment
         d3 = fresh([(m, ic)])[0]
-        d3.set_tiles([0, corner, 0, 0])
-        assert list(d3.tiles()) == [0, corner, 0, 0]
+        d3.set_tiles(tiles)
+        assert list(d3.tiles()) == tiles
         d3.run(4)
         assert_same(snapshot(d3), snaps[3][2])
-    with pytest.raises(AthGpuError):         # the z-march is for the corner kernel only
-        fresh([(m, ic)])[0].set_tiles([6, 0, 0, 0])
+    for bad in ([6, 0, 0, 0], [0, 0, 7, 0]):   # the z-march is for the corner kernel only
+        with pytest.raises(AthGpuError):
+            fresh([(m, ic)])[0].set_tiles(bad)
