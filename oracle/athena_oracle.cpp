@@ -14,12 +14,14 @@
 //     errors (fast/Alfven/slow/entropy at Nx1=32, generated from the compiled
 //     reference and committed under tests/golden/), mhd_linwave.py thresholds,
 //     L/R-going symmetry, conservation and div(B) invariants;
-//   * Riemann solvers: consistency F(U,U)=F(U), HLLE==HLLD on hydrodynamic states
-//     with no transverse field is NOT assumed; region-select and degeneracy branches
-//     are covered by brute-force sampled states with mirror symmetry checks;
-//   * DC/PLM/PPM: exactness on linear data, identity on constants;
+//   * Riemann solvers (tests/test_oracle_pins.py): identical states give the
+//     closed-form physical MHD flux (HLLE, HLLD); supersonic states give the upwind
+//     flux; HLLE with B = 0 equals a textbook Einfeldt HLLE of the Euler equations;
+//   * DC/PLM/PPM (ora_reconstruct_x1, tests/test_oracle_pins.py): constants exact,
+//     linear data exact, PPM exact on cubic cell averages, PPM third order or better
+//     through smooth extrema, mirror symmetry, no new extrema at a step;
 //   * SR con2prim: PrimitiveToConserved o ConservedToPrimitive round trip.
-// Functions with no independent pin are listed as "parity unpinned" in DESIGN.md.
+// What none of these fixes is listed as "parity unpinned" in DESIGN.md.
 //========================================================================================
 #include <algorithm>
 #include <cfloat>
@@ -1131,6 +1133,46 @@ int ora_threads(void) {
 
 // Single-interface Riemann solve for known-answer vectors.  w* are rotated-frame
 // primitive states (7 each), out = 7 fluxes (rotated frame).
+// Test entry: Reconstruct (DC, PLM or PPM; order 1, 2, 3) of one pencil along x1 of a
+// uniform grid of n cells, ng of them ghosts on each side.  q holds the 7 reconstructed
+// variables cell by cell in rows (IDN, IVX, IVY, IVZ, IPR, By, Bz: q[v * n + i]); wl, wr
+// receive the left and right states at the faces ng .. n - ng (q layout, nf = n - 2ng + 1
+// faces per row), floors dfloor / pfloor applied as in PPM.
+void ora_reconstruct_x1(int order, int n, int ng, const double *q, double dfloor,
+                        double pfloor, double *wl, double *wr) {
+  Mesh m;
+  std::memset(&m.c, 0, sizeof(m.c));
+  m.dfloor = dfloor;
+  m.pfloor = pfloor;
+  Block b;
+  b.nx1 = n - 2 * ng; b.nx2 = b.nx3 = 1; b.ng = ng;
+  b.is = ng; b.ie = n - ng - 1; b.js = b.je = b.ks = b.ke = 0;
+  b.nc1 = n; b.nc2 = b.nc3 = 1;
+  b.x1f.resize(n + 1); b.x1v.resize(n); b.dx1f.resize(n);
+  for (int i = 0; i <= n; ++i) b.x1f[i] = static_cast<Real>(i - ng) / b.nx1;
+  for (int i = 0; i < n; ++i) {
+    b.x1v[i] = 0.5 * (b.x1f[i] + b.x1f[i + 1]);
+    b.dx1f[i] = b.x1f[i + 1] - b.x1f[i];
+  }
+  Arr w, bcc, l, r;
+  w.alloc(NHYDRO, 1, 1, n);
+  bcc.alloc(3, 1, 1, n);
+  l.alloc(NWAVE, 1, 1, n + 1);
+  r.alloc(NWAVE, 1, 1, n + 1);
+  for (int i = 0; i < n; ++i) {
+    for (int v = 0; v < NHYDRO; ++v) w(v, 0, 0, i) = q[v * n + i];
+    bcc(IB2, 0, 0, i) = q[IBY * n + i];
+    bcc(IB3, 0, 0, i) = q[IBZ * n + i];
+  }
+  Reconstruct(m, b, order, 0, 0, 0, b.is - 1, b.ie + 1, w, bcc, l, r);
+  const int nf = b.nx1 + 1;
+  for (int v = 0; v < NWAVE; ++v)
+    for (int f = 0; f < nf; ++f) {
+      wl[v * nf + f] = l(v, 0, 0, b.is + f);
+      wr[v * nf + f] = r(v, 0, 0, b.is + f);
+    }
+}
+
 void ora_riemann(int solver, int relativistic, double gamma, int n, const double *wl,
                  const double *wr, const double *bx, double *out) {
   Mesh m;
