@@ -83,6 +83,19 @@ def test_turbulence_multiblock(rsolver, xorder, ng):
     check(m, o, d)
 
 
+@pytest.mark.parametrize("bnx,rsolver,xorder,ng", [((16, 8, 4), "hlld", 2, 2),
+                                                   ((4, 16, 8), "hlle", 3, 3),
+                                                   ((8, 4, 16), "hlld", 3, 4)])
+def test_linear_wave_noncubic_blocks(bnx, rsolver, xorder, ng):
+    # MeshBlocks of unequal sides and NGHOST up to 4: every work map, the corner
+    # kernel's z-march tile width and the ghost exchange's numbering per direction
+    print("case linear_wave 32x16x16 blocks", bnx, rsolver, xorder, ng, flush=True)
+    m = linear_wave_mesh(32, bnx=bnx, rsolver=rsolver, xorder=xorder, ng=ng)
+    lw = LinearWave(m, wave_flag=1, amp=1e-6)
+    o, d = both(m, [lw.block(g) for g in range(m.nblocks)], ncycles=6)
+    check(m, o, d, bitwise=True)
+
+
 def test_conservation_and_divb_gpu():
     print("case conservation 32^3/16^3", flush=True)
     m = turbulence_mesh(32, 16)
