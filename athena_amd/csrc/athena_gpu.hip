@@ -35,6 +35,13 @@ int fail(const std::string &msg) {
   return 1;
 }
 
+}  // namespace
+
+// The library's error text, shared with the other translation units (refine.hip).
+int athgpu::set_error(const std::string &msg) { return fail(msg); }
+
+namespace {
+
 #define HIPCHK(expr)                                                               \
   do {                                                                             \
     hipError_t e_ = (expr);                                                        \

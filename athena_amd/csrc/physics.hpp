@@ -11,7 +11,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace athgpu {
+
+// Sets the text ath_gpu_last_error returns and returns 1 (defined in athena_gpu.hip).
+int set_error(const std::string &msg);
 
 constexpr int IDN = 0, IM1 = 1, IM2 = 2, IM3 = 3, IEN = 4;
 constexpr int IVX = 1, IVY = 2, IVZ = 3, IPR = 4, IBY = 5, IBZ = 6;

@@ -1,0 +1,478 @@
+//========================================================================================
+// refine.hip -- MeshRefinement operators for static mesh refinement on MI355X (gfx950):
+// restriction of cell-centred values and face fields to a MeshBlock's coarse buffer,
+// prolongation from the coarse buffer (minmod-limited linear for cells and for the fine
+// faces on coarse faces), and the divergence-preserving prolongation of the fine faces
+// inside coarse cells (Toth & Roe 2002).  3-D, uniform-Cartesian branches of
+// mesh_refinement.cpp:116-150, 272-300, 328-372, 402-430, 464-525, 835-872 (and the
+// matching Cartesian blocks of ProlongateSharedFieldX2/X3), 1223-1338, with each
+// expression in the reference's association order (built with -ffp-contract=off).
+//
+// Geometry (the reference's, meshblock.cpp:66,93-94): a block has nx cells per
+// direction and ng ghosts (nc = nx + 2 ng); its coarse buffer has nx/2 cells and ng
+// ghosts (ncc = nx/2 + 2 ng); fine index = (coarse index - ng) * 2 + ng.  A batch is nb
+// blocks of one shape stored back to back: cell arrays (nvar, nc3, nc2, nc1) per block,
+// face component d with one more entry along d, coordinate tables per block packed as
+// x1f (n1+1), x1v (n1), dx1f (n1), then x2, x3.  Cartesian metrics: volume dx1 dx2 dx3,
+// face areas dx2 dx3 / dx1 dx3 / dx1 dx2, edge lengths dx2, dx3; face-centre positions
+// along transverse directions are the cell centres (cartesian.cpp:105-118).
+//
+// One thread per (block, component, coarse position), linear and i fastest.  Every
+// index a launch touches is checked on the host against the arrays it addresses.
+//========================================================================================
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/athena_gpu.h"
+#include "physics.hpp"
+
+namespace {
+
+constexpr int RWG = 256;
+
+struct RShape {
+  int nx1, nx2, nx3, ng;
+  int nc1, nc2, nc3, ncc1, ncc2, ncc3;
+  long fcl, ccl;          // coordinate table lengths (fine, coarse) per block
+};
+
+struct RRange {
+  int i0, i1, j0, j1, k0, k1;
+  __host__ __device__ int ni() const { return i1 - i0 + 1; }
+  __host__ __device__ int nj() const { return j1 - j0 + 1; }
+  __host__ __device__ int nk() const { return k1 - k0 + 1; }
+};
+
+// Coordinate tables of one grid (n1 x n2 x n3 cells).
+struct RGrid {
+  const double *x1f, *x1v, *dx1f, *x2f, *x2v, *dx2f, *x3f, *x3v, *dx3f;
+  __device__ RGrid(const double *c, int n1, int n2, int n3) {
+    x1f = c; x1v = x1f + n1 + 1; dx1f = x1v + n1;
+    x2f = dx1f + n1; x2v = x2f + n2 + 1; dx2f = x2v + n2;
+    x3f = dx2f + n2; x3v = x3f + n3 + 1; dx3f = x3v + n3;
+  }
+};
+
+__device__ __forceinline__ double rsign(double x) { return (x < 0.0) ? -1.0 : 1.0; }
+__device__ __forceinline__ double rmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double minmod_c(double gm, double gp) {
+  return 0.5 * (rsign(gm) + rsign(gp)) * rmin(fabs(gm), fabs(gp));
+}
+
+__device__ __forceinline__ long at(int k, int j, int i, int n2, int n1) {
+  return (static_cast<long>(k) * n2 + j) * n1 + i;
+}
+
+// thread -> (block, component, k, j, i) over nb * ncomp * |range|; false past the end
+__device__ __forceinline__ bool rmap(const RRange &r, int nb, int ncomp, int &b, int &n,
+                                     int &k, int &j, int &i) {
+  const long t = static_cast<long>(blockIdx.x) * RWG + threadIdx.x;
+  const long per = static_cast<long>(r.ni()) * r.nj() * r.nk();
+  if (t >= per * ncomp * nb) return false;
+  const long bn = t / per;
+  long q = t - bn * per;
+  b = static_cast<int>(bn / ncomp);
+  n = static_cast<int>(bn - static_cast<long>(b) * ncomp);
+  i = r.i0 + static_cast<int>(q % r.ni());
+  q /= r.ni();
+  j = r.j0 + static_cast<int>(q % r.nj());
+  k = r.k0 + static_cast<int>(q / r.nj());
+  return true;
+}
+
+__global__ void __launch_bounds__(RWG) k_restrict_cc(RShape S, RRange r, int nb, int nvar,
+                                                      const double *fcoord,
+                                                      const double *fine, double *coarse) {
+  int b, n, ck, cj, ci;
+  if (!rmap(r, nb, nvar, b, n, ck, cj, ci)) return;
+  const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
+  const long fn = static_cast<long>(S.nc1) * S.nc2 * S.nc3;
+  const long cn = static_cast<long>(S.ncc1) * S.ncc2 * S.ncc3;
+  const double *f = fine + (static_cast<long>(b) * nvar + n) * fn;
+  const int k = (ck - S.ng) * 2 + S.ng, j = (cj - S.ng) * 2 + S.ng, i = (ci - S.ng) * 2 + S.ng;
+  auto vol = [&](int kk, int jj, int ii) { return F.dx1f[ii] * F.dx2f[jj] * F.dx3f[kk]; };
+  auto fv = [&](int kk, int jj, int ii) { return f[at(kk, jj, ii, S.nc2, S.nc1)]; };
+  const double v000 = vol(k, j, i), v010 = vol(k, j + 1, i);
+  const double v001 = vol(k, j, i + 1), v011 = vol(k, j + 1, i + 1);
+  const double v100 = vol(k + 1, j, i), v110 = vol(k + 1, j + 1, i);
+  const double v101 = vol(k + 1, j, i + 1), v111 = vol(k + 1, j + 1, i + 1);
+  const double tvol = ((v000 + v010) + (v001 + v011)) + ((v100 + v110) + (v101 + v111));
+  coarse[(static_cast<long>(b) * nvar + n) * cn + at(ck, cj, ci, S.ncc2, S.ncc1)] =
+      (((fv(k, j, i) * v000 + fv(k, j + 1, i) * v010) +
+        (fv(k, j, i + 1) * v001 + fv(k, j + 1, i + 1) * v011)) +
+       ((fv(k + 1, j, i) * v100 + fv(k + 1, j + 1, i) * v110) +
+        (fv(k + 1, j, i + 1) * v101 + fv(k + 1, j + 1, i + 1) * v111))) /
+      tvol;
+}
+
+template <int comp>
+__global__ void __launch_bounds__(RWG) k_restrict_fc(RShape S, RRange r, int nb,
+                                                      const double *fcoord,
+                                                      const double *fine, double *coarse) {
+  int b, n, ck, cj, ci;
+  if (!rmap(r, nb, 1, b, n, ck, cj, ci)) return;
+  const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
+  const int fn1 = S.nc1 + (comp == 0), fn2 = S.nc2 + (comp == 1), fn3 = S.nc3 + (comp == 2);
+  const int cn1 = S.ncc1 + (comp == 0), cn2 = S.ncc2 + (comp == 1), cn3 = S.ncc3 + (comp == 2);
+  const double *f = fine + static_cast<long>(b) * fn1 * fn2 * fn3;
+  auto fv = [&](int kk, int jj, int ii) { return f[at(kk, jj, ii, fn2, fn1)]; };
+  const int k = (ck - S.ng) * 2 + S.ng, j = (cj - S.ng) * 2 + S.ng, i = (ci - S.ng) * 2 + S.ng;
+  double v;
+  if (comp == 0) {
+    const double a00 = F.dx2f[j] * F.dx3f[k], a01 = F.dx2f[j + 1] * F.dx3f[k];
+    const double a10 = F.dx2f[j] * F.dx3f[k + 1], a11 = F.dx2f[j + 1] * F.dx3f[k + 1];
+    const double tarea = a00 + a01 + a10 + a11;
+    v = (fv(k, j, i) * a00 + fv(k, j + 1, i) * a01 + fv(k + 1, j, i) * a10 +
+         fv(k + 1, j + 1, i) * a11) / tarea;
+  } else if (comp == 1) {
+    const double a00 = F.dx1f[i] * F.dx3f[k], a01 = F.dx1f[i + 1] * F.dx3f[k];
+    const double a10 = F.dx1f[i] * F.dx3f[k + 1], a11 = F.dx1f[i + 1] * F.dx3f[k + 1];
+    const double tarea = a00 + a01 + a10 + a11;
+    v = (fv(k, j, i) * a00 + fv(k, j, i + 1) * a01 + fv(k + 1, j, i) * a10 +
+         fv(k + 1, j, i + 1) * a11) / tarea;
+  } else {
+    const double a00 = F.dx1f[i] * F.dx2f[j], a01 = F.dx1f[i + 1] * F.dx2f[j];
+    const double a10 = F.dx1f[i] * F.dx2f[j + 1], a11 = F.dx1f[i + 1] * F.dx2f[j + 1];
+    const double tarea = a00 + a01 + a10 + a11;
+    v = (fv(k, j, i) * a00 + fv(k, j, i + 1) * a01 + fv(k, j + 1, i) * a10 +
+         fv(k, j + 1, i + 1) * a11) / tarea;
+  }
+  coarse[static_cast<long>(b) * cn1 * cn2 * cn3 + at(ck, cj, ci, cn2, cn1)] = v;
+}
+
+__global__ void __launch_bounds__(RWG) k_prolongate_cc(RShape S, RRange r, int nb, int nvar,
+                                                        const double *fcoord,
+                                                        const double *ccoord,
+                                                        const double *coarse, double *fine) {
+  int b, n, k, j, i;
+  if (!rmap(r, nb, nvar, b, n, k, j, i)) return;
+  const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
+  const RGrid C(ccoord + b * S.ccl, S.ncc1, S.ncc2, S.ncc3);
+  const long fn = static_cast<long>(S.nc1) * S.nc2 * S.nc3;
+  const long cn = static_cast<long>(S.ncc1) * S.ncc2 * S.ncc3;
+  const double *c = coarse + (static_cast<long>(b) * nvar + n) * cn;
+  double *f = fine + (static_cast<long>(b) * nvar + n) * fn;
+  auto cv = [&](int kk, int jj, int ii) { return c[at(kk, jj, ii, S.ncc2, S.ncc1)]; };
+  const int fk = (k - S.ng) * 2 + S.ng, fj = (j - S.ng) * 2 + S.ng, fi = (i - S.ng) * 2 + S.ng;
+  const double x3m = C.x3v[k - 1], x3c = C.x3v[k], x3p = C.x3v[k + 1];
+  const double dx3m = x3c - x3m, dx3p = x3p - x3c;
+  const double dx3fm = x3c - F.x3v[fk], dx3fp = F.x3v[fk + 1] - x3c;
+  const double x2m = C.x2v[j - 1], x2c = C.x2v[j], x2p = C.x2v[j + 1];
+  const double dx2m = x2c - x2m, dx2p = x2p - x2c;
+  const double dx2fm = x2c - F.x2v[fj], dx2fp = F.x2v[fj + 1] - x2c;
+  const double x1m = C.x1v[i - 1], x1c = C.x1v[i], x1p = C.x1v[i + 1];
+  const double dx1m = x1c - x1m, dx1p = x1p - x1c;
+  const double dx1fm = x1c - F.x1v[fi], dx1fp = F.x1v[fi + 1] - x1c;
+  const double cc = cv(k, j, i);
+  const double gx1c = minmod_c((cc - cv(k, j, i - 1)) / dx1m, (cv(k, j, i + 1) - cc) / dx1p);
+  const double gx2c = minmod_c((cc - cv(k, j - 1, i)) / dx2m, (cv(k, j + 1, i) - cc) / dx2p);
+  const double gx3c = minmod_c((cc - cv(k - 1, j, i)) / dx3m, (cv(k + 1, j, i) - cc) / dx3p);
+  auto fw = [&](int kk, int jj, int ii, double v) { f[at(kk, jj, ii, S.nc2, S.nc1)] = v; };
+  fw(fk, fj, fi, cc - (gx1c * dx1fm + gx2c * dx2fm + gx3c * dx3fm));
+  fw(fk, fj, fi + 1, cc + (gx1c * dx1fp - gx2c * dx2fm - gx3c * dx3fm));
+  fw(fk, fj + 1, fi, cc - (gx1c * dx1fm - gx2c * dx2fp + gx3c * dx3fm));
+  fw(fk, fj + 1, fi + 1, cc + (gx1c * dx1fp + gx2c * dx2fp - gx3c * dx3fm));
+  fw(fk + 1, fj, fi, cc - (gx1c * dx1fm + gx2c * dx2fm - gx3c * dx3fp));
+  fw(fk + 1, fj, fi + 1, cc + (gx1c * dx1fp - gx2c * dx2fm + gx3c * dx3fp));
+  fw(fk + 1, fj + 1, fi, cc - (gx1c * dx1fm - gx2c * dx2fp - gx3c * dx3fp));
+  fw(fk + 1, fj + 1, fi + 1, cc + (gx1c * dx1fp + gx2c * dx2fp + gx3c * dx3fp));
+}
+
+template <int comp>
+__global__ void __launch_bounds__(RWG) k_prolongate_fc_shared(RShape S, RRange r, int nb,
+                                                               const double *fcoord,
+                                                               const double *ccoord,
+                                                               const double *coarse,
+                                                               double *fine) {
+  int b, n, k, j, i;
+  if (!rmap(r, nb, 1, b, n, k, j, i)) return;
+  const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
+  const RGrid C(ccoord + b * S.ccl, S.ncc1, S.ncc2, S.ncc3);
+  const int fn1 = S.nc1 + (comp == 0), fn2 = S.nc2 + (comp == 1), fn3 = S.nc3 + (comp == 2);
+  const int cn1 = S.ncc1 + (comp == 0), cn2 = S.ncc2 + (comp == 1), cn3 = S.ncc3 + (comp == 2);
+  const double *c = coarse + static_cast<long>(b) * cn1 * cn2 * cn3;
+  double *f = fine + static_cast<long>(b) * fn1 * fn2 * fn3;
+  auto cv = [&](int kk, int jj, int ii) { return c[at(kk, jj, ii, cn2, cn1)]; };
+  auto fw = [&](int kk, int jj, int ii, double v) { f[at(kk, jj, ii, fn2, fn1)] = v; };
+  const int fk = (k - S.ng) * 2 + S.ng, fj = (j - S.ng) * 2 + S.ng, fi = (i - S.ng) * 2 + S.ng;
+  const double cc = cv(k, j, i);
+  if (comp == 0) {
+    const double x3m = C.x3v[k - 1], x3c = C.x3v[k], x3p = C.x3v[k + 1];
+    const double dx3m = x3c - x3m, dx3p = x3p - x3c;
+    const double fx3m = F.x3v[fk], fx3p = F.x3v[fk + 1];
+    const double x2m = C.x2v[j - 1], x2c = C.x2v[j], x2p = C.x2v[j + 1];
+    const double dx2m = x2c - x2m, dx2p = x2p - x2c;
+    const double fx2m = F.x2v[fj], fx2p = F.x2v[fj + 1];
+    const double gx2c = minmod_c((cc - cv(k, j - 1, i)) / dx2m, (cv(k, j + 1, i) - cc) / dx2p);
+    const double gx3c = minmod_c((cc - cv(k - 1, j, i)) / dx3m, (cv(k + 1, j, i) - cc) / dx3p);
+    fw(fk, fj, fi, cc - gx2c * (x2c - fx2m) - gx3c * (x3c - fx3m));
+    fw(fk, fj + 1, fi, cc + gx2c * (fx2p - x2c) - gx3c * (x3c - fx3m));
+    fw(fk + 1, fj, fi, cc - gx2c * (x2c - fx2m) + gx3c * (fx3p - x3c));
+    fw(fk + 1, fj + 1, fi, cc + gx2c * (fx2p - x2c) + gx3c * (fx3p - x3c));
+  } else if (comp == 1) {
+    const double x3m = C.x3v[k - 1], x3c = C.x3v[k], x3p = C.x3v[k + 1];
+    const double dx3m = x3c - x3m, dx3p = x3p - x3c;
+    const double fx3m = F.x3v[fk], fx3p = F.x3v[fk + 1];
+    const double x1m = C.x1v[i - 1], x1c = C.x1v[i], x1p = C.x1v[i + 1];
+    const double dx1m = x1c - x1m, dx1p = x1p - x1c;
+    const double fx1m = F.x1v[fi], fx1p = F.x1v[fi + 1];
+    const double gx1c = minmod_c((cc - cv(k, j, i - 1)) / dx1m, (cv(k, j, i + 1) - cc) / dx1p);
+    const double gx3c = minmod_c((cc - cv(k - 1, j, i)) / dx3m, (cv(k + 1, j, i) - cc) / dx3p);
+    fw(fk, fj, fi, cc - gx1c * (x1c - fx1m) - gx3c * (x3c - fx3m));
+    fw(fk, fj, fi + 1, cc + gx1c * (fx1p - x1c) - gx3c * (x3c - fx3m));
+    fw(fk + 1, fj, fi, cc - gx1c * (x1c - fx1m) + gx3c * (fx3p - x3c));
+    fw(fk + 1, fj, fi + 1, cc + gx1c * (fx1p - x1c) + gx3c * (fx3p - x3c));
+  } else {
+    const double x2m = C.x2v[j - 1], x2c = C.x2v[j], x2p = C.x2v[j + 1];
+    const double dx2m = x2c - x2m, dx2p = x2p - x2c;
+    const double fx2m = F.x2v[fj], fx2p = F.x2v[fj + 1];
+    const double x1m = C.x1v[i - 1], x1c = C.x1v[i], x1p = C.x1v[i + 1];
+    const double dx1m = x1c - x1m, dx1p = x1p - x1c;
+    const double fx1m = F.x1v[fi], fx1p = F.x1v[fi + 1];
+    const double gx1c = minmod_c((cc - cv(k, j, i - 1)) / dx1m, (cv(k, j, i + 1) - cc) / dx1p);
+    const double gx2c = minmod_c((cc - cv(k, j - 1, i)) / dx2m, (cv(k, j + 1, i) - cc) / dx2p);
+    fw(fk, fj, fi, cc - gx1c * (x1c - fx1m) - gx2c * (x2c - fx2m));
+    fw(fk, fj, fi + 1, cc + gx1c * (fx1p - x1c) - gx2c * (x2c - fx2m));
+    fw(fk, fj + 1, fi, cc - gx1c * (x1c - fx1m) + gx2c * (fx2p - x2c));
+    fw(fk, fj + 1, fi + 1, cc + gx1c * (fx1p - x1c) + gx2c * (fx2p - x2c));
+  }
+}
+
+__global__ void __launch_bounds__(RWG) k_prolongate_fc_internal(RShape S, RRange r, int nb,
+                                                                 const double *fcoord,
+                                                                 double *b1, double *b2,
+                                                                 double *b3) {
+  int b, n, k, j, i;
+  if (!rmap(r, nb, 1, b, n, k, j, i)) return;
+  const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
+  const long n1f = static_cast<long>(S.nc3) * S.nc2 * (S.nc1 + 1);
+  const long n2f = static_cast<long>(S.nc3) * (S.nc2 + 1) * S.nc1;
+  const long n3f = static_cast<long>(S.nc3 + 1) * S.nc2 * S.nc1;
+  double *x1f = b1 + b * n1f, *x2f = b2 + b * n2f, *x3f = b3 + b * n3f;
+  auto X1 = [&](int kk, int jj, int ii) -> double & { return x1f[at(kk, jj, ii, S.nc2, S.nc1 + 1)]; };
+  auto X2 = [&](int kk, int jj, int ii) -> double & { return x2f[at(kk, jj, ii, S.nc2 + 1, S.nc1)]; };
+  auto X3 = [&](int kk, int jj, int ii) -> double & { return x3f[at(kk, jj, ii, S.nc2, S.nc1)]; };
+  auto A1 = [&](int kk, int jj) { return F.dx2f[jj] * F.dx3f[kk]; };
+  auto A2 = [&](int kk, int ii) { return F.dx1f[ii] * F.dx3f[kk]; };
+  auto A3 = [&](int jj, int ii) { return F.dx1f[ii] * F.dx2f[jj]; };
+  const int fk = (k - S.ng) * 2 + S.ng, fj = (j - S.ng) * 2 + S.ng, fi = (i - S.ng) * 2 + S.ng;
+  double Uxx = 0.0, Vyy = 0.0, Wzz = 0.0;
+  double Uxyz = 0.0, Vxyz = 0.0, Wxyz = 0.0;
+#pragma unroll
+  for (int jj = 0; jj < 2; jj++) {
+    const int js = 2 * jj - 1, fjj = fj + jj, fjp = fj + 2 * jj;
+#pragma unroll
+    for (int ii = 0; ii < 2; ii++) {
+      const int is = 2 * ii - 1, fii = fi + ii, fip = fi + 2 * ii;
+      Uxx += is * (js * (X2(fk, fjp, fii) * A2(fk, fii) + X2(fk + 1, fjp, fii) * A2(fk + 1, fii)) +
+                   (X3(fk + 2, fjj, fii) * A3(fjj, fii) - X3(fk, fjj, fii) * A3(fjj, fii)));
+      Vyy += js * ((X3(fk + 2, fjj, fii) * A3(fjj, fii) - X3(fk, fjj, fii) * A3(fjj, fii)) +
+                   is * (X1(fk, fjj, fip) * A1(fk, fjj) + X1(fk + 1, fjj, fip) * A1(fk + 1, fjj)));
+      Wzz += is * (X1(fk + 1, fjj, fip) * A1(fk + 1, fjj) - X1(fk, fjj, fip) * A1(fk, fjj)) +
+             js * (X2(fk + 1, fjp, fii) * A2(fk + 1, fii) - X2(fk, fjp, fii) * A2(fk, fii));
+      Uxyz += is * js * (X1(fk + 1, fjj, fip) * A1(fk + 1, fjj) - X1(fk, fjj, fip) * A1(fk, fjj));
+      Vxyz += is * js * (X2(fk + 1, fjp, fii) * A2(fk + 1, fii) - X2(fk, fjp, fii) * A2(fk, fii));
+      Wxyz += is * js * (X3(fk + 2, fjj, fii) * A3(fjj, fii) - X3(fk, fjj, fii) * A3(fjj, fii));
+    }
+  }
+  const double s1 = F.dx1f[fi] + F.dx1f[fi + 1], s2 = F.dx2f[fj] + F.dx2f[fj + 1];
+  const double s3 = F.dx3f[fk] + F.dx3f[fk + 1];
+  const double Sdx1 = s1 * s1, Sdx2 = s2 * s2, Sdx3 = s3 * s3;
+  Uxx *= 0.125; Vyy *= 0.125; Wzz *= 0.125;
+  Uxyz *= 0.125 / (Sdx2 + Sdx3);
+  Vxyz *= 0.125 / (Sdx1 + Sdx3);
+  Wxyz *= 0.125 / (Sdx1 + Sdx2);
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const double t3 = kk ? 1.0 : -1.0, t2 = jj ? 1.0 : -1.0;
+      X1(fk + kk, fj + jj, fi + 1) =
+          (0.5 * (X1(fk + kk, fj + jj, fi) * A1(fk + kk, fj + jj) +
+                  X1(fk + kk, fj + jj, fi + 2) * A1(fk + kk, fj + jj)) +
+           Uxx + t3 * Sdx3 * Vxyz + t2 * Sdx2 * Wxyz) / A1(fk + kk, fj + jj);
+    }
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+      const double t3 = kk ? 1.0 : -1.0, t1 = ii ? 1.0 : -1.0;
+      X2(fk + kk, fj + 1, fi + ii) =
+          (0.5 * (X2(fk + kk, fj, fi + ii) * A2(fk + kk, fi + ii) +
+                  X2(fk + kk, fj + 2, fi + ii) * A2(fk + kk, fi + ii)) +
+           Vyy + t3 * Sdx3 * Uxyz + t1 * Sdx1 * Wxyz) / A2(fk + kk, fi + ii);
+    }
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+      const double t2 = jj ? 1.0 : -1.0, t1 = ii ? 1.0 : -1.0;
+      X3(fk + 1, fj + jj, fi + ii) =
+          (0.5 * (X3(fk + 2, fj + jj, fi + ii) * A3(fj + jj, fi + ii) +
+                  X3(fk, fj + jj, fi + ii) * A3(fj + jj, fi + ii)) +
+           Wzz + t2 * Sdx2 * Uxyz + t1 * Sdx1 * Vxyz) / A3(fj + jj, fi + ii);
+    }
+}
+
+//----------------------------------------------------------------------------------------
+// Host side: shapes, range checks, launches.
+
+int rfail(const std::string &m) { return athgpu::set_error(m); }
+
+bool make_shape(const int *shape, RShape &S) {
+  if (!shape) return false;
+  S.nx1 = shape[0]; S.nx2 = shape[1]; S.nx3 = shape[2]; S.ng = shape[3];
+  if (S.nx1 < 2 || S.nx2 < 2 || S.nx3 < 2 || S.ng < 1) return false;
+  if (S.nx1 % 2 || S.nx2 % 2 || S.nx3 % 2) return false;
+  S.nc1 = S.nx1 + 2 * S.ng; S.nc2 = S.nx2 + 2 * S.ng; S.nc3 = S.nx3 + 2 * S.ng;
+  S.ncc1 = S.nx1 / 2 + 2 * S.ng; S.ncc2 = S.nx2 / 2 + 2 * S.ng; S.ncc3 = S.nx3 / 2 + 2 * S.ng;
+  S.fcl = 3L * S.nc1 + 1 + 3L * S.nc2 + 1 + 3L * S.nc3 + 1;
+  S.ccl = 3L * S.ncc1 + 1 + 3L * S.ncc2 + 1 + 3L * S.ncc3 + 1;
+  return true;
+}
+
+// Per direction: the coarse indices [lo, hi] must lie in [clo, chi] and the fine indices
+// they address, f(lo) + flo .. f(hi) + fhi with f(c) = (c - ng) * 2 + ng, in [0, fmax].
+bool dir_ok(int lo, int hi, int clo, int chi, int ng, int flo, int fhi, int fmax) {
+  if (lo > hi || lo < clo || hi > chi) return false;
+  const int a = (lo - ng) * 2 + ng + flo, b = (hi - ng) * 2 + ng + fhi;
+  return a >= 0 && b <= fmax;
+}
+
+bool range_of(const int *r, RRange &R) {
+  if (!r) return false;
+  R = RRange{r[0], r[1], r[2], r[3], r[4], r[5]};
+  return true;
+}
+
+int launch_check(hipStream_t s) {
+  (void)s;
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return rfail(std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+dim3 rgrid(const RRange &R, int nb, int ncomp) {
+  const long n = static_cast<long>(R.ni()) * R.nj() * R.nk() * ncomp * nb;
+  return dim3(static_cast<unsigned>((n + RWG - 1) / RWG));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ath_gpu_restrict_cc(const int *shape, int nb, int nvar, const double *fcoord,
+                        const double *fine, double *coarse, const int *range, void *stream) {
+  RShape S;
+  RRange R;
+  if (!make_shape(shape, S) || !range_of(range, R) || nb < 1 || nvar < 1 || !fcoord ||
+      !fine || !coarse)
+    return rfail("ath_gpu_restrict_cc: bad argument");
+  // coarse cells in the buffer; fine cells i, i+1 of each in the block
+  if (!dir_ok(R.i0, R.i1, 0, S.ncc1 - 1, S.ng, 0, 1, S.nc1 - 1) ||
+      !dir_ok(R.j0, R.j1, 0, S.ncc2 - 1, S.ng, 0, 1, S.nc2 - 1) ||
+      !dir_ok(R.k0, R.k1, 0, S.ncc3 - 1, S.ng, 0, 1, S.nc3 - 1))
+    return rfail("ath_gpu_restrict_cc: range outside the arrays");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_restrict_cc, rgrid(R, nb, nvar), dim3(RWG), 0, st, S, R, nb, nvar,
+                     fcoord, fine, coarse);
+  return launch_check(st);
+}
+
+int ath_gpu_restrict_fc(const int *shape, int nb, int comp, const double *fcoord,
+                        const double *fine, double *coarse, const int *range, void *stream) {
+  RShape S;
+  RRange R;
+  if (!make_shape(shape, S) || !range_of(range, R) || nb < 1 || comp < 0 || comp > 2 ||
+      !fcoord || !fine || !coarse)
+    return rfail("ath_gpu_restrict_fc: bad argument");
+  // along comp: coarse faces 0..ncc and fine faces 0..nc; across: like cells
+  const int lo[3] = {R.i0, R.j0, R.k0}, hi[3] = {R.i1, R.j1, R.k1};
+  const int nc[3] = {S.nc1, S.nc2, S.nc3}, ncc[3] = {S.ncc1, S.ncc2, S.ncc3};
+  for (int d = 0; d < 3; ++d) {
+    const bool ok = (d == comp) ? dir_ok(lo[d], hi[d], 0, ncc[d], S.ng, 0, 0, nc[d])
+                                : dir_ok(lo[d], hi[d], 0, ncc[d] - 1, S.ng, 0, 1, nc[d] - 1);
+    if (!ok) return rfail("ath_gpu_restrict_fc: range outside the arrays");
+  }
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g = rgrid(R, nb, 1);
+  if (comp == 0)
+    hipLaunchKernelGGL(k_restrict_fc<0>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
+  else if (comp == 1)
+    hipLaunchKernelGGL(k_restrict_fc<1>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
+  else
+    hipLaunchKernelGGL(k_restrict_fc<2>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
+  return launch_check(st);
+}
+
+int ath_gpu_prolongate_cc(const int *shape, int nb, int nvar, const double *fcoord,
+                          const double *ccoord, const double *coarse, double *fine,
+                          const int *range, void *stream) {
+  RShape S;
+  RRange R;
+  if (!make_shape(shape, S) || !range_of(range, R) || nb < 1 || nvar < 1 || !fcoord ||
+      !ccoord || !coarse || !fine)
+    return rfail("ath_gpu_prolongate_cc: bad argument");
+  // coarse cells with both neighbours in the buffer; fine cells fi, fi+1 in the block
+  if (!dir_ok(R.i0, R.i1, 1, S.ncc1 - 2, S.ng, 0, 1, S.nc1 - 1) ||
+      !dir_ok(R.j0, R.j1, 1, S.ncc2 - 2, S.ng, 0, 1, S.nc2 - 1) ||
+      !dir_ok(R.k0, R.k1, 1, S.ncc3 - 2, S.ng, 0, 1, S.nc3 - 1))
+    return rfail("ath_gpu_prolongate_cc: range outside the arrays");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_prolongate_cc, rgrid(R, nb, nvar), dim3(RWG), 0, st, S, R, nb, nvar,
+                     fcoord, ccoord, coarse, fine);
+  return launch_check(st);
+}
+
+int ath_gpu_prolongate_fc_shared(const int *shape, int nb, int comp, const double *fcoord,
+                                 const double *ccoord, const double *coarse, double *fine,
+                                 const int *range, void *stream) {
+  RShape S;
+  RRange R;
+  if (!make_shape(shape, S) || !range_of(range, R) || nb < 1 || comp < 0 || comp > 2 ||
+      !fcoord || !ccoord || !coarse || !fine)
+    return rfail("ath_gpu_prolongate_fc_shared: bad argument");
+  // along comp: coarse faces 0..ncc onto fine faces 0..nc (no neighbours read); across:
+  // coarse positions with both neighbours, fine fi, fi+1 in the block
+  const int lo[3] = {R.i0, R.j0, R.k0}, hi[3] = {R.i1, R.j1, R.k1};
+  const int nc[3] = {S.nc1, S.nc2, S.nc3}, ncc[3] = {S.ncc1, S.ncc2, S.ncc3};
+  for (int d = 0; d < 3; ++d) {
+    const bool ok = (d == comp) ? dir_ok(lo[d], hi[d], 0, ncc[d], S.ng, 0, 0, nc[d])
+                                : dir_ok(lo[d], hi[d], 1, ncc[d] - 2, S.ng, 0, 1, nc[d] - 1);
+    if (!ok) return rfail("ath_gpu_prolongate_fc_shared: range outside the arrays");
+  }
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g = rgrid(R, nb, 1);
+  if (comp == 0)
+    hipLaunchKernelGGL(k_prolongate_fc_shared<0>, g, dim3(RWG), 0, st, S, R, nb, fcoord,
+                       ccoord, coarse, fine);
+  else if (comp == 1)
+    hipLaunchKernelGGL(k_prolongate_fc_shared<1>, g, dim3(RWG), 0, st, S, R, nb, fcoord,
+                       ccoord, coarse, fine);
+  else
+    hipLaunchKernelGGL(k_prolongate_fc_shared<2>, g, dim3(RWG), 0, st, S, R, nb, fcoord,
+                       ccoord, coarse, fine);
+  return launch_check(st);
+}
+
+int ath_gpu_prolongate_fc_internal(const int *shape, int nb, const double *fcoord,
+                                   double *b1, double *b2, double *b3, const int *range,
+                                   void *stream) {
+  RShape S;
+  RRange R;
+  if (!make_shape(shape, S) || !range_of(range, R) || nb < 1 || !fcoord || !b1 || !b2 ||
+      !b3)
+    return rfail("ath_gpu_prolongate_fc_internal: bad argument");
+  // each coarse cell reads fine faces f .. f+2 along each direction and cells f, f+1
+  if (!dir_ok(R.i0, R.i1, 0, S.ncc1 - 1, S.ng, 0, 1, S.nc1 - 1) ||
+      !dir_ok(R.j0, R.j1, 0, S.ncc2 - 1, S.ng, 0, 1, S.nc2 - 1) ||
+      !dir_ok(R.k0, R.k1, 0, S.ncc3 - 1, S.ng, 0, 1, S.nc3 - 1))
+    return rfail("ath_gpu_prolongate_fc_internal: range outside the arrays");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_prolongate_fc_internal, rgrid(R, nb, 1), dim3(RWG), 0, st, S, R, nb,
+                     fcoord, b1, b2, b3);
+  return launch_check(st);
+}
+
+}  // extern "C"

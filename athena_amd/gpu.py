@@ -36,7 +36,9 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_ensemble_set_tiles", "ath_gpu_ensemble_autotune",
            "ath_gpu_ensemble_stream", "ath_gpu_mesh_set_tiles", "ath_gpu_mesh_get_tiles",
            "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history",
-           "ath_gpu_ghost_position"]
+           "ath_gpu_ghost_position", "ath_gpu_restrict_cc", "ath_gpu_restrict_fc",
+           "ath_gpu_prolongate_cc", "ath_gpu_prolongate_fc_shared",
+           "ath_gpu_prolongate_fc_internal"]
 NKERNELS = 18
 
 
@@ -123,6 +125,11 @@ def load():
     L.ath_gpu_mesh_autotune.argtypes = [P, I, D]
     L.ath_gpu_kernel_box.argtypes = [I, I, I, IP, IP]
     L.ath_gpu_ghost_position.argtypes = [I, I, I, I, IP]
+    L.ath_gpu_restrict_cc.argtypes = [IP, I, I, P, P, P, IP, P]
+    L.ath_gpu_restrict_fc.argtypes = [IP, I, I, P, P, P, IP, P]
+    L.ath_gpu_prolongate_cc.argtypes = [IP, I, I, P, P, P, P, IP, P]
+    L.ath_gpu_prolongate_fc_shared.argtypes = [IP, I, I, P, P, P, P, IP, P]
+    L.ath_gpu_prolongate_fc_internal.argtypes = [IP, I, P, P, P, P, IP, P]
     L.ath_gpu_history.argtypes = [P, D]
     _lib = L
     return L
@@ -484,6 +491,46 @@ def kernel_box(kind, nx, ng):
     n = np.zeros(3, dtype=np.int32)
     _check(load().ath_gpu_kernel_box(int(kind), int(nx), int(ng), _ip(lo), _ip(n)))
     return lo, n
+
+
+class Refinement:
+    """MeshRefinement operators on device arrays (mesh_refinement.cpp, 3-D uniform
+    Cartesian): ``shape`` = (nx1, nx2, nx3, ng) of the fine blocks, nb blocks per call
+    stored back to back; arguments are device addresses (ints, e.g. a torch tensor's
+    data_ptr()), ``r`` the coarse index range (si, ei, sj, ej, sk, ek)."""
+
+    def __init__(self, nx, ng, stream=None):
+        self.shape = np.array(list(nx) + [ng], dtype=np.int32)
+        self.stream = stream
+
+    def _r(self, r):
+        return np.ascontiguousarray(r, dtype=np.int32)
+
+    def restrict_cc(self, nb, nvar, fcoord, fine, coarse, r):
+        r = self._r(r)
+        _check(load().ath_gpu_restrict_cc(_ip(self.shape), nb, nvar, fcoord, fine, coarse,
+                                          _ip(r), self.stream))
+
+    def restrict_fc(self, nb, comp, fcoord, fine, coarse, r):
+        r = self._r(r)
+        _check(load().ath_gpu_restrict_fc(_ip(self.shape), nb, comp, fcoord, fine, coarse,
+                                          _ip(r), self.stream))
+
+    def prolongate_cc(self, nb, nvar, fcoord, ccoord, coarse, fine, r):
+        r = self._r(r)
+        _check(load().ath_gpu_prolongate_cc(_ip(self.shape), nb, nvar, fcoord, ccoord,
+                                            coarse, fine, _ip(r), self.stream))
+
+    def prolongate_fc_shared(self, nb, comp, fcoord, ccoord, coarse, fine, r):
+        r = self._r(r)
+        _check(load().ath_gpu_prolongate_fc_shared(_ip(self.shape), nb, comp, fcoord,
+                                                   ccoord, coarse, fine, _ip(r),
+                                                   self.stream))
+
+    def prolongate_fc_internal(self, nb, fcoord, b1, b2, b3, r):
+        r = self._r(r)
+        _check(load().ath_gpu_prolongate_fc_internal(_ip(self.shape), nb, fcoord, b1, b2,
+                                                     b3, _ip(r), self.stream))
 
 
 def ghost_position(nx, ng, array, t):

@@ -176,6 +176,37 @@ int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n);
  * (cc/bvals_cc.cpp:287-427, fc/bvals_fc.cpp:344-688) copy there; no device call. */
 int ath_gpu_ghost_position(int nx, int ng, int array, int t, int *pos);
 
+/* ---- Static mesh refinement: MeshRefinement operators -------------------------------
+ * The 3-D uniform-Cartesian branches of MeshRefinement (mesh_refinement.cpp), on device
+ * arrays of nb MeshBlocks of one shape stored back to back, asynchronous on stream.
+ * shape = {nx1, nx2, nx3, ng} (even nx): a block has nx + 2 ng cells per direction and
+ * its coarse buffer nx/2 + 2 ng (meshblock.cpp:66,93-94); fine index = (coarse index -
+ * ng) * 2 + ng.  Cell arrays are (nvar, n3, n2, n1) per block, face component d has one
+ * more entry along d.  fcoord / ccoord: per block the fine / coarse coordinate tables
+ * x1f (n1+1), x1v (n1), dx1f (n1), then the same for x2 and x3.  range = {si, ei, sj, ej,
+ * sk, ek}: the coarse index range, as the reference's arguments; a range whose stencil
+ * leaves the arrays is rejected before any launch. */
+/* RestrictCellCenteredValues (mesh_refinement.cpp:116-150): volume averages. */
+int ath_gpu_restrict_cc(const int *shape, int nb, int nvar, const double *fcoord,
+                        const double *fine, double *coarse, const int *range, void *stream);
+/* RestrictFieldX1/X2/X3 (mesh_refinement.cpp:272-430), comp 0..2: area averages. */
+int ath_gpu_restrict_fc(const int *shape, int nb, int comp, const double *fcoord,
+                        const double *fine, double *coarse, const int *range, void *stream);
+/* ProlongateCellCenteredValues (mesh_refinement.cpp:464-525): minmod-limited linear. */
+int ath_gpu_prolongate_cc(const int *shape, int nb, int nvar, const double *fcoord,
+                          const double *ccoord, const double *coarse, double *fine,
+                          const int *range, void *stream);
+/* ProlongateSharedFieldX1/X2/X3, Cartesian branch (mesh_refinement.cpp:835-872 and the
+ * matching blocks of :914-1221): the four fine faces on each coarse face. */
+int ath_gpu_prolongate_fc_shared(const int *shape, int nb, int comp, const double *fcoord,
+                                 const double *ccoord, const double *coarse, double *fine,
+                                 const int *range, void *stream);
+/* ProlongateInternalField (mesh_refinement.cpp:1223-1338): the twelve fine faces inside
+ * each coarse cell of range, divergence-preserving; b1, b2, b3 the fine face arrays. */
+int ath_gpu_prolongate_fc_internal(const int *shape, int nb, const double *fcoord,
+                                   double *b1, double *b2, double *b3, const int *range,
+                                   void *stream);
+
 /* ---- Learned initial guess for SR primitive recovery --------------------------------
  * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds
  * the Newman-Hamlin iteration with the previous gas pressure.  Here a 6 -> 64 -> 64 -> 1

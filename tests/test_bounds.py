@@ -95,3 +95,73 @@ def test_ghost_exchange_covers_every_ghost_once_from_an_active_source(nx, ng):
         assert seen == ghosts
         with pytest.raises(RuntimeError):
             ghost_position(nx, ng, array, len(ghosts))
+
+
+def _refine_touch_ok(op, d, comp, lo, hi, nx, ng):
+    """Whether every index the refinement kernel `op` touches along direction d, for
+    coarse indices lo..hi, lies inside its arrays (refine.hip's stencils): fine index
+    f = (c - ng) * 2 + ng; cells have nc = nx + 2 ng entries, faces along their normal
+    nc + 1; coarse arrays ncc = nx/2 + 2 ng (+1 for faces along the normal)."""
+    nc, ncc = nx + 2 * ng, nx // 2 + 2 * ng
+    if lo > hi:
+        return False
+    cs = range(lo, hi + 1)
+    fs = [(c - ng) * 2 + ng for c in cs]
+    cells = lambda idx, n: all(0 <= v <= n - 1 for v in idx)
+    if op == "restrict_cc" or (op in ("restrict_fc", "internal") and d != comp):
+        return cells(cs, ncc) and cells(fs + [f + 1 for f in fs], nc)
+    if op == "restrict_fc" or (op == "shared" and d == comp):
+        return cells(cs, ncc + 1) and cells(fs, nc + 1)
+    if op in ("prolongate_cc", "shared"):
+        return (cells([c - 1 for c in cs] + [c + 1 for c in cs], ncc)
+                and cells(fs + [f + 1 for f in fs], nc))
+    # internal, along the face component's own normal: faces f, f+1, f+2
+    return cells(cs, ncc) and cells(fs + [f + 2 for f in fs], nc + 1)
+
+
+@pytest.mark.parametrize("ng", [2, 3])
+def test_refinement_ranges_are_checked_before_any_launch(ng):
+    # every (lo, hi) along every direction: the library accepts exactly the ranges whose
+    # accesses stay inside the arrays, and rejects the others on the host before any
+    # device call (accepted ones reach the launch, which has no device here)
+    from athena_amd.gpu import AthGpuError, Refinement
+    nx = (8, 6, 4)
+    R = Refinement(nx, ng)
+    p = 4096
+    calls = {"restrict_cc": lambda r: R.restrict_cc(1, 1, p, p, p, r),
+             "prolongate_cc": lambda r: R.prolongate_cc(1, 1, p, p, p, p, r)}
+    for comp in range(3):
+        calls["restrict_fc%d" % comp] = lambda r, c=comp: R.restrict_fc(1, c, p, p, p, r)
+        calls["shared%d" % comp] = (lambda r, c=comp:
+                                    R.prolongate_fc_shared(1, c, p, p, p, p, r))
+    calls["internal"] = lambda r: R.prolongate_fc_internal(1, p, p, p, p, r)
+
+    def accepted(fn, r):
+        try:
+            fn(r)
+        except AthGpuError as e:
+            return "range outside" not in str(e) and "bad argument" not in str(e)
+        return True
+
+    base = [[ng, ng + n // 2 - 1] for n in nx]          # a valid range per direction
+    for name, fn in calls.items():
+        op = name.rstrip("012")
+        comp = int(name[-1]) if name[-1].isdigit() else (3 if op == "internal" else -1)
+        for d in range(3):
+            ncc = nx[d] // 2 + 2 * ng
+            for lo in range(-2, ncc + 3):
+                for hi in range(lo - 1, ncc + 3):
+                    r = [v for pair in base for v in pair]
+                    r[2 * d], r[2 * d + 1] = lo, hi
+                    if op == "internal":
+                        want = all(_refine_touch_ok("internal", dd, dd if dd == d else -1,
+                                                    r[2 * dd], r[2 * dd + 1], nx[dd], ng)
+                                   and _refine_touch_ok("internal", dd, -1, r[2 * dd],
+                                                        r[2 * dd + 1], nx[dd], ng)
+                                   for dd in range(3))
+                    else:
+                        want = all(_refine_touch_ok(op, dd, comp, r[2 * dd], r[2 * dd + 1],
+                                                    nx[dd], ng) for dd in range(3))
+                    assert accepted(fn, r) == want, (name, d, lo, hi)
+    with pytest.raises(AthGpuError, match="bad argument"):
+        Refinement((7, 8, 8), 2).restrict_cc(1, 1, p, p, p, [2, 4, 2, 5, 2, 5])   # odd nx
