@@ -70,6 +70,16 @@ MEASURED_TRAFFIC = {
     "ghost_exchange": 2298980494,
 }
 TRAFFIC_TILES = ["linear", "zmarch8", "linear", "linear"]
+# FP64 FLOP per launch (64 x (ADD + MUL + 2 FMA) of the SQ_INSTS_VALU_*_F64 counters) on
+# the same workload: profiles/vl2_hlld_plm_turbulence256_fp64_valu.md.
+MEASURED_FP64_FLOP = {
+    "flux_xyz_plm": 47786607216,
+    "flux_xyz_dc": 31422754608,
+    "update_u_b_ct": 4173332480,
+    "corner_emf": 2138603520,
+    "cons2prim": 3371907300,
+}
+FP64_PEAK = 78.6e12
 
 
 def kernel_algorithmic_bytes(name, m):
@@ -529,6 +539,11 @@ def main():
             "algorithmic_bytes_per_launch": abytes}
     if roof["traffic"]:
         roof["traffic_GBps"] = roof["traffic"] / dom_avg_s / 1e9
+    if measured and dom in MEASURED_FP64_FLOP:
+        fl = MEASURED_FP64_FLOP[dom]
+        roof["fp64_valu"] = {"flop_per_launch": fl, "achieved": fl / dom_avg_s / 1e12,
+                             "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                             "frac": fl / dom_avg_s / FP64_PEAK}
     step_roof = {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE,
                  "achieved": zcs * BYTES_PER_ZONE_CYCLE / 1e9, "unit": "GB/s",
                  "frac": zcs * BYTES_PER_ZONE_CYCLE / HBM_PEAK}
