@@ -2,8 +2,9 @@
 // refine.hip -- MeshRefinement operators for static mesh refinement on MI355X (gfx950):
 // restriction of cell-centred values and face fields to a MeshBlock's coarse buffer,
 // prolongation from the coarse buffer (minmod-limited linear for cells and for the fine
-// faces on coarse faces), and the divergence-preserving prolongation of the fine faces
-// inside coarse cells (Toth & Roe 2002).  3-D, uniform-Cartesian branches of
+// faces on coarse faces), the divergence-preserving prolongation of the fine faces
+// inside coarse cells (Toth & Roe 2002), and the restriction of edge EMFs used by the
+// EMF correction between levels.  3-D, uniform-Cartesian branches of
 // mesh_refinement.cpp:116-150, 272-300, 328-372, 402-430, 464-525, 835-872 (and the
 // matching Cartesian blocks of ProlongateSharedFieldX2/X3), 1223-1338, with each
 // expression in the reference's association order (built with -ffp-contract=off).
@@ -315,6 +316,35 @@ __global__ void __launch_bounds__(RWG) k_prolongate_fc_internal(RShape S, RRange
     }
 }
 
+// The EMF restriction of FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser
+// (flux_correction_fc.cpp:315-420): a coarse edge along comp is the length-weighted
+// average of its two fine edges.  Edge arrays: along x1 (n3+1, n2+1, n1), along x2
+// (n3+1, n2, n1+1), along x3 (n3, n2+1, n1+1).
+template <int comp>
+__global__ void __launch_bounds__(RWG) k_restrict_emf(RShape S, RRange r, int nb,
+                                                       const double *fcoord,
+                                                       const double *fine, double *coarse) {
+  int b, n, ck, cj, ci;
+  if (!rmap(r, nb, 1, b, n, ck, cj, ci)) return;
+  const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
+  const int fn1 = S.nc1 + (comp != 0), fn2 = S.nc2 + (comp != 1), fn3 = S.nc3 + (comp != 2);
+  const int cn1 = S.ncc1 + (comp != 0), cn2 = S.ncc2 + (comp != 1), cn3 = S.ncc3 + (comp != 2);
+  const double *f = fine + static_cast<long>(b) * fn1 * fn2 * fn3;
+  const int k = (ck - S.ng) * 2 + S.ng, j = (cj - S.ng) * 2 + S.ng, i = (ci - S.ng) * 2 + S.ng;
+  double v;
+  if (comp == 0) {
+    const double l1 = F.dx1f[i], l2 = F.dx1f[i + 1];
+    v = (f[at(k, j, i, fn2, fn1)] * l1 + f[at(k, j, i + 1, fn2, fn1)] * l2) / (l1 + l2);
+  } else if (comp == 1) {
+    const double l1 = F.dx2f[j], l2 = F.dx2f[j + 1];
+    v = (f[at(k, j, i, fn2, fn1)] * l1 + f[at(k, j + 1, i, fn2, fn1)] * l2) / (l1 + l2);
+  } else {
+    const double l1 = F.dx3f[k], l2 = F.dx3f[k + 1];
+    v = (f[at(k, j, i, fn2, fn1)] * l1 + f[at(k + 1, j, i, fn2, fn1)] * l2) / (l1 + l2);
+  }
+  coarse[static_cast<long>(b) * cn1 * cn2 * cn3 + at(ck, cj, ci, cn2, cn1)] = v;
+}
+
 //----------------------------------------------------------------------------------------
 // Host side: shapes, range checks, launches.
 
@@ -472,6 +502,33 @@ int ath_gpu_prolongate_fc_internal(const int *shape, int nb, const double *fcoor
   const hipStream_t st = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_prolongate_fc_internal, rgrid(R, nb, 1), dim3(RWG), 0, st, S, R, nb,
                      fcoord, b1, b2, b3);
+  return launch_check(st);
+}
+
+int ath_gpu_restrict_emf(const int *shape, int nb, int comp, const double *fcoord,
+                         const double *fine, double *coarse, const int *range, void *stream) {
+  RShape S;
+  RRange R;
+  if (!make_shape(shape, S) || !range_of(range, R) || nb < 1 || comp < 0 || comp > 2 ||
+      !fcoord || !fine || !coarse)
+    return rfail("ath_gpu_restrict_emf: bad argument");
+  // along comp: coarse edges 0..ncc-1 over fine edges f, f+1; across: edge positions
+  // at faces, coarse 0..ncc and fine 0..nc
+  const int lo[3] = {R.i0, R.j0, R.k0}, hi[3] = {R.i1, R.j1, R.k1};
+  const int nc[3] = {S.nc1, S.nc2, S.nc3}, ncc[3] = {S.ncc1, S.ncc2, S.ncc3};
+  for (int d = 0; d < 3; ++d) {
+    const bool ok = (d == comp) ? dir_ok(lo[d], hi[d], 0, ncc[d] - 1, S.ng, 0, 1, nc[d] - 1)
+                                : dir_ok(lo[d], hi[d], 0, ncc[d], S.ng, 0, 0, nc[d]);
+    if (!ok) return rfail("ath_gpu_restrict_emf: range outside the arrays");
+  }
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g = rgrid(R, nb, 1);
+  if (comp == 0)
+    hipLaunchKernelGGL(k_restrict_emf<0>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
+  else if (comp == 1)
+    hipLaunchKernelGGL(k_restrict_emf<1>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
+  else
+    hipLaunchKernelGGL(k_restrict_emf<2>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
   return launch_check(st);
 }
 

@@ -38,7 +38,7 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history",
            "ath_gpu_ghost_position", "ath_gpu_restrict_cc", "ath_gpu_restrict_fc",
            "ath_gpu_prolongate_cc", "ath_gpu_prolongate_fc_shared",
-           "ath_gpu_prolongate_fc_internal"]
+           "ath_gpu_prolongate_fc_internal", "ath_gpu_restrict_emf"]
 NKERNELS = 18
 
 
@@ -130,6 +130,7 @@ def load():
     L.ath_gpu_prolongate_cc.argtypes = [IP, I, I, P, P, P, P, IP, P]
     L.ath_gpu_prolongate_fc_shared.argtypes = [IP, I, I, P, P, P, P, IP, P]
     L.ath_gpu_prolongate_fc_internal.argtypes = [IP, I, P, P, P, P, IP, P]
+    L.ath_gpu_restrict_emf.argtypes = [IP, I, I, P, P, P, IP, P]
     L.ath_gpu_history.argtypes = [P, D]
     _lib = L
     return L
@@ -526,6 +527,11 @@ class Refinement:
         _check(load().ath_gpu_prolongate_fc_shared(_ip(self.shape), nb, comp, fcoord,
                                                    ccoord, coarse, fine, _ip(r),
                                                    self.stream))
+
+    def restrict_emf(self, nb, comp, fcoord, fine, coarse, r):
+        r = self._r(r)
+        _check(load().ath_gpu_restrict_emf(_ip(self.shape), nb, comp, fcoord, fine, coarse,
+                                           _ip(r), self.stream))
 
     def prolongate_fc_internal(self, nb, fcoord, b1, b2, b3, r):
         r = self._r(r)

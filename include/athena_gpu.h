@@ -206,6 +206,14 @@ int ath_gpu_prolongate_fc_shared(const int *shape, int nb, int comp, const doubl
 int ath_gpu_prolongate_fc_internal(const int *shape, int nb, const double *fcoord,
                                    double *b1, double *b2, double *b3, const int *range,
                                    void *stream);
+/* The EMF restriction of FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser
+ * (flux_correction_fc.cpp:315-420), comp 0..2: each coarse edge along comp is the
+ * length-weighted average of its two fine edges.  Edge arrays per block: along x1
+ * (n3+1, n2+1, n1), along x2 (n3+1, n2, n1+1), along x3 (n3, n2+1, n1+1).  (The face-flux
+ * restriction of the cell-centred flux correction, flux_correction_cc.cpp:77-120, is
+ * ath_gpu_restrict_fc applied to each flux component: the same area average.) */
+int ath_gpu_restrict_emf(const int *shape, int nb, int comp, const double *fcoord,
+                         const double *fine, double *coarse, const int *range, void *stream);
 
 /* ---- Learned initial guess for SR primitive recovery --------------------------------
  * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds

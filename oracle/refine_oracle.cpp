@@ -344,4 +344,38 @@ void ora_prolongate_fc_internal(const int *shape, const double *fcoord, double *
   }
 }
 
+// The EMF restriction of FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser
+// (flux_correction_fc.cpp:315-420, 3-D Cartesian): a coarse edge along direction comp is
+// the length-weighted average of the two fine edges it contains.  Edge arrays of a grid
+// with n cells: along x1 (n3+1, n2+1, n1), along x2 (n3+1, n2, n1+1), along x3
+// (n3, n2+1, n1+1).
+void ora_restrict_emf(const int *shape, int comp, const double *fcoord, const double *fine,
+                      double *coarse, const int *r) {
+  const Shape S(shape);
+  const Grid F(fcoord, S.nc1, S.nc2, S.nc3);
+  const int fn1 = S.nc1 + (comp != 0), fn2 = S.nc2 + (comp != 1);
+  const int cn1 = S.ncc1 + (comp != 0), cn2 = S.ncc2 + (comp != 1);
+  Arr3 f{const_cast<Real *>(fine), fn2, fn1};
+  Arr3 c{coarse, cn2, cn1};
+  for (int ck = r[4]; ck <= r[5]; ++ck) {
+    const int k = (ck - S.cks) * 2 + S.ks;
+    for (int cj = r[2]; cj <= r[3]; ++cj) {
+      const int j = (cj - S.cjs) * 2 + S.js;
+      for (int ci = r[0]; ci <= r[1]; ++ci) {
+        const int i = (ci - S.cis) * 2 + S.is;
+        if (comp == 0) {
+          const Real l1 = F.dx1f[i], l2 = F.dx1f[i + 1];
+          c(ck, cj, ci) = (f(k, j, i) * l1 + f(k, j, i + 1) * l2) / (l1 + l2);
+        } else if (comp == 1) {
+          const Real l1 = F.dx2f[j], l2 = F.dx2f[j + 1];
+          c(ck, cj, ci) = (f(k, j, i) * l1 + f(k, j + 1, i) * l2) / (l1 + l2);
+        } else {
+          const Real l1 = F.dx3f[k], l2 = F.dx3f[k + 1];
+          c(ck, cj, ci) = (f(k, j, i) * l1 + f(k + 1, j, i) * l2) / (l1 + l2);
+        }
+      }
+    }
+  }
+}
+
 }  // extern "C"

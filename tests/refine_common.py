@@ -62,6 +62,7 @@ def lib():
     L.ora_prolongate_cc.argtypes = [P, I, P, P, P, P, P]
     L.ora_prolongate_fc_shared.argtypes = [P, I, P, P, P, P, P]
     L.ora_prolongate_fc_internal.argtypes = [P, P, P, P, P, P]
+    L.ora_restrict_emf.argtypes = [P, I, P, P, P, P]
     return L
 
 
@@ -93,6 +94,17 @@ def ora_prolongate_fc_shared(shape, comp, fc, cc, coarse, fine, r):
 def ora_prolongate_fc_internal(shape, fc, b1, b2, b3, r):
     lib().ora_prolongate_fc_internal(_p(shape), _p(fc), _p(b1), _p(b2), _p(b3),
                                      _p(np.asarray(r, dtype=np.int32)))
+
+
+def ora_restrict_emf(shape, comp, fc, fine, coarse, r):
+    lib().ora_restrict_emf(_p(shape), comp, _p(fc), _p(fine), _p(coarse),
+                           _p(np.asarray(r, dtype=np.int32)))
+
+
+def edge_shape(n, comp):
+    """(n3, n2, n1) of the edges along comp of a grid with n = (n1, n2, n3) cells."""
+    n1, n2, n3 = n
+    return (n3 + (comp != 2), n2 + (comp != 1), n1 + (comp != 0))
 
 
 def face_shape(n, comp):

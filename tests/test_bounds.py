@@ -110,6 +110,10 @@ def _refine_touch_ok(op, d, comp, lo, hi, nx, ng):
     cells = lambda idx, n: all(0 <= v <= n - 1 for v in idx)
     if op == "restrict_cc" or (op in ("restrict_fc", "internal") and d != comp):
         return cells(cs, ncc) and cells(fs + [f + 1 for f in fs], nc)
+    if op == "emf":                    # edges along comp: cells along comp, faces across
+        if d == comp:
+            return cells(cs, ncc) and cells(fs + [f + 1 for f in fs], nc)
+        return cells(cs, ncc + 1) and cells(fs, nc + 1)
     if op == "restrict_fc" or (op == "shared" and d == comp):
         return cells(cs, ncc + 1) and cells(fs, nc + 1)
     if op in ("prolongate_cc", "shared"):
@@ -135,6 +139,8 @@ def test_refinement_ranges_are_checked_before_any_launch(ng):
         calls["shared%d" % comp] = (lambda r, c=comp:
                                     R.prolongate_fc_shared(1, c, p, p, p, p, r))
     calls["internal"] = lambda r: R.prolongate_fc_internal(1, p, p, p, p, r)
+    for comp in range(3):
+        calls["emf%d" % comp] = lambda r, c=comp: R.restrict_emf(1, c, p, p, p, r)
 
     def accepted(fn, r):
         try:

@@ -7,9 +7,9 @@ import pytest
 import torch
 
 from athena_amd.gpu import Refinement
-from refine_common import (coords, dims, face_shape, ora_prolongate_cc,
+from refine_common import (coords, dims, edge_shape, face_shape, ora_prolongate_cc,
                            ora_prolongate_fc_internal, ora_prolongate_fc_shared,
-                           ora_restrict_cc, ora_restrict_fc, shape_of)
+                           ora_restrict_cc, ora_restrict_emf, ora_restrict_fc, shape_of)
 
 pytestmark = pytest.mark.gpu
 
@@ -124,3 +124,25 @@ def test_internal_face_prolongation(nx, ng):
     torch.cuda.synchronize()
     for c in range(3):
         assert np.array_equal(d[c].cpu().numpy(), want[c]), c
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+@pytest.mark.parametrize("nx,ng", SHAPES)
+def test_emf_restriction(nx, ng, comp):
+    print("case refine emf", nx, ng, comp, flush=True)
+    rng, s, fn, cn, fcs, ccs = _setup(nx, ng, 13 + comp)
+    R = Refinement(nx, ng)
+    fine = rng.normal(size=(NB,) + edge_shape(fn, comp))
+    coarse0 = rng.normal(size=(NB,) + edge_shape(cn, comp))
+    r = _ranges(s)
+    for d in range(3):
+        if d != comp:
+            r[2 * d + 1] += 1
+    want = coarse0.copy()
+    for b in range(NB):
+        ora_restrict_emf(s, comp, fcs[b], np.ascontiguousarray(fine[b]), want[b], r)
+    fc_d = _dev(np.concatenate(fcs))
+    f_d, c_d = _dev(fine), _dev(coarse0)
+    R.restrict_emf(NB, comp, fc_d.data_ptr(), f_d.data_ptr(), c_d.data_ptr(), r)
+    torch.cuda.synchronize()
+    assert np.array_equal(c_d.cpu().numpy(), want)

@@ -9,10 +9,10 @@ discrete curl of an edge potential, and reproduces linear divergence-free fields
 import numpy as np
 import pytest
 
-from refine_common import (active_coarse_range, block_coords, dims, face_shape,
-                           ora_prolongate_cc, ora_prolongate_fc_internal,
-                           ora_prolongate_fc_shared, ora_restrict_cc, ora_restrict_fc,
-                           shape_of, tables)
+from refine_common import (active_coarse_range, block_coords, dims, edge_shape,
+                           face_shape, ora_prolongate_cc, ora_prolongate_fc_internal,
+                           ora_prolongate_fc_shared, ora_restrict_cc, ora_restrict_emf,
+                           ora_restrict_fc, shape_of, tables)
 
 SHAPES = [((8, 8, 8), 2), ((8, 4, 6), 3), ((4, 6, 8), 4)]
 COEF = np.array([0.7, 1.3, -0.4, 2.1])            # f = c0 + c1 x + c2 y + c3 z
@@ -196,3 +196,31 @@ def test_internal_prolongation_reproduces_linear_divergence_free_fields(nx, ng):
         a = [ng, ng + nx1 - 1, ng, ng + nx2 - 1, ng, ng + nx3 - 1]
         a[2 * comp + 1] += 1
         np.testing.assert_allclose(sel(fb[comp], a), sel(want, a), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+@pytest.mark.parametrize("nx,ng", SHAPES)
+def test_emf_restriction_exact_on_linear_edge_fields(nx, ng, comp):
+    # the length-weighted average of the two fine edges of a coarse edge is the line
+    # average over the coarse edge: exact for a field linear along the edge (and any
+    # transverse dependence, since both fine edges sit on the same line)
+    s = shape_of(nx, ng)
+    fn, cn = dims(s)
+    fc, cc = block_coords(s)
+    ft, ct = tables(fc, fn), tables(cc, cn)
+
+    def edge_field(t):
+        xs = [t["x1f"], t["x2f"], t["x3f"]]
+        xs[comp] = t["x%dv" % (comp + 1)]
+        z, y, x = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+        transverse = (y * z, x * z, x * y)[comp]      # both fine edges share these
+        return lin(x, y, z) + transverse
+
+    fine = np.ascontiguousarray(edge_field(ft))
+    coarse = np.zeros(edge_shape(cn, comp))
+    r = active_coarse_range(s)
+    for d in range(3):
+        if d != comp:
+            r[2 * d + 1] += 1
+    ora_restrict_emf(s, comp, fc, fine, coarse, r)
+    np.testing.assert_allclose(sel(coarse, r), sel(edge_field(ct), r), rtol=1e-14, atol=1e-14)
