@@ -1082,8 +1082,13 @@ __device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int l
   }
 }
 
+// At least 3 wavefronts per SIMD: the SR recovery needs ~190 VGPRs unbounded (2 waves);
+// capped at 168 it spills a few registers to scratch and runs faster (the Newtonian
+// instantiation needs 62 and is unaffected).
+constexpr int C2P_MIN_WAVES = 3;
+
 template <bool SR, bool LEARNED>
-__global__ void __launch_bounds__(WG) k_cons2prim(MeshK M, C2PArgs a, TileK T) {
+__global__ void __launch_bounds__(WG, C2P_MIN_WAVES) k_cons2prim(MeshK M, C2PArgs a, TileK T) {
   c2p_body<SR, LEARNED>(M, a, xcd_logical_wg(M.xcd_remap), gridDim.x, blockIdx.x, T);
 }
 
@@ -1303,7 +1308,7 @@ __global__ void __launch_bounds__(WG) k_update_ens(const EnsMember *__restrict__
 }
 
 template <bool SR>
-__global__ void __launch_bounds__(WG) k_c2p_ens(const EnsMember *__restrict__ E,
+__global__ void __launch_bounds__(WG, C2P_MIN_WAVES) k_c2p_ens(const EnsMember *__restrict__ E,
                                                  const int2 *__restrict__ map, int nwork) {
   for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
     const int2 mw = ens_item(map, v);

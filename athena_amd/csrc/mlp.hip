@@ -259,8 +259,10 @@ __global__ void __launch_bounds__(256) k_mlp_seed(MlpK K, int n, const double *_
 
 // Batched SR primitive recovery with the solver mode of physics.hpp (reference NH, Newton
 // from the register seed, or Newton from the learned seed in `seed`).
+// bounded to 3 wavefronts per SIMD: ~190 VGPRs unbounded; at 168 a few registers spill
+// and the solve runs 4-8 % faster
 template <int MODE>
-__global__ void __launch_bounds__(256) k_sr_c2p_modes(EosK eos, int n, double *cons,
+__global__ void __launch_bounds__(256, 3) k_sr_c2p_modes(EosK eos, int n, double *cons,
                                                       double *prim, const double *bcc,
                                                       const double *__restrict__ seed,
                                                       int *iters, int *evals, int *fallback) {

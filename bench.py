@@ -50,6 +50,9 @@ class HipEvents:
     def record(self, e, stream):
         assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
 
+    def destroy(self, e):
+        assert self.h.hipEventDestroy(e) == 0
+
     def elapsed_ms(self, a, b):
         assert self.h.hipEventSynchronize(b) == 0
         ms = ctypes.c_float()
@@ -172,24 +175,43 @@ def bench_sr_c2p(args):
     evs, fbs = torch.empty_like(it), torch.empty_like(it)
     ev = HipEvents()
     res = {}
+    # A batch launches K calls back to back, each on its own copy of the states (filled
+    # beforehand, cold seed), so no host latency sits between them and no call finds its
+    # input in the caches.  Untimed batches over every mode run first for a fixed wall
+    # time: the GPU clocks need ~150 calls of ~0.2 ms to settle (a cold call runs ~25 %
+    # slower), which a few warm-up calls per mode would not cover.
+    Cs = [torch.empty_like(C0) for _ in range(args.steps)]
+    Ps = [torch.empty_like(C0) for _ in range(args.steps)]
+
+    def batch(mode):
+        for Ci, Pi in zip(Cs, Ps):
+            Ci.copy_(C0)
+            Pi.zero_()                      # cold: the seed register holds nothing
+        torch.cuda.synchronize()
+        e0, e1 = ev.new(), ev.new()
+        ev.record(e0, 0)
+        for Ci, Pi in zip(Cs, Ps):
+            gpu.sr_con2prim_device(mode, n, Ci.data_ptr(), Pi.data_ptr(), B.data_ptr(),
+                                   ml.GAMMA_AD, FL, FL, mlp=net)
+        ev.record(e1, 0)
+        ms = ev.elapsed_ms(e0, e1)
+        ev.destroy(e0)
+        ev.destroy(e1)
+        return ms
+
+    t_end = time.perf_counter() + args.c2p_warm_s
+    while time.perf_counter() < t_end:
+        for mode in (0, 1, 2):
+            batch(mode)
     for mode, name in ((0, "reference"), (1, "newton"), (2, "learned")):
-        total = 0.0
-        # untimed pass with the per-cell counters, then timed passes without them
-        for r in range(1 + args.warmup + args.steps):
-            C.copy_(C0)
-            P.zero_()                       # cold: the seed register holds nothing
-            torch.cuda.synchronize()
-            e0, e1 = ev.new(), ev.new()
-            ev.record(e0, 0)
-            gpu.sr_con2prim_device(mode, n, C.data_ptr(), P.data_ptr(), B.data_ptr(),
-                                   ml.GAMMA_AD, FL, FL, mlp=net,
-                                   iters=it.data_ptr() if r == 0 else None,
-                                   evals=evs.data_ptr() if r == 0 else None,
-                                   fallback=fbs.data_ptr() if r == 0 else None)
-            ev.record(e1, 0)
-            ms = ev.elapsed_ms(e0, e1)
-            if r > args.warmup:
-                total += ms
+        # untimed call with the per-cell counters
+        C.copy_(C0)
+        P.zero_()
+        gpu.sr_con2prim_device(mode, n, C.data_ptr(), P.data_ptr(), B.data_ptr(),
+                               ml.GAMMA_AD, FL, FL, mlp=net, iters=it.data_ptr(),
+                               evals=evs.data_ptr(), fallback=fbs.data_ptr())
+        for r in range(1 + args.warmup):    # warm-up batches, then the timed one
+            total = batch(mode)
         e = evs.cpu().numpy().astype(np.float64)
         res[name] = {"cell_solves_per_s": n * args.steps / (total * 1e-3),
                      "ms_per_call": total / args.steps,
@@ -458,6 +480,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="mesh cells per direction")
     ap.add_argument("--bn", type=int, default=32, help="MeshBlock cells per direction")
     ap.add_argument("--c2p-n", type=int, default=128, help="states per direction (sr_c2p)")
+    ap.add_argument("--c2p-warm-s", type=float, default=1.0,
+                    help="seconds of untimed calls over every mode before timing (sr_c2p)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default thread-to-cell mapping (turbulence)")
