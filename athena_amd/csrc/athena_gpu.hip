@@ -62,7 +62,9 @@ struct MeshK {
   long ne1, ne2, ne3;         // x1e, x2e, x3e edge-array sizes per block
   int cstride;                // doubles of coordinate table per block
   int coff[3];                // offset of direction d inside the table
-  const int *nbr;             // [nblocks][27] same-level neighbour gid
+  const int *nbr;             // [nblocks][27] same-level neighbour gid, -1 across an
+                              // outflow face of the mesh
+  int outflow;                // 1: some mesh face is an outflow boundary
   const double *coord;        // [nblocks][cstride]: per d: dxf[nc], pf[nc], mf[nc]
   EosK eos;
   double cfl;
@@ -657,20 +659,42 @@ __global__ void __launch_bounds__(WG) k_corner_march(MeshK M, CornerArgs ca, Til
 // by 1/2 (face) or 1/nedge_fine = 1/4 (edge) (flux_correction_fc.cpp:51-…, 689-…,
 // 1355-1548; bvals_base.cpp:327-700).  Raw EMFs are never modified, so every thread
 // reads the un-averaged neighbour values exactly like the packed send buffers.
+// Scale of an edge EMF shared by cnt blocks (1/nedge_fine, bvals_fc.cpp:1129-1193 and
+// flux_correction_fc.cpp:1478-1545; a block alone on its edge keeps its value).
+__device__ __forceinline__ double edge_scale(double v, int cnt) {
+  if (cnt == 1) return v;
+  return (cnt == 2) ? v * 0.5 : ((cnt == 4) ? v * 0.25 : v * (1.0 / 3.0));
+}
+
+// With outflow faces a neighbour may be missing (nbr = -1): it adds nothing, a face with
+// no neighbour is not averaged (flux_correction_fc.cpp:1363-1366) and an edge is scaled
+// by the number of blocks around it.
 __device__ __forceinline__ double emf3(const MeshK &M, const double *e, int g, int k,
                                        int j, int i) {
   double v = e[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)];
   const int o1 = (i == M.is) ? -1 : ((i == M.ie + 1) ? 1 : 0);
   const int o2 = (j == M.js) ? -1 : ((j == M.je + 1) ? 1 : 0);
   if (!o1 && !o2) return v;
-  if (o1) v += e[static_cast<long>(nbr(M, g, o1, 0, 0)) * M.ne3 + e3idx(M, k, j, i - o1 * M.nx1)];
-  if (o2) v += e[static_cast<long>(nbr(M, g, 0, o2, 0)) * M.ne3 + e3idx(M, k, j - o2 * M.nx2, i)];
-  if (o1 && o2) {
-    v += e[static_cast<long>(nbr(M, g, o1, o2, 0)) * M.ne3 +
-           e3idx(M, k, j - o2 * M.nx2, i - o1 * M.nx1)];
-    return v * 0.25;
+  if (!M.outflow) {
+    if (o1) v += e[static_cast<long>(nbr(M, g, o1, 0, 0)) * M.ne3 + e3idx(M, k, j, i - o1 * M.nx1)];
+    if (o2) v += e[static_cast<long>(nbr(M, g, 0, o2, 0)) * M.ne3 + e3idx(M, k, j - o2 * M.nx2, i)];
+    if (o1 && o2) {
+      v += e[static_cast<long>(nbr(M, g, o1, o2, 0)) * M.ne3 +
+             e3idx(M, k, j - o2 * M.nx2, i - o1 * M.nx1)];
+      return v * 0.25;
+    }
+    return v * 0.5;
   }
-  return v * 0.5;
+  const int na = o1 ? nbr(M, g, o1, 0, 0) : -1, nb = o2 ? nbr(M, g, 0, o2, 0) : -1;
+  if (na >= 0) v += e[static_cast<long>(na) * M.ne3 + e3idx(M, k, j, i - o1 * M.nx1)];
+  if (nb >= 0) v += e[static_cast<long>(nb) * M.ne3 + e3idx(M, k, j - o2 * M.nx2, i)];
+  if (o1 && o2) {
+    const int ne = nbr(M, g, o1, o2, 0);
+    if (ne >= 0)
+      v += e[static_cast<long>(ne) * M.ne3 + e3idx(M, k, j - o2 * M.nx2, i - o1 * M.nx1)];
+    return edge_scale(v, 1 + (na >= 0) + (nb >= 0) + (ne >= 0));
+  }
+  return (na >= 0 || nb >= 0) ? v * 0.5 : v;
 }
 __device__ __forceinline__ double emf2(const MeshK &M, const double *e, int g, int k,
                                        int j, int i) {
@@ -678,14 +702,26 @@ __device__ __forceinline__ double emf2(const MeshK &M, const double *e, int g, i
   const int o1 = (i == M.is) ? -1 : ((i == M.ie + 1) ? 1 : 0);
   const int o3 = (k == M.ks) ? -1 : ((k == M.ke + 1) ? 1 : 0);
   if (!o1 && !o3) return v;
-  if (o1) v += e[static_cast<long>(nbr(M, g, o1, 0, 0)) * M.ne2 + e2idx(M, k, j, i - o1 * M.nx1)];
-  if (o3) v += e[static_cast<long>(nbr(M, g, 0, 0, o3)) * M.ne2 + e2idx(M, k - o3 * M.nx3, j, i)];
-  if (o1 && o3) {
-    v += e[static_cast<long>(nbr(M, g, o1, 0, o3)) * M.ne2 +
-           e2idx(M, k - o3 * M.nx3, j, i - o1 * M.nx1)];
-    return v * 0.25;
+  if (!M.outflow) {
+    if (o1) v += e[static_cast<long>(nbr(M, g, o1, 0, 0)) * M.ne2 + e2idx(M, k, j, i - o1 * M.nx1)];
+    if (o3) v += e[static_cast<long>(nbr(M, g, 0, 0, o3)) * M.ne2 + e2idx(M, k - o3 * M.nx3, j, i)];
+    if (o1 && o3) {
+      v += e[static_cast<long>(nbr(M, g, o1, 0, o3)) * M.ne2 +
+             e2idx(M, k - o3 * M.nx3, j, i - o1 * M.nx1)];
+      return v * 0.25;
+    }
+    return v * 0.5;
   }
-  return v * 0.5;
+  const int na = o1 ? nbr(M, g, o1, 0, 0) : -1, nb = o3 ? nbr(M, g, 0, 0, o3) : -1;
+  if (na >= 0) v += e[static_cast<long>(na) * M.ne2 + e2idx(M, k, j, i - o1 * M.nx1)];
+  if (nb >= 0) v += e[static_cast<long>(nb) * M.ne2 + e2idx(M, k - o3 * M.nx3, j, i)];
+  if (o1 && o3) {
+    const int ne = nbr(M, g, o1, 0, o3);
+    if (ne >= 0)
+      v += e[static_cast<long>(ne) * M.ne2 + e2idx(M, k - o3 * M.nx3, j, i - o1 * M.nx1)];
+    return edge_scale(v, 1 + (na >= 0) + (nb >= 0) + (ne >= 0));
+  }
+  return (na >= 0 || nb >= 0) ? v * 0.5 : v;
 }
 __device__ __forceinline__ double emf1(const MeshK &M, const double *e, int g, int k,
                                        int j, int i) {
@@ -693,14 +729,26 @@ __device__ __forceinline__ double emf1(const MeshK &M, const double *e, int g, i
   const int o2 = (j == M.js) ? -1 : ((j == M.je + 1) ? 1 : 0);
   const int o3 = (k == M.ks) ? -1 : ((k == M.ke + 1) ? 1 : 0);
   if (!o2 && !o3) return v;
-  if (o2) v += e[static_cast<long>(nbr(M, g, 0, o2, 0)) * M.ne1 + e1idx(M, k, j - o2 * M.nx2, i)];
-  if (o3) v += e[static_cast<long>(nbr(M, g, 0, 0, o3)) * M.ne1 + e1idx(M, k - o3 * M.nx3, j, i)];
-  if (o2 && o3) {
-    v += e[static_cast<long>(nbr(M, g, 0, o2, o3)) * M.ne1 +
-           e1idx(M, k - o3 * M.nx3, j - o2 * M.nx2, i)];
-    return v * 0.25;
+  if (!M.outflow) {
+    if (o2) v += e[static_cast<long>(nbr(M, g, 0, o2, 0)) * M.ne1 + e1idx(M, k, j - o2 * M.nx2, i)];
+    if (o3) v += e[static_cast<long>(nbr(M, g, 0, 0, o3)) * M.ne1 + e1idx(M, k - o3 * M.nx3, j, i)];
+    if (o2 && o3) {
+      v += e[static_cast<long>(nbr(M, g, 0, o2, o3)) * M.ne1 +
+             e1idx(M, k - o3 * M.nx3, j - o2 * M.nx2, i)];
+      return v * 0.25;
+    }
+    return v * 0.5;
   }
-  return v * 0.5;
+  const int na = o2 ? nbr(M, g, 0, o2, 0) : -1, nb = o3 ? nbr(M, g, 0, 0, o3) : -1;
+  if (na >= 0) v += e[static_cast<long>(na) * M.ne1 + e1idx(M, k, j - o2 * M.nx2, i)];
+  if (nb >= 0) v += e[static_cast<long>(nb) * M.ne1 + e1idx(M, k - o3 * M.nx3, j, i)];
+  if (o2 && o3) {
+    const int ne = nbr(M, g, 0, o2, o3);
+    if (ne >= 0)
+      v += e[static_cast<long>(ne) * M.ne1 + e1idx(M, k - o3 * M.nx3, j - o2 * M.nx2, i)];
+    return edge_scale(v, 1 + (na >= 0) + (nb >= 0) + (ne >= 0));
+  }
+  return (na >= 0 || nb >= 0) ? v * 0.5 : v;
 }
 
 //========================================================================================
@@ -916,11 +964,21 @@ __device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, i
   }
   int i, j, k;
   ghost_pos(t, n1, n2, n3, ng, i, j, k);
-  const int p1 = ghost_off(i, n1, ng), p2 = ghost_off(j, n2, ng), p3 = ghost_off(k, n3, ng);
+  int p1 = ghost_off(i, n1, ng), p2 = ghost_off(j, n2, ng), p3 = ghost_off(k, n3, ng);
+  int si = i - p1 * M.nx1, sj = j - p2 * M.nx2, sk = k - p3 * M.nx3;
+  if (M.outflow) {
+    // No neighbour across an outflow face: the zero-gradient copy of the block's own
+    // boundary layer (cc/outflow_cc.cpp, fc/outflow_fc.cpp), the last active cell or, for
+    // a face component along its normal, the boundary face (n - ng - 1 = ie + 1).
+    // Directions that keep a neighbour still take its values, so an edge or corner ghost
+    // gets what the reference's exchange followed by the boundary copy leaves there.
+    if (p1 && nbr(M, g, p1, 0, 0) < 0) { si = (p1 < 0) ? ng : n1 - ng - 1; p1 = 0; }
+    if (p2 && nbr(M, g, 0, p2, 0) < 0) { sj = (p2 < 0) ? ng : n2 - ng - 1; p2 = 0; }
+    if (p3 && nbr(M, g, 0, 0, p3) < 0) { sk = (p3 < 0) ? ng : n3 - ng - 1; p3 = 0; }
+  }
   const int src = nbr(M, g, p1, p2, p3);
   const long own = (static_cast<long>(k) * n2 + j) * n1 + i;
-  const long from =
-      (static_cast<long>(k - p3 * M.nx3) * n2 + (j - p2 * M.nx2)) * n1 + (i - p1 * M.nx1);
+  const long from = (static_cast<long>(sk) * n2 + sj) * n1 + si;
   double *dst = arr + static_cast<long>(g) * stride + own;
   const double *s0 = arr + static_cast<long>(src) * stride + from;
   if (ncomp == NHYDRO) {
@@ -936,6 +994,62 @@ __device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, i
 
 __global__ void __launch_bounds__(WG) k_ghost(MeshK M, GhostArgs a) {
   ghost_body(M, a, xcd_logical_wg(M.xcd_remap), gridDim.x);
+}
+
+//========================================================================================
+// Kernel 3c: BoundaryValues::ApplyPhysicalBoundaries (bvals.cpp:453-…) for outflow
+// faces, after ConservedToPrimitive.  Every cell ghost position across an outflow face
+// takes the primitives of the cell the boundary functions copy from (the boundary layer
+// in each outflow direction, cc/outflow_cc.cpp, applied to the primitives as
+// time_integrator.cpp:2291 and mesh.cpp:1868 select) and rebuilds its conserved
+// variables with EquationOfState::PrimitiveToConserved (adiabatic_mhd.cpp:99-136) from
+// them and its own cell-centred field, which the ConservedToPrimitive kernel has already
+// computed from the copied face ghosts (bvals.cpp:517-523).  The source cell lies on the
+// active side of every outflow face, so no thread reads what another one writes.
+// Newtonian MHD only.
+struct PhysArgs {
+  double *u, *prim;
+  const double *bcc;
+};
+
+__device__ __forceinline__ void phys_bnd_body(const MeshK &M, const PhysArgs &a, int lw,
+                                              int nwg) {
+  const int nch = nwg / M.nblocks;
+  const int g = lw / nch;
+  const int t = (lw - g * nch) * WG + threadIdx.x;
+  const int n1 = M.nc1, n2 = M.nc2, n3 = M.nc3, ng = M.ng;
+  if (t >= ghost_count(n1, n2, n3, ng)) return;
+  int i, j, k;
+  ghost_pos(t, n1, n2, n3, ng, i, j, k);
+  const int p1 = ghost_off(i, n1, ng), p2 = ghost_off(j, n2, ng), p3 = ghost_off(k, n3, ng);
+  int si = i, sj = j, sk = k;
+  bool phys = false;
+  if (p1 && nbr(M, g, p1, 0, 0) < 0) { si = (p1 < 0) ? M.is : M.ie; phys = true; }
+  if (p2 && nbr(M, g, 0, p2, 0) < 0) { sj = (p2 < 0) ? M.js : M.je; phys = true; }
+  if (p3 && nbr(M, g, 0, 0, p3) < 0) { sk = (p3 < 0) ? M.ks : M.ke; phys = true; }
+  if (!phys) return;
+  const long nc = M.ncell;
+  const long b0 = static_cast<long>(g) * NHYDRO * nc;
+  const long c = cidx(M, k, j, i), src = cidx(M, sk, sj, si);
+  double w[NHYDRO];
+#pragma unroll
+  for (int q = 0; q < NHYDRO; ++q) w[q] = a.prim[b0 + q * nc + src];
+#pragma unroll
+  for (int q = 0; q < NHYDRO; ++q) a.prim[b0 + q * nc + c] = w[q];
+  const double *bc = a.bcc + static_cast<long>(g) * 3 * nc + c;
+  const double bcc1 = bc[0], bcc2 = bc[nc], bcc3 = bc[2 * nc];
+  double *u = a.u + b0 + c;
+  u[IDN * nc] = w[IDN];
+  u[IM1 * nc] = w[IVX] * w[IDN];
+  u[IM2 * nc] = w[IVY] * w[IDN];
+  u[IM3 * nc] = w[IVZ] * w[IDN];
+  u[IEN * nc] = w[IPR] * M.eos.igm1 +
+                0.5 * (w[IDN] * (w[IVX] * w[IVX] + w[IVY] * w[IVY] + w[IVZ] * w[IVZ]) +
+                       (bcc1 * bcc1 + bcc2 * bcc2 + bcc3 * bcc3));
+}
+
+__global__ void __launch_bounds__(WG) k_phys_bnd(MeshK M, PhysArgs a) {
+  phys_bnd_body(M, a, blockIdx.x, gridDim.x);
 }
 
 //========================================================================================
@@ -1414,7 +1528,7 @@ enum KernelId {
   KID_FLUX = 0,          // + 3*dir + (order-1): 9 flux variants
   KID_CORNER = 9, KID_UPDATE_U = 10, KID_SEED = 11, KID_CONS2PRIM = 12,
   KID_NEW_DT = 13, KID_GHOST = 14, KID_FLUX3 = 15,   // + (order-1): 3 fused variants
-  KID_COUNT = 18
+  KID_PHYS_BND = 18, KID_COUNT = 19
 };
 const char *kernel_name(int id) {
   static const char *flux[9] = {
@@ -1431,6 +1545,7 @@ const char *kernel_name(int id) {
     case KID_FLUX3: return "flux_xyz_dc";
     case KID_FLUX3 + 1: return "flux_xyz_plm";
     case KID_FLUX3 + 2: return "flux_xyz_ppm";
+    case KID_PHYS_BND: return "physical_boundary";
     default: return "unknown";
   }
 }
@@ -1716,6 +1831,19 @@ int launch_ghost_args(ath_mesh *m, const GhostArgs &ga) {
   return 0;
 }
 
+// Outflow faces after ConservedToPrimitive (nothing to do on a periodic mesh).
+int launch_phys_bnd(ath_mesh *m, const C2PArgs &pa) {
+  const MeshK &M = m->M;
+  if (!M.outflow) return 0;
+  ProfScope ps(m, KID_PHYS_BND);
+  const int grid =
+      static_cast<int>((ghost_count(M.nc1, M.nc2, M.nc3, M.ng) + WG - 1) / WG) * M.nblocks;
+  hipLaunchKernelGGL(k_phys_bnd, dim3(grid), dim3(WG), 0, m->stream, M,
+                     PhysArgs{pa.u, pa.prim, pa.bcc});
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 void swap_stage_regs(Regs &r) {   // the two swaps of one stage
   std::swap(r.u, r.u1);
   for (int c = 0; c < 3; ++c) std::swap(r.b[c], r.bb[c]);
@@ -1735,6 +1863,7 @@ int stage(ath_mesh *m, int stg) {
   if (launch_update_args(m, a.ua, a.beta, a.first)) return 1;
   if (launch_ghost_args(m, a.ga)) return 1;
   if (launch_cons2prim_args(m, a.pa)) return 1;
+  if (launch_phys_bnd(m, a.pa)) return 1;
   swap_stage_regs(r);
   set_regs(m, r);
   m->nstage += 1;
@@ -1964,6 +2093,14 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   if (!(d.cfl > 0.0) || d.cfl > 0.5) return fail("cfl_number must be in (0, 0.5] for 3-D VL2");
   for (int q = 0; q < 3; ++q)
     if (d.bnx[q] < d.ng) return fail("meshblock nx must be >= nghost");
+  for (int q = 0; q < 3; ++q) {
+    if (d.bc[2 * q] < 0 || d.bc[2 * q] > 1 || d.bc[2 * q + 1] < 0 || d.bc[2 * q + 1] > 1)
+      return fail("boundary flags must be 0 (periodic) or 1 (outflow)");
+    if ((d.bc[2 * q] == 0) != (d.bc[2 * q + 1] == 0))
+      return fail("a periodic boundary needs periodic on the opposite face too");
+    if (d.bc[2 * q] && d.relativistic)
+      return fail("outflow boundaries are implemented for Newtonian MHD only");
+  }
 
   ath_mesh *m = new ath_mesh;
   m->d = d;
@@ -2002,6 +2139,8 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   M.c2p_mode = C2P_REFERENCE;
   M.mlp = MlpK{};
   M.c2p_stats = nullptr;
+  M.outflow = 0;
+  for (int f = 0; f < 6; ++f) M.outflow |= d.bc[f];
 
   // neighbour table and coordinates (coordinates.cpp:129-146, cartesian.cpp:23-72)
   const int nb = M.nblocks;
@@ -2012,11 +2151,17 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
     for (int o3 = -1; o3 <= 1; ++o3)
       for (int o2 = -1; o2 <= 1; ++o2)
         for (int o1 = -1; o1 <= 1; ++o1) {
+          const int o[3] = {o1, o2, o3};
+          bool phys = false;   // across an outflow face of the mesh: no neighbour
+          for (int q = 0; q < 3; ++q) {
+            const int l = lx[q] + o[q];
+            if ((l < 0 && d.bc[2 * q]) || (l >= m->nbx[q] && d.bc[2 * q + 1])) phys = true;
+          }
           int l1 = (lx[0] + o1 + m->nbx[0]) % m->nbx[0];
           int l2 = (lx[1] + o2 + m->nbx[1]) % m->nbx[1];
           int l3 = (lx[2] + o3 + m->nbx[2]) % m->nbx[2];
           nbrh[g * 27 + (o3 + 1) * 9 + (o2 + 1) * 3 + (o1 + 1)] =
-              l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
+              phys ? -1 : l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
         }
     for (int q = 0; q < 3; ++q) {
       const int nx = d.bnx[q], ng = d.ng, nc = nx + 2 * ng, nbq = m->nbx[q];
@@ -2163,12 +2308,14 @@ int ath_gpu_get_block(ath_mesh_t m, int gid, int what, double *host) {
 
 int ath_gpu_initialize(ath_mesh_t m, double tlim) {
   if (!m) return fail("null argument");
-  const MeshK &M = m->M;
   Clock c0{0.0, std::numeric_limits<double>::max(), tlim, 0.0};
   HIPCHK(hipMemcpyAsync(m->clk, &c0, sizeof(Clock), hipMemcpyHostToDevice, m->stream));
   if (launch_ghost_args(m, GhostArgs{m->u, {m->b[0], m->b[1], m->b[2]}})) return 1;
   // ConservedToPrimitive(u, w1, b, w, bcc): output register is w
   if (launch_cons2prim(m, m->w, true)) return 1;
+  if (launch_phys_bnd(m, C2PArgs{m->u, m->w, m->bcc, m->b[0], m->b[1], m->b[2], nullptr,
+                                 m->lseed}))
+    return 1;
   if (launch_new_dt(m, 0)) return 1;
   HIPCHK(hipStreamSynchronize(m->stream));
   return 0;
@@ -2275,6 +2422,10 @@ int ath_gpu_ensemble_create(const ath_mesh_t *members, int n, ath_ensemble_t *ou
     if (m->M.c2p_mode != C2P_REFERENCE) {
       delete e;
       return fail("ensemble members must use the reference primitive recovery");
+    }
+    if (m->M.outflow) {
+      delete e;
+      return fail("ensemble members must be periodic meshes");
     }
     e->mem.push_back(m);
     e->parity.push_back(m->nstage & 1);

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .mesh import MeshDesc, RSOLVER
+from .mesh import BOUNDARY, MeshDesc, RSOLVER
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libathena_gpu.so")
@@ -39,7 +39,7 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_ghost_position", "ath_gpu_restrict_cc", "ath_gpu_restrict_fc",
            "ath_gpu_prolongate_cc", "ath_gpu_prolongate_fc_shared",
            "ath_gpu_prolongate_fc_internal", "ath_gpu_restrict_emf"]
-NKERNELS = 18
+NKERNELS = 19
 
 
 class AthGpuError(RuntimeError):
@@ -54,7 +54,7 @@ class MeshDescC(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("dfloor", ctypes.c_double),
                 ("pfloor", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("sigma_max", ctypes.c_double), ("beta_min", ctypes.c_double),
-                ("gamma_max", ctypes.c_double)]
+                ("gamma_max", ctypes.c_double), ("bc", ctypes.c_int * 6)]
 
 
 _lib = None
@@ -172,11 +172,12 @@ def desc_of(m: MeshDesc) -> MeshDescC:
     c.dfloor, c.pfloor = m.dfloor, m.pfloor
     c.cfl = m.cfl
     c.sigma_max, c.beta_min, c.gamma_max = m.sigma_max, m.beta_min, m.gamma_max
+    c.bc[:] = [BOUNDARY[f] for f in m.bc]
     return c
 
 
 class GpuMesh:
-    """Device mirror of one uniform periodic mesh (all registers resident in HBM)."""
+    """Device mirror of one uniform mesh (all registers resident in HBM)."""
 
     def __init__(self, m: MeshDesc):
         L = load()

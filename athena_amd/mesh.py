@@ -12,7 +12,8 @@ uniform grid:
   ``cartesian.cpp:23-72``);
 * AthenaArray shapes of every register (``athena_arrays.hpp``: i fastest).
 
-Only periodic boundaries are modelled: every same-level neighbour exists.
+Faces are periodic or outflow (``<mesh> ix1_bc`` .. ``ox3_bc``); every same-level
+neighbour exists except across an outflow face of the mesh.
 """
 from __future__ import annotations
 
@@ -27,13 +28,14 @@ IVX, IVY, IVZ, IPR = 1, 2, 3, 4
 IB1, IB2, IB3 = 0, 1, 2
 
 RSOLVER = {"hlld": 0, "hlle": 1}
+BOUNDARY = {"periodic": 0, "outflow": 1}     # <mesh> ix1_bc .. ox3_bc
 FLT_MIN = 1.17549435082228750797e-38          # std::numeric_limits<float>::min()
 DEFAULT_FLOOR = math.sqrt(1024 * FLT_MIN)     # adiabatic_mhd.cpp:29-31
 
 
 @dataclass
 class MeshDesc:
-    """Input-file level description of one uniform, periodic mesh.
+    """Input-file level description of one uniform mesh (periodic or outflow faces).
 
     Field names follow the athinput keys (``<mesh> nx1..3, x1min..``,
     ``<meshblock> nx1..3``, ``<hydro> gamma, dfloor, pfloor``, ``<time> cfl_number,
@@ -55,6 +57,7 @@ class MeshDesc:
     sigma_max: float = -1.0       # SR floors, <0: reference defaults (0, 0, 1000)
     beta_min: float = -1.0
     gamma_max: float = -1.0
+    bc: tuple = ("periodic",) * 6   # ix1_bc, ox1_bc, ix2_bc, ox2_bc, ix3_bc, ox3_bc
 
     def __post_init__(self):
         self.nx = tuple(int(v) for v in self.nx)
@@ -72,6 +75,15 @@ class MeshDesc:
             raise ValueError("xorder must be 1, 2 or 3")
         if self.rsolver not in RSOLVER:
             raise ValueError("rsolver must be one of %s" % sorted(RSOLVER))
+        self.bc = tuple(self.bc)
+        if len(self.bc) != 6 or any(f not in BOUNDARY for f in self.bc):
+            raise ValueError("bc must be 6 flags out of %s" % sorted(BOUNDARY))
+        for d in range(3):
+            if (self.bc[2 * d] == "periodic") != (self.bc[2 * d + 1] == "periodic"):
+                raise ValueError("x%d: periodic on one side needs periodic on the other"
+                                 % (d + 1))
+        if self.relativistic and any(f != "periodic" for f in self.bc):
+            raise ValueError("outflow boundaries are implemented for Newtonian MHD only")
 
     # ---------------------------------------------------------------- geometry
     @property

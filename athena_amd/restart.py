@@ -62,8 +62,8 @@ def default_params(m: MeshDesc, tlim=1.0e300, problem_id="gpu"):
         mesh["nx%d" % d] = str(m.nx[d - 1])
         mesh["x%dmin" % d] = repr(float(m.xmin[d - 1]))
         mesh["x%dmax" % d] = repr(float(m.xmax[d - 1]))
-        mesh["ix%d_bc" % d] = "periodic"
-        mesh["ox%d_bc" % d] = "periodic"
+        mesh["ix%d_bc" % d] = m.bc[2 * d - 2]
+        mesh["ox%d_bc" % d] = m.bc[2 * d - 1]
     mb = {"nx%d" % d: str(m.bnx[d - 1]) for d in (1, 2, 3)}
     hydro = {"gamma": repr(float(m.gamma))}
     dfl, pfl = m.floors

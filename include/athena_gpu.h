@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-/* Input-file level description of one uniform periodic mesh.  Mirrors the athinput
+/* Input-file level description of one uniform mesh.  Mirrors the athinput
  * keys read by Mesh (mesh.cpp:72-…), Reconstruction (reconstruction.cpp:39-…:
  * time/xorder), EquationOfState (adiabatic_mhd.cpp:26-31, adiabatic_mhd_sr.cpp:36-48)
  * and the configure switches --flux, -s, --nghost (configure.py:80-353). */
@@ -38,6 +38,9 @@ typedef struct ath_mesh_desc {
   double dfloor, pfloor;   /* < 0 selects the reference default sqrt(1024*FLT_MIN) */
   double cfl;         /* <time> cfl_number */
   double sigma_max, beta_min, gamma_max;   /* SR floors; < 0 selects 0, 0, 1000 */
+  int bc[6];          /* <mesh> ix1_bc, ox1_bc, ix2_bc, ox2_bc, ix3_bc, ox3_bc: 0 periodic,
+                         1 outflow (bvals.cpp:453-…, cc/outflow_cc.cpp, fc/outflow_fc.cpp;
+                         Newtonian MHD only; a zero-filled descriptor is fully periodic) */
 } ath_mesh_desc;
 
 typedef struct ath_mesh *ath_mesh_t;

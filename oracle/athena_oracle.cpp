@@ -555,14 +555,17 @@ void ComputeCornerE(Mesh &m, Block &b) {
 }
 
 //----------------------------------------------------------------------------------------
-// Same-level EMF flux correction for a uniform periodic mesh:
+// Same-level EMF flux correction on a uniform mesh:
 // SendFluxCorrection/LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313,
 // 623-678), SetFluxBoundarySameLevel (:689-…), AverageFluxBoundary (:1355-1548),
 // neighbour order of BoundaryBase::SearchAndSetNeighbors (bvals_base.cpp:327-700):
 // faces -x1,+x1,-x2,+x2,-x3,+x3, then x1x2 edges, x1x3 edges, x2x3 edges, each edge
 // list ordered (outer offset, inner offset) = (-,-),(+,-),(-,+),(+,+).
 // Every block packs its un-averaged values before receiving (RECV after SEND,
-// time_integrator.cpp:1048-1049), so the additions read a snapshot.
+// time_integrator.cpp:1048-1049), so the additions read a snapshot.  A face on an
+// outflow boundary has no neighbour and is not averaged (:1363-1366); an edge is
+// divided by the number of blocks that share it (CountFineEdges, bvals_fc.cpp:1129-1193,
+// skipped when that number is 1).
 void AverageEMF(Mesh &m) {
   const int nbk = static_cast<int>(m.blocks.size());
   std::vector<Arr> s1(nbk), s2(nbk), s3(nbk);
@@ -571,13 +574,14 @@ void AverageEMF(Mesh &m) {
   for (int g = 0; g < nbk; ++g) {
     Block &b = m.blocks[g];
     int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
-    int nx1 = b.nx1, nx2 = b.nx2, nx3 = b.nx3;
     // face neighbours
     for (int f = 0; f < 6; ++f) {
       int dir = f / 2, side = f % 2;   // side 0 = inner, 1 = outer
       int o[3] = {0, 0, 0};
       o[dir] = side ? 1 : -1;
-      Block &n = m.blocks[b.nb[o[2] + 1][o[1] + 1][o[0] + 1]];
+      const int nid = b.nb[o[2] + 1][o[1] + 1][o[0] + 1];
+      if (nid < 0) continue;
+      Block &n = m.blocks[nid];
       Arr &n1 = s1[n.gid], &n2 = s2[n.gid], &n3 = s3[n.gid];
       if (dir == 0) {
         int i = side ? ie + 1 : is;
@@ -602,30 +606,42 @@ void AverageEMF(Mesh &m) {
           for (int i = is; i <= ie + 1; i++) b.e2(k, j, i) += n2(kn, j, i);
       }
     }
-    // edge neighbours
+    // edge neighbours; nshare[eid] counts the blocks around the edge (CountFineEdges)
+    int nshare[12];
     for (int eid = 0; eid < 12; ++eid) {
       int lo = (eid & 1) ? 1 : -1, hi = (eid & 2) ? 1 : -1;
+      int fa, fb, ed;                  // the two face neighbours and the edge neighbour
+      if (eid < 4) {
+        fa = b.nb[1][1][lo + 1]; fb = b.nb[1][hi + 1][1]; ed = b.nb[1][hi + 1][lo + 1];
+      } else if (eid < 8) {
+        fa = b.nb[1][1][lo + 1]; fb = b.nb[hi + 1][1][1]; ed = b.nb[hi + 1][1][lo + 1];
+      } else {
+        fa = b.nb[1][lo + 1][1]; fb = b.nb[hi + 1][1][1]; ed = b.nb[hi + 1][lo + 1][1];
+      }
+      nshare[eid] = 1 + (fa >= 0) + (fb >= 0) + (ed >= 0);
+      if (ed < 0) continue;
+      Block &n = m.blocks[ed];
       if (eid < 4) {          // x1x2 edge, e3
-        Block &n = m.blocks[b.nb[1][hi + 1][lo + 1]];
         int i = (lo > 0) ? ie + 1 : is, j = (hi > 0) ? je + 1 : js;
         int in = (lo > 0) ? n.is : n.ie + 1, jn = (hi > 0) ? n.js : n.je + 1;
         for (int k = ks; k <= ke; k++) b.e3(k, j, i) += s3[n.gid](k, jn, in);
       } else if (eid < 8) {   // x1x3 edge, e2
-        Block &n = m.blocks[b.nb[hi + 1][1][lo + 1]];
         int i = (lo > 0) ? ie + 1 : is, k = (hi > 0) ? ke + 1 : ks;
         int in = (lo > 0) ? n.is : n.ie + 1, kn = (hi > 0) ? n.ks : n.ke + 1;
         for (int j = js; j <= je; j++) b.e2(k, j, i) += s2[n.gid](kn, j, in);
       } else {                // x2x3 edge, e1
-        Block &n = m.blocks[b.nb[hi + 1][lo + 1][1]];
         int j = (lo > 0) ? je + 1 : js, k = (hi > 0) ? ke + 1 : ks;
         int jn = (lo > 0) ? n.js : n.je + 1, kn = (hi > 0) ? n.ks : n.ke + 1;
         for (int i = is; i <= ie; i++) b.e1(k, j, i) += s1[n.gid](kn, jn, i);
       }
     }
-    (void)nx1; (void)nx2; (void)nx3;
-    // AverageFluxBoundary: faces x0.5 (interior of each face), edges x1/nedge_fine=0.25
+    // AverageFluxBoundary: faces with a neighbour x0.5 (interior of each face), edges
+    // x1/nshare
     for (int f = 0; f < 6; ++f) {
       int dir = f / 2, side = f % 2;
+      int o[3] = {0, 0, 0};
+      o[dir] = side ? 1 : -1;
+      if (b.nb[o[2] + 1][o[1] + 1][o[0] + 1] < 0) continue;
       if (dir == 0) {
         int i = side ? ie + 1 : is;
         for (int k = ks + 1; k <= ke; k++)
@@ -646,8 +662,9 @@ void AverageEMF(Mesh &m) {
           for (int i = is + 1; i <= ie; i++) b.e2(k, j, i) *= 0.5;
       }
     }
-    const Real div = 1.0 / static_cast<Real>(4);
     for (int eid = 0; eid < 12; ++eid) {
+      if (nshare[eid] == 1) continue;
+      const Real div = 1.0 / static_cast<Real>(nshare[eid]);
       int lo = (eid & 1), hi = (eid & 2);
       if (eid < 4) {
         int i = lo ? ie + 1 : is, j = hi ? je + 1 : js;
@@ -792,11 +809,134 @@ void Cons2PrimNewtonian(Mesh &m, Block &b, Arr &cons, Arr &prim, int il, int iu,
 }
 
 
+// Range: the active cells plus the ghost layers of every side with a neighbour
+// (time_integrator.cpp:2206-2212, mesh.cpp:1824-1831).
 void ConservedToPrimitive(Mesh &m, Block &b, Arr &cons, Arr &prim_old, Arr &prim) {
-  int il = b.is - b.ng, iu = b.ie + b.ng, jl = b.js - b.ng, ju = b.je + b.ng;
-  int kl = b.ks - b.ng, ku = b.ke + b.ng;
+  int il = b.is, iu = b.ie, jl = b.js, ju = b.je, kl = b.ks, ku = b.ke;
+  if (b.nb[1][1][0] >= 0) il -= b.ng;
+  if (b.nb[1][1][2] >= 0) iu += b.ng;
+  if (b.nb[1][0][1] >= 0) jl -= b.ng;
+  if (b.nb[1][2][1] >= 0) ju += b.ng;
+  if (b.nb[0][1][1] >= 0) kl -= b.ng;
+  if (b.nb[2][1][1] >= 0) ku += b.ng;
   if (m.c.relativistic) Cons2PrimSR(m, b, cons, prim_old, prim, il, iu, jl, ju, kl, ku);
   else Cons2PrimNewtonian(m, b, cons, prim, il, iu, jl, ju, kl, ku);
+}
+
+// EquationOfState::PrimitiveToConserved, adiabatic MHD (adiabatic_mhd.cpp:99-136)
+void Prim2ConsNewtonian(Mesh &m, Block &b, int il, int iu, int jl, int ju, int kl, int ku) {
+  Real igm1 = 1.0 / (m.c.gamma - 1.0);
+  for (int k = kl; k <= ku; ++k)
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        const Real w_d = b.w(IDN, k, j, i), w_vx = b.w(IVX, k, j, i);
+        const Real w_vy = b.w(IVY, k, j, i), w_vz = b.w(IVZ, k, j, i);
+        const Real w_p = b.w(IPR, k, j, i);
+        const Real bcc1 = b.bcc(IB1, k, j, i), bcc2 = b.bcc(IB2, k, j, i);
+        const Real bcc3 = b.bcc(IB3, k, j, i);
+        b.u(IDN, k, j, i) = w_d;
+        b.u(IM1, k, j, i) = w_vx * w_d;
+        b.u(IM2, k, j, i) = w_vy * w_d;
+        b.u(IM3, k, j, i) = w_vz * w_d;
+        b.u(IEN, k, j, i) = w_p * igm1 + 0.5 * (w_d * (SQR(w_vx) + SQR(w_vy) + SQR(w_vz)) +
+                                                (SQR(bcc1) + SQR(bcc2) + SQR(bcc3)));
+      }
+}
+
+// BoundaryValues::ApplyPhysicalBoundaries (bvals.cpp:453-…) with outflow on the faces
+// whose flag is set: per face, in the order inner x1, outer x1, inner x2, outer x2,
+// inner x3, outer x3, the zero-gradient copies of the primitives
+// (cc/outflow_cc.cpp) and of the three face-field components (fc/outflow_fc.cpp), then
+// the cell-centred field and PrimitiveToConserved over that face's ghost cells.  The
+// transverse ranges of the x1 and x2 faces reach into the ghost layers of every direction
+// without a boundary function (bvals.cpp:457-470); those of the x3 faces cover every
+// cell (bvals.cpp:600-601), so each ghost cell is written by exactly one face.
+void ApplyPhysicalBoundaries(Mesh &m, Block &b) {
+  const int ng = b.ng;
+  const int is = b.is, ie = b.ie, js = b.js, je = b.je, ks = b.ks, ke = b.ke;
+  bool fn[6];
+  fn[0] = b.nb[1][1][0] < 0; fn[1] = b.nb[1][1][2] < 0;
+  fn[2] = b.nb[1][0][1] < 0; fn[3] = b.nb[1][2][1] < 0;
+  fn[4] = b.nb[0][1][1] < 0; fn[5] = b.nb[2][1][1] < 0;
+  if (!(fn[0] || fn[1] || fn[2] || fn[3] || fn[4] || fn[5])) return;
+  int bis = is - ng, bie = ie + ng, bjs = js, bje = je, bks = ks, bke = ke;
+  if (!fn[2]) bjs = js - ng;
+  if (!fn[3]) bje = je + ng;
+  if (!fn[4]) bks = ks - ng;
+  if (!fn[5]) bke = ke + ng;
+  // x1 faces: il = is / iu = ie, j in [bjs, bje], k in [bks, bke]
+  for (int side = 0; side < 2; ++side) {
+    if (!fn[side]) continue;
+    const int src = side ? ie : is;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int k = bks; k <= bke; ++k)
+        for (int j = bjs; j <= bje; ++j)
+          for (int q = 1; q <= ng; ++q) b.w(n, k, j, side ? ie + q : is - q) = b.w(n, k, j, src);
+    for (int k = bks; k <= bke; ++k)
+      for (int j = bjs; j <= bje; ++j)
+        for (int q = 1; q <= ng; ++q)
+          b.b1f(k, j, side ? ie + q + 1 : is - q) = b.b1f(k, j, side ? ie + 1 : is);
+    for (int k = bks; k <= bke; ++k)
+      for (int j = bjs; j <= bje + 1; ++j)
+        for (int q = 1; q <= ng; ++q) b.b2f(k, j, side ? ie + q : is - q) = b.b2f(k, j, src);
+    for (int k = bks; k <= bke + 1; ++k)
+      for (int j = bjs; j <= bje; ++j)
+        for (int q = 1; q <= ng; ++q) b.b3f(k, j, side ? ie + q : is - q) = b.b3f(k, j, src);
+    const int lo = side ? ie + 1 : is - ng, hi = side ? ie + ng : is - 1;
+    CellCenteredField(b, lo, hi, bjs, bje, bks, bke);
+    Prim2ConsNewtonian(m, b, lo, hi, bjs, bje, bks, bke);
+  }
+  // x2 faces: i in [bis, bie], jl = js / ju = je, k in [bks, bke]
+  for (int side = 0; side < 2; ++side) {
+    if (!fn[2 + side]) continue;
+    const int src = side ? je : js;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int k = bks; k <= bke; ++k)
+        for (int q = 1; q <= ng; ++q)
+          for (int i = bis; i <= bie; ++i) b.w(n, k, side ? je + q : js - q, i) = b.w(n, k, src, i);
+    for (int k = bks; k <= bke; ++k)
+      for (int q = 1; q <= ng; ++q)
+        for (int i = bis; i <= bie + 1; ++i)
+          b.b1f(k, side ? je + q : js - q, i) = b.b1f(k, src, i);
+    for (int k = bks; k <= bke; ++k)
+      for (int q = 1; q <= ng; ++q)
+        for (int i = bis; i <= bie; ++i)
+          b.b2f(k, side ? je + q + 1 : js - q, i) = b.b2f(k, side ? je + 1 : js, i);
+    for (int k = bks; k <= bke + 1; ++k)
+      for (int q = 1; q <= ng; ++q)
+        for (int i = bis; i <= bie; ++i)
+          b.b3f(k, side ? je + q : js - q, i) = b.b3f(k, src, i);
+    const int lo = side ? je + 1 : js - ng, hi = side ? je + ng : js - 1;
+    CellCenteredField(b, bis, bie, lo, hi, bks, bke);
+    Prim2ConsNewtonian(m, b, bis, bie, lo, hi, bks, bke);
+  }
+  // x3 faces: i and j over every cell, ghosts included (bvals.cpp:600-601), kl = ks /
+  // ku = ke
+  bjs = js - ng;
+  bje = je + ng;
+  for (int side = 0; side < 2; ++side) {
+    if (!fn[4 + side]) continue;
+    const int src = side ? ke : ks;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int q = 1; q <= ng; ++q)
+        for (int j = bjs; j <= bje; ++j)
+          for (int i = bis; i <= bie; ++i) b.w(n, side ? ke + q : ks - q, j, i) = b.w(n, src, j, i);
+    for (int q = 1; q <= ng; ++q)
+      for (int j = bjs; j <= bje; ++j)
+        for (int i = bis; i <= bie + 1; ++i)
+          b.b1f(side ? ke + q : ks - q, j, i) = b.b1f(src, j, i);
+    for (int q = 1; q <= ng; ++q)
+      for (int j = bjs; j <= bje + 1; ++j)
+        for (int i = bis; i <= bie; ++i)
+          b.b2f(side ? ke + q : ks - q, j, i) = b.b2f(src, j, i);
+    for (int q = 1; q <= ng; ++q)
+      for (int j = bjs; j <= bje; ++j)
+        for (int i = bis; i <= bie; ++i)
+          b.b3f(side ? ke + q + 1 : ks - q, j, i) = b.b3f(side ? ke + 1 : ks, j, i);
+    const int lo = side ? ke + 1 : ks - ng, hi = side ? ke + ng : ks - 1;
+    CellCenteredField(b, bis, bie, bjs, bje, lo, hi);
+    Prim2ConsNewtonian(m, b, bis, bie, bjs, bje, lo, hi);
+  }
 }
 
 //----------------------------------------------------------------------------------------
@@ -868,6 +1008,7 @@ void ExchangeCC(Mesh &m, Arr Block::*field, int nvar) {
       for (int o2 = -1; o2 <= 1; ++o2)
         for (int o1 = -1; o1 <= 1; ++o1) {
           if (!o1 && !o2 && !o3) continue;
+          if (b.nb[o3 + 1][o2 + 1][o1 + 1] < 0) continue;   // physical boundary
           Block &n = m.blocks[b.nb[o3 + 1][o2 + 1][o1 + 1]];
           int s[3], e[3], nxd[3] = {b.nx1, b.nx2, b.nx3}, o[3] = {o1, o2, o3};
           int lo[3] = {b.is, b.js, b.ks}, hi[3] = {b.ie, b.je, b.ke};
@@ -897,6 +1038,7 @@ void ExchangeFC(Mesh &m) {
       for (int o2 = -1; o2 <= 1; ++o2)
         for (int o1 = -1; o1 <= 1; ++o1) {
           if (!o1 && !o2 && !o3) continue;
+          if (b.nb[o3 + 1][o2 + 1][o1 + 1] < 0) continue;   // physical boundary
           Block &n = m.blocks[b.nb[o3 + 1][o2 + 1][o1 + 1]];
           int o[3] = {o1, o2, o3};
           for (int comp = 0; comp < 3; ++comp) {
@@ -971,6 +1113,7 @@ void Stage(Mesh &m, int stage, Real tlim) {
     Block &b = m.blocks[g];
     ConservedToPrimitive(m, b, b.u, b.w, b.w1);
     b.w.swap(b.w1);
+    ApplyPhysicalBoundaries(m, b);
     if (stage == 2) NewBlockTimeStep(m, b);
   }
   (void)tlim;
@@ -984,6 +1127,7 @@ void Initialize(Mesh &m, Real tlim) {
   for (int g = 0; g < nbk; ++g) {
     Block &b = m.blocks[g];
     ConservedToPrimitive(m, b, b.u, b.w1, b.w);
+    ApplyPhysicalBoundaries(m, b);
     NewBlockTimeStep(m, b);
   }
   NewTimeStep(m, tlim);
@@ -1007,6 +1151,10 @@ using namespace ora;
 extern "C" {
 
 void *ora_new(const ora_cfg *cfg) {
+  for (int f = 0; f < 6; ++f) {
+    if (cfg->bc[f] != 0 && cfg->bc[f] != 1) return nullptr;
+    if (cfg->bc[f] && cfg->relativistic) return nullptr;   // outflow: Newtonian MHD only
+  }
   Mesh *m = new Mesh;
   m->c = *cfg;
   Real fmin = std::numeric_limits<float>::min();
@@ -1050,10 +1198,18 @@ void *ora_new(const ora_cfg *cfg) {
     for (int o3 = -1; o3 <= 1; ++o3)
       for (int o2 = -1; o2 <= 1; ++o2)
         for (int o1 = -1; o1 <= 1; ++o1) {
+          // no neighbour across an outflow face of the mesh (nblevel = -1 in the
+          // reference, bvals_base.cpp:327-700)
+          const int o[3] = {o1, o2, o3};
+          bool phys = false;
+          for (int d = 0; d < 3; ++d) {
+            const int l = b.lx[d] + o[d];
+            if ((l < 0 && cfg->bc[2 * d]) || (l >= m->nbx[d] && cfg->bc[2 * d + 1])) phys = true;
+          }
           int l1 = (b.lx[0] + o1 + m->nbx[0]) % m->nbx[0];
           int l2 = (b.lx[1] + o2 + m->nbx[1]) % m->nbx[1];
           int l3 = (b.lx[2] + o3 + m->nbx[2]) % m->nbx[2];
-          b.nb[o3 + 1][o2 + 1][o1 + 1] = l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
+          b.nb[o3 + 1][o2 + 1][o1 + 1] = phys ? -1 : l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
         }
   }
   return m;

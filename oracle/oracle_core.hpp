@@ -60,6 +60,7 @@ typedef struct {
   double dfloor, pfloor;   // <0 selects the reference default sqrt(1024*FLT_MIN)
   double cfl;
   double sigma_max, beta_min, gamma_max;   // SR floors (<0 -> reference defaults)
+  int bc[6];          // ix1, ox1, ix2, ox2, ix3, ox3: 0 periodic, 1 outflow
 } ora_cfg;
 }
 
@@ -79,7 +80,8 @@ struct Block {
   Arr wx1, wx2, wx3;                       // CT upwind weights
   Arr e1, e2, e3;                          // corner (edge) EMFs
   Arr cce;                                 // cell-centred E (3, nc3, nc2, nc1)
-  int nb[3][3][3];                         // neighbour gid at offset (ox3+1, ox2+1, ox1+1)
+  int nb[3][3][3];                         // neighbour gid at offset (ox3+1, ox2+1, ox1+1),
+                                           // -1 across an outflow mesh boundary
   Real new_dt;
 };
 

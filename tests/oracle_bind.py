@@ -10,7 +10,7 @@ import subprocess
 
 import numpy as np
 
-from athena_amd.mesh import MeshDesc, RSOLVER
+from athena_amd.mesh import BOUNDARY, MeshDesc, RSOLVER
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -29,7 +29,7 @@ class OraCfg(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("dfloor", ctypes.c_double),
                 ("pfloor", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("sigma_max", ctypes.c_double), ("beta_min", ctypes.c_double),
-                ("gamma_max", ctypes.c_double)]
+                ("gamma_max", ctypes.c_double), ("bc", ctypes.c_int * 6)]
 
 
 _lib = None
@@ -88,16 +88,19 @@ def cfg_of(m: MeshDesc) -> OraCfg:
     c.dfloor, c.pfloor = m.dfloor, m.pfloor
     c.cfl = m.cfl
     c.sigma_max, c.beta_min, c.gamma_max = m.sigma_max, m.beta_min, m.gamma_max
+    c.bc[:] = [BOUNDARY[f] for f in m.bc]
     return c
 
 
 class OracleMesh:
-    """CPU restatement of one uniform periodic mesh."""
+    """CPU restatement of one uniform mesh (periodic or outflow faces)."""
 
     def __init__(self, m: MeshDesc):
         self.m = m
         self._cfg = cfg_of(m)
         self.h = lib().ora_new(ctypes.byref(self._cfg))
+        if not self.h:
+            raise ValueError("the oracle refused this mesh description")
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -28,6 +28,6 @@ def test_python_mirror_struct_matches_header_order():
     text = open(os.path.join(ROOT, "include", "athena_gpu.h")).read()
     body = text[text.index("typedef struct ath_mesh_desc"):text.index("} ath_mesh_desc;")]
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    names = re.findall(r"\b(\w+)(?:\[3\])?\s*[,;]", body)
+    names = re.findall(r"\b(\w+)(?:\[\d+\])?\s*[,;]", body)
     fields = [f[0] for f in gpu.MeshDescC._fields_]
     assert names == fields, (names, fields)
