@@ -7,7 +7,7 @@ import pytest
 
 from athena_amd.gpu import AthGpuError, Ensemble, GpuMesh
 from athena_amd.pgen.blast import Blast, blast_ensemble, blast_mesh
-from harness import compare
+from harness import compare, div_b
 from oracle_bind import OracleMesh
 
 pytestmark = pytest.mark.gpu
@@ -139,18 +139,30 @@ def test_blast_member_matches_oracle():
 
 
 def test_full_ensemble_builds_and_steps():
+    """The benchmark's 48 blast waves (31.6 M cells): every member advances, conserves
+    mass, momentum and energy to round-off (periodic boxes; device history sums) and
+    keeps div B at round-off."""
     print("case 48-member ensemble, 2 cycles", flush=True)
     members = blast_ensemble()
     meshes = fresh(members)
+    h0 = [d.history() for d in meshes]
     ens = Ensemble(meshes)
     n, cells = ens.size()
     assert n == 48 and cells == sum(m.nx[0] ** 3 for m, _ in members)
     ens.run(2)
-    for d in meshes:
+    for q, d in enumerate(meshes):
         t, dt, nc = d.state()
         assert nc == 2 and t > 0 and np.isfinite(dt)
         u = d.get(0, "u")
         assert np.all(np.isfinite(u))
+        h1 = d.history()
+        assert abs(h1[0] - h0[q][0]) <= 1e-13 * h0[q][0], q
+        assert abs(h1[7] - h0[q][7]) <= 1e-13 * h0[q][7], q
+        scale = np.sqrt(2.0 * h0[q][0] * h1[7])
+        assert np.max(np.abs(h1[1:4] - h0[q][1:4])) <= 1e-13 * scale, q
+    for q in range(3):
+        m = members[q][0]
+        assert div_b(m, lambda g, k: meshes[q].get(g, k)) <= 1e-12, q
 
 
 def test_single_mesh_tiles_and_autotune(solo):
