@@ -2642,6 +2642,8 @@ int ath_gpu_mesh_get_tiles(ath_mesh_t m, int *tile_idx) {
 // left as it was.
 int ath_gpu_mesh_autotune(ath_mesh_t m, int reps, double *best_ms) {
   if (!m) return fail("null argument");
+  // the tuner times the ensemble kernels, which have no physical-boundary step
+  if (m->M.outflow) return fail("autotune: meshes with outflow faces keep their tiles");
   ath_mesh_t one[1] = {m};
   ath_ensemble_t e = nullptr;
   if (ath_gpu_ensemble_create(one, 1, &e)) return 1;

@@ -165,7 +165,8 @@ int ath_gpu_ensemble_autotune(ath_ensemble_t ens, int reps, double *best_ms);
 int ath_gpu_ensemble_stream(ath_ensemble_t ens, void **stream);
 
 /* Tile shape (index as for ensembles) of each kernel kind of a single mesh, and the
- * same autotuner for a single mesh (its state is left as it was). */
+ * same autotuner for a single mesh (its state is left as it was; periodic meshes only,
+ * like ensembles). */
 int ath_gpu_mesh_set_tiles(ath_mesh_t mesh, const int *tile_idx);
 int ath_gpu_mesh_get_tiles(ath_mesh_t mesh, int *tile_idx);
 int ath_gpu_mesh_autotune(ath_mesh_t mesh, int reps, double *best_ms);

@@ -80,3 +80,15 @@ def test_outflow_noncubic_blocks():
     t = Turbulence(m)
     o, d, _ = run_both(m, [t.block(g) for g in range(m.nblocks)], ncycles=8)
     assert_identical(m, o, d)
+
+
+def test_outflow_mesh_refuses_ensemble_and_autotune():
+    from athena_amd.gpu import AthGpuError, Ensemble
+    m = rj2a_mesh(64, shock_dir=3)
+    d = GpuMesh(m)
+    d.load([ShockTube(m, 3).block(0)])
+    d.initialize(RJ2A_TLIM)
+    with pytest.raises(AthGpuError, match="periodic"):
+        Ensemble([d])
+    with pytest.raises(AthGpuError, match="outflow"):
+        d.autotune()
