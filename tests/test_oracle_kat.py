@@ -48,8 +48,10 @@ def test_oracle_matches_reference_linear_wave_errors_64(case):
     if not os.environ.get("ATH_SLOW"):
         pytest.skip("set ATH_SLOW=1 for the 64^3 known-answer runs (~1 min each)")
     rms, _, _ = run_linwave(case["nx1"], case["wave_flag"], case["vflow"], case["tlim"])
+    # the printed digits of the reference's own uniform-grid run; mhd_linwave.py's
+    # absolute thresholds (thresholds_at_64) belong to its static-refinement runs,
+    # which the uniform errors exceed, so they are not asserted here
     assert float("%e" % rms) == case["rms_l1"], (rms, case)
-    assert rms <= GOLD["thresholds_at_64"][case["name"]]
 
 
 def test_convergence_ratio_from_golden():
