@@ -653,10 +653,11 @@ __global__ void __launch_bounds__(WG) k_corner_march(MeshK M, CornerArgs ca, Til
 }
 
 //========================================================================================
-// Same-level EMF flux correction on a uniform periodic mesh, evaluated where CT reads
-// the EMF.  A surface EMF is own + face neighbour(s) + edge neighbour, summed in the
-// reference's neighbour order (x1 faces, x2 faces, x3 faces, then the edge) and scaled
-// by 1/2 (face) or 1/nedge_fine = 1/4 (edge) (flux_correction_fc.cpp:51-…, 689-…,
+// Same-level EMF flux correction on a uniform mesh, evaluated where CT reads the EMF.
+// A surface EMF is own + face neighbour(s) + edge neighbour, summed in the reference's
+// neighbour order (x1 faces, x2 faces, x3 faces, then the edge) and scaled by 1/2
+// (face) or 1/nedge_fine (edge: 1/4 with every neighbour present; outflow faces below)
+// (flux_correction_fc.cpp:51-…, 689-…,
 // 1355-1548; bvals_base.cpp:327-700).  Raw EMFs are never modified, so every thread
 // reads the un-averaged neighbour values exactly like the packed send buffers.
 // Scale of an edge EMF shared by cnt blocks (1/nedge_fine, bvals_fc.cpp:1129-1193 and

@@ -995,10 +995,12 @@ void NewTimeStep(Mesh &m, Real tlim) {
 }
 
 //----------------------------------------------------------------------------------------
-// Same-level ghost exchange for a uniform periodic mesh:
+// Same-level ghost exchange on a uniform mesh:
 // cell-centred LoadBoundaryBufferSameLevel/SetBoundarySameLevel
 // (cc/bvals_cc.cpp:287-307,386-427) and face-centred (fc/bvals_fc.cpp:344-401,583-688).
-// Packed ranges are always interior values, so the exchange is a copy from the owner.
+// Packed ranges are always interior values, so the exchange is a copy from the owner;
+// across an outflow face there is no neighbour and ApplyPhysicalBoundaries fills the
+// ghosts after the primitive recovery.
 void ExchangeCC(Mesh &m, Arr Block::*field, int nvar) {
   const int nbk = static_cast<int>(m.blocks.size());
 #pragma omp parallel for schedule(dynamic, 1)
