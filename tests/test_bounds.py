@@ -15,7 +15,8 @@ STENCIL = {1: (1, 0), 2: (2, 1), 3: (3, 2)}   # cells below / above a face
 
 
 @pytest.mark.parametrize("nx,ng,order", [(8, 2, 1), (8, 2, 2), (16, 3, 3), (32, 2, 2),
-                                         (32, 3, 3), (12, 4, 3), (16, 2, 1)])
+                                         (32, 3, 3), (12, 4, 3), (16, 2, 1), (2, 2, 2),
+                                         (3, 3, 3)])
 def test_flux_box_reads_inside_the_block(nx, ng, order):
     if order == 3 and ng < 3:
         pytest.skip("PPM needs NGHOST >= 3")
@@ -35,7 +36,7 @@ def test_flux_box_reads_inside_the_block(nx, ng, order):
     assert all(lo[d] == is_ - 1 and lo[d] + n[d] - 1 == ie + 1 for d in range(3))
 
 
-@pytest.mark.parametrize("nx,ng", [(8, 2), (32, 2), (16, 3)])
+@pytest.mark.parametrize("nx,ng", [(8, 2), (32, 2), (16, 3), (2, 2), (3, 3)])
 def test_corner_update_and_recovery_boxes(nx, ng):
     nc = nx + 2 * ng
     is_, ie = ng, ng + nx - 1
@@ -59,7 +60,7 @@ def test_boxes_fit_every_tile_shape():
         assert chunks * 256 >= n[0] * n[1] * n[2]
 
 
-@pytest.mark.parametrize("nx,ng", [(8, 2), (6, 3), (4, 4), (5, 2)])
+@pytest.mark.parametrize("nx,ng", [(8, 2), (6, 3), (4, 4), (5, 2), (2, 2), (3, 3)])
 def test_ghost_exchange_covers_every_ghost_once_from_an_active_source(nx, ng):
     # the exchange kernel's numbering, read back through the library: for the cell array
     # and each face component, ghost numbers 0..count-1 hit every ghost position exactly

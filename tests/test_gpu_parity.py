@@ -96,6 +96,18 @@ def test_linear_wave_noncubic_blocks(bnx, rsolver, xorder, ng):
     check(m, o, d, bitwise=True)
 
 
+@pytest.mark.parametrize("nx1,bnx,rsolver,xorder,ng", [(32, (2, 2, 2), "hlld", 2, 2),
+                                                       (24, (3, 3, 3), "hlle", 3, 3)])
+def test_linear_wave_smallest_blocks(nx1, bnx, rsolver, xorder, ng):
+    # MeshBlocks as small as the ghost width allows: each ghost layer is a neighbour's
+    # whole active region and the corner z-march covers fewer planes than its length
+    print("case linear_wave smallest blocks", nx1, bnx, rsolver, xorder, ng, flush=True)
+    m = linear_wave_mesh(nx1, bnx=bnx, rsolver=rsolver, xorder=xorder, ng=ng)
+    lw = LinearWave(m, wave_flag=0, amp=1e-6)
+    o, d = both(m, [lw.block(g) for g in range(m.nblocks)], ncycles=4)
+    check(m, o, d, bitwise=True)
+
+
 def test_conservation_and_divb_gpu():
     print("case conservation 32^3/16^3", flush=True)
     m = turbulence_mesh(32, 16)
