@@ -16,12 +16,19 @@
 //     L/R-going symmetry, conservation and div(B) invariants;
 //   * Riemann solvers (tests/test_oracle_pins.py): identical states give the
 //     closed-form physical MHD flux (HLLE, HLLD); supersonic states give the upwind
-//     flux; HLLE with B = 0 equals a textbook Einfeldt HLLE of the Euler equations;
+//     flux; HLLE equals a textbook HLLE with Einfeldt bounds, of the Euler equations
+//     for B = 0 and of ideal MHD with the Cargo & Gallice Roe-average fast speed
+//     otherwise;
 //   * DC/PLM/PPM (ora_reconstruct_x1, tests/test_oracle_pins.py): constants exact,
 //     linear data exact, PPM exact on cubic cell averages, PPM third order or better
 //     through smooth extrema, mirror symmetry, no new extrema at a step;
-//   * SR con2prim: PrimitiveToConserved o ConservedToPrimitive round trip.
-// What none of these fixes is listed as "parity unpinned" in DESIGN.md.
+//   * outflow faces (ApplyPhysicalBoundaries, the EMF edge counts): the RJ2a shock
+//     tube's error and convergence criteria of the reference's test, equal results
+//     along x1/x2/x3, and the ghost-zone semantics cell by cell;
+//   * SR con2prim: PrimitiveToConserved o ConservedToPrimitive round trip; the SR path
+//     as a whole: the reference's SR shock-tube data and SR wave convergence orders.
+// Parity unpinned: the branches of PPM's Colella-Sekora limiter that act only near
+// discontinuities (checked for creating no new extrema, not value by value).
 //========================================================================================
 #include <algorithm>
 #include <cfloat>

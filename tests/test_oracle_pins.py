@@ -12,7 +12,9 @@ independently of the oracle's own formulas:
   states give the upwind physical flux; HLLE with B = 0 equals the textbook HLLE of
   the Euler equations with Einfeldt's wave-speed bounds from Roe averages
   (Einfeldt 1988, SIAM J. Numer. Anal. 25, 294; Toro, Riemann Solvers, 3rd ed., 10.6),
-  written here from those sources."""
+  and with B != 0 the textbook MHD HLLE with the Roe-average fast speed of Cargo &
+  Gallice (1997) as Stone et al. (2008, appendix B) state it, both written here from
+  those sources."""
 import ctypes
 
 import numpy as np
@@ -252,3 +254,60 @@ def test_hlle_without_field_is_the_textbook_euler_hlle():
     alt = np.array([euler_hlle(wl[q], wr[q], einfeldt=False) for q in range(len(wl))])
     differs = np.any(np.abs(f[:, :5] - alt) > 1e-8 * scale, axis=1)
     assert differs.mean() > 0.2
+
+
+def mhd_hlle(wl, wr, bx, gam=GAM, xy=True):
+    """Textbook HLLE flux of ideal MHD: Einfeldt's bounds SL = min(vx_roe - cf_roe,
+    vxL - cfL), SR = max(vx_roe + cf_roe, vxR + cfR) with the fast speed of the Roe
+    average of Cargo & Gallice (1997, J. Comput. Phys. 136, 446), as Stone et al. (2008,
+    ApJS 178, 137, appendix B) state it: rho = sqrt(rhoL rhoR), v and H (total enthalpy
+    per mass, magnetic pressure included) sqrt(rho)-weighted, Bt weighted the other way,
+    a^2 = (gamma-1)(H - v^2/2 - B^2/rho) - (gamma-2) X,
+    bt*^2 = ((gamma-1) - (gamma-2) Y) Bt^2/rho, X = |BtL - BtR|^2 / (2 (sqrt(rhoL) +
+    sqrt(rhoR))^2), Y = (rhoL + rhoR) / (2 rho), cf^2 = ((a^2 + b^2) + sqrt((a^2 + b^2)^2
+    - 4 a^2 bx^2)) / 2 with b^2 = bx^2 + bt*^2, bx^2 = Bx^2/rho; then the HLL flux with
+    the bounds clipped at zero.  xy=False drops the X and Y terms (X = 0, Y = 1)."""
+    def cons(w):
+        rho, vx, vy, vz, p, by, bz = w
+        e = p / (gam - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz) \
+            + 0.5 * (bx * bx + by * by + bz * bz)
+        return np.array([rho, rho * vx, rho * vy, rho * vz, e, by, bz])
+
+    sl, sr = np.sqrt(wl[0]), np.sqrt(wr[0])
+    rho = sl * sr
+    v = (sl * wl[1:4] + sr * wr[1:4]) / (sl + sr)
+    hl = (cons(wl)[4] + wl[4] + 0.5 * (bx * bx + wl[5] ** 2 + wl[6] ** 2)) / wl[0]
+    hr = (cons(wr)[4] + wr[4] + 0.5 * (bx * bx + wr[5] ** 2 + wr[6] ** 2)) / wr[0]
+    h = (sl * hl + sr * hr) / (sl + sr)
+    bt = (sr * wl[5:7] + sl * wr[5:7]) / (sl + sr)
+    X = np.sum((wl[5:7] - wr[5:7]) ** 2) / (2.0 * (sl + sr) ** 2) if xy else 0.0
+    Y = (wl[0] + wr[0]) / (2.0 * rho) if xy else 1.0
+    a2 = (gam - 1.0) * (h - 0.5 * np.dot(v, v) - (bx * bx + np.dot(bt, bt)) / rho) \
+        - (gam - 2.0) * X
+    bx2 = bx * bx / rho
+    b2 = bx2 + ((gam - 1.0) - (gam - 2.0) * Y) * np.dot(bt, bt) / rho
+    cf = np.sqrt(0.5 * ((a2 + b2) + np.sqrt((a2 + b2) ** 2 - 4.0 * a2 * bx2)))
+    bm = min(v[0] - cf, wl[1] - _fast_speed(wl, bx, gam), 0.0)
+    bp = max(v[0] + cf, wr[1] + _fast_speed(wr, bx, gam), 0.0)
+    fl, fr = physical_flux(wl, bx, gam), physical_flux(wr, bx, gam)
+    return (bp * fl - bm * fr + bp * bm * (cons(wr) - cons(wl))) / (bp - bm), a2
+
+
+def test_hlle_with_field_is_the_textbook_mhd_hlle():
+    """The Roe-average fast speed with the X and Y terms (hlle_mhd.cpp:62-118) against
+    the published formulas: a dropped or mis-signed X or Y term changes the flux of a
+    large share of these states."""
+    rng = np.random.default_rng(51)
+    wl, bx = _states(rng, 3000)
+    wr, _ = _states(rng, 3000)
+    f = ob.riemann(1, 0, GAM, wl, wr, bx)
+    want, a2 = zip(*(mhd_hlle(wl[q], wr[q], bx[q]) for q in range(len(wl))))
+    want, a2 = np.array(want), np.array(a2)
+    ok = a2 > 0.0                      # the oracle clamps a negative a^2 at zero
+    assert ok.mean() > 0.9
+    scale = np.abs(want).max(axis=1, keepdims=True) + 1e-3
+    err = np.abs(f - want) / scale
+    assert np.all(err[ok] <= 1e-12), err[ok].max()
+    alt = np.array([mhd_hlle(wl[q], wr[q], bx[q], xy=False)[0] for q in range(len(wl))])
+    differs = np.any(np.abs(f - alt) > 1e-8 * scale, axis=1)
+    assert differs[ok].mean() > 0.2, differs[ok].mean()
