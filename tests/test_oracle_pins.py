@@ -7,7 +7,9 @@ independently of the oracle's own formulas:
   cell averages (a second-order interface formula misses them by ~1e-4); PPM converges
   at third order or better through smooth extrema (a limiter that clips extrema drops
   to first or second order there); mirror images of the data give mirrored states;
-  a step creates no new extrema;
+  a step creates no new extrema; isolated spikes are flattened to the cell average
+  (Colella & Sekora 2008); parabolas of monotone cells are monotone (Colella &
+  Woodward 1984, eq. 1.10);
 * HLLE / HLLD: identical states give the closed-form physical MHD flux; supersonic
   states give the upwind physical flux; HLLE with B = 0 equals the textbook HLLE of
   the Euler equations with Einfeldt's wave-speed bounds from Roe averages
@@ -134,6 +136,53 @@ def test_mirror_symmetry(order):
     # mirror face, and vice versa
     np.testing.assert_allclose(wlm[:, ::-1], wr, rtol=1e-14, atol=0)
     np.testing.assert_allclose(wrm[:, ::-1], wl, rtol=1e-14, atol=0)
+
+
+def _cell_states(q, ng=3):
+    """(q_minus, q_plus) of the cells ng .. n-ng-1 from the PPM face states."""
+    wl, wr = reconstruct(3, q, ng)
+    return wr[:, :-1], wl[:, 1:]
+
+
+def test_ppm_flattens_discontinuous_extrema():
+    # Colella & Sekora (2008): at an extremum where the second differences disagree in
+    # sign (an isolated spike or dip, not a smooth peak) the parabola is flattened to
+    # the cell average, both interface values equal to it
+    n, ng = 28, 3
+    x = np.arange(n, dtype=float)
+    q = np.tile(10.0 + 0.05 * x, (7, 1))
+    spikes = {8: 20.0, 15: 2.0, 21: 30.0}
+    for c, v in spikes.items():
+        q[:, c] = v
+    qm, qp = _cell_states(q, ng)
+    for c, v in spikes.items():
+        assert np.all(qm[:, c - ng] == v) and np.all(qp[:, c - ng] == v), c
+
+
+def test_ppm_parabolas_are_monotone_in_monotone_cells():
+    # Colella & Woodward (1984), eq. 1.10: in a cell whose neighbours rise (or fall)
+    # monotonically the limited parabola has no interior extremum, which holds exactly
+    # when |q0 - q_minus| <= 2 |q_plus - q0| and |q_plus - q0| <= 2 |q0 - q_minus|
+    rng = np.random.default_rng(61)
+    n, ng = 64, 3
+    steps = rng.uniform(0.01, 0.1, size=(7, n - 1))
+    jump = rng.uniform(size=(7, n - 1)) < 0.15
+    steps[jump] *= rng.uniform(20.0, 200.0, size=jump.sum())       # steep fronts
+    q = np.concatenate([np.full((7, 1), 1.0), 1.0 + np.cumsum(steps, axis=1)], axis=1)
+    q[3:5] = q[3:5, ::-1].copy()                                     # two falling rows
+    qm, qp = _cell_states(q, ng)
+    q0 = q[:, ng:n - ng]
+    dm, dp = q0 - qm, qp - q0
+    tol = 1e-12 * np.abs(q0)
+    assert np.all(dm * dp >= -tol)
+    assert np.all(np.abs(dm) <= 2.0 * np.abs(dp) + tol)
+    assert np.all(np.abs(dp) <= 2.0 * np.abs(dm) + tol)
+    # the data exercise the limiter: the unlimited fourth-order interface values
+    # (7/12 (q_i + q_i+1) - 1/12 (q_i-1 + q_i+2)) break the condition in many cells
+    f4 = 7.0 / 12.0 * (q[:, 1:-2] + q[:, 2:-1]) - 1.0 / 12.0 * (q[:, :-3] + q[:, 3:])
+    um, up = q0 - f4[:, ng - 2:n - ng - 2], f4[:, ng - 1:n - ng - 1] - q0
+    broken = (np.abs(um) > 2.0 * np.abs(up)) | (np.abs(up) > 2.0 * np.abs(um))
+    assert broken.mean() > 0.1, broken.mean()
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
