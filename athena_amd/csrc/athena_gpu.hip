@@ -1005,14 +1005,15 @@ __global__ void __launch_bounds__(WG) k_ghost(MeshK M, GhostArgs a) {
 // time_integrator.cpp:2291 and mesh.cpp:1868 select) and rebuilds its conserved
 // variables with EquationOfState::PrimitiveToConserved (adiabatic_mhd.cpp:99-136) from
 // them and its own cell-centred field, which the ConservedToPrimitive kernel has already
-// computed from the copied face ghosts (bvals.cpp:517-523).  The source cell lies on the
+// computed from the copied face ghosts (bvals.cpp:517-523); SR meshes use the SR
+// PrimitiveToConserved (adiabatic_mhd_sr.cpp:276-332).  The source cell lies on the
 // active side of every outflow face, so no thread reads what another one writes.
-// Newtonian MHD only.
 struct PhysArgs {
   double *u, *prim;
   const double *bcc;
 };
 
+template <bool SR>
 __device__ __forceinline__ void phys_bnd_body(const MeshK &M, const PhysArgs &a, int lw,
                                               int nwg) {
   const int nch = nwg / M.nblocks;
@@ -1040,17 +1041,35 @@ __device__ __forceinline__ void phys_bnd_body(const MeshK &M, const PhysArgs &a,
   const double *bc = a.bcc + static_cast<long>(g) * 3 * nc + c;
   const double bcc1 = bc[0], bcc2 = bc[nc], bcc3 = bc[2 * nc];
   double *u = a.u + b0 + c;
-  u[IDN * nc] = w[IDN];
-  u[IM1 * nc] = w[IVX] * w[IDN];
-  u[IM2 * nc] = w[IVY] * w[IDN];
-  u[IM3 * nc] = w[IVZ] * w[IDN];
-  u[IEN * nc] = w[IPR] * M.eos.igm1 +
-                0.5 * (w[IDN] * (w[IVX] * w[IVX] + w[IVY] * w[IVY] + w[IVZ] * w[IVZ]) +
-                       (bcc1 * bcc1 + bcc2 * bcc2 + bcc3 * bcc3));
+  if (!SR) {
+    u[IDN * nc] = w[IDN];
+    u[IM1 * nc] = w[IVX] * w[IDN];
+    u[IM2 * nc] = w[IVY] * w[IDN];
+    u[IM3 * nc] = w[IVZ] * w[IDN];
+    u[IEN * nc] = w[IPR] * M.eos.igm1 +
+                  0.5 * (w[IDN] * (w[IVX] * w[IVX] + w[IVY] * w[IVY] + w[IVZ] * w[IVZ]) +
+                         (bcc1 * bcc1 + bcc2 * bcc2 + bcc3 * bcc3));
+  } else {
+    const double red = M.eos.gamma / (M.eos.gamma - 1.0);
+    const double u1 = w[IVX], u2 = w[IVY], u3 = w[IVZ];
+    const double u0 = sqrt(1.0 + sq(u1) + sq(u2) + sq(u3));
+    const double b0 = bcc1 * u1 + bcc2 * u2 + bcc3 * u3;
+    const double b1 = (bcc1 + b0 * u1) / u0;
+    const double b2 = (bcc2 + b0 * u2) / u0;
+    const double b3 = (bcc3 + b0 * u3) / u0;
+    const double b_sq = -sq(b0) + sq(b1) + sq(b2) + sq(b3);
+    const double wtot_u02 = (w[IDN] + red * w[IPR] + b_sq) * u0 * u0;
+    u[IDN * nc] = w[IDN] * u0;
+    u[IEN * nc] = wtot_u02 - b0 * b0 - (w[IPR] + 0.5 * b_sq);
+    u[IM1 * nc] = wtot_u02 * u1 / u0 - b0 * b1;
+    u[IM2 * nc] = wtot_u02 * u2 / u0 - b0 * b2;
+    u[IM3 * nc] = wtot_u02 * u3 / u0 - b0 * b3;
+  }
 }
 
+template <bool SR>
 __global__ void __launch_bounds__(WG) k_phys_bnd(MeshK M, PhysArgs a) {
-  phys_bnd_body(M, a, blockIdx.x, gridDim.x);
+  phys_bnd_body<SR>(M, a, blockIdx.x, gridDim.x);
 }
 
 //========================================================================================
@@ -1839,8 +1858,12 @@ int launch_phys_bnd(ath_mesh *m, const C2PArgs &pa) {
   ProfScope ps(m, KID_PHYS_BND);
   const int grid =
       static_cast<int>((ghost_count(M.nc1, M.nc2, M.nc3, M.ng) + WG - 1) / WG) * M.nblocks;
-  hipLaunchKernelGGL(k_phys_bnd, dim3(grid), dim3(WG), 0, m->stream, M,
-                     PhysArgs{pa.u, pa.prim, pa.bcc});
+  if (m->d.relativistic)
+    hipLaunchKernelGGL(k_phys_bnd<true>, dim3(grid), dim3(WG), 0, m->stream, M,
+                       PhysArgs{pa.u, pa.prim, pa.bcc});
+  else
+    hipLaunchKernelGGL(k_phys_bnd<false>, dim3(grid), dim3(WG), 0, m->stream, M,
+                       PhysArgs{pa.u, pa.prim, pa.bcc});
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2099,8 +2122,6 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
       return fail("boundary flags must be 0 (periodic) or 1 (outflow)");
     if ((d.bc[2 * q] == 0) != (d.bc[2 * q + 1] == 0))
       return fail("a periodic boundary needs periodic on the opposite face too");
-    if (d.bc[2 * q] && d.relativistic)
-      return fail("outflow boundaries are implemented for Newtonian MHD only");
   }
 
   ath_mesh *m = new ath_mesh;

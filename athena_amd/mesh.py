@@ -82,8 +82,6 @@ class MeshDesc:
             if (self.bc[2 * d] == "periodic") != (self.bc[2 * d + 1] == "periodic"):
                 raise ValueError("x%d: periodic on one side needs periodic on the other"
                                  % (d + 1))
-        if self.relativistic and any(f != "periodic" for f in self.bc):
-            raise ValueError("outflow boundaries are implemented for Newtonian MHD only")
 
     # ---------------------------------------------------------------- geometry
     @property

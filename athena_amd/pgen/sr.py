@@ -66,11 +66,20 @@ MUB = {
 }
 
 
-def shock_tube_mesh(problem, nx1=None, xorder=2, rsolver="hlle", **kw):
+def shock_tube_mesh(problem, nx1=None, xorder=2, rsolver="hlle", outflow=False, **kw):
+    """The tube on a 3-D mesh with two periodic transverse cells.  outflow=True: the
+    reference's own [-0.5, 0.5] between outflow faces; otherwise a periodic box three
+    times as long (the wrap-around waves do not reach [-0.5, 0.5] by tlim)."""
     p = MUB[problem]
-    n = (nx1 or p["nx1"]) * 3
-    return MeshDesc(nx=(n, 2, 2), bnx=kw.pop("bnx", (n, 2, 2)), xmin=(-1.5, -0.5, -0.5),
-                    xmax=(1.5, 0.5, 0.5), gamma=p["gamma"], cfl=0.4, xorder=xorder,
+    if outflow:
+        n = nx1 or p["nx1"]
+        lo, hi = -0.5, 0.5
+        kw.setdefault("bc", ("outflow", "outflow") + ("periodic",) * 4)
+    else:
+        n = (nx1 or p["nx1"]) * 3
+        lo, hi = -1.5, 1.5
+    return MeshDesc(nx=(n, 2, 2), bnx=kw.pop("bnx", (n, 2, 2)), xmin=(lo, -0.5, -0.5),
+                    xmax=(hi, 0.5, 0.5), gamma=p["gamma"], cfl=0.4, xorder=xorder,
                     rsolver=rsolver, relativistic=True, **kw)
 
 

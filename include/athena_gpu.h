@@ -40,7 +40,7 @@ typedef struct ath_mesh_desc {
   double sigma_max, beta_min, gamma_max;   /* SR floors; < 0 selects 0, 0, 1000 */
   int bc[6];          /* <mesh> ix1_bc, ox1_bc, ix2_bc, ox2_bc, ix3_bc, ox3_bc: 0 periodic,
                          1 outflow (bvals.cpp:453-…, cc/outflow_cc.cpp, fc/outflow_fc.cpp;
-                         Newtonian MHD only; a zero-filled descriptor is fully periodic) */
+                         a zero-filled descriptor is fully periodic) */
 } ath_mesh_desc;
 
 typedef struct ath_mesh *ath_mesh_t;

@@ -833,7 +833,25 @@ void ConservedToPrimitive(Mesh &m, Block &b, Arr &cons, Arr &prim_old, Arr &prim
   else Cons2PrimNewtonian(m, b, cons, prim, il, iu, jl, ju, kl, ku);
 }
 
-// EquationOfState::PrimitiveToConserved, adiabatic MHD (adiabatic_mhd.cpp:99-136)
+// EquationOfState::PrimitiveToConserved, adiabatic MHD (adiabatic_mhd.cpp:99-136); SR
+// (adiabatic_mhd_sr.cpp:276-332) cell by cell through Prim2ConsSRCell
+void Prim2ConsNewtonian(Mesh &m, Block &b, int il, int iu, int jl, int ju, int kl, int ku);
+void Prim2Cons(Mesh &m, Block &b, int il, int iu, int jl, int ju, int kl, int ku) {
+  if (!m.c.relativistic) {
+    Prim2ConsNewtonian(m, b, il, iu, jl, ju, kl, ku);
+    return;
+  }
+  for (int k = kl; k <= ku; ++k)
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        Real w[5], bc[3], c[5];
+        for (int n = 0; n < 5; ++n) w[n] = b.w(n, k, j, i);
+        for (int n = 0; n < 3; ++n) bc[n] = b.bcc(n, k, j, i);
+        Prim2ConsSRCell(w, bc, m.c.gamma, c);
+        for (int n = 0; n < 5; ++n) b.u(n, k, j, i) = c[n];
+      }
+}
+
 void Prim2ConsNewtonian(Mesh &m, Block &b, int il, int iu, int jl, int ju, int kl, int ku) {
   Real igm1 = 1.0 / (m.c.gamma - 1.0);
   for (int k = kl; k <= ku; ++k)
@@ -894,7 +912,7 @@ void ApplyPhysicalBoundaries(Mesh &m, Block &b) {
         for (int q = 1; q <= ng; ++q) b.b3f(k, j, side ? ie + q : is - q) = b.b3f(k, j, src);
     const int lo = side ? ie + 1 : is - ng, hi = side ? ie + ng : is - 1;
     CellCenteredField(b, lo, hi, bjs, bje, bks, bke);
-    Prim2ConsNewtonian(m, b, lo, hi, bjs, bje, bks, bke);
+    Prim2Cons(m, b, lo, hi, bjs, bje, bks, bke);
   }
   // x2 faces: i in [bis, bie], jl = js / ju = je, k in [bks, bke]
   for (int side = 0; side < 2; ++side) {
@@ -918,7 +936,7 @@ void ApplyPhysicalBoundaries(Mesh &m, Block &b) {
           b.b3f(k, side ? je + q : js - q, i) = b.b3f(k, src, i);
     const int lo = side ? je + 1 : js - ng, hi = side ? je + ng : js - 1;
     CellCenteredField(b, bis, bie, lo, hi, bks, bke);
-    Prim2ConsNewtonian(m, b, bis, bie, lo, hi, bks, bke);
+    Prim2Cons(m, b, bis, bie, lo, hi, bks, bke);
   }
   // x3 faces: i and j over every cell, ghosts included (bvals.cpp:600-601), kl = ks /
   // ku = ke
@@ -945,7 +963,7 @@ void ApplyPhysicalBoundaries(Mesh &m, Block &b) {
           b.b3f(side ? ke + q + 1 : ks - q, j, i) = b.b3f(side ? ke + 1 : ks, j, i);
     const int lo = side ? ke + 1 : ks - ng, hi = side ? ke + ng : ks - 1;
     CellCenteredField(b, bis, bie, bjs, bje, lo, hi);
-    Prim2ConsNewtonian(m, b, bis, bie, bjs, bje, lo, hi);
+    Prim2Cons(m, b, bis, bie, bjs, bje, lo, hi);
   }
 }
 
@@ -1163,10 +1181,8 @@ using namespace ora;
 extern "C" {
 
 void *ora_new(const ora_cfg *cfg) {
-  for (int f = 0; f < 6; ++f) {
+  for (int f = 0; f < 6; ++f)
     if (cfg->bc[f] != 0 && cfg->bc[f] != 1) return nullptr;
-    if (cfg->bc[f] && cfg->relativistic) return nullptr;   // outflow: Newtonian MHD only
-  }
   Mesh *m = new Mesh;
   m->c = *cfg;
   Real fmin = std::numeric_limits<float>::min();

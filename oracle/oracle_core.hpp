@@ -109,6 +109,7 @@ void PiecewiseParabolic(Mesh &m, Block &b, int d, int k, int j, int il, int iu, 
 void Cons2PrimSR(Mesh &m, Block &b, Arr &cons, Arr &prim_old, Arr &prim, int il, int iu,
                  int jl, int ju, int kl, int ku);
 void CellCenteredField(Block &b, int il, int iu, int jl, int ju, int kl, int ku);
+void Prim2ConsSRCell(const Real prim[5], const Real bc[3], Real gamma_adi, Real cons[5]);
 Real Wval(Block &b, Arr &w, Arr &bcc, int d, int n, int k, int j, int i);
 inline Real SIGN(Real x) { return ((x) < 0.0) ? -1.0 : 1.0; }
 constexpr Real PI = 3.1415926535897932;

@@ -132,8 +132,7 @@ def test_outflow_ghost_semantics(bc):
 def test_outflow_descriptor_checks():
     with pytest.raises(ValueError):
         MeshDesc(nx=(8, 8, 8), bnx=(8, 8, 8), bc=("outflow", "periodic") + ("periodic",) * 4)
-    with pytest.raises(ValueError):
-        MeshDesc(nx=(8, 8, 8), bnx=(8, 8, 8), relativistic=True, rsolver="hlle",
-                 bc=("outflow",) * 6)
+    MeshDesc(nx=(8, 8, 8), bnx=(8, 8, 8), relativistic=True, rsolver="hlle",
+             bc=("outflow",) * 6)                # SR meshes take outflow faces too
     with pytest.raises(ValueError):
         MeshDesc(nx=(8, 8, 8), bnx=(8, 8, 8), bc=("reflecting",) * 6)

@@ -20,9 +20,10 @@ def ref(problem):
     return dict(np.load(os.path.join(GOLDEN, "sr_mhd_shock%d_hlld.npz" % problem)))
 
 
-# The reference runs the tubes in 1-D; here they run in a periodic 3-D box with two
-# transverse cells, so every transverse face solves a Riemann problem with identical
-# left and right states.  HLLE returns exactly F(U) there, but HLLD's wave-fan
+# The reference runs the tubes in 1-D between outflow faces; here they run in 3-D with
+# two periodic transverse cells, either between outflow faces on the reference's
+# [-0.5, 0.5] or in a periodic box three times as long, so every transverse face solves
+# a Riemann problem with identical left and right states.  HLLE returns exactly F(U) there, but HLLD's wave-fan
 # arithmetic leaves round-off in components that vanish analytically (e.g. the z-face
 # flux of B_x when v_z = B_z = 0), so with HLLD the "exactly zero" components of the
 # reference test are bounded at round-off instead.
@@ -41,8 +42,8 @@ def tols(problem, rsolver):
     return MUB[problem]["tols_hlld" if rsolver == "hlld" else "tols"]
 
 
-def run_oracle(problem, rsolver="hlle"):
-    m = shock_tube_mesh(problem, rsolver=rsolver)
+def run_oracle(problem, rsolver="hlle", outflow=False):
+    m = shock_tube_mesh(problem, rsolver=rsolver, outflow=outflow)
     st = ShockTube(m, problem)
     o = OracleMesh(m)
     o.load([st.block(0)])
@@ -55,6 +56,13 @@ def run_oracle(problem, rsolver="hlle"):
 @pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
 def test_oracle_sr_shock_tube_1(rsolver):
     assert_tube(run_oracle(1, rsolver), tols(1, rsolver), ZERO_TOL[rsolver])
+
+
+@pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
+def test_oracle_sr_shock_tube_1_outflow(rsolver):
+    # the reference's own domain [-0.5, 0.5] between outflow faces
+    # (mhd_shocks_hlle.py runs athinput.mub_1 with outflow x1 boundaries)
+    assert_tube(run_oracle(1, rsolver, outflow=True), tols(1, rsolver), ZERO_TOL[rsolver])
 
 
 @pytest.mark.parametrize("rsolver", ["hlle", "hlld"])
