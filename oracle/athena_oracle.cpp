@@ -23,15 +23,14 @@
 //     linear data exact, PPM exact on cubic cell averages, PPM third order or better
 //     through smooth extrema, mirror symmetry, no new extrema at a step, isolated
 //     spikes flattened to the cell average, monotone parabolas in monotone cells
-//     (Colella & Woodward's condition; a disabled overshoot or flattening branch fails);
+//     (Colella & Woodward's condition; a disabled overshoot or flattening branch fails),
+//     hand-derived face values for each branch of the Colella-Sekora interface limiter
+//     and for the extremum limiter's partial flattening (a wrong C = 1.25 fails);
 //   * outflow faces (ApplyPhysicalBoundaries, the EMF edge counts): the RJ2a shock
 //     tube's error and convergence criteria of the reference's test, equal results
 //     along x1/x2/x3, and the ghost-zone semantics cell by cell;
 //   * SR con2prim: PrimitiveToConserved o ConservedToPrimitive round trip; the SR path
 //     as a whole: the reference's SR shock-tube data and SR wave convergence orders.
-// Parity unpinned: the value PPM's Colella-Sekora interface limiter picks next to a
-// discontinuity (its C2 = 1.25 curvature bound), bounded by the checks above but not
-// fixed value by value.
 //========================================================================================
 #include <algorithm>
 #include <cfloat>

@@ -9,7 +9,9 @@ independently of the oracle's own formulas:
   to first or second order there); mirror images of the data give mirrored states;
   a step creates no new extrema; isolated spikes are flattened to the cell average
   (Colella & Sekora 2008); parabolas of monotone cells are monotone (Colella &
-  Woodward 1984, eq. 1.10);
+  Woodward 1984, eq. 1.10); known answers derived by hand from Colella & Sekora's
+  interface limiter (its C |D2| bound, the sign test, the kept fourth-order value)
+  and extremum limiter (the partial flattening rho = D2lim / D2 with C = 1.25);
 * HLLE / HLLD: identical states give the closed-form physical MHD flux; supersonic
   states give the upwind physical flux; HLLE with B = 0 equals the textbook HLLE of
   the Euler equations with Einfeldt's wave-speed bounds from Roe averages
@@ -183,6 +185,85 @@ def test_ppm_parabolas_are_monotone_in_monotone_cells():
     um, up = q0 - f4[:, ng - 2:n - ng - 2], f4[:, ng - 1:n - ng - 1] - q0
     broken = (np.abs(um) > 2.0 * np.abs(up)) | (np.abs(up) > 2.0 * np.abs(um))
     assert broken.mean() > 0.1, broken.mean()
+
+
+def _cs_interface(qm1, q0, qp1, qp2, c=1.25, factor=3.0, same_sign=True, limit=True):
+    """Colella & Sekora (2008), eqs. 18-19, written here from the paper: the fourth-
+    order value at the face between q0 and qp1 and, when it leaves [q0, qp1], its
+    replacement 1/2 (q0 + qp1) - D2lim / 6 with D2 = 3 (q0 - 2 q_face + qp1) bounded by
+    C times the cell curvatures on either side when all three agree in sign, else 0.
+    The keyword arguments plant the mistakes that the known answers must catch."""
+    qf = 7.0 / 12.0 * (q0 + qp1) - 1.0 / 12.0 * (qm1 + qp2)
+    if not limit or (qf - q0) * (qp1 - qf) >= 0.0:
+        return qf
+    d2, d2l, d2r = factor * (q0 - 2.0 * qf + qp1), qm1 - 2.0 * q0 + qp1, q0 - 2.0 * qp1 + qp2
+    s = np.sign(d2)
+    lim = 0.0
+    if not same_sign or (np.sign(d2l) == s and np.sign(d2r) == s):
+        lim = s * min(c * abs(d2l), c * abs(d2r), abs(d2))
+    return 0.5 * (q0 + qp1) - lim / 6.0
+
+
+# (q_i-2, q_i-1, q_i, q_i+1, q_i+2) and the face value between q_i and q_i+1 derived by
+# hand from eqs. 18-19: the C |D2_R| bound binds (1.05 + 1.25 * 0.1 / 6); the cell
+# curvatures disagree in sign, so the face is the mean; |D2| binds, which keeps the
+# overshooting fourth-order value 7/12 * 1 + 11/12
+CS_CASES = [((0.0, 0.0, 1.0, 1.1, 1.1), 1.05 + 0.125 / 6.0),
+            ((-1.0, -0.5, 0.0, 1.0, -6.0), 0.5),
+            ((-12.0, -6.0, 0.0, 1.0, -5.0), 1.5)]
+
+
+@pytest.mark.parametrize("case", range(len(CS_CASES)))
+def test_ppm_interface_limiter_known_answers(case):
+    st, face = CS_CASES[case]
+    ref = _cs_interface(*st[1:])
+    assert abs(ref - face) <= 1e-14 * abs(face)
+    # each case leaves the fourth-order range, and together they catch a wrong C, a
+    # dropped factor 3, a missing sign test and a missing limiter
+    qf = _cs_interface(*st[1:], limit=False)
+    assert (qf - st[2]) * (st[3] - qf) < 0.0
+    wrong = [_cs_interface(*st[1:], c=1.0), _cs_interface(*st[1:], c=2.0),
+             _cs_interface(*st[1:], factor=1.0), _cs_interface(*st[1:], same_sign=False),
+             qf]
+    assert case != 0 or all(abs(v - face) > 1e-3 for v in wrong[:2])
+    assert case != 1 or all(abs(v - face) > 1e-3 for v in wrong[3:])
+    assert case != 2 or all(abs(v - face) > 1e-3 for v in (wrong[0], wrong[2]))
+    # the cell left of the face rises monotonically and the face value keeps its
+    # parabola monotone (|q_plus - q_i| < 2 |q_i - q_minus| with the unlimited q_minus),
+    # so the extremum and monotonicity steps leave it: the face state is this value
+    qminus = _cs_interface(*st[:4])
+    assert st[1] < st[2] < st[3] and st[1] <= qminus <= st[2]
+    assert abs(face - st[2]) < 2.0 * abs(st[2] - qminus)
+    ng, n, i = 3, 16, 8
+    q = np.empty(n)
+    q[:i - 2], q[i - 2:i + 3], q[i + 3:] = st[0], st, st[4]
+    q = np.tile(q, (7, 1))
+    wl, _ = reconstruct(3, q, ng)
+    assert np.all(np.abs(wl[:, i + 1 - ng] - face) <= 1e-14 * abs(face)), wl[:, i + 1 - ng]
+    # the mirror image exercises the other interface of the PPM cell loop
+    _, wr = reconstruct(3, q[:, ::-1], ng)
+    assert np.all(np.abs(wr[:, n - 1 - i - ng] - face) <= 1e-14 * abs(face))
+
+
+def test_ppm_extremum_limiter_known_answer():
+    # Colella & Sekora (2008), eqs. 20-23: at an extremum whose cell curvatures all
+    # agree in sign the parabola is scaled by rho = D2lim / D2, D2 = 6 (q_- + q_+ - 2 q)
+    # and D2lim = min(C |D2_c,i-1|, C |D2_c,i|, C |D2_c,i+1|, |D2|).  For the symmetric
+    # peak 0, 0.51, 1, 0.51, 0 the fourth-order faces f lie inside their cells (no
+    # interface limiting), |D2_c| is smallest beside the peak (0.02) and
+    # rho (1 - f) = C 0.02 / (12 (1 - f)) (1 - f), so both faces are 1 - C 0.02 / 12
+    f = 7.0 / 12.0 * 1.51 - 1.0 / 12.0 * 0.51
+    assert 0.51 < f < 1.0
+    face = 1.0 - 1.25 * 0.02 / 12.0
+    for wrong in (f, 1.0, 1.0 - 0.02 / 12.0, 1.0 - 2.0 * 0.02 / 12.0):
+        assert abs(wrong - face) > 1e-4          # no limiter, full flattening, C = 1, 2
+    ng, n, i = 3, 16, 8
+    q = np.zeros(n)
+    q[i - 1], q[i], q[i + 1] = 0.51, 1.0, 0.51
+    q = np.tile(q, (7, 1))
+    wl, wr = reconstruct(3, q, ng)
+    assert np.all(np.abs(wr[:, i - ng] - face) <= 1e-13), wr[:, i - ng]
+    assert np.all(np.abs(wl[:, i + 1 - ng] - face) <= 1e-13), wl[:, i + 1 - ng]
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
