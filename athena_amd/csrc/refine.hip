@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/athena_gpu.h"
 #include "physics.hpp"
@@ -388,6 +389,163 @@ dim3 rgrid(const RRange &R, int nb, int ncomp) {
   return dim3(static_cast<unsigned>((n + RWG - 1) / RWG));
 }
 
+//----------------------------------------------------------------------------------------
+// MeshBlock tree of a 3-D mesh with static refinement (meshblock_tree.cpp).  Nodes live in
+// one vector and refer to their children by index; a node without children is a leaf,
+// that is a MeshBlock.  Logical levels count from the node that covers the whole
+// power-of-two box (level 0); the root grid sits at root_level.
+struct TNode {
+  int level;
+  long long lx[3];
+  int child[8];   // index n = i + 2 j + 4 k of the child at (2 lx + (i, j, k)), -1 if none
+  bool leaf() const {
+    for (int n = 0; n < 8; ++n) if (child[n] >= 0) return false;
+    return true;
+  }
+};
+
+constexpr long kTreeMaxNodes = 1L << 24;
+constexpr int kTreeMaxLevel = 40;
+
+struct Tree {
+  std::vector<TNode> nd;
+  long long nrb[3];
+  bool periodic[6];   // inner/outer x1, x2, x3
+  int root_level = 0;
+  bool overflow = false;
+
+  int make(int level, long long l1, long long l2, long long l3) {
+    if (static_cast<long>(nd.size()) >= kTreeMaxNodes) { overflow = true; return -1; }
+    TNode t;
+    t.level = level; t.lx[0] = l1; t.lx[1] = l2; t.lx[2] = l3;
+    for (int n = 0; n < 8; ++n) t.child[n] = -1;
+    nd.push_back(t);
+    return static_cast<int>(nd.size()) - 1;
+  }
+
+  // CreateRootGrid (meshblock_tree.cpp:87-112): children exist where they overlap the
+  // nrb[0] x nrb[1] x nrb[2] root blocks.
+  void create_root(int id) {
+    const int level = nd[id].level;
+    if (level == root_level || overflow) return;
+    const long long fac = 1LL << (root_level - level - 1);
+    for (int n = 0; n < 8; ++n) {
+      const int i = n & 1, j = (n >> 1) & 1, k = (n >> 2) & 1;
+      const long long c1 = nd[id].lx[0] * 2 + i, c2 = nd[id].lx[1] * 2 + j,
+                      c3 = nd[id].lx[2] * 2 + k;
+      if (c3 * fac < nrb[2] && c2 * fac < nrb[1] && c1 * fac < nrb[0]) {
+        const int c = make(level + 1, c1, c2, c3);
+        if (c < 0) return;
+        nd[id].child[n] = c;
+        create_root(c);
+      }
+    }
+  }
+
+  // Refine (meshblock_tree.cpp:166-258): the eight children, then every neighbour of the
+  // refined node at its own level is made to exist (periodic faces wrap, others end the
+  // search), which keeps neighbouring leaves within one level of each other.
+  void refine(int id) {
+    if (!nd[id].leaf() || overflow) return;
+    const int level = nd[id].level;
+    const long long l0[3] = {nd[id].lx[0], nd[id].lx[1], nd[id].lx[2]};
+    for (int n = 0; n < 8; ++n) {
+      const int c = make(level + 1, l0[0] * 2 + (n & 1), l0[1] * 2 + ((n >> 1) & 1),
+                         l0[2] * 2 + ((n >> 2) & 1));
+      if (c < 0) return;
+      nd[id].child[n] = c;
+    }
+    long long nmax[3];
+    for (int q = 0; q < 3; ++q) nmax[q] = nrb[q] << (level - root_level);
+    for (int oz = -1; oz <= 1; ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+          if (ox == 0 && oy == 0 && oz == 0) continue;
+          const int o[3] = {ox, oy, oz};
+          long long l[3];
+          bool skip = false;
+          for (int q = 0; q < 3 && !skip; ++q) {
+            l[q] = l0[q] + o[q];
+            if (l[q] < 0) { if (periodic[2 * q]) l[q] = nmax[q] - 1; else skip = true; }
+            else if (l[q] >= nmax[q]) { if (periodic[2 * q + 1]) l[q] = 0; else skip = true; }
+          }
+          if (!skip) add(level, l);
+          if (overflow) return;
+        }
+  }
+
+  // AddMeshBlock (meshblock_tree.cpp:116-135): walk from the top, refining leaves on the
+  // way, until a node of the requested level exists at l.
+  void add(int level, const long long *l) {
+    int id = 0;
+    while (nd[id].level < level && !overflow) {
+      if (nd[id].leaf()) refine(id);
+      if (overflow) return;
+      const int sh = level - nd[id].level - 1;
+      const int n = static_cast<int>(((l[0] >> sh) & 1) + 2 * ((l[1] >> sh) & 1) +
+                                     4 * ((l[2] >> sh) & 1));
+      id = nd[id].child[n];
+      if (id < 0) { overflow = true; return; }   // outside the root grid: not reachable
+    }
+  }
+
+  // GetMeshBlockList (meshblock_tree.cpp:336-356): leaves depth first, children in
+  // index order, which is the Z order of the MeshBlock ids.
+  void leaves(int id, std::vector<int> &out) const {
+    if (nd[id].leaf()) { out.push_back(id); return; }
+    for (int n = 0; n < 8; ++n) if (nd[id].child[n] >= 0) leaves(nd[id].child[n], out);
+  }
+
+  // FindNeighbor (meshblock_tree.cpp:362-468): the node at the neighbour's position and
+  // level, or the coarser leaf that contains it; -1 across a non-periodic face, -3 when
+  // the tree has no node there (never for a tree built here).
+  int neighbor(int id, int o1, int o2, int o3) const {
+    const int ll = nd[id].level;
+    const int o[3] = {o1, o2, o3};
+    long long l[3];
+    for (int q = 0; q < 3; ++q) {
+      const long long nmax = nrb[q] << (ll - root_level);
+      l[q] = nd[id].lx[q] + o[q];
+      if (l[q] < 0) { if (!periodic[2 * q]) return -1; l[q] = nmax - 1; }
+      else if (l[q] >= nmax) { if (!periodic[2 * q + 1]) return -1; l[q] = 0; }
+    }
+    if (ll < 1) return 0;
+    int bt = 0;
+    for (int level = 0; level < ll; ++level) {
+      if (nd[bt].leaf()) return (level == ll - 1) ? bt : -3;
+      const int sh = ll - level - 1;
+      const int n = static_cast<int>(((l[0] >> sh) & 1) + 2 * ((l[1] >> sh) & 1) +
+                                     4 * ((l[2] >> sh) & 1));
+      bt = nd[bt].child[n];
+      if (bt < 0) return -3;
+    }
+    return bt;
+  }
+};
+
+// Uniform mesh generator at logical face index `index` of `nrange` (mesh.hpp:457-472 with
+// sym_interval, :535-538).
+double tree_face_x(long long index, long long nrange, double xmin, double xmax) {
+  const long long no = index - nrange / 2, noc = index - (nrange + 1) / 2;
+  const double x = static_cast<double>(no + noc) / (2.0 * static_cast<double>(nrange));
+  return 0.5 * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+// The logical index range [lo, hi) at ref_lev that a refinement region covers along one
+// direction, widened to whole sibling pairs (mesh.cpp:400-447).
+void region_range(long long nrb, int ref_lev, double rmin, double rmax, double xmin,
+                  double xmax, long long *lo, long long *hi) {
+  const long long lxmax = nrb * (1LL << ref_lev);
+  long long a = 0, b = 0;
+  for (a = 0; a < lxmax; ++a)
+    if (tree_face_x(a + 1, lxmax, xmin, xmax) > rmin) break;
+  for (b = a; b < lxmax; ++b)
+    if (tree_face_x(b + 1, lxmax, xmin, xmax) >= rmax) break;
+  if (a % 2 == 1) --a;
+  if (b % 2 == 0) ++b;
+  *lo = a; *hi = b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -530,6 +688,84 @@ int ath_gpu_restrict_emf(const int *shape, int nb, int comp, const double *fcoor
   else
     hipLaunchKernelGGL(k_restrict_emf<2>, g, dim3(RWG), 0, st, S, R, nb, fcoord, fine, coarse);
   return launch_check(st);
+}
+
+int ath_gpu_tree_build(const int *nrb, const int *bc, const double *xmin, const double *xmax,
+                   const ath_refine_region *reg, int nreg, long long *locs, int *nbr,
+                   int cap, int *nblocks, int *root_level) {
+  if (!nrb || !bc || !xmin || !xmax || (nreg > 0 && !reg) || nreg < 0 || !nblocks)
+    return rfail("ath_gpu_tree_build: bad argument");
+  Tree T;
+  long long nbmax = 1;
+  for (int q = 0; q < 3; ++q) {
+    if (nrb[q] < 1) return rfail("ath_gpu_tree_build: root blocks per direction must be >= 1");
+    if (!(xmin[q] < xmax[q])) return rfail("ath_gpu_tree_build: xmin must be below xmax");
+    T.nrb[q] = nrb[q];
+    nbmax = (nrb[q] > nbmax) ? nrb[q] : nbmax;
+    if ((bc[2 * q] != 0 && bc[2 * q] != 1) || (bc[2 * q + 1] != 0 && bc[2 * q + 1] != 1) ||
+        ((bc[2 * q] == 0) != (bc[2 * q + 1] == 0)))
+      return rfail("ath_gpu_tree_build: bc flags are 0 (periodic, on both faces) or 1 (outflow)");
+    T.periodic[2 * q] = bc[2 * q] == 0;
+    T.periodic[2 * q + 1] = bc[2 * q + 1] == 0;
+  }
+  for (T.root_level = 0; (1LL << T.root_level) < nbmax; ++T.root_level) {}
+  for (int r = 0; r < nreg; ++r) {
+    const ath_refine_region &R = reg[r];
+    if (R.level < 1 || R.level + T.root_level > kTreeMaxLevel)
+      return rfail("ath_gpu_tree_build: region level must be in 1 .. " +
+                   std::to_string(kTreeMaxLevel - T.root_level));
+    for (int q = 0; q < 3; ++q) {
+      if (R.xmin[q] > R.xmax[q]) return rfail("ath_gpu_tree_build: invalid refinement region");
+      if (R.xmin[q] < xmin[q] || R.xmax[q] > xmax[q])
+        return rfail("ath_gpu_tree_build: refinement region must lie inside the mesh");
+    }
+  }
+  T.make(0, 0, 0, 0);
+  T.create_root(0);
+  for (int r = 0; r < nreg && !T.overflow; ++r) {
+    const ath_refine_region &R = reg[r];
+    long long lo[3], hi[3];
+    for (int q = 0; q < 3; ++q)
+      region_range(nrb[q], R.level, R.xmin[q], R.xmax[q], xmin[q], xmax[q], &lo[q], &hi[q]);
+    const int lrlev = R.level + T.root_level;
+    for (long long k = lo[2]; k < hi[2] && !T.overflow; k += 2)
+      for (long long j = lo[1]; j < hi[1] && !T.overflow; j += 2)
+        for (long long i = lo[0]; i < hi[0] && !T.overflow; i += 2) {
+          const long long l[3] = {i, j, k};
+          T.add(lrlev, l);
+        }
+  }
+  if (T.overflow)
+    return rfail("ath_gpu_tree_build: more than " + std::to_string(kTreeMaxNodes) + " tree nodes");
+  std::vector<int> lv;
+  T.leaves(0, lv);
+  *nblocks = static_cast<int>(lv.size());
+  if (root_level) *root_level = T.root_level;
+  if (static_cast<long>(lv.size()) > cap)
+    return rfail("ath_gpu_tree_build: " + std::to_string(lv.size()) + " MeshBlocks, cap " +
+                 std::to_string(cap));
+  std::vector<int> gid(T.nd.size(), -1);
+  for (size_t g = 0; g < lv.size(); ++g) gid[lv[g]] = static_cast<int>(g);
+  for (size_t g = 0; g < lv.size(); ++g) {
+    const TNode &t = T.nd[lv[g]];
+    if (locs) {
+      locs[4 * g] = t.level;
+      for (int q = 0; q < 3; ++q) locs[4 * g + 1 + q] = t.lx[q];
+    }
+    if (!nbr) continue;
+    for (int o3 = -1; o3 <= 1; ++o3)
+      for (int o2 = -1; o2 <= 1; ++o2)
+        for (int o1 = -1; o1 <= 1; ++o1) {
+          const int b = T.neighbor(lv[g], o1, o2, o3);
+          int v;
+          if (b == -3) return rfail("ath_gpu_tree_build: neighbour search failed");
+          if (b < 0) v = -1;
+          else if (!T.nd[b].leaf()) v = -2;
+          else v = gid[b];
+          nbr[27 * g + (o3 + 1) * 9 + (o2 + 1) * 3 + (o1 + 1)] = v;
+        }
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -38,12 +38,17 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history",
            "ath_gpu_ghost_position", "ath_gpu_restrict_cc", "ath_gpu_restrict_fc",
            "ath_gpu_prolongate_cc", "ath_gpu_prolongate_fc_shared",
-           "ath_gpu_prolongate_fc_internal", "ath_gpu_restrict_emf"]
+           "ath_gpu_prolongate_fc_internal", "ath_gpu_restrict_emf", "ath_gpu_tree_build"]
 NKERNELS = 19
 
 
 class AthGpuError(RuntimeError):
     pass
+
+
+class RefineRegionC(ctypes.Structure):
+    _fields_ = [("xmin", ctypes.c_double * 3), ("xmax", ctypes.c_double * 3),
+                ("level", ctypes.c_int)]
 
 
 class MeshDescC(ctypes.Structure):
@@ -131,9 +136,41 @@ def load():
     L.ath_gpu_prolongate_fc_shared.argtypes = [IP, I, I, P, P, P, P, IP, P]
     L.ath_gpu_prolongate_fc_internal.argtypes = [IP, I, P, P, P, P, IP, P]
     L.ath_gpu_restrict_emf.argtypes = [IP, I, I, P, P, P, IP, P]
+    L.ath_gpu_tree_build.argtypes = [IP, IP, D, D, ctypes.POINTER(RefineRegionC), I,
+                                     ctypes.POINTER(ctypes.c_longlong), IP, I, IP, IP]
     L.ath_gpu_history.argtypes = [P, D]
     _lib = L
     return L
+
+
+def tree_build(nrb, bc=(0,) * 6, xmin=(0.0,) * 3, xmax=(1.0,) * 3, regions=()):
+    """MeshBlock tree of a statically refined mesh (ath_gpu_tree_build; host only).
+    regions: (xmin[3], xmax[3], level) triples.  Returns (locs, nbr, root_level): locs an
+    (n, 4) int64 array of (level, lx1, lx2, lx3) in Z order, nbr an (n, 3, 3, 3) int32
+    array indexed [o3+1, o2+1, o1+1] (leaf id, -2 finer, -1 none)."""
+    L = load()
+    nrb_a = np.ascontiguousarray(nrb, dtype=np.int32)
+    bc_a = np.ascontiguousarray(bc, dtype=np.int32)
+    lo = np.ascontiguousarray(xmin, dtype=np.float64)
+    hi = np.ascontiguousarray(xmax, dtype=np.float64)
+    reg = (RefineRegionC * max(len(regions), 1))()
+    for r, (a, b, lev) in enumerate(regions):
+        reg[r].xmin[:] = [float(v) for v in a]
+        reg[r].xmax[:] = [float(v) for v in b]
+        reg[r].level = int(lev)
+    nb, rl = ctypes.c_int(0), ctypes.c_int(0)
+    # first call sizes the output (it fails only on cap when the arguments are valid)
+    rc = L.ath_gpu_tree_build(_ip(nrb_a), _ip(bc_a), _dp(lo), _dp(hi), reg, len(regions), None,
+                          None, 0, ctypes.byref(nb), ctypes.byref(rl))
+    if rc != 0 and nb.value == 0:
+        _check(rc)
+    n = nb.value
+    locs = np.zeros((n, 4), dtype=np.int64)
+    nbr = np.zeros((n, 27), dtype=np.int32)
+    _check(L.ath_gpu_tree_build(_ip(nrb_a), _ip(bc_a), _dp(lo), _dp(hi), reg, len(regions),
+                            locs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                            _ip(nbr), n, ctypes.byref(nb), ctypes.byref(rl)))
+    return locs, nbr.reshape(n, 3, 3, 3), rl.value
 
 
 def _check(rc):

@@ -219,6 +219,32 @@ int ath_gpu_prolongate_fc_internal(const int *shape, int nb, const double *fcoor
 int ath_gpu_restrict_emf(const int *shape, int nb, int comp, const double *fcoord,
                          const double *fine, double *coarse, const int *range, void *stream);
 
+/* ---- Static mesh refinement: the MeshBlock tree ------------------------------------
+ * A refinement region (<refinementN> x1min .. x3max, level; mesh.cpp:351-398): the box
+ * inside the mesh and its level relative to the root grid (>= 1). */
+typedef struct {
+  double xmin[3];
+  double xmax[3];
+  int level;
+} ath_refine_region;
+
+/* MeshBlockTree of a 3-D mesh of nrb[0] x nrb[1] x nrb[2] root MeshBlocks on
+ * [xmin, xmax] with static refinement: CreateRootGrid, then AddMeshBlock over every
+ * region's logical range (mesh.cpp:306,400-482; meshblock_tree.cpp:87-258), which refines
+ * neighbours so that adjacent MeshBlocks (faces, edges, corners; periodic faces wrap)
+ * differ by at most one level.  bc as ath_mesh_desc.bc.  Host only, no GPU needed.
+ * Writes *nblocks, *root_level (the logical level of the root grid) and, when cap is at
+ * least *nblocks: locs[4 g .. 4 g + 3] = (level, lx1, lx2, lx3) of MeshBlock g in the
+ * reference's Z order (GetMeshBlockList, meshblock_tree.cpp:336-356), and, if nbr is not
+ * null, nbr[27 g + (o3+1) 9 + (o2+1) 3 + (o1+1)] for each offset o in {-1,0,1}^3: the
+ * MeshBlock at that side on the same or the next coarser level (FindNeighbor,
+ * meshblock_tree.cpp:362-468), -2 when that side is refined further (finer
+ * neighbours), -1 across a non-periodic face.  locs or nbr may be null.  Fails when a
+ * region is empty, outside the mesh or deeper than 40 levels, or cap < *nblocks. */
+int ath_gpu_tree_build(const int *nrb, const int *bc, const double *xmin, const double *xmax,
+                   const ath_refine_region *reg, int nreg, long long *locs, int *nbr,
+                   int cap, int *nblocks, int *root_level);
+
 /* ---- Learned initial guess for SR primitive recovery --------------------------------
  * No counterpart in the reference: its SR recovery (adiabatic_mhd_sr.cpp:404-522) seeds
  * the Newman-Hamlin iteration with the previous gas pressure.  Here a 6 -> 64 -> 64 -> 1
