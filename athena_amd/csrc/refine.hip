@@ -531,16 +531,30 @@ double tree_face_x(long long index, long long nrange, double xmin, double xmax) 
   return 0.5 * (xmin + xmax) + (x * xmax - x * xmin);
 }
 
+// First v in [lo, hi) with pred(v) true, or hi; pred must be false then true along v.
+template <class Pred>
+long long first_true(long long lo, long long hi, Pred pred) {
+  while (lo < hi) {
+    const long long mid = lo + (hi - lo) / 2;
+    if (pred(mid)) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
 // The logical index range [lo, hi) at ref_lev that a refinement region covers along one
-// direction, widened to whole sibling pairs (mesh.cpp:400-447).
+// direction, widened to whole sibling pairs (mesh.cpp:400-447).  The reference scans the
+// indices in order for the first right face beyond rmin and the first at or beyond rmax;
+// face positions grow with the index, so a bisection finds the same indices in
+// O(ref_lev) steps for deep levels.
 void region_range(long long nrb, int ref_lev, double rmin, double rmax, double xmin,
                   double xmax, long long *lo, long long *hi) {
   const long long lxmax = nrb * (1LL << ref_lev);
-  long long a = 0, b = 0;
-  for (a = 0; a < lxmax; ++a)
-    if (tree_face_x(a + 1, lxmax, xmin, xmax) > rmin) break;
-  for (b = a; b < lxmax; ++b)
-    if (tree_face_x(b + 1, lxmax, xmin, xmax) >= rmax) break;
+  long long a = first_true(0, lxmax, [&](long long v) {
+    return tree_face_x(v + 1, lxmax, xmin, xmax) > rmin;
+  });
+  long long b = first_true(a, lxmax, [&](long long v) {
+    return tree_face_x(v + 1, lxmax, xmin, xmax) >= rmax;
+  });
   if (a % 2 == 1) --a;
   if (b % 2 == 0) ++b;
   *lo = a; *hi = b;

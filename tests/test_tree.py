@@ -132,3 +132,23 @@ def test_region_on_block_edges():
 def test_rejects_bad_input(kw, msg):
     with pytest.raises(AthGpuError, match=msg):
         tree_build((2, 2, 2), **kw)
+
+
+def test_deep_region_matches_an_ordered_scan():
+    # the region's logical range (mesh.cpp:400-416, an ordered scan in the reference) at
+    # level 18: the deepest leaves are the sibling-widened cells containing the region
+    from athena_amd.restart import root_level as rlev
+    nrb, lev = (2, 2, 2), 18
+    a, b = (0.3, 0.41, 0.77), (0.3000001, 0.4100001, 0.7700001)
+    locs, nbr, rl = tree_build(nrb, regions=[(a, b, lev)])
+    assert rl == rlev(nrb)
+    deep = locs[locs[:, 0] == rl + lev]
+    n = nrb[0] << lev
+    for q in range(3):
+        faces = 0.5 + (np.arange(1, n + 1) / n - 0.5)          # right faces, [0, 1] mesh
+        lo = int(np.argmax(faces > a[q]))
+        hi = int(np.argmax(faces >= b[q]))
+        lo -= lo % 2
+        hi += 1 - hi % 2            # odd: the pair (hi - 1, hi) is the last one added
+        assert deep[:, 1 + q].min() == lo and deep[:, 1 + q].max() == hi, q
+    assert len(deep) % 8 == 0
