@@ -3,10 +3,10 @@
 //
 // Data layout in HBM: one pool per register, blocks back to back, each block an
 // AthenaArray ((n,k,j,i), i fastest).  Kernels take the whole mesh in one launch
-// (grid.y = block id), so a VL2 stage is five launches regardless of block count:
-//   flux x1/x2/x3 (reconstruct + Riemann) -> corner EMF -> update (u and b, with the
-//   same-level EMF averaging folded in and the ghost exchange done as a push from the
-//   owning cell) -> primitive recovery (+ CFL reduction on the last stage).
+// (work-group id -> (MeshBlock, chunk)), so a VL2 stage is five launches regardless of
+// block count: fluxes x1/x2/x3 (reconstruct + Riemann) -> corner EMFs -> update (u and
+// b, with the same-level EMF averaging folded in) -> ghost exchange (a gather from the
+// neighbours' active cells) -> primitive recovery (+ CFL reduction on the last stage).
 // The mesh clock (time, dt) lives in device memory, so cycles run without host
 // synchronisation.
 #include <hip/hip_runtime.h>

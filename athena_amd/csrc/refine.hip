@@ -404,7 +404,8 @@ struct TNode {
   }
 };
 
-constexpr long kTreeMaxNodes = 1L << 24;
+// About 64 bytes per node: the cap bounds the host vector near 256 MiB.
+constexpr long kTreeMaxNodes = 1L << 22;
 constexpr int kTreeMaxLevel = 40;
 
 struct Tree {
@@ -733,6 +734,23 @@ int ath_gpu_tree_build(const int *nrb, const int *bc, const double *xmin, const 
       if (R.xmin[q] < xmin[q] || R.xmax[q] > xmax[q])
         return rfail("ath_gpu_tree_build: refinement region must lie inside the mesh");
     }
+  }
+  {   // the MeshBlocks the regions alone ask for, before any node is made
+    double want = static_cast<double>(nrb[0]) * nrb[1] * nrb[2];
+    for (int r = 0; r < nreg; ++r) {
+      long long lo[3], hi[3];
+      double v = 1.0;
+      for (int q = 0; q < 3; ++q) {
+        region_range(nrb[q], reg[r].level, reg[r].xmin[q], reg[r].xmax[q], xmin[q], xmax[q],
+                     &lo[q], &hi[q]);
+        v *= static_cast<double>(hi[q] - lo[q]);
+      }
+      want += v;
+    }
+    if (want > static_cast<double>(kTreeMaxNodes))
+      return rfail("ath_gpu_tree_build: the regions need about " +
+                   std::to_string(static_cast<long long>(want)) + " MeshBlocks, more than " +
+                   std::to_string(kTreeMaxNodes) + " tree nodes");
   }
   T.make(0, 0, 0, 0);
   T.create_root(0);

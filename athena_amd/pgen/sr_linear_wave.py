@@ -97,7 +97,19 @@ def _quartic(p, gamma):
 
 def eigen(wave_flag, p=BACKGROUND, gamma=GAMMA):
     """(lambda, delta_rho, delta_pgas, delta_u[4], delta_b[4]) of gr_linear_wave.cpp:
-    116-335 (MHD branch)."""
+    116-335 (MHD branch), scaled to unit L2 norm over (delta_rho, delta_pgas,
+    delta_u[0..3], delta_b[0..3]) as gr_linear_wave.cpp:400-411 does before the
+    amplitude is applied."""
+    lam, drho, dp, du, db = _eigen_raw(wave_flag, p, gamma)
+    size = drho * drho + dp * dp
+    for mu in range(4):
+        size += du[mu] * du[mu] + db[mu] * db[mu]
+    size = math.sqrt(size)
+    return (lam, drho / size, dp / size, [x / size for x in du], [x / size for x in db])
+
+
+def _eigen_raw(wave_flag, p=BACKGROUND, gamma=GAMMA):
+    """The un-normalised eigenvector of gr_linear_wave.cpp:116-335 (MHD branch)."""
     if not 0 <= wave_flag <= 6:
         raise ValueError("wave_flag must be 0..6")
     u, b = _background(p, gamma)

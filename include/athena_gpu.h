@@ -5,7 +5,8 @@
  * and corner EMFs, CT weights) resident in HBM in the reference's AthenaArray layout
  * (athena_arrays.hpp: (n,k,j,i), i fastest, one contiguous array per block), so a host
  * AthenaArray can be copied in or out with one memcpy per register.  Blocks of one
- * mesh are numbered row-major, x1 fastest.  All boundaries are periodic.
+ * mesh are numbered row-major, x1 fastest.  Each pair of opposite mesh faces is either
+ * periodic or outflow (ath_mesh_desc.bc).
  *
  * Every function returns 0 on success and a non-zero status otherwise; the message is
  * available from ath_gpu_last_error().  A non-zero status must be mapped to the
@@ -240,7 +241,9 @@ typedef struct {
  * MeshBlock at that side on the same or the next coarser level (FindNeighbor,
  * meshblock_tree.cpp:362-468), -2 when that side is refined further (finer
  * neighbours), -1 across a non-periodic face.  locs or nbr may be null.  Fails when a
- * region is empty, outside the mesh or deeper than 40 levels, or cap < *nblocks. */
+ * region is inside-out or outside the mesh, deeper than 40 levels or would need more than
+ * 2^22 tree nodes, or cap < *nblocks (a region of zero width is accepted, as the
+ * reference accepts it). */
 int ath_gpu_tree_build(const int *nrb, const int *bc, const double *xmin, const double *xmax,
                    const ath_refine_region *reg, int nreg, long long *locs, int *nbr,
                    int cap, int *nblocks, int *root_level);

@@ -6,13 +6,15 @@ tst/regression/scripts/tests/sr/mhd_convergence.py (7 waves of another backgroun
 periodic transverse cells and cfl 0.5 (the 3-D VL2 limit, time_integrator.cpp:219-222;
 the test's input file has 0.8, legal only in 1-D).
 
-The test's convergence criterion (error ratio <= 0.4 from 16 to 32 cells) holds for
-every wave.  Its absolute bounds at 32 cells (2-4e-8) are not met: the errors here are
-1.7-3x larger, identical on the oracle, dominated by the PLM reconstruction (a 10x
-smaller CFL number leaves them unchanged, PPM cuts them 3x) although the initial states
-are exact eigenvectors and the SR HLLD resolves isolated Alfven and contact jumps to
-1e-7.  That test is kept as a strict expected failure so the gap stays visible.
+Both of the test's criteria hold for every wave on the device: the error ratio from 16
+to 32 cells is <= 0.4 and the error at 32 cells is within the per-wave bound (2-4e-8).
+The device also prints, in the same number of cycles, the errors a build of the
+reference prints for the left-going fast, right-going Alfven and entropy waves
+(tests/golden/sr_linear_wave_errors.json).
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -23,6 +25,9 @@ from harness import compare
 from oracle_bind import OracleMesh
 
 pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "sr_linear_wave_errors.json")))
 
 
 def device_rms(wave_flag, res):
@@ -55,11 +60,18 @@ def test_sr_linwave_convergence(errors):
         assert ratio <= ERROR_RATIO, (w, ratio)
 
 
-@pytest.mark.xfail(strict=True, reason="PLM errors 1.7-3x the reference test's absolute "
-                                       "bounds, on the oracle as on the device")
 def test_sr_linwave_reference_bounds(errors):
     for w in range(7):
         assert errors[w, 32][0] <= HIGH_RES_ERRORS[w], (NAMES[w], errors[w, 32][0])
+
+
+def test_sr_linwave_reference_digits(errors):
+    for c in GOLD["cases"]:
+        rms, n = errors[c["wave_flag"], c["res"]]
+        print("%-18s %d: rms %e cycles %d (reference %e, %d)"
+              % (c["name"], c["res"], rms, n, c["rms_l1"], c["cycles"]))
+        assert n == c["cycles"], c
+        assert abs(rms - c["rms_l1"]) <= 1e-6 * c["rms_l1"], (rms, c)
 
 
 @pytest.mark.parametrize("wave_flag", [5, 2])

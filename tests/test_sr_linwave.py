@@ -8,9 +8,17 @@ tst/regression/scripts/tests/sr/sr_mhd_linwave.py).
 * The speeds agree with the test's own calculator and come in the order
   fast < Alfven < slow < entropy < slow < Alfven < fast of Anton et al.
 * The oracle run converges at the rate the test demands (ratio <= 0.4 from 16 to 32
-  cells) for the right-going fast and left-going Alfven waves; all 7 run on the device
-  (tests/test_gpu_sr_linwave.py).
+  cells) and meets its absolute bound at 32 cells for the right-going fast and
+  left-going Alfven waves; all 7 run on the device (tests/test_gpu_sr_linwave.py).
+* The oracle reproduces, to the 7 printed digits and in the same number of cycles, the
+  RMS errors that a build of the reference prints for the left-going fast,
+  right-going Alfven and entropy waves (tests/golden/sr_linear_wave_errors.json).  The
+  eigenvector is scaled to unit L2 norm as the reference does before the amplitude is
+  applied; without that scaling the fast wave's error is 3.0x the reference's.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -18,6 +26,9 @@ from athena_amd.pgen.sr_linear_wave import (BACKGROUND, ERROR_RATIO, GAMMA, SRLi
                                             _background, eigen, sr_linear_wave_mesh,
                                             wavespeed)
 from oracle_bind import OracleMesh
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "sr_linear_wave_errors.json")))
 
 
 def _conserved_and_flux(eps, wave_flag, tweak=0.0):
@@ -68,18 +79,37 @@ def test_wavespeeds():
         assert abs(eigen(w)[0] - speeds[w]) <= 1e-12 * abs(speeds[w]), w
 
 
+def oracle_rms(wave_flag, res):
+    m = sr_linear_wave_mesh(res)
+    lw = SRLinearWave(m, wave_flag)
+    o = OracleMesh(m)
+    o.load([lw.block(g) for g in range(m.nblocks)])
+    tl = lw.tlim()
+    o.initialize(tl)
+    n = o.run(1 << 30, tl)
+    return lw.errors(lambda g, k: o.get(g, k))[0], n
+
+
+def test_eigenvector_unit_norm():
+    for w in range(7):
+        lam, drho, dp, du, db = eigen(w)
+        size = drho ** 2 + dp ** 2 + sum(x * x for x in du) + sum(x * x for x in db)
+        assert abs(size - 1.0) <= 1e-14, (w, size)
+
+
 @pytest.mark.parametrize("wave_flag", [6, 1])
 def test_oracle_converges(wave_flag):
-    rms = {}
-    for res in (16, 32):
-        m = sr_linear_wave_mesh(res)
-        lw = SRLinearWave(m, wave_flag)
-        o = OracleMesh(m)
-        o.load([lw.block(g) for g in range(m.nblocks)])
-        tl = lw.tlim()
-        o.initialize(tl)
-        o.run(1 << 30, tl)
-        rms[res] = lw.errors(lambda g, k: o.get(g, k))[0]
+    rms = {res: oracle_rms(wave_flag, res)[0] for res in (16, 32)}
     print("wave %d: rms %e (16) %e (32), ratio %.3f"
           % (wave_flag, rms[16], rms[32], rms[32] / rms[16]))
     assert rms[32] / rms[16] <= ERROR_RATIO
+    assert rms[32] <= GOLD["high_res_bounds"][wave_flag]
+
+
+@pytest.mark.parametrize("case", [c for c in GOLD["cases"]
+                                  if c["res"] == 16 or c["wave_flag"] != 3],
+                         ids=lambda c: "%d-%d" % (c["wave_flag"], c["res"]))
+def test_oracle_matches_reference_sr_errors(case):
+    rms, n = oracle_rms(case["wave_flag"], case["res"])
+    assert n == case["cycles"]
+    assert float("%e" % rms) == case["rms_l1"], (rms, case)

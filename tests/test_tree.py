@@ -152,3 +152,16 @@ def test_deep_region_matches_an_ordered_scan():
         hi += 1 - hi % 2            # odd: the pair (hi - 1, hi) is the last one added
         assert deep[:, 1 + q].min() == lo and deep[:, 1 + q].max() == hi, q
     assert len(deep) % 8 == 0
+
+
+def test_oversized_region_fails_before_building():
+    # level 9 over the whole of an 8^3 root grid asks for 2^36 blocks: refused from the
+    # region ranges alone, without growing the node vector first
+    import time
+    t0 = time.time()
+    with pytest.raises(AthGpuError, match="tree nodes"):
+        tree_build((8, 8, 8), regions=[((0.0,) * 3, (1.0,) * 3, 9)])
+    assert time.time() - t0 < 1.0
+    # a zero-width region is accepted, as the reference accepts it
+    locs, _, _ = tree_build((4, 4, 4), regions=[((0.3, 0.3, 0.3), (0.3, 0.3, 0.3), 1)])
+    assert len(locs) >= 64
