@@ -1086,105 +1086,112 @@ struct C2PArgs {
   const double *lseed;        // learned pressure seeds (C2P_LEARNED)
 };
 
-// Every thread of the work-group runs the whole body (no early return): the dt
-// reduction at its end synchronises the work-group.  `slot` is the work-group's entry
-// in dt_partial.
+// ConservedToPrimitive of one cell (k, j, i) of block g: bcc from the faces, the
+// primitives into prim (the conserved variables rewritten where a floor applies) and,
+// with with_dt, the cell's Hydro::NewBlockTimeStep minimum into *my_dt.
 template <bool SR, bool LEARNED>
-__device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int lw, int nwg,
-                                         int slot, TileK T) {
+__device__ __forceinline__ void c2p_cell(const MeshK &M, const C2PArgs &a, int g, int k, int j,
+                                         int i, bool with_dt, double *my_dt, C2PStats *st) {
   double *__restrict__ u = a.u;
   double *__restrict__ prim = a.prim;
   double *__restrict__ bcc = a.bcc;
   const double *__restrict__ b1 = a.b1;
   const double *__restrict__ b2 = a.b2;
   const double *__restrict__ b3 = a.b3;
-  double *__restrict__ dt_partial = a.dt_partial;
   const double *__restrict__ lseed = a.lseed;
+  const long n = M.ncell;
+  const long c = cidx(M, k, j, i);
+  const double *B1 = b1 + static_cast<long>(g) * M.nf1;
+  const double *B2 = b2 + static_cast<long>(g) * M.nf2;
+  const double *B3 = b3 + static_cast<long>(g) * M.nf3;
+  const double b1l = B1[f1idx(M, k, j, i)], b1r = B1[f1idx(M, k, j, i + 1)];
+  const double b2l = B2[f2idx(M, k, j, i)], b2r = B2[f2idx(M, k, j + 1, i)];
+  const double b3l = B3[f3idx(M, k, j, i)], b3r = B3[f3idx(M, k + 1, j, i)];
+  double bc[3];
+  bc[IB1] = 0.5 * b1l + 0.5 * b1r;
+  bc[IB2] = 0.5 * b2l + 0.5 * b2r;
+  bc[IB3] = 0.5 * b3l + 0.5 * b3r;
+  double *bb = bcc + static_cast<long>(g) * 3 * n;
+  bb[IB1 * n + c] = bc[IB1];
+  bb[IB2 * n + c] = bc[IB2];
+  bb[IB3 * n + c] = bc[IB3];
+  double *ub = u + static_cast<long>(g) * NHYDRO * n;
+  double *pb = prim + static_cast<long>(g) * NHYDRO * n;
+  double w[NHYDRO];
+  if (!SR) {
+    // adiabatic_mhd.cpp:41-90
+    const double gm1 = M.eos.gamma - 1.0;
+    double u_d = ub[IDN * n + c];
+    const double u_m1 = ub[IM1 * n + c], u_m2 = ub[IM2 * n + c], u_m3 = ub[IM3 * n + c];
+    double u_e = ub[IEN * n + c];
+    const double ud_in = u_d, ue_in = u_e;
+    u_d = (u_d > M.eos.dfloor) ? u_d : M.eos.dfloor;
+    w[IDN] = u_d;
+    double di = 1.0 / u_d;
+    w[IVX] = u_m1 * di;
+    w[IVY] = u_m2 * di;
+    w[IVZ] = u_m3 * di;
+    double pb_ = 0.5 * (sq(bc[IB1]) + sq(bc[IB2]) + sq(bc[IB3]));
+    double e_k = 0.5 * di * (sq(u_m1) + sq(u_m2) + sq(u_m3));
+    double w_p = gm1 * (u_e - e_k - pb_);
+    u_e = (w_p > M.eos.pfloor) ? u_e : ((M.eos.pfloor / gm1) + e_k + pb_);
+    w_p = (w_p > M.eos.pfloor) ? w_p : M.eos.pfloor;
+    w[IPR] = w_p;
+    if (u_d != ud_in) ub[IDN * n + c] = u_d;
+    if (u_e != ue_in) ub[IEN * n + c] = u_e;
+  } else {
+    double cons[NHYDRO];
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q) { cons[q] = ub[q * n + c]; w[q] = pb[q * n + c]; }
+    double cin[NHYDRO];
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q) cin[q] = cons[q];
+    sr_cons2prim_cell(cons, w, bc, M.eos, LEARNED ? C2P_LEARNED : M.c2p_mode,
+                      LEARNED ? lseed[static_cast<long>(g) * n + c] : 0.0, st);
+#pragma unroll
+    for (int q = 0; q < NHYDRO; ++q)
+      if (cons[q] != cin[q]) ub[q * n + c] = cons[q];
+  }
+#pragma unroll
+  for (int q = 0; q < NHYDRO; ++q) pb[q * n + c] = w[q];
+
+  if (with_dt && i >= M.is && i <= M.ie && j >= M.js && j <= M.je && k >= M.ks &&
+      k <= M.ke) {
+    const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
+    double dt1 = dx1[i], dt2 = dx2[j], dt3 = dx3[k];
+    if (!SR) {
+      double wi[NWAVE];
+      wi[IDN] = w[IDN]; wi[IVX] = w[IVX]; wi[IVY] = w[IVY]; wi[IVZ] = w[IVZ];
+      wi[IPR] = w[IPR];
+      double bx = bc[IB1] + fabs(b1l - bc[IB1]);
+      wi[IBY] = bc[IB2]; wi[IBZ] = bc[IB3];
+      double cf = fast_speed(wi, bx, M.eos.gamma);
+      dt1 /= dmax(0.0, (fabs(wi[IVX]) + cf));
+      wi[IBY] = bc[IB3]; wi[IBZ] = bc[IB1];
+      bx = bc[IB2] + fabs(b2l - bc[IB2]);
+      cf = fast_speed(wi, bx, M.eos.gamma);
+      dt2 /= dmax(0.0, (fabs(wi[IVY]) + cf));
+      wi[IBY] = bc[IB1]; wi[IBZ] = bc[IB2];
+      bx = bc[IB3] + fabs(b3l - bc[IB3]);
+      cf = fast_speed(wi, bx, M.eos.gamma);
+      dt3 /= dmax(0.0, (fabs(wi[IVZ]) + cf));
+    }
+    *my_dt = dmin(dmin(dt1, dt2), dt3);
+  }
+}
+
+// Every thread of the work-group runs the whole body (no early return): the dt
+// reduction at its end synchronises the work-group.  `slot` is the work-group's entry
+// in dt_partial.
+template <bool SR, bool LEARNED>
+__device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int lw, int nwg,
+                                         int slot, TileK T) {
+  double *__restrict__ dt_partial = a.dt_partial;
   int g, i, j, k;
   const bool live = map_kind(M, KK_C2P, lw, nwg, T, g, i, j, k);
   double my_dt = DBL_MAX;
   C2PStats st{0, 0};
-  if (live) {
-    const long n = M.ncell;
-    const long c = cidx(M, k, j, i);
-    const double *B1 = b1 + static_cast<long>(g) * M.nf1;
-    const double *B2 = b2 + static_cast<long>(g) * M.nf2;
-    const double *B3 = b3 + static_cast<long>(g) * M.nf3;
-    const double b1l = B1[f1idx(M, k, j, i)], b1r = B1[f1idx(M, k, j, i + 1)];
-    const double b2l = B2[f2idx(M, k, j, i)], b2r = B2[f2idx(M, k, j + 1, i)];
-    const double b3l = B3[f3idx(M, k, j, i)], b3r = B3[f3idx(M, k + 1, j, i)];
-    double bc[3];
-    bc[IB1] = 0.5 * b1l + 0.5 * b1r;
-    bc[IB2] = 0.5 * b2l + 0.5 * b2r;
-    bc[IB3] = 0.5 * b3l + 0.5 * b3r;
-    double *bb = bcc + static_cast<long>(g) * 3 * n;
-    bb[IB1 * n + c] = bc[IB1];
-    bb[IB2 * n + c] = bc[IB2];
-    bb[IB3 * n + c] = bc[IB3];
-    double *ub = u + static_cast<long>(g) * NHYDRO * n;
-    double *pb = prim + static_cast<long>(g) * NHYDRO * n;
-    double w[NHYDRO];
-    if (!SR) {
-      // adiabatic_mhd.cpp:41-90
-      const double gm1 = M.eos.gamma - 1.0;
-      double u_d = ub[IDN * n + c];
-      const double u_m1 = ub[IM1 * n + c], u_m2 = ub[IM2 * n + c], u_m3 = ub[IM3 * n + c];
-      double u_e = ub[IEN * n + c];
-      const double ud_in = u_d, ue_in = u_e;
-      u_d = (u_d > M.eos.dfloor) ? u_d : M.eos.dfloor;
-      w[IDN] = u_d;
-      double di = 1.0 / u_d;
-      w[IVX] = u_m1 * di;
-      w[IVY] = u_m2 * di;
-      w[IVZ] = u_m3 * di;
-      double pb_ = 0.5 * (sq(bc[IB1]) + sq(bc[IB2]) + sq(bc[IB3]));
-      double e_k = 0.5 * di * (sq(u_m1) + sq(u_m2) + sq(u_m3));
-      double w_p = gm1 * (u_e - e_k - pb_);
-      u_e = (w_p > M.eos.pfloor) ? u_e : ((M.eos.pfloor / gm1) + e_k + pb_);
-      w_p = (w_p > M.eos.pfloor) ? w_p : M.eos.pfloor;
-      w[IPR] = w_p;
-      if (u_d != ud_in) ub[IDN * n + c] = u_d;
-      if (u_e != ue_in) ub[IEN * n + c] = u_e;
-    } else {
-      double cons[NHYDRO];
-#pragma unroll
-      for (int q = 0; q < NHYDRO; ++q) { cons[q] = ub[q * n + c]; w[q] = pb[q * n + c]; }
-      double cin[NHYDRO];
-#pragma unroll
-      for (int q = 0; q < NHYDRO; ++q) cin[q] = cons[q];
-      sr_cons2prim_cell(cons, w, bc, M.eos, LEARNED ? C2P_LEARNED : M.c2p_mode,
-                        LEARNED ? lseed[static_cast<long>(g) * n + c] : 0.0, &st);
-#pragma unroll
-      for (int q = 0; q < NHYDRO; ++q)
-        if (cons[q] != cin[q]) ub[q * n + c] = cons[q];
-    }
-#pragma unroll
-    for (int q = 0; q < NHYDRO; ++q) pb[q * n + c] = w[q];
-
-    if (dt_partial && i >= M.is && i <= M.ie && j >= M.js && j <= M.je && k >= M.ks &&
-        k <= M.ke) {
-      const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
-      double dt1 = dx1[i], dt2 = dx2[j], dt3 = dx3[k];
-      if (!SR) {
-        double wi[NWAVE];
-        wi[IDN] = w[IDN]; wi[IVX] = w[IVX]; wi[IVY] = w[IVY]; wi[IVZ] = w[IVZ];
-        wi[IPR] = w[IPR];
-        double bx = bc[IB1] + fabs(b1l - bc[IB1]);
-        wi[IBY] = bc[IB2]; wi[IBZ] = bc[IB3];
-        double cf = fast_speed(wi, bx, M.eos.gamma);
-        dt1 /= dmax(0.0, (fabs(wi[IVX]) + cf));
-        wi[IBY] = bc[IB3]; wi[IBZ] = bc[IB1];
-        bx = bc[IB2] + fabs(b2l - bc[IB2]);
-        cf = fast_speed(wi, bx, M.eos.gamma);
-        dt2 /= dmax(0.0, (fabs(wi[IVY]) + cf));
-        wi[IBY] = bc[IB1]; wi[IBZ] = bc[IB2];
-        bx = bc[IB3] + fabs(b3l - bc[IB3]);
-        cf = fast_speed(wi, bx, M.eos.gamma);
-        dt3 /= dmax(0.0, (fabs(wi[IVZ]) + cf));
-      }
-      my_dt = dmin(dmin(dt1, dt2), dt3);
-    }
-  }
+  if (live) c2p_cell<SR, LEARNED>(M, a, g, k, j, i, dt_partial != nullptr, &my_dt, &st);
   if (SR && M.c2p_stats) {   // wavefront sums, one atomic per wavefront
     unsigned long long ev = st.evals, fb = st.fallback, nc = live ? 1 : 0;
 #pragma unroll
@@ -1456,6 +1463,219 @@ __global__ void __launch_bounds__(1024) k_new_dt_ens(const EnsMember *__restrict
                                                       int advance) {
   const EnsMember &e = E[blockIdx.x];
   new_dt_body(e.pa.dt_partial, e.nwg[3], e.clk, e.M.cfl, advance);
+}
+
+//========================================================================================
+// Per-MeshBlock kernels over caller-owned arrays (ath_gpu_block_*): one MeshBlock whose
+// registers the caller keeps in device memory.  The MeshK describes that one block
+// (g = 0 everywhere) with no neighbours, so the EMF accessors of the update return the
+// caller's EMFs unchanged: the reference's CT and AddFluxDivergence see only their own
+// MeshBlock's arrays, the neighbour correction having been applied before by the caller.
+
+// Reconstruction::{DonorCell,PiecewiseLinear,PiecewiseParabolic}X{DIR+1} on pencil
+// (k, j) for cells il .. iu (dc.cpp:24-103, plm.cpp:27-463, ppm.cpp:44-923): each cell's
+// right-edge value goes to the left state of the face above it and its left-edge value
+// to the right state of its own face.  X1 writes wl(n, i+1) and wr(n, i); X2 and X3 write
+// wl(n, i) (the face above cell j or k) and wr(n, i) (its own face).  Pencils (NWAVE,
+// nc1).  The transverse field components are (IB2, IB3) along x1, (IB3, IB1) along x2
+// and (IB1, IB2) along x3.
+template <int DIR, int ORDER>
+__global__ void __launch_bounds__(WG) k_blk_recon(MeshK M, int k, int j, int il, int iu,
+                                                  const double *__restrict__ w,
+                                                  const double *__restrict__ bcc,
+                                                  double *__restrict__ wl,
+                                                  double *__restrict__ wr) {
+  const int i = il + static_cast<int>(blockIdx.x * WG + threadIdx.x);
+  if (i > iu) return;
+  const long s = (DIR == 0) ? 1 : ((DIR == 1) ? M.nc1 : static_cast<long>(M.nc1) * M.nc2);
+  const long c = cidx(M, k, j, i);
+  constexpr int cby = (DIR == 0) ? IB2 : ((DIR == 1) ? IB3 : IB1);
+  constexpr int cbz = (DIR == 0) ? IB3 : ((DIR == 1) ? IB1 : IB2);
+  const int fc = (DIR == 0) ? i : ((DIR == 1) ? j : k);
+  const double *ct = ctab(M, 0, DIR);
+  const int ncd = (DIR == 0) ? M.nc1 : ((DIR == 1) ? M.nc2 : M.nc3);
+  const int il_out = (DIR == 0) ? i + 1 : i;     // pencil index of the left state
+  for (int v = 0; v < NWAVE; ++v) {
+    const double *src;
+    if (v < NHYDRO) src = w + v * M.ncell;
+    else if (v == IBY) src = bcc + cby * M.ncell;
+    else src = bcc + cbz * M.ncell;
+    double lo, hi;   // left-edge and right-edge values of the cell
+    if (ORDER == 1) {
+      lo = hi = src[c];
+    } else if (ORDER == 2) {
+      const double q0 = src[c];
+      const double dwm = vl_slope(src[c - s], q0, src[c + s]);
+      hi = q0 + ct[ncd + fc] * dwm;
+      lo = q0 - ct[2 * ncd + fc] * dwm;
+    } else {
+      double q[5];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) q[m] = src[c + (m - 2) * s];
+      ppm_cell(q, lo, hi);
+      if (v == IDN) {
+        lo = (lo > M.eos.dfloor) ? lo : M.eos.dfloor;
+        hi = (hi > M.eos.dfloor) ? hi : M.eos.dfloor;
+      } else if (v == IPR) {
+        lo = (lo > M.eos.pfloor) ? lo : M.eos.pfloor;
+        hi = (hi > M.eos.pfloor) ? hi : M.eos.pfloor;
+      }
+    }
+    wl[v * M.nc1 + il_out] = hi;
+    wr[v * M.nc1 + i] = lo;
+  }
+}
+
+// Hydro::RiemannSolver(k, j, il, iu, ivx, bx, wl, wr, flx, ey, ez, wct, dxw) of hlld.cpp,
+// hlle_mhd.cpp, hlle_mhd_rel.cpp or hlld_rel.cpp (the solver loop and its epilogue):
+// states rotated by ivx, outputs at face (k, j, i) of the face arrays along ivx, the CT
+// weight from dxw(i) and dt (hydro.cpp:154-158).
+template <int DIR, int RS>
+__global__ void __launch_bounds__(WG) k_blk_riemann(MeshK M, int k, int j, int il, int iu,
+                                                    const double *__restrict__ bx,
+                                                    const double *__restrict__ wl,
+                                                    const double *__restrict__ wr,
+                                                    double *__restrict__ flx,
+                                                    double *__restrict__ ey,
+                                                    double *__restrict__ ez,
+                                                    double *__restrict__ wct,
+                                                    const double *__restrict__ dxw, double dt) {
+  const int i = il + static_cast<int>(blockIdx.x * WG + threadIdx.x);
+  if (i > iu) return;
+  constexpr int ivx = IVX + DIR;
+  constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  const int p = M.nc1;
+  double a[NWAVE], b[NWAVE], f[NWAVE];
+  a[IDN] = wl[IDN * p + i]; a[IVX] = wl[ivx * p + i]; a[IVY] = wl[ivy * p + i];
+  a[IVZ] = wl[ivz * p + i]; a[IPR] = wl[IPR * p + i]; a[IBY] = wl[IBY * p + i];
+  a[IBZ] = wl[IBZ * p + i];
+  b[IDN] = wr[IDN * p + i]; b[IVX] = wr[ivx * p + i]; b[IVY] = wr[ivy * p + i];
+  b[IVZ] = wr[ivz * p + i]; b[IPR] = wr[IPR * p + i]; b[IBY] = wr[IBY * p + i];
+  b[IBZ] = wr[IBZ * p + i];
+  long fi, nf;
+  if (DIR == 0) { fi = f1idx(M, k, j, i); nf = M.nf1; }
+  else if (DIR == 1) { fi = f2idx(M, k, j, i); nf = M.nf2; }
+  else { fi = f3idx(M, k, j, i); nf = M.nf3; }
+  const double bxi = bx[fi];
+  if (RS == 0) hlld(a, b, bxi, M.eos, f);
+  else if (RS == 1) hlle(a, b, bxi, M.eos, f);
+  else if (RS == 2) sr_hlle(a, b, bxi, M.eos, f);
+  else sr_hlld(a, b, bxi, M.eos, f);
+  flx[IDN * nf + fi] = f[IDN];
+  flx[ivx * nf + fi] = f[IVX];
+  flx[ivy * nf + fi] = f[IVY];
+  flx[ivz * nf + fi] = f[IVZ];
+  flx[IEN * nf + fi] = f[IEN];
+  ey[fi] = -f[IBY];
+  ez[fi] = f[IBZ];
+  wct[fi] = ct_weight(f[IDN], a[IDN], b[IDN], dxw[i], dt);
+}
+
+// Hydro::CalculateFluxes on one block with dt by value (the fused three-direction body).
+template <int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_blk_flux3(MeshK M, const double *__restrict__ w,
+                                                  const double *__restrict__ bcc, FaceOut o1,
+                                                  FaceOut o2, FaceOut o3, double dt) {
+  flux3_body<ORDER, RS>(M, w, bcc, o1, o2, o3, dt, blockIdx.x, gridDim.x, TileK{0, 0, 0});
+}
+
+// Field::CT (what & 2) and Hydro::AddFluxDivergence (what & 1) of one block, in place on
+// the caller's b_out / u_out.
+__global__ void __launch_bounds__(WG) k_blk_update(MeshK M, UpdateArgs a, double wght,
+                                                   int what) {
+  int g, i, j, k;
+  if (!map_kind(M, KK_UPDATE, blockIdx.x, gridDim.x, TileK{0, 0, 0}, g, i, j, k)) return;
+  const bool ci = i <= M.ie, cj = j <= M.je, ck = k <= M.ke;
+  if ((what & 1) && ci && cj && ck)
+    update_cell_u(M, g, k, j, i, a.uout, a.uout, a.f1, a.f2, a.f3, wght, 0);
+  if (what & 2) {
+    if (cj && ck)
+      update_face_b<0>(M, g, k, j, i, a.bout[0], a.bout[0], a.e1, a.e2, a.e3, wght, 0);
+    if (ci && ck)
+      update_face_b<1>(M, g, k, j, i, a.bout[1], a.bout[1], a.e1, a.e2, a.e3, wght, 0);
+    if (ci && cj)
+      update_face_b<2>(M, g, k, j, i, a.bout[2], a.bout[2], a.e1, a.e2, a.e3, wght, 0);
+  }
+}
+
+// EquationOfState::ConservedToPrimitive over the box lo[] .. lo[] + n[] - 1 of one block.
+template <bool SR>
+__global__ void __launch_bounds__(WG, C2P_MIN_WAVES) k_blk_c2p(MeshK M, C2PArgs a, int il,
+                                                               int jl, int kl, int ni, int nj,
+                                                               int nk) {
+  const int t = static_cast<int>(blockIdx.x * WG + threadIdx.x);
+  if (t >= ni * nj * nk) return;
+  const int i = il + t % ni, j = jl + (t / ni) % nj, k = kl + t / (ni * nj);
+  double dt = DBL_MAX;
+  c2p_cell<SR, false>(M, a, 0, k, j, i, false, &dt, nullptr);
+}
+
+// Hydro::NewBlockTimeStep (new_blockdt.cpp:45-230) of one block from its primitives,
+// cell-centred and face fields: per work-group minima of the active cells' time steps
+// into part[], then k_blk_dt_final takes their minimum and applies cfl_number.
+template <bool SR>
+__global__ void __launch_bounds__(WG) k_blk_dt(MeshK M, const double *__restrict__ w,
+                                               const double *__restrict__ bcc,
+                                               const double *__restrict__ b1,
+                                               const double *__restrict__ b2,
+                                               const double *__restrict__ b3,
+                                               double *__restrict__ part) {
+  const int t = static_cast<int>(blockIdx.x * WG + threadIdx.x);
+  double my = DBL_MAX;
+  if (t < M.nx1 * M.nx2 * M.nx3) {
+    const int i = M.is + t % M.nx1, j = M.js + (t / M.nx1) % M.nx2;
+    const int k = M.ks + t / (M.nx1 * M.nx2);
+    const double *dx1 = ctab(M, 0, 0), *dx2 = ctab(M, 0, 1), *dx3 = ctab(M, 0, 2);
+    double dt1 = dx1[i], dt2 = dx2[j], dt3 = dx3[k];
+    if (!SR) {
+      const long n = M.ncell, c = cidx(M, k, j, i);
+      const double bc1 = bcc[IB1 * n + c], bc2 = bcc[IB2 * n + c], bc3 = bcc[IB3 * n + c];
+      double wi[NWAVE];
+      wi[IDN] = w[IDN * n + c]; wi[IVX] = w[IVX * n + c]; wi[IVY] = w[IVY * n + c];
+      wi[IVZ] = w[IVZ * n + c]; wi[IPR] = w[IPR * n + c];
+      double bx = bc1 + fabs(b1[f1idx(M, k, j, i)] - bc1);
+      wi[IBY] = bc2; wi[IBZ] = bc3;
+      double cf = fast_speed(wi, bx, M.eos.gamma);
+      dt1 /= dmax(0.0, (fabs(wi[IVX]) + cf));
+      wi[IBY] = bc3; wi[IBZ] = bc1;
+      bx = bc2 + fabs(b2[f2idx(M, k, j, i)] - bc2);
+      cf = fast_speed(wi, bx, M.eos.gamma);
+      dt2 /= dmax(0.0, (fabs(wi[IVY]) + cf));
+      wi[IBY] = bc1; wi[IBZ] = bc2;
+      bx = bc3 + fabs(b3[f3idx(M, k, j, i)] - bc3);
+      cf = fast_speed(wi, bx, M.eos.gamma);
+      dt3 /= dmax(0.0, (fabs(wi[IVZ]) + cf));
+    }
+    my = dmin(dmin(dt1, dt2), dt3);
+  }
+  __shared__ double sm[WG / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) my = dmin(my, __shfl_down(my, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = my;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sm[0];
+    for (int q = 1; q < WG / 64; ++q) m = dmin(m, sm[q]);
+    part[blockIdx.x] = m;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_blk_dt_final(const double *__restrict__ part,
+                                                       int npart, double cfl,
+                                                       double *__restrict__ dt_out) {
+  __shared__ double sm[1024 / 64];
+  double v = DBL_MAX;
+  for (int q = threadIdx.x; q < npart; q += 1024) v = dmin(v, part[q]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sm[0];
+    for (int q = 1; q < 1024 / 64; ++q) m = dmin(m, sm[q]);
+    *dt_out = m * cfl;   // min_dt_hyperbolic *= cfl_number (new_blockdt.cpp:154)
+  }
 }
 
 // Batched single-interface Riemann solves (known-answer vectors)
@@ -2088,6 +2308,34 @@ int ens_check(ath_ensemble *e) {
   }
   return 0;
 }
+
+}  // namespace
+
+//========================================================================================
+// One MeshBlock over caller-owned arrays (ath_gpu_block_*): its geometry, coordinate
+// table and equation of state on the device, as a MeshBlock owns its Coordinates and
+// EquationOfState.
+
+struct ath_block {
+  MeshK M;
+  int rs = 0;              // solver_code
+  int relativistic = 0;
+  int *nbr_d = nullptr;
+  double *coord_d = nullptr;
+};
+
+namespace {
+
+int blk_check_stream(hipStream_t st) {
+  HIPCHK(hipGetLastError());
+  (void)st;
+  return 0;
+}
+
+bool in_box(int lo, int hi, int n) { return lo >= 0 && hi < n && lo <= hi; }
+
+// Threads of a pencil launch over il .. iu.
+dim3 pencil_grid(int il, int iu) { return dim3(static_cast<unsigned>((iu - il + WG) / WG)); }
 
 }  // namespace
 
@@ -2852,6 +3100,288 @@ int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
   (void)hipFree(d);
   (void)hipFree(di);
   return rc;
+}
+
+
+//----------------------------------------------------------------------------------------
+// Per-MeshBlock entries over caller-owned device arrays
+
+int ath_gpu_block_create(const ath_block_desc *d, ath_block_t *out) {
+  if (!d || !out) return fail("null argument");
+  *out = nullptr;
+  for (int q = 0; q < 3; ++q)
+    if (d->nx[q] < 2) return fail("block nx must be >= 2 in every direction (3-D only)");
+  if (d->ng < 2) return fail("nghost must be >= 2");
+  for (int q = 0; q < 3; ++q)
+    if (d->nx[q] < d->ng) return fail("block nx must be >= nghost");
+  if (d->rsolver < 0 || d->rsolver > 1) return fail("rsolver must be 0 (HLLD) or 1 (HLLE)");
+  if (!(d->gamma > 1.0)) return fail("gamma must exceed 1");
+  const double *xf[3] = {d->x1f, d->x2f, d->x3f}, *xv[3] = {d->x1v, d->x2v, d->x3v};
+  const double *dxf[3] = {d->dx1f, d->dx2f, d->dx3f};
+  for (int q = 0; q < 3; ++q)
+    if (!xf[q] || !xv[q] || !dxf[q]) return fail("null coordinate array");
+  ath_block *b = new ath_block;
+  MeshK &M = b->M;
+  M = MeshK{};
+  M.nx1 = d->nx[0]; M.nx2 = d->nx[1]; M.nx3 = d->nx[2]; M.ng = d->ng;
+  M.nc1 = M.nx1 + 2 * M.ng; M.nc2 = M.nx2 + 2 * M.ng; M.nc3 = M.nx3 + 2 * M.ng;
+  M.is = M.ng; M.ie = M.ng + M.nx1 - 1;
+  M.js = M.ng; M.je = M.ng + M.nx2 - 1;
+  M.ks = M.ng; M.ke = M.ng + M.nx3 - 1;
+  M.nblocks = 1;
+  M.ncell = static_cast<long>(M.nc1) * M.nc2 * M.nc3;
+  M.nf1 = static_cast<long>(M.nc3) * M.nc2 * (M.nc1 + 1);
+  M.nf2 = static_cast<long>(M.nc3) * (M.nc2 + 1) * M.nc1;
+  M.nf3 = static_cast<long>(M.nc3 + 1) * M.nc2 * M.nc1;
+  M.ne1 = static_cast<long>(M.nc3 + 1) * (M.nc2 + 1) * M.nc1;
+  M.ne2 = static_cast<long>(M.nc3 + 1) * M.nc2 * (M.nc1 + 1);
+  M.ne3 = static_cast<long>(M.nc3) * (M.nc2 + 1) * (M.nc1 + 1);
+  M.coff[0] = 0;
+  M.coff[1] = 3 * M.nc1;
+  M.coff[2] = 3 * (M.nc1 + M.nc2);
+  M.cstride = 3 * (M.nc1 + M.nc2 + M.nc3);
+  const double fmin = static_cast<double>(std::numeric_limits<float>::min());
+  M.eos.gamma = d->gamma;
+  M.eos.igm1 = 1.0 / (d->gamma - 1.0);
+  M.eos.dfloor = (d->dfloor < 0.0) ? std::sqrt(1024 * fmin) : d->dfloor;
+  M.eos.pfloor = (d->pfloor < 0.0) ? std::sqrt(1024 * fmin) : d->pfloor;
+  M.eos.sigma_max = (d->sigma_max < 0.0) ? 0.0 : d->sigma_max;
+  M.eos.beta_min = (d->beta_min < 0.0) ? 0.0 : d->beta_min;
+  M.eos.gamma_max = (d->gamma_max < 0.0) ? 1000.0 : d->gamma_max;
+  M.cfl = d->cfl;
+  M.outflow = 1;          // no neighbour on any side: the EMF accessors return raw values
+  M.c2p_mode = C2P_REFERENCE;
+  b->relativistic = d->relativistic ? 1 : 0;
+  b->rs = d->relativistic ? ((d->rsolver == 0) ? 3 : 2) : d->rsolver;
+  // coordinate table: per direction dxNf, the PLM face factors (plm.cpp:149-155)
+  std::vector<double> tab(M.cstride);
+  const int nc[3] = {M.nc1, M.nc2, M.nc3};
+  for (int q = 0; q < 3; ++q) {
+    double *t = tab.data() + M.coff[q];
+    for (int i = 0; i < nc[q]; ++i) {
+      t[i] = dxf[q][i];
+      t[nc[q] + i] = (xf[q][i + 1] - xv[q][i]) / dxf[q][i];
+      t[2 * nc[q] + i] = (xv[q][i] - xf[q][i]) / dxf[q][i];
+    }
+  }
+  std::vector<int> nb(27, -1);
+  if (hipMalloc(reinterpret_cast<void **>(&b->coord_d), sizeof(double) * tab.size()) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&b->nbr_d), sizeof(int) * 27) != hipSuccess ||
+      hipMemcpy(b->coord_d, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->nbr_d, nb.data(), sizeof(int) * 27, hipMemcpyHostToDevice) != hipSuccess) {
+    if (b->coord_d) (void)hipFree(b->coord_d);
+    if (b->nbr_d) (void)hipFree(b->nbr_d);
+    delete b;
+    return fail("ath_gpu_block_create: device allocation or upload failed");
+  }
+  M.coord = b->coord_d;
+  M.nbr = b->nbr_d;
+  *out = b;
+  return 0;
+}
+
+int ath_gpu_block_destroy(ath_block_t b) {
+  if (!b) return 0;
+  (void)hipFree(b->coord_d);
+  (void)hipFree(b->nbr_d);
+  delete b;
+  return 0;
+}
+
+int ath_gpu_block_reconstruct(ath_block_t b, int dir, int order, int k, int j, int il, int iu,
+                              const double *w, const double *bcc, double *wl, double *wr,
+                              void *stream) {
+  if (!b || !w || !bcc || !wl || !wr) return fail("null argument");
+  if (dir < 0 || dir > 2) return fail("dir must be 0, 1 or 2");
+  if (order < 1 || order > 3) return fail("order must be 1, 2 or 3");
+  const MeshK &M = b->M;
+  if (order == 3 && M.ng < 3) return fail("PPM needs nghost >= 3");
+  const int e = order - 1;   // stencil half-width
+  const int pos[3] = {0, j, k}, n[3] = {M.nc1, M.nc2, M.nc3};
+  if (!in_box(il, iu, M.nc1) || !in_box(j, j, M.nc2) || !in_box(k, k, M.nc3))
+    return fail("pencil outside the block");
+  if (dir == 0 && (il - e < 0 || iu + std::max(e, 1) >= M.nc1))
+    return fail("reconstruction stencil leaves the block along x1");
+  if (dir > 0 && (pos[dir] - e < 0 || pos[dir] + e >= n[dir]))
+    return fail("reconstruction stencil leaves the block");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g = pencil_grid(il, iu);
+#define LR(D, O) hipLaunchKernelGGL((k_blk_recon<D, O>), g, dim3(WG), 0, st, M, k, j, il, iu, w, bcc, wl, wr)
+#define LR_O(D)                 \
+  do {                          \
+    if (order == 1) LR(D, 1);   \
+    else if (order == 2) LR(D, 2); \
+    else LR(D, 3);              \
+  } while (0)
+  if (dir == 0) LR_O(0);
+  else if (dir == 1) LR_O(1);
+  else LR_O(2);
+#undef LR_O
+#undef LR
+  return blk_check_stream(st);
+}
+
+int ath_gpu_block_riemann(ath_block_t b, int k, int j, int il, int iu, int ivx,
+                          const double *bx, const double *wl, const double *wr, double *flx,
+                          double *ey, double *ez, double *wct, const double *dxw, double dt,
+                          void *stream) {
+  if (!b || !bx || !wl || !wr || !flx || !ey || !ez || !wct || !dxw) return fail("null argument");
+  if (ivx < 1 || ivx > 3) return fail("ivx must be IVX, IVY or IVZ (1, 2, 3)");
+  const MeshK &M = b->M;
+  const int d = ivx - 1;
+  // face-array extents along (i, j, k) for the direction of ivx
+  const int n1 = M.nc1 + (d == 0), n2 = M.nc2 + (d == 1), n3 = M.nc3 + (d == 2);
+  if (!in_box(il, iu, n1) || iu >= M.nc1 || !in_box(j, j, n2) || !in_box(k, k, n3))
+    return fail("pencil outside the face arrays");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g = pencil_grid(il, iu);
+  const int rs = b->rs;
+#define LRS(D, R) hipLaunchKernelGGL((k_blk_riemann<D, R>), g, dim3(WG), 0, st, M, k, j, il, iu, bx, wl, wr, flx, ey, ez, wct, dxw, dt)
+#define LRS_R(D)                   \
+  do {                             \
+    if (rs == 0) LRS(D, 0);        \
+    else if (rs == 1) LRS(D, 1);   \
+    else if (rs == 2) LRS(D, 2);   \
+    else LRS(D, 3);                \
+  } while (0)
+  if (d == 0) LRS_R(0);
+  else if (d == 1) LRS_R(1);
+  else LRS_R(2);
+#undef LRS_R
+#undef LRS
+  return blk_check_stream(st);
+}
+
+int ath_gpu_block_calculate_fluxes(ath_block_t b, const double *w, const double *b1,
+                                   const double *b2, const double *b3, const double *bcc,
+                                   int order, double dt, const ath_block_regs *r,
+                                   void *stream) {
+  if (!b || !w || !b1 || !b2 || !b3 || !bcc || !r) return fail("null argument");
+  if (order < 1 || order > 3) return fail("order must be 1, 2 or 3");
+  const MeshK &M = b->M;
+  if (order == 3 && M.ng < 3) return fail("PPM needs nghost >= 3");
+  for (int q = 0; q < 3; ++q)
+    if (!r->flux[q] || !r->wght[q]) return fail("null flux or weight array");
+  if (!r->e3_x1f || !r->e2_x1f || !r->e1_x2f || !r->e3_x2f || !r->e2_x3f || !r->e1_x3f)
+    return fail("null face EMF array");
+  FaceOut o[3];
+  o[0] = FaceOut{b1, r->flux[0], r->e3_x1f, r->e2_x1f, r->wght[0]};
+  o[1] = FaceOut{b2, r->flux[1], r->e1_x2f, r->e3_x2f, r->wght[1]};
+  o[2] = FaceOut{b3, r->flux[2], r->e2_x3f, r->e1_x3f, r->wght[2]};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(static_cast<unsigned>(kind_chunks(M, KK_FLUX, TileK{0, 0, 0})));
+  const int rs = b->rs;
+#define LF(O, R) hipLaunchKernelGGL((k_blk_flux3<O, R>), g, dim3(WG), 0, st, M, w, bcc, o[0], o[1], o[2], dt)
+#define LF_O(R)                         \
+  do {                                  \
+    if (order == 1) LF(1, R);           \
+    else if (order == 2) LF(2, R);      \
+    else LF(3, R);                      \
+  } while (0)
+  if (rs == 0) LF_O(0);
+  else if (rs == 1) LF_O(1);
+  else if (rs == 2) LF_O(2);
+  else LF_O(3);
+#undef LF_O
+#undef LF
+  return blk_check_stream(st);
+}
+
+int ath_gpu_block_compute_corner_e(ath_block_t b, const double *w, const double *bcc,
+                                   const ath_block_regs *r, void *stream) {
+  if (!b || !w || !bcc || !r) return fail("null argument");
+  for (int q = 0; q < 3; ++q)
+    if (!r->e[q] || !r->wght[q]) return fail("null edge EMF or weight array");
+  if (!r->e3_x1f || !r->e2_x1f || !r->e1_x2f || !r->e3_x2f || !r->e2_x3f || !r->e1_x3f)
+    return fail("null face EMF array");
+  const MeshK &M = b->M;
+  CornerArgs ca{w, bcc, r->e2_x1f, r->e3_x1f, r->e1_x2f, r->e3_x2f, r->e1_x3f, r->e2_x3f,
+                r->wght[0], r->wght[1], r->wght[2], r->e[0], r->e[1], r->e[2]};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const TileK T{0, 0, 0};
+  const dim3 g(static_cast<unsigned>(kind_chunks(M, KK_CORNER, T)));
+  if (b->relativistic) hipLaunchKernelGGL((k_corner<true>), g, dim3(WG), 0, st, M, ca, T);
+  else hipLaunchKernelGGL((k_corner<false>), g, dim3(WG), 0, st, M, ca, T);
+  return blk_check_stream(st);
+}
+
+static int blk_update(ath_block_t b, const ath_block_regs *r, double wght, double *u_out,
+                      double *const *b_out, int what, void *stream) {
+  const MeshK &M = b->M;
+  UpdateArgs a{};
+  if (what & 1) {
+    for (int q = 0; q < 3; ++q)
+      if (!r->flux[q]) return fail("null flux array");
+    a.uout = u_out;
+    a.ubase = u_out;
+    a.f1 = r->flux[0]; a.f2 = r->flux[1]; a.f3 = r->flux[2];
+  }
+  if (what & 2) {
+    for (int q = 0; q < 3; ++q)
+      if (!r->e[q] || !b_out[q]) return fail("null edge EMF or field array");
+    for (int q = 0; q < 3; ++q) { a.bin[q] = b_out[q]; a.bout[q] = b_out[q]; }
+    a.e1 = r->e[0]; a.e2 = r->e[1]; a.e3 = r->e[2];
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(static_cast<unsigned>(kind_chunks(M, KK_UPDATE, TileK{0, 0, 0})));
+  hipLaunchKernelGGL(k_blk_update, g, dim3(WG), 0, st, M, a, wght, what);
+  return blk_check_stream(st);
+}
+
+int ath_gpu_block_ct(ath_block_t b, double wght, const ath_block_regs *r, double *b1_out,
+                     double *b2_out, double *b3_out, void *stream) {
+  if (!b || !r || !b1_out || !b2_out || !b3_out) return fail("null argument");
+  double *bo[3] = {b1_out, b2_out, b3_out};
+  return blk_update(b, r, wght, nullptr, bo, 2, stream);
+}
+
+int ath_gpu_block_add_flux_divergence(ath_block_t b, double wght, const ath_block_regs *r,
+                                      double *u_out, void *stream) {
+  if (!b || !r || !u_out) return fail("null argument");
+  double *bo[3] = {nullptr, nullptr, nullptr};
+  return blk_update(b, r, wght, u_out, bo, 1, stream);
+}
+
+int ath_gpu_block_cons2prim(ath_block_t b, double *cons, const double *prim_old,
+                            const double *b1, const double *b2, const double *b3,
+                            double *prim, double *bcc, int il, int iu, int jl, int ju, int kl,
+                            int ku, void *stream) {
+  (void)prim_old;   // both recoveries read their seed (SR) from prim itself
+  if (!b || !cons || !b1 || !b2 || !b3 || !prim || !bcc) return fail("null argument");
+  const MeshK &M = b->M;
+  if (!in_box(il, iu, M.nc1) || !in_box(jl, ju, M.nc2) || !in_box(kl, ku, M.nc3))
+    return fail("range outside the block");
+  C2PArgs a{cons, prim, bcc, b1, b2, b3, nullptr, nullptr};
+  const int ni = iu - il + 1, nj = ju - jl + 1, nk = ku - kl + 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 g(static_cast<unsigned>((static_cast<long>(ni) * nj * nk + WG - 1) / WG));
+  if (b->relativistic)
+    hipLaunchKernelGGL((k_blk_c2p<true>), g, dim3(WG), 0, st, M, a, il, jl, kl, ni, nj, nk);
+  else
+    hipLaunchKernelGGL((k_blk_c2p<false>), g, dim3(WG), 0, st, M, a, il, jl, kl, ni, nj, nk);
+  return blk_check_stream(st);
+}
+
+int ath_gpu_block_new_dt(ath_block_t b, const double *w, const double *bcc, const double *b1,
+                         const double *b2, const double *b3, double *dt_out, void *stream) {
+  if (!b || !w || !bcc || !b1 || !b2 || !b3 || !dt_out) return fail("null argument");
+  const MeshK &M = b->M;
+  const int npart = (M.nx1 * M.nx2 * M.nx3 + WG - 1) / WG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double *part = nullptr;
+  HIPCHK(hipMalloc(reinterpret_cast<void **>(&part), sizeof(double) * npart));
+  if (b->relativistic)
+    hipLaunchKernelGGL((k_blk_dt<true>), dim3(npart), dim3(WG), 0, st, M, w, bcc, b1, b2, b3, part);
+  else
+    hipLaunchKernelGGL((k_blk_dt<false>), dim3(npart), dim3(WG), 0, st, M, w, bcc, b1, b2, b3, part);
+  hipLaunchKernelGGL(k_blk_dt_final, dim3(1), dim3(1024), 0, st, part, npart, M.cfl, dt_out);
+  const hipError_t le = hipGetLastError();
+  const hipError_t se = hipStreamSynchronize(st);   // part is freed below
+  (void)hipFree(part);
+  if (le != hipSuccess) return fail(std::string("ath_gpu_block_new_dt: ") + hipGetErrorString(le));
+  if (se != hipSuccess) return fail(std::string("ath_gpu_block_new_dt: ") + hipGetErrorString(se));
+  return 0;
 }
 
 }  // extern "C"

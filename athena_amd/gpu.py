@@ -38,7 +38,12 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_mesh_autotune", "ath_gpu_kernel_box", "ath_gpu_history",
            "ath_gpu_ghost_position", "ath_gpu_restrict_cc", "ath_gpu_restrict_fc",
            "ath_gpu_prolongate_cc", "ath_gpu_prolongate_fc_shared",
-           "ath_gpu_prolongate_fc_internal", "ath_gpu_restrict_emf", "ath_gpu_tree_build"]
+           "ath_gpu_prolongate_fc_internal", "ath_gpu_restrict_emf", "ath_gpu_tree_build",
+           "ath_gpu_block_create", "ath_gpu_block_destroy", "ath_gpu_block_reconstruct",
+           "ath_gpu_block_riemann", "ath_gpu_block_calculate_fluxes",
+           "ath_gpu_block_compute_corner_e", "ath_gpu_block_ct",
+           "ath_gpu_block_add_flux_divergence", "ath_gpu_block_cons2prim",
+           "ath_gpu_block_new_dt"]
 NKERNELS = 19
 
 
@@ -60,6 +65,22 @@ class MeshDescC(ctypes.Structure):
                 ("pfloor", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("sigma_max", ctypes.c_double), ("beta_min", ctypes.c_double),
                 ("gamma_max", ctypes.c_double), ("bc", ctypes.c_int * 6)]
+
+
+class BlockDescC(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int * 3), ("ng", ctypes.c_int)] + \
+        [(n, ctypes.POINTER(ctypes.c_double)) for n in
+         ("x1f", "x2f", "x3f", "x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")] + \
+        [("rsolver", ctypes.c_int), ("relativistic", ctypes.c_int)] + \
+        [(n, ctypes.c_double) for n in
+         ("gamma", "dfloor", "pfloor", "sigma_max", "beta_min", "gamma_max", "cfl")]
+
+
+class BlockRegsC(ctypes.Structure):
+    _fields_ = [("flux", ctypes.c_void_p * 3)] + \
+        [(n, ctypes.c_void_p) for n in
+         ("e3_x1f", "e2_x1f", "e1_x2f", "e3_x2f", "e2_x3f", "e1_x3f")] + \
+        [("wght", ctypes.c_void_p * 3), ("e", ctypes.c_void_p * 3)]
 
 
 _lib = None
@@ -139,6 +160,18 @@ def load():
     L.ath_gpu_tree_build.argtypes = [IP, IP, D, D, ctypes.POINTER(RefineRegionC), I,
                                      ctypes.POINTER(ctypes.c_longlong), IP, I, IP, IP]
     L.ath_gpu_history.argtypes = [P, D]
+    BR = ctypes.POINTER(BlockRegsC)
+    Dd = ctypes.c_double
+    L.ath_gpu_block_create.argtypes = [ctypes.POINTER(BlockDescC), ctypes.POINTER(P)]
+    L.ath_gpu_block_destroy.argtypes = [P]
+    L.ath_gpu_block_reconstruct.argtypes = [P, I, I, I, I, I, I, P, P, P, P, P]
+    L.ath_gpu_block_riemann.argtypes = [P, I, I, I, I, I, P, P, P, P, P, P, P, P, Dd, P]
+    L.ath_gpu_block_calculate_fluxes.argtypes = [P, P, P, P, P, P, I, Dd, BR, P]
+    L.ath_gpu_block_compute_corner_e.argtypes = [P, P, P, BR, P]
+    L.ath_gpu_block_ct.argtypes = [P, Dd, BR, P, P, P, P]
+    L.ath_gpu_block_add_flux_divergence.argtypes = [P, Dd, BR, P, P]
+    L.ath_gpu_block_cons2prim.argtypes = [P, P, P, P, P, P, P, P, I, I, I, I, I, I, P]
+    L.ath_gpu_block_new_dt.argtypes = [P, P, P, P, P, P, P, P]
     _lib = L
     return L
 
@@ -583,3 +616,91 @@ def ghost_position(nx, ng, array, t):
     pos = np.zeros(9, dtype=np.int32)
     _check(load().ath_gpu_ghost_position(int(nx), int(ng), int(array), int(t), _ip(pos)))
     return pos
+
+
+class Block:
+    """One MeshBlock whose registers the caller keeps in device memory
+    (ath_gpu_block_*): the reference's per-MeshBlock functions with its own signatures.
+    Array arguments are device addresses (ints, e.g. a torch tensor's data_ptr());
+    ``regs`` maps the names of ath_block_regs (flux1..3, e3_x1f, e2_x1f, e1_x2f, e3_x2f,
+    e2_x3f, e1_x3f, wght1..3, e1..e3) to device addresses.  Calls are asynchronous on
+    ``stream`` (None: the default stream)."""
+
+    def __init__(self, m: MeshDesc, gid=0, stream=None):
+        from .mesh import block_coords
+        c = block_coords(m, gid)
+        self._keep = [np.ascontiguousarray(getattr(c, n), dtype=np.float64) for n in
+                      ("x1f", "x2f", "x3f", "x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f")]
+        d = BlockDescC()
+        d.nx[:] = list(m.bnx)
+        d.ng = m.ng
+        for name, a in zip(("x1f", "x2f", "x3f", "x1v", "x2v", "x3v", "dx1f", "dx2f", "dx3f"),
+                           self._keep):
+            setattr(d, name, _dp(a))
+        d.rsolver = RSOLVER[m.rsolver]
+        d.relativistic = int(m.relativistic)
+        d.gamma, d.dfloor, d.pfloor = m.gamma, m.dfloor, m.pfloor
+        d.sigma_max, d.beta_min, d.gamma_max = m.sigma_max, m.beta_min, m.gamma_max
+        d.cfl = m.cfl
+        self._desc = d
+        h = ctypes.c_void_p()
+        _check(load().ath_gpu_block_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.stream = stream
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().ath_gpu_block_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def regs(r):
+        c = BlockRegsC()
+        for q in range(3):
+            c.flux[q] = r.get("flux%d" % (q + 1))
+            c.wght[q] = r.get("wght%d" % (q + 1))
+            c.e[q] = r.get("e%d" % (q + 1))
+        for n in ("e3_x1f", "e2_x1f", "e1_x2f", "e3_x2f", "e2_x3f", "e1_x3f"):
+            setattr(c, n, r.get(n))
+        return c
+
+    def reconstruct(self, dir, order, k, j, il, iu, w, bcc, wl, wr):
+        _check(load().ath_gpu_block_reconstruct(self.h, dir, order, k, j, il, iu, w, bcc, wl,
+                                                wr, self.stream))
+
+    def riemann(self, k, j, il, iu, ivx, bx, wl, wr, flx, ey, ez, wct, dxw, dt):
+        _check(load().ath_gpu_block_riemann(self.h, k, j, il, iu, ivx, bx, wl, wr, flx, ey, ez,
+                                            wct, dxw, dt, self.stream))
+
+    def calculate_fluxes(self, w, b1, b2, b3, bcc, order, dt, regs):
+        r = self.regs(regs)
+        _check(load().ath_gpu_block_calculate_fluxes(self.h, w, b1, b2, b3, bcc, order, dt,
+                                                     ctypes.byref(r), self.stream))
+
+    def compute_corner_e(self, w, bcc, regs):
+        r = self.regs(regs)
+        _check(load().ath_gpu_block_compute_corner_e(self.h, w, bcc, ctypes.byref(r),
+                                                     self.stream))
+
+    def ct(self, wght, regs, b1, b2, b3):
+        r = self.regs(regs)
+        _check(load().ath_gpu_block_ct(self.h, wght, ctypes.byref(r), b1, b2, b3, self.stream))
+
+    def add_flux_divergence(self, wght, regs, u):
+        r = self.regs(regs)
+        _check(load().ath_gpu_block_add_flux_divergence(self.h, wght, ctypes.byref(r), u,
+                                                        self.stream))
+
+    def cons2prim(self, cons, prim_old, b1, b2, b3, prim, bcc, rng):
+        il, iu, jl, ju, kl, ku = (int(v) for v in rng)
+        _check(load().ath_gpu_block_cons2prim(self.h, cons, prim_old, b1, b2, b3, prim, bcc,
+                                              il, iu, jl, ju, kl, ku, self.stream))
+
+    def new_dt(self, w, bcc, b1, b2, b3, dt_out):
+        _check(load().ath_gpu_block_new_dt(self.h, w, bcc, b1, b2, b3, dt_out, self.stream))

@@ -133,6 +133,105 @@ int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
 
 const char *ath_gpu_last_error(void);
 
+/* ---- One MeshBlock over caller-owned device arrays -----------------------------------
+ * The reference's per-MeshBlock functions with their own signatures, for a caller that
+ * keeps a MeshBlock's AthenaArrays in device memory itself (one array per register,
+ * AthenaArray layout, shapes as for ath_gpu_set_block with nc = nx + 2 ng).  A block
+ * object holds what the MeshBlock's Coordinates and EquationOfState hold; the register
+ * arrays stay the caller's.  Every call is asynchronous on `stream` (a hipStream_t; null
+ * for the default stream) unless stated otherwise, and acts on that one block only:
+ * neighbour exchange and EMF correction between blocks are the caller's, exactly as in
+ * the reference, where they are separate tasks. */
+typedef struct ath_block_desc {
+  int nx[3];                        /* block_size.nx1 .. nx3 */
+  int ng;                           /* NGHOST */
+  const double *x1f, *x2f, *x3f;    /* host Coordinates::x1f .. x3f (nc + 1 each) */
+  const double *x1v, *x2v, *x3v;    /* host Coordinates::x1v .. x3v (nc each) */
+  const double *dx1f, *dx2f, *dx3f; /* host Coordinates::dx1f .. dx3f (nc each) */
+  int rsolver;                      /* 0 HLLD, 1 HLLE (Newtonian and SR, as ath_mesh_desc) */
+  int relativistic;
+  double gamma, dfloor, pfloor;     /* < 0 floors select the reference defaults */
+  double sigma_max, beta_min, gamma_max;
+  double cfl;                       /* Mesh::cfl_number, used by ath_gpu_block_new_dt */
+} ath_block_desc;
+
+typedef struct ath_block *ath_block_t;
+
+/* Outputs of the flux and EMF functions: Hydro::flux[X1DIR..X3DIR] (NHYDRO, face array),
+ * the face EMFs of Field (e3_x1f = -F(By) and e2_x1f = F(Bz) on x1 faces, e1_x2f and
+ * e3_x2f on x2 faces, e2_x3f and e1_x3f on x3 faces: the ey, ez arguments of the
+ * RiemannSolver calls at calculate_fluxes.cpp:102, 223, 343), Field::wght.x1f .. x3f and
+ * Field::e.x1e .. x3e. */
+typedef struct ath_block_regs {
+  double *flux[3];
+  double *e3_x1f, *e2_x1f, *e1_x2f, *e3_x2f, *e2_x3f, *e1_x3f;
+  double *wght[3];
+  double *e[3];
+} ath_block_regs;
+
+/* Coordinates and EquationOfState of one MeshBlock (coordinates.cpp:129-146,
+ * adiabatic_mhd.cpp:26-31, adiabatic_mhd_sr.cpp:36-48); synchronous. */
+int ath_gpu_block_create(const ath_block_desc *desc, ath_block_t *out);
+int ath_gpu_block_destroy(ath_block_t blk);
+
+/* Reconstruction::DonorCellX{dir+1} / PiecewiseLinearX{dir+1} / PiecewiseParabolicX{dir+1}
+ * for order 1 / 2 / 3 (reconstruction.hpp:59-105; dc.cpp, plm.cpp, ppm.cpp) on pencil
+ * (k, j), cells il .. iu: w (NHYDRO, nc3, nc2, nc1), bcc (3, nc3, nc2, nc1); wl, wr pencils
+ * (NWAVE, nc1).  x1 writes wl(n, i+1), wr(n, i); x2 and x3 write wl(n, i), wr(n, i) for the
+ * cell j (x2) or k (x3), as the reference does. */
+int ath_gpu_block_reconstruct(ath_block_t blk, int dir, int order, int k, int j, int il,
+                              int iu, const double *w, const double *bcc, double *wl,
+                              double *wr, void *stream);
+
+/* Hydro::RiemannSolver(k, j, il, iu, ivx, bx, wl, wr, flx, ey, ez, wct, dxw) (hydro.hpp:
+ * 82-87; hlld.cpp, hlle_mhd.cpp, hlle_mhd_rel.cpp, hlld_rel.cpp by the block's rsolver
+ * and dynamics): ivx = 1, 2, 3 (IVX, IVY, IVZ); bx, ey, ez, wct face arrays and flx the
+ * (NHYDRO, face array) of that direction; wl, wr, dxw pencils; dt is Mesh::dt for
+ * GetWeightForCT (hydro.cpp:154-158). */
+int ath_gpu_block_riemann(ath_block_t blk, int k, int j, int il, int iu, int ivx,
+                          const double *bx, const double *wl, const double *wr, double *flx,
+                          double *ey, double *ez, double *wct, const double *dxw, double dt,
+                          void *stream);
+
+/* Hydro::CalculateFluxes(w, b, bcc, order) (hydro.hpp:72-73, calculate_fluxes.cpp:38-415;
+ * MHD ranges): writes out->flux, the face EMFs and out->wght, with Mesh::dt = dt. */
+int ath_gpu_block_calculate_fluxes(ath_block_t blk, const double *w, const double *b1,
+                                   const double *b2, const double *b3, const double *bcc,
+                                   int order, double dt, const ath_block_regs *out,
+                                   void *stream);
+
+/* Field::ComputeCornerE(w, bcc) (field.hpp:62, calculate_corner_e.cpp:129-230): reads the
+ * face EMFs and weights of r, writes r->e. */
+int ath_gpu_block_compute_corner_e(ath_block_t blk, const double *w, const double *bcc,
+                                   const ath_block_regs *r, void *stream);
+
+/* Field::CT(wght, b_out) (field.hpp:65, ct.cpp:31-116): b_out -= wght * curl(r->e) on the
+ * active faces, in place. */
+int ath_gpu_block_ct(ath_block_t blk, double wght, const ath_block_regs *r, double *b1_out,
+                     double *b2_out, double *b3_out, void *stream);
+
+/* Hydro::AddFluxDivergence(wght, u_out) (hydro.hpp:67, add_flux_divergence.cpp:39-96):
+ * u_out -= wght * div(r->flux) on the active cells, in place. */
+int ath_gpu_block_add_flux_divergence(ath_block_t blk, double wght, const ath_block_regs *r,
+                                      double *u_out, void *stream);
+
+/* EquationOfState::ConservedToPrimitive(cons, prim_old, b, prim, bcc, pco, il, iu, jl, ju,
+ * kl, ku) (eos.hpp:37-40; adiabatic_mhd.cpp:41-90 or adiabatic_mhd_sr.cpp:72-272) with
+ * Field::CalculateCellCenteredField over the same range: writes prim and bcc, rewrites
+ * cons where a floor applies.  Like the reference, the SR recovery seeds from prim
+ * itself; prim_old is not read. */
+int ath_gpu_block_cons2prim(ath_block_t blk, double *cons, const double *prim_old,
+                            const double *b1, const double *b2, const double *b3,
+                            double *prim, double *bcc, int il, int iu, int jl, int ju,
+                            int kl, int ku, void *stream);
+
+/* Hydro::NewBlockTimeStep() (hydro.hpp:66, new_blockdt.cpp:45-230, ideal MHD): the block's
+ * new_block_dt (cfl applied) from w, bcc and b, into the device double *dt_out.
+ * Allocates and frees its per-work-group scratch, so it synchronises `stream`. */
+int ath_gpu_block_new_dt(ath_block_t blk, const double *w, const double *bcc,
+                         const double *b1, const double *b2, const double *b3,
+                         double *dt_out, void *stream);
+
 /* ---- Ensembles ------------------------------------------------------------------------
  * Independent meshes of uneven sizes advanced together: each VL2 stage kernel is
  * launched once for all members (persistent work-groups over a (member, work-group)

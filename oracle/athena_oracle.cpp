@@ -1247,7 +1247,9 @@ void ora_free(void *h) { delete static_cast<Mesh *>(h); }
 int ora_nblocks(void *h) { return static_cast<int>(static_cast<Mesh *>(h)->blocks.size()); }
 
 // what: 0=u 1=w 2=bcc 3=b.x1f 4=b.x2f 5=b.x3f 6=w1 7..9 = x1f,x2f,x3f coords
-//       10..12 = x1v,x2v,x3v 13..15 = dx1f..dx3f
+//       10..12 = x1v,x2v,x3v 13..15 = dx1f..dx3f 16..18 = flux[X1DIR..X3DIR]
+//       19..21 = e.x1e..x3e 22..27 = face EMFs e3_x1f, e2_x1f, e1_x2f, e3_x2f, e2_x3f,
+//       e1_x3f 28..30 = wght.x1f..x3f
 static Arr *pick(Block &b, int what) {
   switch (what) {
     case 0: return &b.u;
@@ -1257,6 +1259,21 @@ static Arr *pick(Block &b, int what) {
     case 4: return &b.b2f;
     case 5: return &b.b3f;
     case 6: return &b.w1;
+    case 16: return &b.flux[0];
+    case 17: return &b.flux[1];
+    case 18: return &b.flux[2];
+    case 19: return &b.e1;
+    case 20: return &b.e2;
+    case 21: return &b.e3;
+    case 22: return &b.e3x1;
+    case 23: return &b.e2x1;
+    case 24: return &b.e1x2;
+    case 25: return &b.e3x2;
+    case 26: return &b.e2x3;
+    case 27: return &b.e1x3;
+    case 28: return &b.wx1;
+    case 29: return &b.wx2;
+    case 30: return &b.wx3;
     default: return nullptr;
   }
 }
@@ -1290,6 +1307,66 @@ int ora_get(void *h, int gid, int what, double *dst) {
 }
 
 void ora_initialize(void *h, double tlim) { Initialize(*static_cast<Mesh *>(h), tlim); }
+
+// Test entry: one per-MeshBlock function of the path on block gid's registers, as the
+// reference calls it on one MeshBlock (no neighbour exchange, no EMF averaging).
+//   fn 0 Hydro::CalculateFluxes(w, b, bcc, order) with the mesh's dt for the CT weights
+//   fn 1 Field::ComputeCornerE(w, bcc)
+//   fn 2 Field::CT(wght, b)                     fn 3 Hydro::AddFluxDivergence(wght, u)
+//   fn 4 EquationOfState::ConservedToPrimitive(u, w1, b, w, bcc) over range
+//        {il, iu, jl, ju, kl, ku}
+//   fn 5 Hydro::NewBlockTimeStep (out[0] = the block's new dt, cfl applied)
+//   fn 6 Reconstruction X{d+1} of order `order` on the pencil (k, j) = range[4], range[2]
+//        for cells range[0] .. range[1], d = range[5]; out = wl then wr, (NWAVE, nc1 + 1)
+// Returns 0, or -1 for an unknown fn.
+int ora_block_call(void *h, int gid, int fn, int order, double wght, const int *range,
+                   double *out) {
+  Mesh &m = *static_cast<Mesh *>(h);
+  Block &b = m.blocks[gid];
+  switch (fn) {
+    case 0: CalculateFluxes(m, b, order); return 0;
+    case 1: ComputeCornerE(m, b); return 0;
+    case 2: CT(b, wght); return 0;
+    case 3: AddFluxDivergence(b, wght, b.u); return 0;
+    case 4:
+      if (m.c.relativistic)
+        Cons2PrimSR(m, b, b.u, b.w1, b.w, range[0], range[1], range[2], range[3], range[4],
+                    range[5]);
+      else
+        Cons2PrimNewtonian(m, b, b.u, b.w, range[0], range[1], range[2], range[3], range[4],
+                           range[5]);
+      return 0;
+    case 5: NewBlockTimeStep(m, b); out[0] = b.new_dt; return 0;
+    case 6: {
+      Arr wl, wr;
+      wl.alloc(NWAVE, 1, 1, b.nc1 + 1);
+      wr.alloc(NWAVE, 1, 1, b.nc1 + 1);
+      Reconstruct(m, b, order, range[5], range[4], range[2], range[0], range[1], b.w, b.bcc,
+                  wl, wr);
+      std::memcpy(out, wl.v.data(), wl.v.size() * 8);
+      std::memcpy(out + wl.v.size(), wr.v.data(), wr.v.size() * 8);
+      return 0;
+    }
+    default: return -1;
+  }
+}
+
+// Test entry: Hydro::RiemannSolver on pencil (k, j), faces il .. iu along direction d, of
+// block gid, from the caller's wl, wr (NWAVE, nc1 + 1 each), writing the block's flux[d],
+// face EMFs and CT weights with the mesh's dt.
+void ora_block_riemann(void *h, int gid, int d, int k, int j, int il, int iu,
+                       const double *wl, const double *wr) {
+  Mesh &m = *static_cast<Mesh *>(h);
+  Block &b = m.blocks[gid];
+  Arr l, r;
+  l.alloc(NWAVE, 1, 1, b.nc1 + 1);
+  r.alloc(NWAVE, 1, 1, b.nc1 + 1);
+  std::memcpy(l.v.data(), wl, l.v.size() * 8);
+  std::memcpy(r.v.data(), wr, r.v.size() * 8);
+  if (d == 0) RiemannPencil(m, b, 0, k, j, il, iu, b.b1f, l, r, b.flux[0], b.e3x1, b.e2x1, b.wx1);
+  else if (d == 1) RiemannPencil(m, b, 1, k, j, il, iu, b.b2f, l, r, b.flux[1], b.e1x2, b.e3x2, b.wx2);
+  else RiemannPencil(m, b, 2, k, j, il, iu, b.b3f, l, r, b.flux[2], b.e2x3, b.e1x3, b.wx3);
+}
 
 // Runs up to ncycles cycles while time < tlim; returns the number of cycles done.
 int ora_run(void *h, int ncycles, double tlim) {

@@ -18,7 +18,9 @@ LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 
 WHAT = {"u": 0, "w": 1, "bcc": 2, "b1": 3, "b2": 4, "b3": 5, "w1": 6,
         "x1f": 7, "x2f": 8, "x3f": 9, "x1v": 10, "x2v": 11, "x3v": 12,
-        "dx1f": 13, "dx2f": 14, "dx3f": 15}
+        "dx1f": 13, "dx2f": 14, "dx3f": 15, "flux1": 16, "flux2": 17, "flux3": 18,
+        "e1": 19, "e2": 20, "e3": 21, "e3_x1f": 22, "e2_x1f": 23, "e1_x2f": 24,
+        "e3_x2f": 25, "e2_x3f": 26, "e1_x3f": 27, "wght1": 28, "wght2": 29, "wght3": 30}
 
 
 class OraCfg(ctypes.Structure):
@@ -57,6 +59,9 @@ def lib():
         L.ora_state.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int)]
         L.ora_set_dt.argtypes = [P, ctypes.c_double]
         L.ora_threads.restype = ctypes.c_int
+        L.ora_block_call.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.POINTER(ctypes.c_int), D]
+        L.ora_block_riemann.argtypes = [P] + [ctypes.c_int] * 6 + [D, D]
         L.ora_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   D, D, D, D]
         L.ora_sr_con2prim.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + \
@@ -140,6 +145,20 @@ class OracleMesh:
 
     def block(self, gid):
         return {k: self.get(gid, k) for k in ("u", "w", "bcc", "b1", "b2", "b3")}
+
+    def call(self, gid, fn, order=0, wght=0.0, rng=(0, 0, 0, 0, 0, 0), out_size=1):
+        """One per-MeshBlock function on block gid (ora_block_call's fn codes)."""
+        r = np.ascontiguousarray(rng, dtype=np.int32)
+        out = np.zeros(out_size)
+        rc = lib().ora_block_call(self.h, gid, fn, order, wght,
+                                  r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(out))
+        assert rc == 0
+        return out
+
+    def riemann_pencil(self, gid, d, k, j, il, iu, wl, wr):
+        wl = np.ascontiguousarray(wl, dtype=np.float64)
+        wr = np.ascontiguousarray(wr, dtype=np.float64)
+        lib().ora_block_riemann(self.h, gid, d, k, j, il, iu, _dp(wl), _dp(wr))
 
 
 def riemann(solver, relativistic, gamma, wl, wr, bx):

@@ -3,6 +3,7 @@ header declares (no device calls)."""
 import ctypes
 import os
 import re
+import subprocess
 
 from athena_amd import gpu
 
@@ -31,3 +32,35 @@ def test_python_mirror_struct_matches_header_order():
     names = re.findall(r"\b(\w+)(?:\[\d+\])?\s*[,;]", body)
     fields = [f[0] for f in gpu.MeshDescC._fields_]
     assert names == fields, (names, fields)
+
+
+def build_caller(outdir):
+    """Compile and link tests/abi_caller.cpp with g++ against the header and the library
+    (host code only: hipMalloc and friends come from the HIP runtime library)."""
+    exe = os.path.join(outdir, "abi_caller")
+    libdir = os.path.dirname(gpu.LIBPATH)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "abi_caller.cpp"), "-o", exe,
+           "-L" + libdir, "-lathena_gpu", "-Wl,-rpath," + libdir,
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    return exe
+
+
+def test_cpp_caller_compiles_and_links(tmp_path):
+    # the header is valid C++ for an outside caller and every entry it uses links
+    exe = build_caller(str(tmp_path))
+    assert os.path.exists(exe)
+
+
+def test_python_block_desc_matches_header_order():
+    text = open(os.path.join(ROOT, "include", "athena_gpu.h")).read()
+    body = text[text.index("typedef struct ath_block_desc"):text.index("} ath_block_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"\*?(\w+)(?:\[\d+\])?\s*[,;]", body)
+    fields = [f[0] for f in gpu.BlockDescC._fields_]
+    assert names == fields, (names, fields)
+    body = text[text.index("typedef struct ath_block_regs"):text.index("} ath_block_regs;")]
+    names = re.findall(r"\*?(\w+)(?:\[\d+\])?\s*[,;]", body)
+    assert names == [f[0] for f in gpu.BlockRegsC._fields_], names
