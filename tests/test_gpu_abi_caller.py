@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 def write_input(path, m, blocks, ncycles):
     with open(path, "wb") as f:
-        f.write(struct.pack("=14i", *m.nx, *m.bnx, m.ng, m.xorder, RSOLVER[m.rsolver],
+        f.write(struct.pack("=16i", *m.nx, *m.bnx, m.ng, m.xorder, RSOLVER[m.rsolver],
                             int(m.relativistic), *[BOUNDARY[b] for b in m.bc]))
         f.write(struct.pack("=i", ncycles))
         f.write(struct.pack("=8d", *m.xmin, *m.xmax, m.gamma, m.cfl))
