@@ -83,6 +83,18 @@ MEASURED_FP64_FLOP = {
     "cons2prim": 3371907300,
 }
 FP64_PEAK = 78.6e12
+TRAFFIC_SOURCE = "profiles/vl2_hlld_plm_turbulence256_hbm_traffic.md"
+FP64_SOURCE = "profiles/vl2_hlld_plm_turbulence256_fp64_valu.md"
+
+# The oracle against the reference's own CPU build on identical inputs (a cubic 64^3
+# MeshBlock, diagonal fast wave, VL2+PLM+HLLD, 10 cycles, one thread, same container):
+# 6.96e5 zone-cycles per CPU-second for the oracle against 5.48e5 for the reference
+# compiled from its sources with its own configure line (g++ -O3).  The oracle is this
+# much faster than the reference; the bench divides its CPU figure by it.
+ORACLE_OVER_REFERENCE = 6.96e5 / 5.48e5
+CALIBRATION_BASIS = ("cubic 64^3 MeshBlock, diagonal fast wave, VL2+PLM+HLLD, 10 cycles, 1 "
+                     "thread: oracle 6.96e5, reference built from source 5.48e5 "
+                     "zone-cycles per CPU-second")
 
 
 def kernel_algorithmic_bytes(name, m):
@@ -93,6 +105,10 @@ def kernel_algorithmic_bytes(name, m):
       flux_xyz_*:  w, bcc (8 doubles) on the stencil box (n+2e)^3 (e = 1 DC, 2 PLM,
                    3 PPM), normal face fields (3), and per face flux(5)+EMF(2)+weight(1)
                    on 3 (n+1)(n+2)^2 faces;
+      flux_div_*:  the same reads and per face EMF(2)+weight(1) writes, plus U^n (5) read
+                   and U (5) written on the active cells (the fluxes stay on chip);
+      update_b_ct: 3 face fields read and written and 3 edge EMF components read on the
+                   active faces;
       corner_emf:  v, bcc (6) on (n+2)^3 cells, 6 face EMFs + 3 weights on the faces,
                    3 edge EMFs written;
       update_u_b_ct: U^n (5) and 3 face fields on the active region, 15 flux and 3 edge
@@ -109,6 +125,11 @@ def kernel_algorithmic_bytes(name, m):
     if name.startswith("flux_xyz_"):
         e = {"dc": 1, "plm": 2, "ppm": 3}[name.split("_")[2]]
         per = 8 * (n + 2 * e) ** 3 + faces + 8 * faces
+    elif name.startswith("flux_div_"):
+        e = {"dc": 1, "plm": 2, "ppm": 3}[name.split("_")[2]]
+        per = 8 * (n + 2 * e) ** 3 + faces + 3 * faces + 10 * n ** 3
+    elif name == "update_b_ct":
+        per = 3 * (n + 1) * n * n + 3 * (n + 1) ** 2 * n + 3 * (n + 1) * n * n
     elif name == "corner_emf":
         per = 6 * (n + 2) ** 3 + 9 * (n + 1) * (n + 2) ** 2 + 3 * (n + 1) ** 2 * (n + 2)
     elif name == "update_u_b_ct":
@@ -150,7 +171,10 @@ def cpu_baseline(sample_blocks=(4, 4, 4), ncycles=2, min_seconds=10.0):
     return {"value": zc / dt, "unit": "zone-cycles/s", "cores": int(oralib().ora_threads()),
             "kind": "port",
             "sample": "%d^3 cells as %d MeshBlocks of 32^3, %d VL2 cycles (%.2f s)"
-                      % (n, m.nblocks, done, dt)}
+                      % (n, m.nblocks, done, dt),
+            "calibration": {"oracle_over_reference": ORACLE_OVER_REFERENCE,
+                            "basis": CALIBRATION_BASIS,
+                            "reference_equivalent_value": zc / dt / ORACLE_OVER_REFERENCE}}
 
 
 C2P_BYTES_PER_SOLVE = 112   # read D, M, E, Bcc and a seed; write 5 primitives
@@ -449,7 +473,11 @@ def cpu_baseline_sr(n=128, ncycles=2, min_seconds=10.0):
     return {"value": done * n ** 3 / dt, "unit": "zone-cycles/s",
             "cores": int(oralib().ora_threads()), "kind": "port",
             "sample": "%d^3 cells as %d MeshBlocks of 32^3, %d VL2 cycles (%.2f s)"
-                      % (n, m.nblocks, done, dt)}
+                      % (n, m.nblocks, done, dt),
+            "calibration": {"oracle_over_reference": ORACLE_OVER_REFERENCE,
+                            "basis": CALIBRATION_BASIS,
+                            "reference_equivalent_value":
+                                done * n ** 3 / dt / ORACLE_OVER_REFERENCE}}
 
 
 def copy_calibration():
@@ -563,11 +591,16 @@ def main():
             "algorithmic_bytes_per_launch": abytes}
     if roof["traffic"]:
         roof["traffic_GBps"] = roof["traffic"] / dom_avg_s / 1e9
+        roof["traffic_source"] = ("%s: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this "
+                                  "workload (256^3 as 512 MeshBlocks of 32^3, tiles %s), "
+                                  "not counters of this run" % (TRAFFIC_SOURCE, TRAFFIC_TILES))
     if measured and dom in MEASURED_FP64_FLOP:
         fl = MEASURED_FP64_FLOP[dom]
         roof["fp64_valu"] = {"flop_per_launch": fl, "achieved": fl / dom_avg_s / 1e12,
                              "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                             "frac": fl / dom_avg_s / FP64_PEAK}
+                             "frac": fl / dom_avg_s / FP64_PEAK,
+                             "flop_source": "%s: SQ_INSTS_VALU_*_F64 counters of this "
+                                            "workload, not of this run" % FP64_SOURCE}
     step_roof = {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE,
                  "achieved": zcs * BYTES_PER_ZONE_CYCLE / 1e9, "unit": "GB/s",
                  "frac": zcs * BYTES_PER_ZONE_CYCLE / HBM_PEAK}
