@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2322,7 +2323,15 @@ struct ath_block {
   int relativistic = 0;
   int *nbr_d = nullptr;
   double *coord_d = nullptr;
+  double *scratch_d = nullptr;   // this device's copy of `scratch` (ath_gpu_block_new_dt_noalloc)
 };
+
+// Per-work-group minima of ath_gpu_block_new_dt_noalloc, one double per 256 active cells:
+// at most kDtScratchMax partials (16,777,216 active cells per MeshBlock).  Every device
+// holds its own copy until the process ends; host code never names the array directly
+// but uses the address hipGetSymbolAddress returns for the current device.
+constexpr int kDtScratchMax = 65536;
+static __device__ __align__(256) double scratch[kDtScratchMax];
 
 namespace {
 
@@ -2333,6 +2342,26 @@ int blk_check_stream(hipStream_t st) {
 }
 
 bool in_box(int lo, int hi, int n) { return lo >= 0 && hi < n && lo <= hi; }
+
+// Device address of `scratch` on the current device, looked up once per device.
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_addr;
+
+int scratch_address(double **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (static_cast<int>(g_scratch_addr.size()) <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
 
 // Threads of a pencil launch over il .. iu.
 dim3 pencil_grid(int il, int iu) { return dim3(static_cast<unsigned>((iu - il + WG) / WG)); }
@@ -3177,6 +3206,12 @@ int ath_gpu_block_create(const ath_block_desc *d, ath_block_t *out) {
   }
   M.coord = b->coord_d;
   M.nbr = b->nbr_d;
+  if (scratch_address(&b->scratch_d) != 0) {
+    (void)hipFree(b->coord_d);
+    (void)hipFree(b->nbr_d);
+    delete b;
+    return 1;
+  }
   *out = b;
   return 0;
 }
@@ -3382,6 +3417,28 @@ int ath_gpu_block_new_dt(ath_block_t b, const double *w, const double *bcc, cons
   if (le != hipSuccess) return fail(std::string("ath_gpu_block_new_dt: ") + hipGetErrorString(le));
   if (se != hipSuccess) return fail(std::string("ath_gpu_block_new_dt: ") + hipGetErrorString(se));
   return 0;
+}
+
+int ath_gpu_block_new_dt_noalloc(ath_block_t b, const double *w, const double *bcc,
+                                 const double *b1, const double *b2, const double *b3,
+                                 double *dt_out, void *stream) {
+  if (!b || !w || !bcc || !b1 || !b2 || !b3 || !dt_out) return fail("null argument");
+  const MeshK &M = b->M;
+  const long ncell = static_cast<long>(M.nx1) * M.nx2 * M.nx3;
+  const long npart = (ncell + WG - 1) / WG;
+  if (npart > kDtScratchMax)
+    return fail("ath_gpu_block_new_dt_noalloc: " + std::to_string(ncell) +
+                " active cells exceed the scratch limit of " +
+                std::to_string(static_cast<long>(kDtScratchMax) * WG));
+  double *part = b->scratch_d;   // every slot 0 .. npart-1 is written before it is read
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->relativistic)
+    hipLaunchKernelGGL((k_blk_dt<true>), dim3(npart), dim3(WG), 0, st, M, w, bcc, b1, b2, b3, part);
+  else
+    hipLaunchKernelGGL((k_blk_dt<false>), dim3(npart), dim3(WG), 0, st, M, w, bcc, b1, b2, b3, part);
+  hipLaunchKernelGGL(k_blk_dt_final, dim3(1), dim3(1024), 0, st, part, static_cast<int>(npart),
+                     M.cfl, dt_out);
+  return blk_check_stream(st);
 }
 
 }  // extern "C"

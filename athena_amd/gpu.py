@@ -43,7 +43,7 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_block_riemann", "ath_gpu_block_calculate_fluxes",
            "ath_gpu_block_compute_corner_e", "ath_gpu_block_ct",
            "ath_gpu_block_add_flux_divergence", "ath_gpu_block_cons2prim",
-           "ath_gpu_block_new_dt"]
+           "ath_gpu_block_new_dt", "ath_gpu_block_new_dt_noalloc"]
 NKERNELS = 19
 
 
@@ -172,6 +172,7 @@ def load():
     L.ath_gpu_block_add_flux_divergence.argtypes = [P, Dd, BR, P, P]
     L.ath_gpu_block_cons2prim.argtypes = [P, P, P, P, P, P, P, P, I, I, I, I, I, I, P]
     L.ath_gpu_block_new_dt.argtypes = [P, P, P, P, P, P, P, P]
+    L.ath_gpu_block_new_dt_noalloc.argtypes = [P, P, P, P, P, P, P, P]
     _lib = L
     return L
 
@@ -704,3 +705,9 @@ class Block:
 
     def new_dt(self, w, bcc, b1, b2, b3, dt_out):
         _check(load().ath_gpu_block_new_dt(self.h, w, bcc, b1, b2, b3, dt_out, self.stream))
+
+    def new_dt_noalloc(self, w, bcc, b1, b2, b3, dt_out):
+        """ath_gpu_block_new_dt_noalloc: the same result through the library's fixed
+        device scratch array, without allocating."""
+        _check(load().ath_gpu_block_new_dt_noalloc(self.h, w, bcc, b1, b2, b3, dt_out,
+                                                   self.stream))

@@ -232,6 +232,18 @@ int ath_gpu_block_new_dt(ath_block_t blk, const double *w, const double *bcc,
                          const double *b1, const double *b2, const double *b3,
                          double *dt_out, void *stream);
 
+/* ath_gpu_block_new_dt without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The per-work-group minima go to a
+ * fixed device array of 65536 doubles (512 KiB, held on each GPU the library runs on
+ * until the process ends), so a MeshBlock may have at most 16,777,216 active cells; a
+ * larger one is refused with an error.  Asynchronous on `stream`.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (different streams or host threads)
+ * race and must be serialised by the caller. */
+int ath_gpu_block_new_dt_noalloc(ath_block_t blk, const double *w, const double *bcc,
+                                 const double *b1, const double *b2, const double *b3,
+                                 double *dt_out, void *stream);
+
 /* ---- Ensembles ------------------------------------------------------------------------
  * Independent meshes of uneven sizes advanced together: each VL2 stage kernel is
  * launched once for all members (persistent work-groups over a (member, work-group)
