@@ -278,11 +278,14 @@ struct FaceOut {
   double *flx, *ey, *ez, *wct;
 };
 
+// One interface along DIR at face (k, j, i) of block g: reconstruction, the Riemann
+// solver, and the face EMFs and CT weight of that face written to o (when emit); the five
+// fluxes are returned in fx[] in conserved order (IDN, IM1, IM2, IM3, IEN).
 template <int DIR, int ORDER, int RS>
-__device__ __forceinline__ void flux_face(const MeshK &M, int g, int k, int j, int i,
+__device__ __forceinline__ void face_flux(const MeshK &M, int g, int k, int j, int i,
                                           const double *__restrict__ wb,
                                           const double *__restrict__ bb, const FaceOut &o,
-                                          double dt) {
+                                          double dt, bool emit, double *fx) {
   const long s = (DIR == 0) ? 1 : ((DIR == 1) ? M.nc1 : static_cast<long>(M.nc1) * M.nc2);
   const long c = cidx(M, k, j, i);
   constexpr int ivx = IVX + DIR;
@@ -345,16 +348,35 @@ __device__ __forceinline__ void flux_face(const MeshK &M, int g, int k, int j, i
   else if (RS == 2) sr_hlle(wl, wr, bx, M.eos, f);
   else sr_hlld(wl, wr, bx, M.eos, f);
 
+  fx[IDN] = f[IDN];
+  fx[ivx] = f[IVX];
+  fx[ivy] = f[IVY];
+  fx[ivz] = f[IVZ];
+  fx[IEN] = f[IEN];
+  if (emit) {
+    o.ey[static_cast<long>(g) * nf + fi] = -f[IBY];
+    o.ez[static_cast<long>(g) * nf + fi] = f[IBZ];
+    const double dxw = ct[fc];
+    o.wct[static_cast<long>(g) * nf + fi] = ct_weight(f[IDN], wl[IDN], wr[IDN], dxw, dt);
+  }
+}
+
+// face_flux with the fluxes written to Hydro::flux[DIR] (o.flx) as well.
+template <int DIR, int ORDER, int RS>
+__device__ __forceinline__ void flux_face(const MeshK &M, int g, int k, int j, int i,
+                                          const double *__restrict__ wb,
+                                          const double *__restrict__ bb, const FaceOut &o,
+                                          double dt) {
+  double fx[NHYDRO];
+  face_flux<DIR, ORDER, RS>(M, g, k, j, i, wb, bb, o, dt, true, fx);
+  long fi;
+  if (DIR == 0) fi = f1idx(M, k, j, i);
+  else if (DIR == 1) fi = f2idx(M, k, j, i);
+  else fi = f3idx(M, k, j, i);
+  const long nf = (DIR == 0) ? M.nf1 : ((DIR == 1) ? M.nf2 : M.nf3);
   double *fb = o.flx + static_cast<long>(g) * NHYDRO * nf;
-  fb[IDN * nf + fi] = f[IDN];
-  fb[ivx * nf + fi] = f[IVX];
-  fb[ivy * nf + fi] = f[IVY];
-  fb[ivz * nf + fi] = f[IVZ];
-  fb[IEN * nf + fi] = f[IEN];
-  o.ey[static_cast<long>(g) * nf + fi] = -f[IBY];
-  o.ez[static_cast<long>(g) * nf + fi] = f[IBZ];
-  const double dxw = ct[fc];
-  o.wct[static_cast<long>(g) * nf + fi] = ct_weight(f[IDN], wl[IDN], wr[IDN], dxw, dt);
+#pragma unroll
+  for (int n = 0; n < NHYDRO; ++n) fb[n * nf + fi] = fx[n];
 }
 
 template <int DIR, int ORDER, int RS>
@@ -402,6 +424,159 @@ __global__ void __launch_bounds__(WG) k_flux3(MeshK M, const double *__restrict_
                                                const Clock *__restrict__ clk, TileK T) {
   flux3_body<ORDER, RS>(M, w, bcc, o1, o2, o3, clk->dt, xcd_logical_wg(M.xcd_remap),
                         gridDim.x, T);
+}
+
+//========================================================================================
+// Kernel 1 (production form): the fluxes fused with Hydro::AddFluxDivergence
+// (add_flux_divergence.cpp:39-96).  A work-group owns a band of rows of one MeshBlock and
+// marches up its planes k = ks-1 .. ke+1; thread t owns the column (j, i) of row
+// rho = t / (nx1+2), column c = t % (nx1+2) of the band (i = is-1+c).  At plane k the
+// thread computes the faces on the low sides of cell (k, j, i) that lie in the
+// reference's flux ranges (calculate_fluxes.cpp:68-76, 176-182, 299-302), as k_flux3
+// does per cell, and writes their EMFs and CT weights as k_flux3 does; one more row of
+// threads computes the x2 faces of the next band's first row, which the update of the
+// band's last row needs (the next band writes their EMFs).  The fluxes themselves never
+// leave the chip: the x3 flux stays with its column, so the thread that owns an active
+// cell first finishes cell (k-1, j, i) with it, then the x1 and x2 fluxes go through LDS
+// to start cell (k, j, i), whose x1 + x2 divergence terms and low x3 flux are carried in
+// LDS to the next plane.  The divergence is summed in the reference's order (x1, x2, x3
+// terms), so U comes out bit for bit as from k_update; Hydro::flux is not written.
+struct FdivBand {
+  int ja, jb;     // active rows whose cells the band updates
+  int r0, r1;     // rows of its x1 and x3 faces (the flux box rows of the band)
+  int q1;         // row of the extra x2 faces (jb + 1); computed by the band iff q1 > r1
+};
+
+// Threads per row of a band and the band count of a block with bands of up to `rows`
+// active rows: as even as possible, nband = ceil(nx2 / rows).
+__host__ __device__ __forceinline__ int fdiv_nband(const MeshK &M, int rows) {
+  return (M.nx2 + rows - 1) / rows;
+}
+
+__host__ __device__ __forceinline__ FdivBand fdiv_band(const MeshK &M, int rows, int b) {
+  const int nband = fdiv_nband(M, rows);
+  FdivBand B;
+  B.ja = M.js + (b * M.nx2) / nband;
+  B.jb = M.js + ((b + 1) * M.nx2) / nband - 1;
+  B.r0 = (b == 0) ? M.js - 1 : B.ja;
+  B.r1 = (b == nband - 1) ? M.je + 1 : B.jb;
+  B.q1 = B.jb + 1;
+  return B;
+}
+
+// Thread rows a band of `rows` active rows needs at most: its ghost row (edge bands)
+// and the extra x2 row.
+__host__ __device__ __forceinline__ int fdiv_thread_rows(int rows) { return rows + 2; }
+
+// LDS doubles of one work-group: per thread the x1 and x2 fluxes of its column at the
+// current plane and the carry of its owned cell (x1 + x2 terms, low x3 flux).
+constexpr int FDIV_LDS_DOUBLES = 4 * NHYDRO * WG;
+
+// The duties of thread t of band B.
+struct FdivThread {
+  bool live;          // a column of the band (or of its extra x2 row)
+  int rho, c, j, i;   // band row, column, block row and column
+  bool x1, x2, x3;    // faces it computes (x3 only for k >= ks)
+  bool emit2;         // writes the EMFs and weight of its x2 face
+  bool own;           // updates cell (., j, i)
+};
+
+__host__ __device__ __forceinline__ FdivThread fdiv_thread(const MeshK &M, const FdivBand &B,
+                                                           int t) {
+  FdivThread T;
+  const int ni = M.nx1 + 2;
+  const int R = B.r1 - B.r0 + 1;
+  const int nrow = R + ((B.q1 > B.r1) ? 1 : 0);
+  T.rho = t / ni;
+  T.c = t - T.rho * ni;
+  T.live = T.rho < nrow;
+  T.i = M.is - 1 + T.c;
+  T.j = (T.rho < R) ? B.r0 + T.rho : B.q1;
+  T.x1 = T.live && T.rho < R && T.c >= 1;
+  T.x2 = T.live && T.j >= M.js;
+  T.x3 = T.live && T.rho < R;
+  T.emit2 = T.j <= B.r1;
+  T.own = T.live && T.rho < R && T.j >= B.ja && T.j <= B.jb && T.i >= M.is && T.i <= M.ie;
+  return T;
+}
+
+template <int ORDER, int RS>
+__global__ void __launch_bounds__(WG) k_flux_div(MeshK M, const double *__restrict__ w,
+                                                     const double *__restrict__ bcc,
+                                                     FaceOut o1, FaceOut o2, FaceOut o3,
+                                                     const double *ubase, double *uout,
+                                                     double beta, int first_stage,
+                                                     const Clock *__restrict__ clk,
+                                                     int rows) {
+  extern __shared__ double lds[];
+  const int nband = fdiv_nband(M, rows);
+  const int g = static_cast<int>(blockIdx.x) / nband;
+  const FdivBand B = fdiv_band(M, rows, static_cast<int>(blockIdx.x) - g * nband);
+  const int t = static_cast<int>(threadIdx.x);
+  const FdivThread T = fdiv_thread(M, B, t);
+  const double dt = clk->dt;
+  const double wght = beta * dt;
+  const int ni = M.nx1 + 2;
+  const int j0 = T.j, i0 = T.i;
+  const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+  const long ub = static_cast<long>(g) * NHYDRO * M.ncell;
+  const double *dx1 = ctab(M, g, 0), *dx2 = ctab(M, g, 1), *dx3 = ctab(M, g, 2);
+  double *F1 = lds;                        // [n][t]: x1 flux of the thread's face
+  double *F2 = lds + NHYDRO * WG;          // [n][t]: x2 flux
+  double *CD = lds + 2 * NHYDRO * WG;      // [n][t]: x1 + x2 terms of the owned cell
+  double *CP = lds + 3 * NHYDRO * WG;      // [n][t]: its low x3 flux
+
+  for (int k = M.ks - 1; k <= M.ke + 1; ++k) {
+    // the column indices are laundered each plane so that nothing derived from them is
+    // hoisted out of the march and held in registers across the Riemann solves
+    int j = j0, i = i0;
+    asm volatile("" : "+v"(j), "+v"(i));
+    double fx[NHYDRO];
+    if (T.x3 && k >= M.ks) {
+      face_flux<2, ORDER, RS>(M, g, k, j, i, wb, bb, o3, dt, true, fx);
+      if (T.own) {
+        if (k - 1 >= M.ks) {               // cell (k-1, j, i): its x3 term, then U
+          const int kk = k - 1;
+          const double a3 = dx1[i] * dx2[j];
+          const double vol = dx1[i] * dx2[j] * dx3[kk];
+          const long c = cidx(M, kk, j, i);
+#pragma unroll
+          for (int n = 0; n < NHYDRO; ++n) {
+            double dflx = CD[n * WG + t];
+            dflx += (a3 * fx[n] - a3 * CP[n * WG + t]);
+            double u0 = ubase[ub + n * M.ncell + c];
+            if (first_stage) u0 = 0.0 + 1.0 * u0;
+            uout[ub + n * M.ncell + c] = u0 - wght * dflx / vol;
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < NHYDRO; ++n) CP[n * WG + t] = fx[n];
+      }
+    }
+    if (T.x1) {
+      face_flux<0, ORDER, RS>(M, g, k, j, i, wb, bb, o1, dt, true, fx);
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) F1[n * WG + t] = fx[n];
+    }
+    if (T.x2) {
+      face_flux<1, ORDER, RS>(M, g, k, j, i, wb, bb, o2, dt, T.emit2, fx);
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) F2[n * WG + t] = fx[n];
+    }
+    __syncthreads();
+    if (T.own && k >= M.ks && k <= M.ke) {  // cell (k, j, i): x1 and x2 terms
+      const double a1 = dx2[j] * dx3[k];
+      const double a2 = dx1[i] * dx3[k];
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) {
+        double d = (a1 * F1[n * WG + t + 1] - a1 * F1[n * WG + t]);
+        d += (a2 * F2[n * WG + t + ni] - a2 * F2[n * WG + t]);
+        CD[n * WG + t] = d;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 //========================================================================================
@@ -845,12 +1020,12 @@ struct UpdateArgs {
 
 __device__ __forceinline__ void update_body(const MeshK &M, const UpdateArgs &a, double beta,
                                             int first_stage, double dt, int lw, int nwg,
-                                            TileK T) {
+                                            TileK T, bool do_u = true) {
   int g, i, j, k;
   if (!map_kind(M, KK_UPDATE, lw, nwg, T, g, i, j, k)) return;
   const double wght = beta * dt;
   const bool ci = i <= M.ie, cj = j <= M.je, ck = k <= M.ke;
-  if (ci && cj && ck)
+  if (do_u && ci && cj && ck)
     update_cell_u(M, g, k, j, i, a.ubase, a.uout, a.f1, a.f2, a.f3, wght, first_stage);
   if (cj && ck)
     update_face_b<0>(M, g, k, j, i, a.bin[0], a.bout[0], a.e1, a.e2, a.e3, wght, first_stage);
@@ -864,6 +1039,14 @@ __global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double bet
                                                 int first_stage,
                                                 const Clock *__restrict__ clk, TileK T) {
   update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+}
+
+// Field::CT alone (after k_flux_div has updated U).
+__global__ void __launch_bounds__(WG) k_update_b(MeshK M, UpdateArgs a, double beta,
+                                                  int first_stage,
+                                                  const Clock *__restrict__ clk, TileK T) {
+  update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap), gridDim.x, T,
+              false);
 }
 
 //========================================================================================
@@ -1239,13 +1422,14 @@ __global__ void __launch_bounds__(WG, C2P_MIN_WAVES) k_cons2prim(MeshK M, C2PArg
 // conserved state and the cell-centred field it is about to see.  Wavefront-collective
 // BF16 MFMA inference; a kernel of its own so that the FP64 recovery keeps its register
 // budget.  Dead lanes carry a dummy state.
-__global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restrict__ u,
-                                                  const double *__restrict__ b1,
-                                                  const double *__restrict__ b2,
-                                                  const double *__restrict__ b3,
-                                                  double *__restrict__ lseed, TileK T) {
+__device__ __forceinline__ void mesh_seed_body(const MeshK &M, const double *__restrict__ u,
+                                               const double *__restrict__ b1,
+                                               const double *__restrict__ b2,
+                                               const double *__restrict__ b3,
+                                               double *__restrict__ lseed, int lw, int nwg,
+                                               TileK T) {
   int g, i, j, k;
-  const bool live = map_kind(M, KK_C2P, xcd_logical_wg(M.xcd_remap), gridDim.x, T, g, i, j, k);
+  const bool live = map_kind(M, KK_C2P, lw, nwg, T, g, i, j, k);
   double cs[NHYDRO] = {1.0, 0.0, 0.0, 0.0, 2.0}, bs[3] = {0.0, 0.0, 0.0};
   const long c = live ? cidx(M, k, j, i) : 0;
   if (live) {
@@ -1263,6 +1447,14 @@ __global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restr
   mlp_features(cs, bs, M.mlp.P, x);
   const double sd = mlp_seed(cs, mlp_wave_infer_bf16(M.mlp, x));
   if (live) lseed[static_cast<long>(g) * M.ncell + c] = sd;
+}
+
+__global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restrict__ u,
+                                                  const double *__restrict__ b1,
+                                                  const double *__restrict__ b2,
+                                                  const double *__restrict__ b3,
+                                                  double *__restrict__ lseed, TileK T) {
+  mesh_seed_body(M, u, b1, b2, b3, lseed, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
 }
 
 //========================================================================================
@@ -1449,13 +1641,25 @@ __global__ void __launch_bounds__(WG) k_update_ens(const EnsMember *__restrict__
   }
 }
 
-template <bool SR>
+template <bool SR, bool LEARNED>
 __global__ void __launch_bounds__(WG, C2P_MIN_WAVES) k_c2p_ens(const EnsMember *__restrict__ E,
                                                  const int2 *__restrict__ map, int nwork) {
   for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
     const int2 mw = ens_item(map, v);
     const EnsMember &e = E[mw.x];
-    c2p_body<SR, false>(e.M, e.pa, mw.y, e.nwg[3], mw.y, e.tile[3]);
+    c2p_body<SR, LEARNED>(e.M, e.pa, mw.y, e.nwg[3], mw.y, e.tile[3]);
+  }
+}
+
+// The learned pressure seeds of every member (k_mesh_seed over the recovery's work map).
+__global__ void __launch_bounds__(WG) k_seed_ens(const EnsMember *__restrict__ E,
+                                                 const int2 *__restrict__ map, int nwork) {
+  for (int v = blockIdx.x; v < nwork; v += gridDim.x) {
+    const int2 mw = ens_item(map, v);
+    const EnsMember &e = E[mw.x];
+    // the recovery only reads the seed register, which this kernel fills
+    mesh_seed_body(e.M, e.pa.u, e.pa.b1, e.pa.b2, e.pa.b3, const_cast<double *>(e.pa.lseed),
+                   mw.y, e.nwg[3], e.tile[3]);
   }
 }
 
@@ -1738,6 +1942,12 @@ struct ath_mesh {
   long ndt_used = 0;          // entries written by the last cons2prim with dt
   // per work kind: linear chunks, the corner kernel's z-march over 8 planes
   TileK tile[4] = {{0, 0, 0}, {-1, 0, 8}, {0, 0, 0}, {0, 0, 0}};
+  int fdiv_rows = 0;          // > 0: stages use k_flux_div with bands of this many rows
+  // ath_gpu_cycle as one captured HIP graph per cycle (ath_gpu_mesh_set_graph)
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<const void *> graph_key;   // what the captured cycle depends on
   long nstage = 0;            // stages run (the register configuration's parity)
   int *nbr_d = nullptr;
   double *coord_d = nullptr;
@@ -1769,7 +1979,8 @@ enum KernelId {
   KID_FLUX = 0,          // + 3*dir + (order-1): 9 flux variants
   KID_CORNER = 9, KID_UPDATE_U = 10, KID_SEED = 11, KID_CONS2PRIM = 12,
   KID_NEW_DT = 13, KID_GHOST = 14, KID_FLUX3 = 15,   // + (order-1): 3 fused variants
-  KID_PHYS_BND = 18, KID_COUNT = 19
+  KID_PHYS_BND = 18, KID_FDIV = 19,                  // + (order-1): fused flux + divergence
+  KID_UPDATE_B = 22, KID_COUNT = 23
 };
 const char *kernel_name(int id) {
   static const char *flux[9] = {
@@ -1787,6 +1998,10 @@ const char *kernel_name(int id) {
     case KID_FLUX3 + 1: return "flux_xyz_plm";
     case KID_FLUX3 + 2: return "flux_xyz_ppm";
     case KID_PHYS_BND: return "physical_boundary";
+    case KID_FDIV: return "flux_div_dc";
+    case KID_FDIV + 1: return "flux_div_plm";
+    case KID_FDIV + 2: return "flux_div_ppm";
+    case KID_UPDATE_B: return "update_b_ct";
     default: return "unknown";
   }
 }
@@ -2051,12 +2266,65 @@ int launch_new_dt(ath_mesh *m, int advance) {
   return 0;
 }
 
-int launch_update_args(ath_mesh *m, const UpdateArgs &ua, double beta, int first) {
+int launch_update_args(ath_mesh *m, const UpdateArgs &ua, double beta, int first,
+                       bool do_u = true) {
   const dim3 grid = kind_grid(m, KK_UPDATE);
   const TileK T = m->tile[KK_UPDATE];
-  ProfScope ps(m, KID_UPDATE_U);
-  hipLaunchKernelGGL(k_update, grid, dim3(WG), 0, m->stream, m->M, ua, beta, first, m->clk,
-                     T);
+  ProfScope ps(m, do_u ? KID_UPDATE_U : KID_UPDATE_B);
+  if (do_u)
+    hipLaunchKernelGGL(k_update, grid, dim3(WG), 0, m->stream, m->M, ua, beta, first, m->clk,
+                       T);
+  else
+    hipLaunchKernelGGL(k_update_b, grid, dim3(WG), 0, m->stream, m->M, ua, beta, first,
+                       m->clk, T);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Rows per band of k_flux_div for a mesh: about 128 owned cells per work-group, at most
+// one per thread; 0 where a row of the block holds more active cells than a work-group
+// has threads.
+// Whether bands of up to `rows` active rows fit one work-group of k_flux_div: every
+// thread row of the band (its ghost row at a mesh edge and the extra x2 row included)
+// takes nx1 + 2 threads.
+bool fdiv_rows_ok(const MeshK &M, int rows) {
+  return rows >= 1 && rows <= M.nx2 &&
+         static_cast<long>(fdiv_thread_rows(rows)) * (M.nx1 + 2) <= WG;
+}
+
+// Band height that ath_gpu_mesh_set_fused(-1) picks: as many rows as a work-group holds;
+// 0 (separate flux and update kernels) where that is fewer than 2 rows, whose extra x2 row
+// would add a third more Riemann solves.
+int fdiv_default_rows(const MeshK &M) {
+  int rows = WG / (M.nx1 + 2) - 2;
+  if (rows > M.nx2) rows = M.nx2;
+  return (rows >= 2 && fdiv_rows_ok(M, rows)) ? rows : 0;
+}
+
+int launch_flux_div_args(ath_mesh *m, const StageArgs &a) {
+  const MeshK &M = m->M;
+  const int rows = m->fdiv_rows;
+  const int rs = solver_code(m->d);
+  const dim3 grid(static_cast<unsigned>(fdiv_nband(M, rows) * M.nblocks));
+  const size_t lds = sizeof(double) * FDIV_LDS_DOUBLES;
+  const UpdateArgs &ua = a.ua;
+  ProfScope ps(m, KID_FDIV + (a.order - 1));
+#define LF(O, R)                                                                          \
+  hipLaunchKernelGGL((k_flux_div<O, R>), grid, dim3(WG), lds, m->stream, M, a.w, a.bcc,     \
+                     a.fo[0], a.fo[1], a.fo[2], ua.ubase, ua.uout, a.beta, a.first, m->clk, \
+                     rows)
+#define LF_O(R)                         \
+  do {                                  \
+    if (a.order == 1) LF(1, R);         \
+    else if (a.order == 2) LF(2, R);    \
+    else LF(3, R);                      \
+  } while (0)
+  if (rs == 0) LF_O(0);
+  else if (rs == 1) LF_O(1);
+  else if (rs == 2) LF_O(2);
+  else LF_O(3);
+#undef LF_O
+#undef LF
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2103,15 +2371,82 @@ void set_regs(ath_mesh *m, const Regs &r) {
 int stage(ath_mesh *m, int stg) {
   Regs r = regs_of(m);
   const StageArgs a = stage_args(m, r, stg);
-  if (launch_flux_args(m, a.order, a.w, a.bcc, a.fo)) return 1;
+  const bool fused = m->fdiv_rows > 0;
+  if (fused) {
+    if (launch_flux_div_args(m, a)) return 1;
+  } else if (launch_flux_args(m, a.order, a.w, a.bcc, a.fo)) {
+    return 1;
+  }
   if (launch_corner_args(m, a.ca)) return 1;
-  if (launch_update_args(m, a.ua, a.beta, a.first)) return 1;
+  if (launch_update_args(m, a.ua, a.beta, a.first, !fused)) return 1;
   if (launch_ghost_args(m, a.ga)) return 1;
   if (launch_cons2prim_args(m, a.pa)) return 1;
   if (launch_phys_bnd(m, a.pa)) return 1;
   swap_stage_regs(r);
   set_regs(m, r);
   m->nstage += 1;
+  return 0;
+}
+
+// Everything a captured cycle bakes in: the register pointers at the start of a cycle
+// (a cycle's two stages swap every pair twice, so they are the same at the start of the
+// next), the fused band height, the tile shapes, the recovery mode and its network.
+std::vector<const void *> cycle_key(const ath_mesh *m) {
+  std::vector<const void *> k = {m->u, m->u1, m->w, m->w1, m->b[0], m->b[1], m->b[2],
+                                 m->bb[0], m->bb[1], m->bb[2], m->lseed, m->M.mlp.P,
+                                 m->M.c2p_stats};
+  k.push_back(reinterpret_cast<const void *>(static_cast<intptr_t>(m->fdiv_rows)));
+  k.push_back(reinterpret_cast<const void *>(static_cast<intptr_t>(m->M.c2p_mode)));
+  for (int q = 0; q < KK_N; ++q) {
+    const TileK &t = m->tile[q];
+    k.push_back(reinterpret_cast<const void *>(
+        static_cast<intptr_t>(((t.tx + 2) << 20) | (t.ty << 10) | t.tz)));
+  }
+  return k;
+}
+
+void drop_cycle_graph(ath_mesh *m) {
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->gexec = nullptr;
+  m->graph = nullptr;
+  m->graph_key.clear();
+}
+
+// One VL2 cycle (two stages and the new dt), launched eagerly.
+int cycle_eager(ath_mesh *m) {
+  if (stage(m, 1)) return 1;
+  if (stage(m, 2)) return 1;
+  return launch_new_dt(m, 1);
+}
+
+// One VL2 cycle replayed from the captured graph, capturing it first when the mesh's
+// configuration differs from the captured one.  The host register bookkeeping advances
+// as the eager cycle would: two stages, registers back where they were.
+int cycle_graph(ath_mesh *m) {
+  const std::vector<const void *> key = cycle_key(m);
+  if (!m->gexec || key != m->graph_key) {
+    drop_cycle_graph(m);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    const long nst = m->nstage;
+    const int rc = cycle_eager(m);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return 1;
+    }
+    if (ec != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    m->graph = g;
+    HIPCHK(hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
+    m->graph_key = key;
+    // capturing recorded the cycle without running it: undo the bookkeeping it advanced
+    if (cycle_key(m) != key) return fail("a captured cycle must leave the registers in place");
+    m->nstage = nst;
+  }
+  HIPCHK(hipGraphLaunch(m->gexec, m->stream));
+  m->nstage += 2;
   return 0;
 }
 
@@ -2146,6 +2481,8 @@ int block_count(ath_mesh *m, int what, long *n, double ***pool) {
 struct ath_ensemble {
   std::vector<ath_mesh *> mem;
   int xorder = 2, rs = 0, sr = 0;
+  int c2p_mode = 0;           // primitive-recovery mode shared by the members
+  bool learned = false;       // c2p_mode == C2P_LEARNED: seed kernel before the recovery
   hipStream_t stream = nullptr;
   EnsMember *tab_d[2] = {nullptr, nullptr};   // stage-1 and stage-2 member tables
   int2 *map_d[KK_N] = {nullptr, nullptr, nullptr, nullptr};
@@ -2280,9 +2617,13 @@ int ens_launch_kind(ath_ensemble *e, int kind, int stg) {
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_ghost_ens, dim3(static_cast<unsigned>(e->nwork_ghost)), dim3(WG), 0,
                        e->stream, T, e->map_ghost_d, e->nwork_ghost);
+  } else if (e->learned) {
+    hipLaunchKernelGGL(k_seed_ens, g, dim3(WG), 0, e->stream, T, mp, nw);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_c2p_ens<true, true>), g, dim3(WG), 0, e->stream, T, mp, nw);
   } else {
-    if (e->sr) hipLaunchKernelGGL(k_c2p_ens<true>, g, dim3(WG), 0, e->stream, T, mp, nw);
-    else hipLaunchKernelGGL(k_c2p_ens<false>, g, dim3(WG), 0, e->stream, T, mp, nw);
+    if (e->sr) hipLaunchKernelGGL((k_c2p_ens<true, false>), g, dim3(WG), 0, e->stream, T, mp, nw);
+    else hipLaunchKernelGGL((k_c2p_ens<false, false>), g, dim3(WG), 0, e->stream, T, mp, nw);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -2440,6 +2781,7 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   M.c2p_stats = nullptr;
   M.outflow = 0;
   for (int f = 0; f < 6; ++f) M.outflow |= d.bc[f];
+  m->fdiv_rows = 0;   // separate flux and update kernels: measured faster (see the header)
 
   // neighbour table and coordinates (coordinates.cpp:129-146, cartesian.cpp:23-72)
   const int nb = M.nblocks;
@@ -2562,6 +2904,7 @@ int ath_gpu_mesh_destroy(ath_mesh_t m) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
+  drop_cycle_graph(m);
   for (void *p : m->allocs) (void)hipFree(p);
   if (m->hist_d) (void)hipFree(m->hist_d);
   (void)hipStreamDestroy(m->stream);
@@ -2644,11 +2987,21 @@ int ath_gpu_cycle(ath_mesh_t m, int ncycles, double tlim, int *done) {
       HIPCHK(hipStreamSynchronize(m->stream));
       if (!(c.time < tlim)) break;
     }
-    if (stage(m, 1)) return 1;
-    if (stage(m, 2)) return 1;
-    if (launch_new_dt(m, 1)) return 1;
+    if (m->use_graph && !m->prof_on) {
+      if (cycle_graph(m)) return 1;
+    } else if (cycle_eager(m)) {
+      return 1;
+    }
   }
   if (done) *done = n;
+  return 0;
+}
+
+int ath_gpu_mesh_set_graph(ath_mesh_t m, int on) {
+  if (!m) return fail("null argument");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  m->use_graph = on != 0;
+  if (!m->use_graph) drop_cycle_graph(m);
   return 0;
 }
 
@@ -2714,13 +3067,14 @@ int ath_gpu_ensemble_create(const ath_mesh_t *members, int n, ath_ensemble_t *ou
       e->xorder = m->d.xorder;
       e->rs = rs;
       e->sr = m->d.relativistic;
+      e->c2p_mode = m->M.c2p_mode;
+      e->learned = m->d.relativistic && m->M.c2p_mode == C2P_LEARNED;
     } else if (m->d.xorder != e->xorder || rs != e->rs || m->d.relativistic != e->sr) {
       delete e;
       return fail("ensemble members must share xorder, Riemann solver and dynamics");
-    }
-    if (m->M.c2p_mode != C2P_REFERENCE) {
+    } else if (m->M.c2p_mode != e->c2p_mode) {
       delete e;
-      return fail("ensemble members must use the reference primitive recovery");
+      return fail("ensemble members must share the primitive-recovery mode");
     }
     if (m->M.outflow) {
       delete e;
@@ -3131,6 +3485,90 @@ int ath_gpu_sr_con2prim_batch(int n, double gamma, double dfloor, double pfloor,
   return rc;
 }
 
+
+int ath_gpu_mesh_set_fused(ath_mesh_t m, int rows) {
+  if (!m) return fail("null argument");
+  const MeshK &M = m->M;
+  if (rows < 0) rows = fdiv_default_rows(M);
+  if (rows > 0 && !fdiv_rows_ok(M, rows))
+    return fail("fused flux kernel: rows must be in 1 .. nx2 with (rows + 2) * (nx1 + 2) <= " +
+                std::to_string(WG));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  m->fdiv_rows = rows;
+  return 0;
+}
+
+int ath_gpu_mesh_get_fused(ath_mesh_t m, int *rows) {
+  if (!m || !rows) return fail("null argument");
+  *rows = m->fdiv_rows;
+  return 0;
+}
+
+int ath_gpu_flux_div_plan(const int *nx, int ng, int rows, int *faces, long max_faces,
+                          long *nfaces, int *cells, long max_cells, long *ncells,
+                          long *lds_doubles) {
+  if (!nx || !nfaces || !ncells || !lds_doubles) return fail("null argument");
+  if (ng < 1 || nx[0] < 1 || nx[1] < 1 || nx[2] < 1) return fail("bad block shape");
+  MeshK M{};
+  M.nx1 = nx[0]; M.nx2 = nx[1]; M.nx3 = nx[2]; M.ng = ng;
+  M.nc1 = M.nx1 + 2 * ng; M.nc2 = M.nx2 + 2 * ng; M.nc3 = M.nx3 + 2 * ng;
+  M.is = ng; M.ie = ng + M.nx1 - 1; M.js = ng; M.je = ng + M.nx2 - 1;
+  M.ks = ng; M.ke = ng + M.nx3 - 1;
+  if (!fdiv_rows_ok(M, rows))
+    return fail("rows must be in 1 .. nx2 with (rows + 2) * (nx1 + 2) <= " +
+                std::to_string(WG));
+  *lds_doubles = FDIV_LDS_DOUBLES;
+  *nfaces = 0;
+  *ncells = 0;
+  auto face = [&](int dir, int k, int j, int i) {
+    if (faces && *nfaces < max_faces) {
+      int *f = faces + 4 * *nfaces;
+      f[0] = dir; f[1] = k; f[2] = j; f[3] = i;
+    }
+    ++*nfaces;
+  };
+  const int ni = M.nx1 + 2;
+  std::vector<FdivThread> th(WG);
+  for (int b = 0; b < fdiv_nband(M, rows); ++b) {
+    const FdivBand B = fdiv_band(M, rows, b);
+    for (int t = 0; t < WG; ++t) th[t] = fdiv_thread(M, B, t);
+    // every live thread of the band's rows lies in its work-group
+    const int R = B.r1 - B.r0 + 1, nrow = R + ((B.q1 > B.r1) ? 1 : 0);
+    if (nrow * ni > WG) return fail("a band needs more threads than a work-group has");
+    for (int k = M.ks - 1; k <= M.ke + 1; ++k) {
+      for (int t = 0; t < WG; ++t) {
+        const FdivThread &T = th[t];
+        if (T.x3 && k >= M.ks) face(2, k, T.j, T.i);
+        if (T.x1) face(0, k, T.j, T.i);
+        if (T.x2 && T.emit2) face(1, k, T.j, T.i);
+        if (!T.own) continue;
+        // the owner's reads: F1[t], F1[t+1], F2[t], F2[t+ni] must hold the x1 faces i and
+        // i+1 and the x2 faces j and j+1 of its cell, computed by those threads
+        if (k >= M.ks && k <= M.ke) {
+          const int src[4] = {t, t + 1, t, t + ni};
+          const int dir[4] = {0, 0, 1, 1};
+          const int dj[4] = {0, 0, 0, 1}, di[4] = {0, 1, 0, 0};
+          for (int q = 0; q < 4; ++q) {
+            if (src[q] < 0 || src[q] >= WG) return fail("LDS read out of range");
+            const FdivThread &S = th[src[q]];
+            const bool has = dir[q] == 0 ? S.x1 : S.x2;
+            if (!has || S.j != T.j + dj[q] || S.i != T.i + di[q])
+              return fail("an owned cell reads a flux its neighbour does not compute");
+          }
+        }
+        if (!T.x3) return fail("an owned cell's column computes no x3 faces");
+        if (k - 1 >= M.ks && k - 1 <= M.ke) {
+          if (cells && *ncells < max_cells) {
+            int *c = cells + 3 * *ncells;
+            c[0] = k - 1; c[1] = T.j; c[2] = T.i;
+          }
+          ++*ncells;
+        }
+      }
+    }
+  }
+  return 0;
+}
 
 //----------------------------------------------------------------------------------------
 // Per-MeshBlock entries over caller-owned device arrays

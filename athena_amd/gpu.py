@@ -43,8 +43,10 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_block_riemann", "ath_gpu_block_calculate_fluxes",
            "ath_gpu_block_compute_corner_e", "ath_gpu_block_ct",
            "ath_gpu_block_add_flux_divergence", "ath_gpu_block_cons2prim",
-           "ath_gpu_block_new_dt", "ath_gpu_block_new_dt_noalloc"]
-NKERNELS = 19
+           "ath_gpu_block_new_dt", "ath_gpu_block_new_dt_noalloc",
+           "ath_gpu_mesh_set_fused", "ath_gpu_mesh_get_fused", "ath_gpu_flux_div_plan",
+           "ath_gpu_mesh_set_graph"]
+NKERNELS = 23
 
 
 class AthGpuError(RuntimeError):
@@ -137,6 +139,12 @@ def load():
     L.ath_gpu_sr_con2prim_device.argtypes = [P, I, I] + [ctypes.c_double] * 6 + \
         [P, P, P, P, P, P, P]
     L.ath_gpu_mesh_set_c2p.argtypes = [P, P, I]
+    LP = ctypes.POINTER(ctypes.c_long)
+    L.ath_gpu_mesh_set_fused.argtypes = [P, I]
+    L.ath_gpu_mesh_set_graph.argtypes = [P, I]
+    L.ath_gpu_mesh_get_fused.argtypes = [P, IP]
+    L.ath_gpu_flux_div_plan.argtypes = [IP, I, I, IP, ctypes.c_long, LP, IP, ctypes.c_long,
+                                        LP, LP]
     L.ath_gpu_mesh_c2p_stats.argtypes = [P, I, ctypes.POINTER(ctypes.c_ulonglong)]
     L.ath_gpu_ensemble_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.ath_gpu_ensemble_destroy.argtypes = [P]
@@ -348,6 +356,20 @@ class GpuMesh:
     def set_tiles(self, tile_idx):
         t = np.ascontiguousarray(tile_idx, dtype=np.int32)
         _check(load().ath_gpu_mesh_set_tiles(self.h, _ip(t)))
+
+    def set_fused(self, rows=-1):
+        """Rows per band of the fused flux + flux-divergence kernel (0: separate flux and
+        update kernels, -1: the default)."""
+        _check(load().ath_gpu_mesh_set_fused(self.h, int(rows)))
+
+    def set_graph(self, on=True):
+        """Replay each cycle of run() as one captured HIP graph (same bits as eager)."""
+        _check(load().ath_gpu_mesh_set_graph(self.h, int(bool(on))))
+
+    def fused(self):
+        r = ctypes.c_int()
+        _check(load().ath_gpu_mesh_get_fused(self.h, ctypes.byref(r)))
+        return r.value
 
     def tiles(self):
         t = np.zeros(4, dtype=np.int32)
@@ -711,3 +733,21 @@ class Block:
         device scratch array, without allocating."""
         _check(load().ath_gpu_block_new_dt_noalloc(self.h, w, bcc, b1, b2, b3, dt_out,
                                                    self.stream))
+
+
+def flux_div_plan(nx, ng, rows):
+    """The fused flux kernel's plan for one MeshBlock shape, checked on the host by the
+    library with the kernel's own index helpers: (faces (n, 4) as dir, k, j, i whose EMFs
+    and weights it writes; cells (n, 3) as k, j, i that it updates; LDS doubles per
+    work-group).  Raises AthGpuError where an LDS access would leave the array or read the
+    wrong face."""
+    nxa = np.ascontiguousarray(nx, dtype=np.int32)
+    nf, nc, nl = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+    _check(load().ath_gpu_flux_div_plan(_ip(nxa), ng, rows, None, 0, ctypes.byref(nf), None, 0,
+                                        ctypes.byref(nc), ctypes.byref(nl)))
+    faces = np.zeros((max(nf.value, 1), 4), dtype=np.int32)
+    cells = np.zeros((max(nc.value, 1), 3), dtype=np.int32)
+    _check(load().ath_gpu_flux_div_plan(_ip(nxa), ng, rows, _ip(faces), nf.value,
+                                        ctypes.byref(nf), _ip(cells), nc.value,
+                                        ctypes.byref(nc), ctypes.byref(nl)))
+    return faces[:nf.value], cells[:nc.value], nl.value

@@ -244,6 +244,33 @@ int ath_gpu_block_new_dt_noalloc(ath_block_t blk, const double *w, const double 
                                  const double *b1, const double *b2, const double *b3,
                                  double *dt_out, void *stream);
 
+/* ath_gpu_cycle with on != 0 replays each cycle (both stages and the new dt) as one
+ * captured HIP graph, recaptured whenever the mesh's registers, tiles, fused band height
+ * or recovery mode differ from the captured ones; the state evolves bit for bit as in the
+ * eager cycle.  Kernel timing (ath_gpu_kernel_timing) runs eagerly.  Off by default.  No
+ * counterpart in the reference. */
+int ath_gpu_mesh_set_graph(ath_mesh_t mesh, int on);
+
+/* Optional fused stage: the fluxes and Hydro::AddFluxDivergence in one kernel that
+ * marches up the planes of bands of `rows` active rows of each MeshBlock, one thread per
+ * (j, i) column, and keeps the fluxes on chip (Hydro::flux is then not written by
+ * ath_gpu_vl2_stage / ath_gpu_cycle; ath_gpu_calculate_fluxes still fills it).  rows = 0
+ * (the default) keeps the separate flux and update kernels, which measured faster on the
+ * 256^3 HLLD and SR HLLE workloads; -1 picks the tallest band a work-group holds.  Both
+ * give the same bits.  No counterpart in the reference. */
+int ath_gpu_mesh_set_fused(ath_mesh_t mesh, int rows);
+int ath_gpu_mesh_get_fused(ath_mesh_t mesh, int *rows);
+
+/* Host check of that kernel's plan for one MeshBlock shape (nx[3], ng) and band height:
+ * enumerates every band and plane with the kernel's own index helpers, fails if an LDS
+ * access leaves the work-group's array or an owned cell reads a position that does not
+ * hold the face it needs, and lists the faces whose EMFs and weight are written
+ * (dir, k, j, i) and the cells updated (k, j, i), with their counts and the LDS doubles of
+ * one work-group.  No GPU needed. */
+int ath_gpu_flux_div_plan(const int *nx, int ng, int rows, int *faces, long max_faces,
+                          long *nfaces, int *cells, long max_cells, long *ncells,
+                          long *lds_doubles);
+
 /* ---- Ensembles ------------------------------------------------------------------------
  * Independent meshes of uneven sizes advanced together: each VL2 stage kernel is
  * launched once for all members (persistent work-groups over a (member, work-group)
@@ -251,8 +278,9 @@ int ath_gpu_block_new_dt_noalloc(ath_block_t blk, const double *w, const double 
  * persistent grid of every kernel are chosen by a built-in autotuner.  No counterpart
  * in the reference, which advances one Mesh per process (main.cpp:457-560); each member
  * evolves exactly as ath_gpu_cycle would advance it on its own, with its own dt.
- * Members must share xorder, Riemann solver and dynamics, be initialised
- * (ath_gpu_initialize) and use the reference primitive recovery; while in an ensemble
+ * Members must share xorder, Riemann solver, dynamics and primitive-recovery mode
+ * (ath_gpu_mesh_set_c2p; with the learned guess a seed kernel runs over all members
+ * before the recovery) and be initialised (ath_gpu_initialize); while in an ensemble
  * they must not run single stages (ath_gpu_vl2_stage), which the ensemble detects. */
 typedef struct ath_ensemble *ath_ensemble_t;
 

@@ -1383,6 +1383,11 @@ void ora_state(void *h, double *time, double *dt, int *ncycle) {
 
 void ora_set_dt(void *h, double dt) { static_cast<Mesh *>(h)->dt = dt; }
 
+void ora_set_clock(void *h, double time, double dt, int ncycle) {
+  Mesh &m = *static_cast<Mesh *>(h);
+  m.time = time; m.dt = dt; m.ncycle = ncycle;
+}
+
 int ora_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -46,6 +46,11 @@ def lib():
         P = ctypes.c_void_p
         D = ctypes.POINTER(ctypes.c_double)
         L.ora_new.restype = P
+        L.ora_restart_read.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int), D,
+                                       ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, D,
+                                       ctypes.c_long]
+        L.ora_restart_error.restype = ctypes.c_char_p
         L.ora_new.argtypes = [ctypes.POINTER(OraCfg)]
         L.ora_free.argtypes = [P]
         L.ora_nblocks.argtypes = [P]
@@ -58,6 +63,7 @@ def lib():
         L.ora_run.restype = ctypes.c_int
         L.ora_state.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_int)]
         L.ora_set_dt.argtypes = [P, ctypes.c_double]
+        L.ora_set_clock.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.ora_threads.restype = ctypes.c_int
         L.ora_block_call.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_int), D]
@@ -138,6 +144,9 @@ class OracleMesh:
     def run(self, ncycles, tlim=1e300):
         return lib().ora_run(self.h, ncycles, tlim)
 
+    def set_clock(self, time, dt, ncycle):
+        lib().ora_set_clock(self.h, float(time), float(dt), int(ncycle))
+
     def state(self):
         t, dt, nc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         lib().ora_state(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(nc))
@@ -217,3 +226,22 @@ def mlp_features(cons, bcc):
     x = np.empty((cons.shape[0], 6))
     lib().ora_mlp_features(cons.shape[0], _dp(cons), _dp(bcc), _dp(x))
     return x
+
+
+def restart_read(path, ng, nblocks_cap, doubles_cap):
+    """The restatement of the reference's restart reader (oracle/restart_oracle.cpp):
+    (ints, reals, locations (n, 4) as level, lx1, lx2, lx3 in the order the data is
+    assigned, data (doubles of every block's u, b.x1f, b.x2f, b.x3f))."""
+    L = lib()
+    ints = (ctypes.c_int * 11)()
+    reals = np.zeros(11)
+    locs = np.zeros(4 * nblocks_cap, dtype=np.int64)
+    data = np.zeros(doubles_cap)
+    rc = L.ora_restart_read(path.encode(), ng, ints,
+                            reals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            locs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                            nblocks_cap, data.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            doubles_cap)
+    if rc != 0:
+        raise RuntimeError(L.ora_restart_error().decode())
+    return list(ints), reals, locs.reshape(-1, 4), data

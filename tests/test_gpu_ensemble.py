@@ -191,3 +191,59 @@ def test_single_mesh_tiles_and_autotune(solo):
     for bad in ([6, 0, 0, 0], [0, 0, 7, 0]):   # the z-march is for the corner kernel only
         with pytest.raises(AthGpuError):
             fresh([(m, ic)])[0].set_tiles(bad)
+
+
+@pytest.mark.parametrize("mode", ["learned", "newton"])
+def test_sr_members_with_recovery_modes_match_solo(mode):
+    """Relativistic members using the learned seed (a seed launch over all members, then
+    the learned recovery) or Newton recovery inside an ensemble, eager and replayed as a
+    HIP graph, evolve bit for bit as on their own in the same mode."""
+    from athena_amd import ml
+    from athena_amd.gpu import C2P_MODES
+    from athena_amd.pgen.sr import ShearLayer, shear_mesh
+    print("case SR ensemble recovery mode %s" % mode, flush=True)
+    net = ml.load_mlp() if mode == "learned" else None
+    specs = [(shear_mesh(16, 8), {}), (shear_mesh(24, 8), {"amp": 3.0e-3}),
+             (shear_mesh(16, 16), {"vshear": 0.3})]
+
+    def make():
+        out = []
+        for m, kw in specs:
+            ic = ShearLayer(m, **kw)
+            d = GpuMesh(m)
+            d.load([ic.block(g) for g in range(m.nblocks)])
+            d.set_c2p(C2P_MODES[mode], net)
+            d.initialize()
+            out.append(d)
+        return out
+
+    solo = make()
+    snaps = []
+    for _ in range(3):
+        for d in solo:
+            d.run(1)
+        snaps.append([snapshot(d) for d in solo])
+    for graph in (False, True):
+        meshes = make()
+        ens = Ensemble(meshes)
+        for c in range(3):
+            ens.run(1, graph=graph)
+            for d, r in zip(meshes, snaps[c]):
+                assert_same(snapshot(d), r)
+        ens.close()
+
+
+def test_members_with_different_recovery_modes_are_refused():
+    from athena_amd.gpu import C2P_MODES
+    from athena_amd.pgen.sr import ShearLayer, shear_mesh
+    meshes = []
+    for mode in ("reference", "newton"):
+        m = shear_mesh(16, 8)
+        ic = ShearLayer(m)
+        d = GpuMesh(m)
+        d.load([ic.block(g) for g in range(m.nblocks)])
+        d.set_c2p(C2P_MODES[mode], None)
+        d.initialize()
+        meshes.append(d)
+    with pytest.raises(AthGpuError, match="primitive-recovery mode"):
+        Ensemble(meshes)

@@ -1,11 +1,16 @@
 """Restart files: the reference's binary layout (outputs/restart.cpp:36-240), block
-order of the reference's tree (meshblock_tree.cpp:87-110, 336-352) and, on the GPU,
-checkpoint/resume that continues bit for bit."""
+order of the reference's tree (meshblock_tree.cpp:87-110, 336-352), the file read back
+by a restatement of the reference's own reader (oracle/restart_oracle.cpp: parameter
+parsing, packed header, ID list, tree reconstruction and block offsets as mesh.cpp and
+meshblock.cpp read them), which also catches planted writer mistakes (block order,
+header padding), and, on the GPU, checkpoint/resume that continues bit for bit, on the
+device and on the oracle from the file read the reference's way."""
 import os
 
 import numpy as np
 import pytest
 
+import oracle_bind
 from athena_amd import restart
 from athena_amd.mesh import MeshDesc
 
@@ -72,6 +77,89 @@ def test_round_trip_and_layout(tmp_path):
                                      + 40 * m.nblocks + m.nblocks * datasize)
 
 
+def _sample(m, seed=0):
+    rng = np.random.default_rng(seed)
+    return {(g, k): rng.normal(size=m.shape(k)) for g in range(m.nblocks)
+            for k in ("u", "b1", "b2", "b3")}
+
+
+def _per_block(m):
+    return sum(int(np.prod(m.shape(k))) for k in ("u", "b1", "b2", "b3"))
+
+
+def _split(m, flat):
+    out, p = {}, 0
+    for k in ("u", "b1", "b2", "b3"):
+        n = int(np.prod(m.shape(k)))
+        out[k] = flat[p:p + n].reshape(m.shape(k))
+        p += n
+    return out
+
+
+def read_as_reference(path, m):
+    """The file read by the restatement of the reference's reader: the header, the block
+    size it takes from <meshblock>, and each block's registers under the location the
+    reference would assign them to."""
+    per = _per_block(m)
+    ints, reals, locs, data = oracle_bind.restart_read(path, m.ng, m.nblocks,
+                                                       per * m.nblocks)
+    blocks = {}
+    for i, (level, l1, l2, l3) in enumerate(locs[:ints[0]]):
+        blocks[m.gid(int(l1), int(l2), int(l3))] = _split(m, data[per * i:per * (i + 1)])
+    return ints, reals, blocks
+
+
+@pytest.mark.parametrize("nx,bnx,ng", [((24, 16, 8), (8, 8, 8), 2), ((32, 32, 32), (8, 16, 16), 3),
+                                       ((16, 16, 16), (16, 16, 16), 2),
+                                       ((48, 24, 40), (8, 8, 8), 4)])
+def test_reference_reader_reads_the_written_file(tmp_path, nx, bnx, ng):
+    m = MeshDesc(nx=nx, bnx=bnx, xmin=(-1.0, 0.0, 0.0), xmax=(1.0, 1.0, 0.5), ng=ng)
+    data = _sample(m)
+    path = str(tmp_path / "r.rst")
+    restart.write(path, m, lambda g, k: data[(g, k)], 0.375, 2.5e-3, 41)
+    ints, reals, blocks = read_as_reference(path, m)
+    assert ints[:3] == [m.nblocks, restart.root_level(m.nbx), 41]
+    assert tuple(ints[3:6]) == m.bnx and tuple(ints[6:9]) == m.nx
+    assert ints[9] == 1                       # the file lists its blocks in tree order
+    assert tuple(reals[:2]) == (0.375, 2.5e-3)
+    assert tuple(reals[2:8]) == (-1.0, 0.0, 0.0, 1.0, 1.0, 0.5)
+    assert tuple(reals[8:11]) == (1.0, 1.0, 1.0)
+    assert sorted(blocks) == list(range(m.nblocks))
+    for g in range(m.nblocks):
+        for k in ("u", "b1", "b2", "b3"):
+            assert np.array_equal(blocks[g][k], data[(g, k)]), (g, k)
+
+
+def test_reference_reader_catches_planted_writer_mistakes(tmp_path, monkeypatch):
+    m = MeshDesc(nx=(32, 16, 16), bnx=(8, 8, 8), xmin=(0.0,) * 3, xmax=(1.0,) * 3)
+    data = _sample(m, 1)
+    get = lambda g, k: data[(g, k)]    # noqa: E731
+    # blocks listed row-major instead of in the tree's order: the reference would put
+    # one block's data under another's location
+    rowmajor = [(l1, l2, l3) for l3 in range(m.nbx[2]) for l2 in range(m.nbx[1])
+                for l1 in range(m.nbx[0])]
+    assert rowmajor != restart.ztree_order(m.nbx)
+    path = str(tmp_path / "order.rst")
+    with monkeypatch.context() as mp:
+        mp.setattr(restart, "ztree_order", lambda nbx: rowmajor)
+        restart.write(path, m, get, 0.0, 1e-3, 0)
+    ints, _, blocks = read_as_reference(path, m)
+    assert ints[9] == 0
+    assert any(not np.array_equal(blocks[g]["u"], data[(g, "u")]) for g in range(m.nblocks))
+    # RegionSize written without its 4 bytes of tail padding: the reader's packed header
+    # is shifted, so the time, dt and block sizes no longer come out
+    path = str(tmp_path / "pad.rst")
+    import struct
+    with monkeypatch.context() as mp:
+        mp.setattr(restart, "REGION", struct.Struct("<9d3i"))
+        restart.write(path, m, get, 0.5, 1e-3, 3)
+    try:
+        ints, reals, _ = read_as_reference(path, m)
+        assert not (reals[0] == 0.5 and reals[1] == 1e-3 and ints[2] == 3)
+    except RuntimeError:
+        pass                                  # refused as broken: also caught
+
+
 @pytest.mark.gpu
 def test_checkpoint_resume_is_bitwise(tmp_path):
     from athena_amd.gpu import GpuMesh
@@ -98,3 +186,15 @@ def test_checkpoint_resume_is_bitwise(tmp_path):
     for g in range(m.nblocks):
         for k in ("u", "w", "bcc", "b1", "b2", "b3"):
             assert np.array_equal(a.get(g, k), b.get(g, k)), (g, k)
+    # the same file read by the restatement of the reference's reader and continued by
+    # the oracle ends where the device run ends
+    ints, reals, rblocks = read_as_reference(path, m)
+    o = oracle_bind.OracleMesh(m)
+    o.load([rblocks[g] for g in range(m.nblocks)])
+    o.initialize()
+    o.set_clock(reals[0], reals[1], ints[2])
+    o.run(3)
+    assert o.state()[:2] == a.state()[:2]
+    for g in range(m.nblocks):
+        for k in ("u", "w", "bcc", "b1", "b2", "b3"):
+            assert np.array_equal(a.get(g, k), o.get(g, k)), ("oracle", g, k)
