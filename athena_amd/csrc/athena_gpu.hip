@@ -72,8 +72,12 @@ struct MeshK {
   int xcd_remap;              // 1: XCD-contiguous work-group numbering (ATH_XCD_REMAP=1)
   int c2p_mode;               // SR primitive recovery: C2PMode
   MlpK mlp;                   // network of C2P_LEARNED
-  unsigned long long *c2p_stats;   // null, or [evals, fallbacks, cells] accumulators
+  unsigned long long *c2p_stats;   // null, or C2P_STAT_SLOTS x [evals, fallbacks, cells]
 };
+
+// The SR recovery counters are spread over this many slots of three accumulators, chosen
+// by wavefront, so that their atomics do not queue on one address; a read sums the slots.
+constexpr int C2P_STAT_SLOTS = 4096;
 
 struct Clock {
   double time, dt, tlim;
@@ -1376,7 +1380,7 @@ __device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int l
   double my_dt = DBL_MAX;
   C2PStats st{0, 0};
   if (live) c2p_cell<SR, LEARNED>(M, a, g, k, j, i, dt_partial != nullptr, &my_dt, &st);
-  if (SR && M.c2p_stats) {   // wavefront sums, one atomic per wavefront
+  if (SR && M.c2p_stats) {   // wavefront sums, one atomic per wavefront into its slot
     unsigned long long ev = st.evals, fb = st.fallback, nc = live ? 1 : 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1385,9 +1389,12 @@ __device__ __forceinline__ void c2p_body(const MeshK &M, const C2PArgs &a, int l
       nc += __shfl_down(nc, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&M.c2p_stats[0], ev);
-      atomicAdd(&M.c2p_stats[1], fb);
-      atomicAdd(&M.c2p_stats[2], nc);
+      const int sl = (static_cast<int>(blockIdx.x) * (WG / 64) + (threadIdx.x >> 6)) %
+                     C2P_STAT_SLOTS;
+      unsigned long long *c = M.c2p_stats + 3 * sl;
+      atomicAdd(&c[0], ev);
+      atomicAdd(&c[1], fb);
+      atomicAdd(&c[2], nc);
     }
   }
   if (dt_partial) {
@@ -3368,15 +3375,20 @@ int ath_gpu_mesh_set_c2p(ath_mesh_t m, ath_mlp_t mlp, int mode) {
 int ath_gpu_mesh_c2p_stats(ath_mesh_t m, int enable, unsigned long long *totals) {
   if (!m) return fail("null argument");
   HIPCHK(hipStreamSynchronize(m->stream));
+  const size_t bytes = 3 * C2P_STAT_SLOTS * sizeof(unsigned long long);
   if (!m->c2p_stats_d) {
-    HIPCHK(hipMalloc(reinterpret_cast<void **>(&m->c2p_stats_d), 3 * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&m->c2p_stats_d), bytes));
     m->allocs.push_back(m->c2p_stats_d);
-    HIPCHK(hipMemset(m->c2p_stats_d, 0, 3 * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(m->c2p_stats_d, 0, bytes));
   }
-  if (totals)
-    HIPCHK(hipMemcpy(totals, m->c2p_stats_d, 3 * sizeof(unsigned long long),
-                     hipMemcpyDeviceToHost));
-  if (enable > 0) HIPCHK(hipMemset(m->c2p_stats_d, 0, 3 * sizeof(unsigned long long)));
+  if (totals) {
+    std::vector<unsigned long long> h(3 * C2P_STAT_SLOTS);
+    HIPCHK(hipMemcpy(h.data(), m->c2p_stats_d, bytes, hipMemcpyDeviceToHost));
+    totals[0] = totals[1] = totals[2] = 0;
+    for (int sl = 0; sl < C2P_STAT_SLOTS; ++sl)
+      for (int q = 0; q < 3; ++q) totals[q] += h[3 * sl + q];
+  }
+  if (enable > 0) HIPCHK(hipMemset(m->c2p_stats_d, 0, bytes));
   if (enable >= 0) m->M.c2p_stats = enable ? m->c2p_stats_d : nullptr;
   return 0;
 }

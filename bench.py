@@ -393,12 +393,14 @@ def bench_sr_shear(args):
         d.run(args.steps)
         ev.record(e1, stream)
         ms = ev.elapsed_ms(e0, e1)
-        # recovery counters (atomics) and per-kernel events only outside the timed cycles
-        d.c2p_stats(1)
+        # per-kernel events, then the recovery counters (atomics), each in cycles of its
+        # own outside the timed ones
         d.kernel_timing(True)
         d.run(2)
         stats = d.kernel_stats()
         d.kernel_timing(False)
+        d.c2p_stats(1)
+        d.run(2)
         evals, fb, nsolve = d.c2p_stats(0)
         zcs = args.steps * cells / (ms * 1e-3)
         runs[name] = {"zone_cycles_per_s": zcs, "ms_per_step": ms / args.steps,
@@ -428,7 +430,7 @@ def bench_sr_shear(args):
     ens.run(args.steps, graph=True)
     ev.record(e1, stream)
     ms = ev.elapsed_ms(e0, e1)
-    ens.run(2, graph=True)
+    ens.run(4, graph=True)                  # the eager runs' timing and counter cycles
     same = all(np.array_equal(d.get(g, "u"), finals["learned"][g]) for g in range(m.nblocks))
     zcs = args.steps * cells / (ms * 1e-3)
     runs["learned_ensemble_graph"] = {"zone_cycles_per_s": zcs, "ms_per_step": ms / args.steps,
