@@ -649,6 +649,17 @@ def main():
     step_roof = {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE,
                  "achieved": zcs * BYTES_PER_ZONE_CYCLE / 1e9, "unit": "GB/s",
                  "frac": zcs * BYTES_PER_ZONE_CYCLE / HBM_PEAK}
+    if measured and all(k in MEASURED_TRAFFIC for k in stats if k != "new_dt"):
+        # the HBM bytes the cycle's launches actually move (the profile's counters times
+        # this run's launches per cycle) over this run's cycle time: how busy HBM is,
+        # beside the compulsory 536-byte figure above
+        per_cycle = sum(MEASURED_TRAFFIC[k] * c / args.steps for k, (_, c) in stats.items()
+                        if k != "new_dt")
+        step_roof["measured_traffic"] = {
+            "bytes_per_cycle": per_cycle,
+            "achieved": per_cycle / (ms / args.steps * 1e-3) / 1e9, "unit": "GB/s",
+            "frac": per_cycle / (ms / args.steps * 1e-3) / HBM_PEAK,
+            "source": "per-launch FETCH_SIZE/WRITE_SIZE of %s" % TRAFFIC_SOURCE}
 
     cpu = None
     if not args.no_cpu:
