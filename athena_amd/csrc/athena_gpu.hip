@@ -69,7 +69,7 @@ struct MeshK {
   const double *coord;        // [nblocks][cstride]: per d: dxf[nc], pf[nc], mf[nc]
   EosK eos;
   double cfl;
-  int xcd_remap;              // 1: XCD-contiguous work-group numbering (ATH_XCD_REMAP=1)
+  int xcd_remap;              // per-kernel XCD-contiguous numbering, bits XCD_FLUX ..
   int c2p_mode;               // SR primitive recovery: C2PMode
   MlpK mlp;                   // network of C2P_LEARNED
   unsigned long long *c2p_stats;   // null, or C2P_STAT_SLOTS x [evals, fallbacks, cells]
@@ -120,6 +120,10 @@ __device__ __forceinline__ const double *ctab(const MeshK &M, int g, int d) {
 // receives form one contiguous logical range keeps a MeshBlock (and its stencil
 // neighbours) inside one L2.  Bijective for any grid size.
 constexpr int NXCD = 8;
+// Bits of MeshK::xcd_remap (environment variable ATH_XCD_REMAP, a decimal mask; 31 remaps
+// every kernel): which kernels number their work-groups XCD-contiguously.
+enum XcdBits { XCD_FLUX = 1, XCD_CORNER = 2, XCD_UPDATE = 4, XCD_GHOST = 8, XCD_C2P = 16 };
+
 __device__ __forceinline__ int xcd_logical_wg(int remap) {
   const int w = blockIdx.x, n = gridDim.x;
   if (!remap) return w;
@@ -387,7 +391,7 @@ template <int DIR, int ORDER, int RS>
 __global__ void __launch_bounds__(WG) k_flux(MeshK M, const double *__restrict__ w,
                                               const double *__restrict__ bcc, FaceOut o,
                                               const Clock *__restrict__ clk) {
-  const int lw = xcd_logical_wg(M.xcd_remap);
+  const int lw = xcd_logical_wg(M.xcd_remap & XCD_FLUX);
   const int nch = gridDim.x / M.nblocks;
   const int g = lw / nch;
   int ni, nj, nk, i0, j0, k0;
@@ -426,7 +430,7 @@ __global__ void __launch_bounds__(WG) k_flux3(MeshK M, const double *__restrict_
                                                const double *__restrict__ bcc, FaceOut o1,
                                                FaceOut o2, FaceOut o3,
                                                const Clock *__restrict__ clk, TileK T) {
-  flux3_body<ORDER, RS>(M, w, bcc, o1, o2, o3, clk->dt, xcd_logical_wg(M.xcd_remap),
+  flux3_body<ORDER, RS>(M, w, bcc, o1, o2, o3, clk->dt, xcd_logical_wg(M.xcd_remap & XCD_FLUX),
                         gridDim.x, T);
 }
 
@@ -824,12 +828,12 @@ __device__ __forceinline__ void corner_march(const MeshK &M, const CornerArgs &c
 
 template <bool SR>
 __global__ void __launch_bounds__(WG) k_corner(MeshK M, CornerArgs ca, TileK T) {
-  corner_body<SR>(M, ca, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+  corner_body<SR>(M, ca, xcd_logical_wg(M.xcd_remap & XCD_CORNER), gridDim.x, T);
 }
 
 template <bool SR>
 __global__ void __launch_bounds__(WG) k_corner_march(MeshK M, CornerArgs ca, TileK T) {
-  corner_march<SR>(M, ca, xcd_logical_wg(M.xcd_remap), gridDim.x, T.tz);
+  corner_march<SR>(M, ca, xcd_logical_wg(M.xcd_remap & XCD_CORNER), gridDim.x, T.tz);
 }
 
 //========================================================================================
@@ -1042,14 +1046,16 @@ __device__ __forceinline__ void update_body(const MeshK &M, const UpdateArgs &a,
 __global__ void __launch_bounds__(WG) k_update(MeshK M, UpdateArgs a, double beta,
                                                 int first_stage,
                                                 const Clock *__restrict__ clk, TileK T) {
-  update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+  update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap & XCD_UPDATE),
+              gridDim.x, T);
 }
 
 // Field::CT alone (after k_flux_div has updated U).
 __global__ void __launch_bounds__(WG) k_update_b(MeshK M, UpdateArgs a, double beta,
                                                   int first_stage,
                                                   const Clock *__restrict__ clk, TileK T) {
-  update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap), gridDim.x, T,
+  update_body(M, a, beta, first_stage, clk->dt, xcd_logical_wg(M.xcd_remap & XCD_UPDATE),
+              gridDim.x, T,
               false);
 }
 
@@ -1182,7 +1188,7 @@ __device__ __forceinline__ void ghost_body(const MeshK &M, const GhostArgs &a, i
 }
 
 __global__ void __launch_bounds__(WG) k_ghost(MeshK M, GhostArgs a) {
-  ghost_body(M, a, xcd_logical_wg(M.xcd_remap), gridDim.x);
+  ghost_body(M, a, xcd_logical_wg(M.xcd_remap & XCD_GHOST), gridDim.x);
 }
 
 //========================================================================================
@@ -1421,7 +1427,7 @@ constexpr int C2P_MIN_WAVES = 3;
 
 template <bool SR, bool LEARNED>
 __global__ void __launch_bounds__(WG, C2P_MIN_WAVES) k_cons2prim(MeshK M, C2PArgs a, TileK T) {
-  c2p_body<SR, LEARNED>(M, a, xcd_logical_wg(M.xcd_remap), gridDim.x, blockIdx.x, T);
+  c2p_body<SR, LEARNED>(M, a, xcd_logical_wg(M.xcd_remap & XCD_C2P), gridDim.x, blockIdx.x, T);
 }
 
 
@@ -1461,7 +1467,7 @@ __global__ void __launch_bounds__(WG) k_mesh_seed(MeshK M, const double *__restr
                                                   const double *__restrict__ b2,
                                                   const double *__restrict__ b3,
                                                   double *__restrict__ lseed, TileK T) {
-  mesh_seed_body(M, u, b1, b2, b3, lseed, xcd_logical_wg(M.xcd_remap), gridDim.x, T);
+  mesh_seed_body(M, u, b1, b2, b3, lseed, xcd_logical_wg(M.xcd_remap & XCD_C2P), gridDim.x, T);
 }
 
 //========================================================================================
@@ -2781,7 +2787,7 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   M.cfl = d.cfl;
   {
     const char *env = std::getenv("ATH_XCD_REMAP");
-    M.xcd_remap = (env && env[0] == '1') ? 1 : 0;
+    M.xcd_remap = env ? (std::atoi(env) & 31) : 0;
   }
   M.c2p_mode = C2P_REFERENCE;
   M.mlp = MlpK{};

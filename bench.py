@@ -65,12 +65,12 @@ class HipEvents:
 # linear), fetched KiB x 2.0 + written KiB x 1.0 (the factors a 512 MiB device copy
 # gives): profiles/vl2_hlld_plm_turbulence256_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    "flux_xyz_plm": 11447612416,
-    "flux_xyz_dc": 8295108608,
-    "update_u_b_ct": 7937490944,
-    "corner_emf": 3592592384,
-    "cons2prim": 3309689856,
-    "ghost_exchange": 2298982400,
+    "flux_xyz_plm": 11398313984,
+    "flux_xyz_dc": 8297477120,
+    "update_u_b_ct": 7938470912,
+    "corner_emf": 3726039040,
+    "cons2prim": 3309647872,
+    "ghost_exchange": 2298912768,
 }
 TRAFFIC_TILES = ["linear", "zmarch8", "linear", "linear"]
 # FP64 FLOP per launch (64 x (ADD + MUL + 2 FMA) of the SQ_INSTS_VALU_*_F64 counters) on
