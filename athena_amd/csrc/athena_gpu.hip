@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -24,6 +25,8 @@
 #include "../../include/athena_gpu.h"
 #include "mlp.hpp"
 #include "physics.hpp"
+#include "refine_ops.hpp"
+#include "smr_plan.hpp"
 
 using namespace athgpu;
 
@@ -66,6 +69,7 @@ struct MeshK {
   const int *nbr;             // [nblocks][27] same-level neighbour gid, -1 across an
                               // outflow face of the mesh
   int outflow;                // 1: some mesh face is an outflow boundary
+  int smr;                    // 1: statically refined mesh (EMFs corrected beforehand)
   const double *coord;        // [nblocks][cstride]: per d: dxf[nc], pf[nc], mf[nc]
   EosK eos;
   double cfl;
@@ -857,6 +861,7 @@ __device__ __forceinline__ double edge_scale(double v, int cnt) {
 __device__ __forceinline__ double emf3(const MeshK &M, const double *e, int g, int k,
                                        int j, int i) {
   double v = e[static_cast<long>(g) * M.ne3 + e3idx(M, k, j, i)];
+  if (M.smr) return v;
   const int o1 = (i == M.is) ? -1 : ((i == M.ie + 1) ? 1 : 0);
   const int o2 = (j == M.js) ? -1 : ((j == M.je + 1) ? 1 : 0);
   if (!o1 && !o2) return v;
@@ -884,6 +889,7 @@ __device__ __forceinline__ double emf3(const MeshK &M, const double *e, int g, i
 __device__ __forceinline__ double emf2(const MeshK &M, const double *e, int g, int k,
                                        int j, int i) {
   double v = e[static_cast<long>(g) * M.ne2 + e2idx(M, k, j, i)];
+  if (M.smr) return v;
   const int o1 = (i == M.is) ? -1 : ((i == M.ie + 1) ? 1 : 0);
   const int o3 = (k == M.ks) ? -1 : ((k == M.ke + 1) ? 1 : 0);
   if (!o1 && !o3) return v;
@@ -911,6 +917,7 @@ __device__ __forceinline__ double emf2(const MeshK &M, const double *e, int g, i
 __device__ __forceinline__ double emf1(const MeshK &M, const double *e, int g, int k,
                                        int j, int i) {
   double v = e[static_cast<long>(g) * M.ne1 + e1idx(M, k, j, i)];
+  if (M.smr) return v;
   const int o2 = (j == M.js) ? -1 : ((j == M.je + 1) ? 1 : 0);
   const int o3 = (k == M.ks) ? -1 : ((k == M.ke + 1) ? 1 : 0);
   if (!o2 && !o3) return v;
@@ -1922,6 +1929,260 @@ __global__ void k_sr_c2p_batch(EosK eos, int n, double *cons, double *prim,
 }
 
 //========================================================================================
+// Static mesh refinement: one work-group per MeshBlock runs that block's step program
+// (built on the host by smr_plan.hpp) box by box, with a work-group barrier at every
+// BARRIER step.  A block writes only its own registers; it reads other blocks' active
+// data (boundary exchange, flux correction) or a snapshot of their EMFs (EMF correction),
+// which no work-group of the same launch writes.
+
+struct SmrArgs {
+  const smrplan::Step *steps;
+  const int *offs;
+  double *u, *w, *bcc;              // the updated state; w takes prolongated primitives
+  double *b[3];
+  double *cu, *cw, *cbcc, *cb[3];   // coarse buffers: conserved, primitive, bcc, faces
+  const double *fco, *cco;          // fine / coarse coordinate tables per block
+  long fcl, ccl;
+  double *flux[3];
+  double *e[3];
+  const double *es[3];              // EMF snapshot
+};
+
+struct SmrDims {
+  int c1, c2, c3;                   // coarse cells per direction (nx/2 + 2 ng)
+  long ccell;
+  long cf[3];                       // coarse face-array sizes
+};
+
+__device__ __forceinline__ SmrDims smr_dims(const MeshK &M) {
+  SmrDims D;
+  D.c1 = M.nx1 / 2 + 2 * M.ng; D.c2 = M.nx2 / 2 + 2 * M.ng; D.c3 = M.nx3 / 2 + 2 * M.ng;
+  D.ccell = static_cast<long>(D.c1) * D.c2 * D.c3;
+  D.cf[0] = static_cast<long>(D.c3) * D.c2 * (D.c1 + 1);
+  D.cf[1] = static_cast<long>(D.c3) * (D.c2 + 1) * D.c1;
+  D.cf[2] = static_cast<long>(D.c3 + 1) * D.c2 * D.c1;
+  return D;
+}
+
+// views of one block's arrays
+__device__ __forceinline__ refops::View cell_view(const MeshK &M, double *p, int g, int n) {
+  return refops::View{p + (static_cast<long>(g) * NHYDRO + n) * M.ncell, M.nc2, M.nc1};
+}
+__device__ __forceinline__ refops::View face_view(const MeshK &M, double *p, int g, int c) {
+  const long nf = (c == 0) ? M.nf1 : ((c == 1) ? M.nf2 : M.nf3);
+  return refops::View{p + static_cast<long>(g) * nf, M.nc2 + (c == 1), M.nc1 + (c == 0)};
+}
+__device__ __forceinline__ refops::View edge_view(const MeshK &M, const double *p, int g,
+                                                  int c) {
+  const long ne = (c == 0) ? M.ne1 : ((c == 1) ? M.ne2 : M.ne3);
+  return refops::View{const_cast<double *>(p) + static_cast<long>(g) * ne, M.nc2 + (c != 1),
+                      M.nc1 + (c != 0)};
+}
+__device__ __forceinline__ refops::View ccell_view(const SmrDims &D, double *p, int g, int n,
+                                                   int nvar) {
+  return refops::View{p + (static_cast<long>(g) * nvar + n) * D.ccell, D.c2, D.c1};
+}
+__device__ __forceinline__ refops::View cface_view(const SmrDims &D, double *p, int g, int c) {
+  return refops::View{p + static_cast<long>(g) * D.cf[c], D.c2 + (c == 1), D.c1 + (c == 0)};
+}
+
+// ConservedToPrimitive on a coarse-buffer cell (adiabatic_mhd.cpp:41-90 with the coarse
+// face fields, as ApplyPhysicalBoundariesOnCoarseLevel calls it).
+__device__ __forceinline__ void smr_coarse_c2p(const MeshK &M, const SmrDims &D,
+                                               const SmrArgs &a, int g, int k, int j, int i) {
+  const refops::View B1 = cface_view(D, a.cb[0], g, 0), B2 = cface_view(D, a.cb[1], g, 1);
+  const refops::View B3 = cface_view(D, a.cb[2], g, 2);
+  const double bc1 = 0.5 * B1(k, j, i) + 0.5 * B1(k, j, i + 1);
+  const double bc2 = 0.5 * B2(k, j, i) + 0.5 * B2(k, j + 1, i);
+  const double bc3 = 0.5 * B3(k, j, i) + 0.5 * B3(k + 1, j, i);
+  ccell_view(D, a.cbcc, g, 0, 3)(k, j, i) = bc1;
+  ccell_view(D, a.cbcc, g, 1, 3)(k, j, i) = bc2;
+  ccell_view(D, a.cbcc, g, 2, 3)(k, j, i) = bc3;
+  double &u_d = ccell_view(D, a.cu, g, IDN, NHYDRO)(k, j, i);
+  double &u_e = ccell_view(D, a.cu, g, IEN, NHYDRO)(k, j, i);
+  const double u_m1 = ccell_view(D, a.cu, g, IM1, NHYDRO)(k, j, i);
+  const double u_m2 = ccell_view(D, a.cu, g, IM2, NHYDRO)(k, j, i);
+  const double u_m3 = ccell_view(D, a.cu, g, IM3, NHYDRO)(k, j, i);
+  const double gm1 = M.eos.gamma - 1.0;
+  u_d = (u_d > M.eos.dfloor) ? u_d : M.eos.dfloor;
+  ccell_view(D, a.cw, g, IDN, NHYDRO)(k, j, i) = u_d;
+  const double di = 1.0 / u_d;
+  ccell_view(D, a.cw, g, IVX, NHYDRO)(k, j, i) = u_m1 * di;
+  ccell_view(D, a.cw, g, IVY, NHYDRO)(k, j, i) = u_m2 * di;
+  ccell_view(D, a.cw, g, IVZ, NHYDRO)(k, j, i) = u_m3 * di;
+  const double pb = 0.5 * (bc1 * bc1 + bc2 * bc2 + bc3 * bc3);
+  const double e_k = 0.5 * di * (u_m1 * u_m1 + u_m2 * u_m2 + u_m3 * u_m3);
+  const double w_p = gm1 * (u_e - e_k - pb);
+  u_e = (w_p > M.eos.pfloor) ? u_e : ((M.eos.pfloor / gm1) + e_k + pb);
+  ccell_view(D, a.cw, g, IPR, NHYDRO)(k, j, i) = (w_p > M.eos.pfloor) ? w_p : M.eos.pfloor;
+}
+
+// bcc from the faces and PrimitiveToConserved on a fine cell (field.cpp:112-180 uniform
+// branch, adiabatic_mhd.cpp:99-136).
+__device__ __forceinline__ void smr_bcc_p2c(const MeshK &M, const SmrArgs &a, int g, int k,
+                                            int j, int i) {
+  const refops::View B1 = face_view(M, a.b[0], g, 0), B2 = face_view(M, a.b[1], g, 1);
+  const refops::View B3 = face_view(M, a.b[2], g, 2);
+  const double bc1 = 0.5 * B1(k, j, i) + 0.5 * B1(k, j, i + 1);
+  const double bc2 = 0.5 * B2(k, j, i) + 0.5 * B2(k, j + 1, i);
+  const double bc3 = 0.5 * B3(k, j, i) + 0.5 * B3(k + 1, j, i);
+  const long c = cidx(M, k, j, i), bb = static_cast<long>(g) * 3 * M.ncell;
+  a.bcc[bb + c] = bc1;
+  a.bcc[bb + M.ncell + c] = bc2;
+  a.bcc[bb + 2 * M.ncell + c] = bc3;
+  const long ub = static_cast<long>(g) * NHYDRO * M.ncell;
+  const double w_d = a.w[ub + c], w_vx = a.w[ub + M.ncell + c];
+  const double w_vy = a.w[ub + 2 * M.ncell + c], w_vz = a.w[ub + 3 * M.ncell + c];
+  const double w_p = a.w[ub + 4 * M.ncell + c];
+  a.u[ub + c] = w_d;
+  a.u[ub + M.ncell + c] = w_vx * w_d;
+  a.u[ub + 2 * M.ncell + c] = w_vy * w_d;
+  a.u[ub + 3 * M.ncell + c] = w_vz * w_d;
+  a.u[ub + 4 * M.ncell + c] =
+      w_p * M.eos.igm1 + 0.5 * (w_d * (w_vx * w_vx + w_vy * w_vy + w_vz * w_vz) +
+                                (bc1 * bc1 + bc2 * bc2 + bc3 * bc3));
+}
+
+// Area-weighted restriction of the fine fluxes of block src whose lowest face is
+// (k, j, i) (LoadFluxBoundaryBufferToCoarser, flux_correction_cc.cpp:77-166).
+__device__ __forceinline__ double smr_flux_restrict(const MeshK &M, const SmrArgs &a, int dir,
+                                                    int src, int n, int k, int j, int i) {
+  const long nf = (dir == 0) ? M.nf1 : ((dir == 1) ? M.nf2 : M.nf3);
+  const refops::View f{a.flux[dir] + (static_cast<long>(src) * NHYDRO + n) * nf,
+                       M.nc2 + (dir == 1), M.nc1 + (dir == 0)};
+  const double *d1 = ctab(M, src, 0), *d2 = ctab(M, src, 1), *d3 = ctab(M, src, 2);
+  if (dir == 0) {
+    const double amm = d2[j] * d3[k], amp = d2[j + 1] * d3[k];
+    const double apm = d2[j] * d3[k + 1], app = d2[j + 1] * d3[k + 1];
+    const double tarea = amm + amp + apm + app;
+    return (f(k, j, i) * amm + f(k, j + 1, i) * amp + f(k + 1, j, i) * apm +
+            f(k + 1, j + 1, i) * app) / tarea;
+  } else if (dir == 1) {
+    const double s00 = d1[i] * d3[k], s01 = d1[i + 1] * d3[k];
+    const double s10 = d1[i] * d3[k + 1], s11 = d1[i + 1] * d3[k + 1];
+    const double tarea = s00 + s01 + s10 + s11;
+    return (f(k, j, i) * s00 + f(k, j, i + 1) * s01 + f(k + 1, j, i) * s10 +
+            f(k + 1, j, i + 1) * s11) / tarea;
+  }
+  const double s00 = d1[i] * d2[j], s01 = d1[i + 1] * d2[j];
+  const double s10 = d1[i] * d2[j + 1], s11 = d1[i + 1] * d2[j + 1];
+  const double tarea = s00 + s01 + s10 + s11;
+  return (f(k, j, i) * s00 + f(k, j, i + 1) * s01 + f(k, j + 1, i) * s10 +
+          f(k, j + 1, i + 1) * s11) / tarea;
+}
+
+__global__ void __launch_bounds__(WG) k_smr(MeshK M, SmrArgs a) {
+  const int g = blockIdx.x;
+  const SmrDims D = smr_dims(M);
+  const refops::RGrid Fg(a.fco + g * a.fcl, M.nc1, M.nc2, M.nc3);
+  const refops::RGrid Cg(a.cco + g * a.ccl, D.c1, D.c2, D.c3);
+  const int s_end = a.offs[g + 1];
+  for (int s = a.offs[g]; s < s_end; ++s) {
+    const smrplan::Step st = a.steps[s];
+    if (st.op == smrplan::BARRIER) {
+      __syncthreads();
+      continue;
+    }
+    const bool nv5 = st.op == smrplan::CC_COPY || st.op == smrplan::CC_RESTRICT ||
+                     st.op == smrplan::CC_TO_COARSE || st.op == smrplan::CC_OWN_RESTRICT ||
+                     st.op == smrplan::PROLONG_CC || st.op == smrplan::FLUX_RESTRICT;
+    const int nb = st.n[0] * st.n[1] * st.n[2];
+    const int total = nv5 ? NHYDRO * nb : nb;
+    for (int t = threadIdx.x; t < total; t += WG) {
+      const int n = t / nb;
+      int q = t - n * nb;
+      const int di = q % st.n[0];
+      q /= st.n[0];
+      const int dj = q % st.n[1], dk = q / st.n[1];
+      const int i = st.d0[0] + di, j = st.d0[1] + dj, k = st.d0[2] + dk;
+      const int si = st.s0[0] + st.mul * di, sj = st.s0[1] + st.mul * dj;
+      const int sk = st.s0[2] + st.mul * dk;
+      const int c = st.comp;
+      switch (st.op) {
+        case smrplan::CC_COPY:
+          cell_view(M, a.u, g, n)(k, j, i) = cell_view(M, a.u, st.src, n)(sk, sj, si);
+          break;
+        case smrplan::CC_RESTRICT: {
+          const refops::RGrid Fs(a.fco + st.src * a.fcl, M.nc1, M.nc2, M.nc3);
+          cell_view(M, a.u, g, n)(k, j, i) = refops::restrict_cc_at(
+              cell_view(M, a.u, st.src, n), Fs, (sk - M.ng) * 2 + M.ng, (sj - M.ng) * 2 + M.ng,
+              (si - M.ng) * 2 + M.ng);
+          break;
+        }
+        case smrplan::CC_TO_COARSE:
+          ccell_view(D, a.cu, g, n, NHYDRO)(k, j, i) = cell_view(M, a.u, st.src, n)(sk, sj, si);
+          break;
+        case smrplan::CC_OWN_RESTRICT:
+          ccell_view(D, a.cu, g, n, NHYDRO)(k, j, i) = refops::restrict_cc_at(
+              cell_view(M, a.u, g, n), Fg, (k - M.ng) * 2 + M.ng, (j - M.ng) * 2 + M.ng,
+              (i - M.ng) * 2 + M.ng);
+          break;
+        case smrplan::FC_COPY:
+          face_view(M, a.b[c], g, c)(k, j, i) = face_view(M, a.b[c], st.src, c)(sk, sj, si);
+          break;
+        case smrplan::FC_RESTRICT: {
+          const refops::RGrid Fs(a.fco + st.src * a.fcl, M.nc1, M.nc2, M.nc3);
+          face_view(M, a.b[c], g, c)(k, j, i) = refops::restrict_fc_at(
+              c, face_view(M, a.b[c], st.src, c), Fs, (sk - M.ng) * 2 + M.ng,
+              (sj - M.ng) * 2 + M.ng, (si - M.ng) * 2 + M.ng);
+          break;
+        }
+        case smrplan::FC_TO_COARSE:
+          cface_view(D, a.cb[c], g, c)(k, j, i) = face_view(M, a.b[c], st.src, c)(sk, sj, si);
+          break;
+        case smrplan::FC_OWN_RESTRICT:
+          cface_view(D, a.cb[c], g, c)(k, j, i) = refops::restrict_fc_at(
+              c, face_view(M, a.b[c], g, c), Fg, (k - M.ng) * 2 + M.ng, (j - M.ng) * 2 + M.ng,
+              (i - M.ng) * 2 + M.ng);
+          break;
+        case smrplan::C2P_COARSE:
+          smr_coarse_c2p(M, D, a, g, k, j, i);
+          break;
+        case smrplan::PROLONG_CC:
+          refops::prolong_cc_at(ccell_view(D, a.cw, g, n, NHYDRO), cell_view(M, a.w, g, n), Fg,
+                                Cg, M.ng, k, j, i);
+          break;
+        case smrplan::PROLONG_FC_SHARED:
+          refops::prolong_fc_shared_at(c, cface_view(D, a.cb[c], g, c),
+                                       face_view(M, a.b[c], g, c), Fg, Cg, M.ng, k, j, i);
+          break;
+        case smrplan::PROLONG_FC_INTERNAL:
+          refops::prolong_fc_internal_at(face_view(M, a.b[0], g, 0), face_view(M, a.b[1], g, 1),
+                                         face_view(M, a.b[2], g, 2), Fg, M.ng, k, j, i);
+          break;
+        case smrplan::BCC_P2C:
+          smr_bcc_p2c(M, a, g, k, j, i);
+          break;
+        case smrplan::FLUX_RESTRICT: {
+          const long nf = (c == 0) ? M.nf1 : ((c == 1) ? M.nf2 : M.nf3);
+          const refops::View f{a.flux[c] + (static_cast<long>(g) * NHYDRO + n) * nf,
+                               M.nc2 + (c == 1), M.nc1 + (c == 0)};
+          f(k, j, i) = smr_flux_restrict(M, a, c, st.src, n, sk, sj, si);
+          break;
+        }
+        case smrplan::EMF_ADD_SAME:
+          edge_view(M, a.e[c], g, c)(k, j, i) += edge_view(M, a.es[c], st.src, c)(sk, sj, si);
+          break;
+        case smrplan::EMF_CLEAR:
+          edge_view(M, a.e[c], g, c)(k, j, i) = 0.0;
+          break;
+        case smrplan::EMF_ADD_FINE: {
+          const refops::RGrid Fs(a.fco + st.src * a.fcl, M.nc1, M.nc2, M.nc3);
+          edge_view(M, a.e[c], g, c)(k, j, i) +=
+              refops::restrict_emf_at(c, edge_view(M, a.es[c], st.src, c), Fs, sk, sj, si);
+          break;
+        }
+        case smrplan::EMF_SCALE:
+          edge_view(M, a.e[c], g, c)(k, j, i) *= st.f;
+          break;
+        default:
+          break;
+      }
+    }
+  }
+}
+
+//========================================================================================
 // Host side
 
 double mesh_gen_x(long index, long nrange) {   // mesh.hpp:457-472, sym_interval = true
@@ -1962,6 +2223,14 @@ struct ath_mesh {
   hipGraphExec_t gexec = nullptr;
   std::vector<const void *> graph_key;   // what the captured cycle depends on
   long nstage = 0;            // stages run (the register configuration's parity)
+  // static mesh refinement (ath_gpu_smr_mesh_create)
+  bool smr = false;
+  std::vector<smrplan::BlockInfo> smr_blocks;
+  smrplan::Step *smr_steps[3] = {nullptr, nullptr, nullptr};   // flux, EMF, boundary
+  int *smr_offs[3] = {nullptr, nullptr, nullptr};
+  double *cu = nullptr, *cw = nullptr, *cbcc = nullptr, *cb[3] = {nullptr, nullptr, nullptr};
+  double *fco = nullptr, *cco = nullptr, *esnap[3] = {nullptr, nullptr, nullptr};
+  long fcl = 0, ccl = 0;
   int *nbr_d = nullptr;
   double *coord_d = nullptr;
   Clock *clk = nullptr;
@@ -1993,7 +2262,8 @@ enum KernelId {
   KID_CORNER = 9, KID_UPDATE_U = 10, KID_SEED = 11, KID_CONS2PRIM = 12,
   KID_NEW_DT = 13, KID_GHOST = 14, KID_FLUX3 = 15,   // + (order-1): 3 fused variants
   KID_PHYS_BND = 18, KID_FDIV = 19,                  // + (order-1): fused flux + divergence
-  KID_UPDATE_B = 22, KID_COUNT = 23
+  KID_UPDATE_B = 22, KID_SMR_FLUX = 23, KID_SMR_EMF = 24, KID_SMR_BND = 25,
+  KID_COUNT = 26
 };
 const char *kernel_name(int id) {
   static const char *flux[9] = {
@@ -2015,6 +2285,9 @@ const char *kernel_name(int id) {
     case KID_FDIV + 1: return "flux_div_plm";
     case KID_FDIV + 2: return "flux_div_ppm";
     case KID_UPDATE_B: return "update_b_ct";
+    case KID_SMR_FLUX: return "smr_flux_correction";
+    case KID_SMR_EMF: return "smr_emf_correction";
+    case KID_SMR_BND: return "smr_boundary";
     default: return "unknown";
   }
 }
@@ -2381,7 +2654,61 @@ void set_regs(ath_mesh *m, const Regs &r) {
   for (int c = 0; c < 3; ++c) { m->b[c] = r.b[c]; m->bb[c] = r.bb[c]; }
 }
 
+SmrArgs smr_args(const ath_mesh *m, int prog, double *u, double *const *b, double *w) {
+  SmrArgs a{};
+  a.steps = m->smr_steps[prog];
+  a.offs = m->smr_offs[prog];
+  a.u = u;
+  a.w = w;
+  a.bcc = m->bcc;
+  for (int c = 0; c < 3; ++c) {
+    a.b[c] = b ? b[c] : nullptr;
+    a.cb[c] = m->cb[c];
+    a.flux[c] = m->flux[c];
+    a.e[c] = m->e[c];
+    a.es[c] = m->esnap[c];
+  }
+  a.cu = m->cu; a.cw = m->cw; a.cbcc = m->cbcc;
+  a.fco = m->fco; a.cco = m->cco;
+  a.fcl = m->fcl; a.ccl = m->ccl;
+  return a;
+}
+
+int launch_smr(ath_mesh *m, int prog, const SmrArgs &a) {
+  static const int kid[3] = {KID_SMR_FLUX, KID_SMR_EMF, KID_SMR_BND};
+  ProfScope ps(m, kid[prog]);
+  hipLaunchKernelGGL(k_smr, dim3(m->M.nblocks), dim3(WG), 0, m->stream, m->M, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// One VL2 stage on a statically refined mesh: fluxes, their correction on coarse/fine
+// faces, corner EMFs and their correction, the update, the boundary program (exchange,
+// restriction, prolongation), and the primitive recovery over every cell.
+int stage_smr(ath_mesh *m, int stg) {
+  Regs r = regs_of(m);
+  const StageArgs a = stage_args(m, r, stg);
+  if (launch_flux_args(m, a.order, a.w, a.bcc, a.fo)) return 1;
+  if (launch_smr(m, 0, smr_args(m, 0, nullptr, nullptr, nullptr))) return 1;
+  if (launch_corner_args(m, a.ca)) return 1;
+  const long ne[3] = {m->M.ne1, m->M.ne2, m->M.ne3};
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpyAsync(m->esnap[c], m->e[c], sizeof(double) * ne[c] * m->M.nblocks,
+                          hipMemcpyDeviceToDevice, m->stream));
+  if (launch_smr(m, 1, smr_args(m, 1, nullptr, nullptr, nullptr))) return 1;
+  if (launch_update_args(m, a.ua, a.beta, a.first, true)) return 1;
+  // boundary program on the updated state; prolongated primitives go to the register the
+  // recovery treats as the previous primitives
+  if (launch_smr(m, 2, smr_args(m, 2, a.ga.u, a.ga.b, r.w))) return 1;
+  if (launch_cons2prim_args(m, a.pa)) return 1;
+  swap_stage_regs(r);
+  set_regs(m, r);
+  m->nstage += 1;
+  return 0;
+}
+
 int stage(ath_mesh *m, int stg) {
+  if (m->smr) return stage_smr(m, stg);
   Regs r = regs_of(m);
   const StageArgs a = stage_args(m, r, stg);
   const bool fused = m->fdiv_rows > 0;
@@ -2729,7 +3056,156 @@ extern "C" {
 
 const char *ath_gpu_last_error(void) { return g_err.c_str(); }
 
-int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
+}  // extern "C"
+
+namespace {
+
+// Coordinate table of one grid in the refinement operators' layout (x1f, x1v, dx1f, then
+// x2, x3) for a block at logical location lx, `shift` levels above the root grid; coarse:
+// the coarse buffer's grid (nx/2 cells, faces at every second fine face; the coarse flag
+// of the Coordinates constructor, coordinates.cpp:30-146).
+void level_table(const ath_mesh_desc &d, const long *lx, int shift, bool coarse,
+                 std::vector<double> &out) {
+  out.clear();
+  for (int q = 0; q < 3; ++q) {
+    const int nxb = d.bnx[q], ng = d.ng, n = coarse ? nxb / 2 : nxb, nc = n + 2 * ng;
+    const long nbq = static_cast<long>(d.nx[q] / d.bnx[q]) << shift;
+    const long nroot = static_cast<long>(d.nx[q]) << shift;
+    const long l = lx[q];
+    const double bmin = (l == 0) ? d.xmin[q] : uniform_gen(mesh_gen_x(l, nbq), d.xmin[q], d.xmax[q]);
+    const double bmax = (l == nbq - 1) ? d.xmax[q]
+                                       : uniform_gen(mesh_gen_x(l + 1, nbq), d.xmin[q], d.xmax[q]);
+    const int il = ng, iu = ng + n - 1;
+    const double dx = (bmax - bmin) / (iu - il + 1);
+    std::vector<double> xf(nc + 1), xv(nc), dxf(nc, dx);
+    for (int i = il - ng; i <= iu + ng + 1; ++i) {
+      const long noff = coarse ? (i - il) * 2 + l * nxb : i - il + l * nxb;
+      xf[i] = uniform_gen(mesh_gen_x(noff, nroot), d.xmin[q], d.xmax[q]);
+    }
+    xf[il] = bmin;
+    xf[iu + 1] = bmax;
+    for (int i = 0; i < nc; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
+    out.insert(out.end(), xf.begin(), xf.end());
+    out.insert(out.end(), xv.begin(), xv.end());
+    out.insert(out.end(), dxf.begin(), dxf.end());
+  }
+}
+
+// Step programs, coarse buffers, coordinate tables and the EMF snapshot of a refined mesh.
+int smr_setup(ath_mesh *m, const std::vector<smrplan::BlockInfo> &sb, int root_level) {
+  const MeshK &M = m->M;
+  const int nb = M.nblocks;
+  const smrplan::Shape S = smrplan::make_shape(m->d.bnx, m->d.ng);
+  m->smr_blocks = sb;
+  smrplan::Program prog[3];
+  try {
+    for (int q = 0; q < 3; ++q) prog[q].offs.push_back(0);
+    for (int g = 0; g < nb; ++g) {
+      smrplan::flux_program(S, sb, g, prog[0]);
+      prog[0].offs.push_back(static_cast<int>(prog[0].steps.size()));
+      smrplan::emf_program(S, sb, g, prog[1]);
+      prog[1].offs.push_back(static_cast<int>(prog[1].steps.size()));
+      smrplan::boundary_program(S, sb, g, prog[2]);
+      prog[2].offs.push_back(static_cast<int>(prog[2].steps.size()));
+    }
+  } catch (const std::exception &e) {
+    return fail(e.what());
+  }
+  std::vector<double> fco, cco, t;
+  for (int g = 0; g < nb; ++g) {
+    const long lx[3] = {static_cast<long>(sb[g].lx[0]), static_cast<long>(sb[g].lx[1]),
+                        static_cast<long>(sb[g].lx[2])};
+    level_table(m->d, lx, sb[g].level - root_level, false, t);
+    if (g == 0) m->fcl = static_cast<long>(t.size());
+    fco.insert(fco.end(), t.begin(), t.end());
+    level_table(m->d, lx, sb[g].level - root_level, true, t);
+    if (g == 0) m->ccl = static_cast<long>(t.size());
+    cco.insert(cco.end(), t.begin(), t.end());
+  }
+  const long cc = static_cast<long>(S.ncc[0]) * S.ncc[1] * S.ncc[2];
+  const long cf[3] = {static_cast<long>(S.ncc[2]) * S.ncc[1] * (S.ncc[0] + 1),
+                      static_cast<long>(S.ncc[2]) * (S.ncc[1] + 1) * S.ncc[0],
+                      static_cast<long>(S.ncc[2] + 1) * S.ncc[1] * S.ncc[0]};
+  const long ne[3] = {M.ne1, M.ne2, M.ne3};
+  int rc = 0;
+  rc |= m->alloc(&m->cu, NHYDRO * cc * nb);
+  rc |= m->alloc(&m->cw, NHYDRO * cc * nb);
+  rc |= m->alloc(&m->cbcc, 3 * cc * nb);
+  for (int c = 0; c < 3 && !rc; ++c) {
+    rc |= m->alloc(&m->cb[c], cf[c] * nb);
+    rc |= m->alloc(&m->esnap[c], ne[c] * nb);
+  }
+  if (!rc) rc |= m->alloc(&m->fco, static_cast<long>(fco.size()));
+  if (!rc) rc |= m->alloc(&m->cco, static_cast<long>(cco.size()));
+  for (int q = 0; q < 3 && !rc; ++q) {
+    void *ps = nullptr, *po = nullptr;
+    const size_t nsb = sizeof(smrplan::Step) * std::max<size_t>(prog[q].steps.size(), 1);
+    const size_t nob = sizeof(int) * prog[q].offs.size();
+    if (hipMalloc(&ps, nsb) != hipSuccess) return fail("hipMalloc failed for SMR programs");
+    m->allocs.push_back(ps);
+    if (hipMalloc(&po, nob) != hipSuccess) return fail("hipMalloc failed for SMR programs");
+    m->allocs.push_back(po);
+    m->smr_steps[q] = static_cast<smrplan::Step *>(ps);
+    m->smr_offs[q] = static_cast<int *>(po);
+    if (!prog[q].steps.empty())
+      HIPCHK(hipMemcpyAsync(ps, prog[q].steps.data(), sizeof(smrplan::Step) * prog[q].steps.size(),
+                            hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(po, prog[q].offs.data(), nob, hipMemcpyHostToDevice, m->stream));
+  }
+  if (rc) return 1;
+  HIPCHK(hipMemcpyAsync(m->fco, fco.data(), sizeof(double) * fco.size(), hipMemcpyHostToDevice,
+                        m->stream));
+  HIPCHK(hipMemcpyAsync(m->cco, cco.data(), sizeof(double) * cco.size(), hipMemcpyHostToDevice,
+                        m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+// The MeshBlock list of a refined mesh: every location inside the root grid at its level,
+// none listed twice or inside another, and together they cover the root grid exactly.
+int smr_blocks_of(const ath_mesh_desc &d, const long long *locs, int nblocks, int root_level,
+                  std::vector<smrplan::BlockInfo> &sb) {
+  if (nblocks < 1 || !locs) return fail("a refined mesh needs at least one MeshBlock");
+  int nrb[3], maxlev = root_level;
+  for (int q = 0; q < 3; ++q) nrb[q] = d.nx[q] / d.bnx[q];
+  sb.resize(nblocks);
+  std::map<std::array<long long, 4>, int> seen;
+  for (int g = 0; g < nblocks; ++g) {
+    smrplan::BlockInfo &b = sb[g];
+    b.level = static_cast<int>(locs[4 * g]);
+    if (b.level < root_level || b.level > root_level + 20)
+      return fail("MeshBlock level outside [root_level, root_level + 20]");
+    maxlev = std::max(maxlev, b.level);
+    for (int q = 0; q < 3; ++q) {
+      b.lx[q] = locs[4 * g + 1 + q];
+      if (b.lx[q] < 0 || b.lx[q] >= (static_cast<long long>(nrb[q]) << (b.level - root_level)))
+        return fail("MeshBlock location outside the mesh");
+    }
+    if (!seen.emplace(std::array<long long, 4>{b.level, b.lx[0], b.lx[1], b.lx[2]}, g).second)
+      return fail("a MeshBlock location is listed twice");
+  }
+  long double vol = 0.0L;
+  for (const auto &b : sb) {
+    vol += std::ldexp(1.0L, 3 * (maxlev - b.level));
+    for (int up = 1; up <= b.level - root_level; ++up)
+      if (seen.count({b.level - up, b.lx[0] >> up, b.lx[1] >> up, b.lx[2] >> up}))
+        return fail("a MeshBlock lies inside another listed MeshBlock");
+  }
+  const long double want = std::ldexp(static_cast<long double>(nrb[0]) * nrb[1] * nrb[2],
+                                      3 * (maxlev - root_level));
+  if (vol != want) return fail("the MeshBlocks do not cover the mesh");
+  try {
+    smrplan::search_neighbors(nrb, root_level, sb);
+  } catch (const std::exception &e) {
+    return fail(e.what());
+  }
+  return 0;
+}
+
+// The device mirror of a uniform mesh (sb null) or of a statically refined one (sb: the
+// MeshBlock list with its neighbour lists, root_level the logical level of the root grid).
+int mesh_build(const ath_mesh_desc *desc, const std::vector<smrplan::BlockInfo> *sb,
+               int root_level, ath_mesh_t *out) {
   if (!desc || !out) return fail("null argument");
   *out = nullptr;
   const ath_mesh_desc &d = *desc;
@@ -2764,7 +3240,7 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   M.js = M.ng; M.je = M.ng + M.nx2 - 1;
   M.ks = M.ng; M.ke = M.ng + M.nx3 - 1;
   for (int q = 0; q < 3; ++q) m->nbx[q] = d.nx[q] / d.bnx[q];
-  M.nblocks = m->nbx[0] * m->nbx[1] * m->nbx[2];
+  M.nblocks = sb ? static_cast<int>(sb->size()) : m->nbx[0] * m->nbx[1] * m->nbx[2];
   M.ncell = static_cast<long>(M.nc1) * M.nc2 * M.nc3;
   M.nf1 = static_cast<long>(M.nc3) * M.nc2 * (M.nc1 + 1);
   M.nf2 = static_cast<long>(M.nc3) * (M.nc2 + 1) * M.nc1;
@@ -2794,6 +3270,8 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   M.c2p_stats = nullptr;
   M.outflow = 0;
   for (int f = 0; f < 6; ++f) M.outflow |= d.bc[f];
+  M.smr = sb ? 1 : 0;
+  m->smr = sb != nullptr;
   m->fdiv_rows = 0;   // separate flux and update kernels: measured faster (see the header)
 
   // neighbour table and coordinates (coordinates.cpp:129-146, cartesian.cpp:23-72)
@@ -2801,7 +3279,12 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   std::vector<int> nbrh(static_cast<size_t>(nb) * 27);
   std::vector<double> coordh(static_cast<size_t>(nb) * M.cstride);
   for (int g = 0; g < nb; ++g) {
-    int lx[3] = {g % m->nbx[0], (g / m->nbx[0]) % m->nbx[1], g / (m->nbx[0] * m->nbx[1])};
+    long lx[3] = {g % m->nbx[0], (g / m->nbx[0]) % m->nbx[1], g / (m->nbx[0] * m->nbx[1])};
+    int shift = 0;   // levels above the root grid
+    if (sb) {
+      for (int q = 0; q < 3; ++q) lx[q] = static_cast<long>((*sb)[g].lx[q]);
+      shift = (*sb)[g].level - root_level;
+    }
     for (int o3 = -1; o3 <= 1; ++o3)
       for (int o2 = -1; o2 <= 1; ++o2)
         for (int o1 = -1; o1 <= 1; ++o1) {
@@ -2811,14 +3294,18 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
             const int l = lx[q] + o[q];
             if ((l < 0 && d.bc[2 * q]) || (l >= m->nbx[q] && d.bc[2 * q + 1])) phys = true;
           }
-          int l1 = (lx[0] + o1 + m->nbx[0]) % m->nbx[0];
-          int l2 = (lx[1] + o2 + m->nbx[1]) % m->nbx[1];
-          int l3 = (lx[2] + o3 + m->nbx[2]) % m->nbx[2];
+          const long l1 = (lx[0] + o1 + m->nbx[0]) % m->nbx[0];
+          const long l2 = (lx[1] + o2 + m->nbx[1]) % m->nbx[1];
+          const long l3 = (lx[2] + o3 + m->nbx[2]) % m->nbx[2];
+          // refined meshes exchange through their step programs: every side has a
+          // neighbour (periodic), recorded as the block itself
           nbrh[g * 27 + (o3 + 1) * 9 + (o2 + 1) * 3 + (o1 + 1)] =
-              phys ? -1 : l1 + m->nbx[0] * (l2 + m->nbx[1] * l3);
+              sb ? g : (phys ? -1 : static_cast<int>(l1 + m->nbx[0] * (l2 + m->nbx[1] * l3)));
         }
     for (int q = 0; q < 3; ++q) {
-      const int nx = d.bnx[q], ng = d.ng, nc = nx + 2 * ng, nbq = m->nbx[q];
+      const int nx = d.bnx[q], ng = d.ng, nc = nx + 2 * ng;
+      const long nbq = static_cast<long>(m->nbx[q]) << shift;   // blocks along q, this level
+      const long nroot = static_cast<long>(d.nx[q]) << shift;    // cells along q, this level
       const long l = lx[q];
       double bmin = (l == 0) ? d.xmin[q] : uniform_gen(mesh_gen_x(l, nbq), d.xmin[q], d.xmax[q]);
       double bmax = (l == nbq - 1) ? d.xmax[q]
@@ -2827,7 +3314,7 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
       const double dx = (bmax - bmin) / (iu - il + 1);
       std::vector<double> xf(nc + 1), xv(nc);
       for (int i = il - ng; i <= iu + ng + 1; ++i)
-        xf[i] = uniform_gen(mesh_gen_x(i - il + l * nx, d.nx[q]), d.xmin[q], d.xmax[q]);
+        xf[i] = uniform_gen(mesh_gen_x(i - il + l * nx, nroot), d.xmin[q], d.xmax[q]);
       xf[il] = bmin;
       xf[iu + 1] = bmax;
       for (int i = 0; i < nc; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
@@ -2906,7 +3393,83 @@ int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
   }
   M.nbr = m->nbr_d;
   M.coord = m->coord_d;
+  if (sb && smr_setup(m, *sb, root_level)) {
+    std::string keep = g_err;
+    cleanup();
+    g_err = keep;
+    return 1;
+  }
   *out = m;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out) {
+  return mesh_build(desc, nullptr, 0, out);
+}
+
+int ath_gpu_smr_mesh_create(const ath_mesh_desc *desc, const long long *locs, int nblocks,
+                            int root_level, ath_mesh_t *out) {
+  if (!desc || !out) return fail("null argument");
+  *out = nullptr;
+  const ath_mesh_desc &d = *desc;
+  for (int f = 0; f < 6; ++f)
+    if (d.bc[f] != 0) return fail("refined meshes are periodic on every face");
+  if (d.relativistic) return fail("refined meshes are Newtonian");
+  for (int q = 0; q < 3; ++q) {
+    if (d.bnx[q] <= 0 || d.nx[q] <= 0 || d.nx[q] % d.bnx[q])
+      return fail("mesh nx must be a positive multiple of meshblock nx in every direction");
+    if (d.bnx[q] % 2 || d.bnx[q] < 2 * d.ng)
+      return fail("refined meshes need even MeshBlock sizes of at least 2 nghost");
+  }
+  if (d.ng % 2)   // MeshRefinement::MeshRefinement (mesh_refinement.cpp:42-48)
+    return fail("nghost must be even with mesh refinement");
+  std::vector<smrplan::BlockInfo> sb;
+  if (smr_blocks_of(d, locs, nblocks, root_level, sb)) return 1;
+  return mesh_build(desc, &sb, root_level, out);
+}
+
+int ath_gpu_smr_plan(const ath_mesh_desc *desc, const long long *locs, int nblocks,
+                     int root_level, int gid, int *nbr_out, int cap, int *nnbr, long *nsteps) {
+  if (!desc || !nnbr || !nsteps) return fail("null argument");
+  const ath_mesh_desc &d = *desc;
+  for (int q = 0; q < 3; ++q)
+    if (d.bnx[q] <= 0 || d.nx[q] % d.bnx[q] || d.bnx[q] % 2 || d.bnx[q] < 2 * d.ng)
+      return fail("refined meshes need even MeshBlock sizes of at least 2 nghost dividing nx");
+  if (d.ng % 2) return fail("nghost must be even with mesh refinement");
+  std::vector<smrplan::BlockInfo> sb;
+  if (smr_blocks_of(d, locs, nblocks, root_level, sb)) return 1;
+  if (gid < 0 || gid >= nblocks) return fail("block id out of range");
+  const smrplan::Shape S = smrplan::make_shape(d.bnx, d.ng);
+  smrplan::Program prog[3];
+  try {
+    for (int g = 0; g < nblocks; ++g) {
+      smrplan::flux_program(S, sb, g, prog[0]);
+      smrplan::emf_program(S, sb, g, prog[1]);
+      smrplan::boundary_program(S, sb, g, prog[2]);
+    }
+  } catch (const std::exception &e) {
+    return fail(e.what());
+  }
+  for (int q = 0; q < 3; ++q) nsteps[q] = static_cast<long>(prog[q].steps.size());
+  const std::vector<smrplan::Nbr> &nb = sb[gid].nb;
+  *nnbr = static_cast<int>(nb.size());
+  for (int q = 0; q < *nnbr && q < cap && nbr_out; ++q) {
+    const smrplan::Nbr &e = nb[q];
+    const int v[9] = {e.gid, e.level, e.ox1, e.ox2, e.ox3, e.type, e.fi1, e.fi2,
+                      (e.type == smrplan::EDGE) ? e.eid : e.fid};
+    for (int t = 0; t < 9; ++t) nbr_out[9 * q + t] = v[t];
+  }
+  return 0;
+}
+
+int ath_gpu_smr_block_level(ath_mesh_t m, int gid, int *level) {
+  if (!m || !level) return fail("null argument");
+  if (gid < 0 || gid >= m->M.nblocks) return fail("block id out of range");
+  *level = m->smr ? m->smr_blocks[gid].level : 0;
   return 0;
 }
 
@@ -2965,7 +3528,11 @@ int ath_gpu_initialize(ath_mesh_t m, double tlim) {
   if (!m) return fail("null argument");
   Clock c0{0.0, std::numeric_limits<double>::max(), tlim, 0.0};
   HIPCHK(hipMemcpyAsync(m->clk, &c0, sizeof(Clock), hipMemcpyHostToDevice, m->stream));
-  if (launch_ghost_args(m, GhostArgs{m->u, {m->b[0], m->b[1], m->b[2]}})) return 1;
+  if (m->smr) {   // boundary program; prolongated primitives land in w, then recovered
+    if (launch_smr(m, 2, smr_args(m, 2, m->u, m->b, m->w))) return 1;
+  } else if (launch_ghost_args(m, GhostArgs{m->u, {m->b[0], m->b[1], m->b[2]}})) {
+    return 1;
+  }
   // ConservedToPrimitive(u, w1, b, w, bcc): output register is w
   if (launch_cons2prim(m, m->w, true)) return 1;
   if (launch_phys_bnd(m, C2PArgs{m->u, m->w, m->bcc, m->b[0], m->b[1], m->b[2], nullptr,
@@ -3310,6 +3877,7 @@ int ath_gpu_mesh_autotune(ath_mesh_t m, int reps, double *best_ms) {
   if (!m) return fail("null argument");
   // the tuner times the ensemble kernels, which have no physical-boundary step
   if (m->M.outflow) return fail("autotune: meshes with outflow faces keep their tiles");
+  if (m->smr) return fail("autotune: refined meshes keep their tiles");
   ath_mesh_t one[1] = {m};
   ath_ensemble_t e = nullptr;
   if (ath_gpu_ensemble_create(one, 1, &e)) return 1;
@@ -3508,6 +4076,7 @@ int ath_gpu_mesh_set_fused(ath_mesh_t m, int rows) {
   if (!m) return fail("null argument");
   const MeshK &M = m->M;
   if (rows < 0) rows = fdiv_default_rows(M);
+  if (rows > 0 && m->smr) return fail("fused flux kernel: refined meshes use separate kernels");
   if (rows > 0 && !fdiv_rows_ok(M, rows))
     return fail("fused flux kernel: rows must be in 1 .. nx2 with (rows + 2) * (nx1 + 2) <= " +
                 std::to_string(WG));

@@ -28,6 +28,7 @@
 
 #include "../../include/athena_gpu.h"
 #include "physics.hpp"
+#include "refine_ops.hpp"
 
 namespace {
 
@@ -46,25 +47,9 @@ struct RRange {
   __host__ __device__ int nk() const { return k1 - k0 + 1; }
 };
 
-// Coordinate tables of one grid (n1 x n2 x n3 cells).
-struct RGrid {
-  const double *x1f, *x1v, *dx1f, *x2f, *x2v, *dx2f, *x3f, *x3v, *dx3f;
-  __device__ RGrid(const double *c, int n1, int n2, int n3) {
-    x1f = c; x1v = x1f + n1 + 1; dx1f = x1v + n1;
-    x2f = dx1f + n1; x2v = x2f + n2 + 1; dx2f = x2v + n2;
-    x3f = dx2f + n2; x3v = x3f + n3 + 1; dx3f = x3v + n3;
-  }
-};
-
-__device__ __forceinline__ double rsign(double x) { return (x < 0.0) ? -1.0 : 1.0; }
-__device__ __forceinline__ double rmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double minmod_c(double gm, double gp) {
-  return 0.5 * (rsign(gm) + rsign(gp)) * rmin(fabs(gm), fabs(gp));
-}
-
-__device__ __forceinline__ long at(int k, int j, int i, int n2, int n1) {
-  return (static_cast<long>(k) * n2 + j) * n1 + i;
-}
+using refops::at;
+using refops::RGrid;
+using refops::View;
 
 // thread -> (block, component, k, j, i) over nb * ncomp * |range|; false past the end
 __device__ __forceinline__ bool rmap(const RRange &r, int nb, int ncomp, int &b, int &n,
@@ -91,21 +76,10 @@ __global__ void __launch_bounds__(RWG) k_restrict_cc(RShape S, RRange r, int nb,
   const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
   const long fn = static_cast<long>(S.nc1) * S.nc2 * S.nc3;
   const long cn = static_cast<long>(S.ncc1) * S.ncc2 * S.ncc3;
-  const double *f = fine + (static_cast<long>(b) * nvar + n) * fn;
+  const View f{const_cast<double *>(fine) + (static_cast<long>(b) * nvar + n) * fn, S.nc2, S.nc1};
   const int k = (ck - S.ng) * 2 + S.ng, j = (cj - S.ng) * 2 + S.ng, i = (ci - S.ng) * 2 + S.ng;
-  auto vol = [&](int kk, int jj, int ii) { return F.dx1f[ii] * F.dx2f[jj] * F.dx3f[kk]; };
-  auto fv = [&](int kk, int jj, int ii) { return f[at(kk, jj, ii, S.nc2, S.nc1)]; };
-  const double v000 = vol(k, j, i), v010 = vol(k, j + 1, i);
-  const double v001 = vol(k, j, i + 1), v011 = vol(k, j + 1, i + 1);
-  const double v100 = vol(k + 1, j, i), v110 = vol(k + 1, j + 1, i);
-  const double v101 = vol(k + 1, j, i + 1), v111 = vol(k + 1, j + 1, i + 1);
-  const double tvol = ((v000 + v010) + (v001 + v011)) + ((v100 + v110) + (v101 + v111));
   coarse[(static_cast<long>(b) * nvar + n) * cn + at(ck, cj, ci, S.ncc2, S.ncc1)] =
-      (((fv(k, j, i) * v000 + fv(k, j + 1, i) * v010) +
-        (fv(k, j, i + 1) * v001 + fv(k, j + 1, i + 1) * v011)) +
-       ((fv(k + 1, j, i) * v100 + fv(k + 1, j + 1, i) * v110) +
-        (fv(k + 1, j, i + 1) * v101 + fv(k + 1, j + 1, i + 1) * v111))) /
-      tvol;
+      refops::restrict_cc_at(f, F, k, j, i);
 }
 
 template <int comp>
@@ -117,30 +91,10 @@ __global__ void __launch_bounds__(RWG) k_restrict_fc(RShape S, RRange r, int nb,
   const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
   const int fn1 = S.nc1 + (comp == 0), fn2 = S.nc2 + (comp == 1), fn3 = S.nc3 + (comp == 2);
   const int cn1 = S.ncc1 + (comp == 0), cn2 = S.ncc2 + (comp == 1), cn3 = S.ncc3 + (comp == 2);
-  const double *f = fine + static_cast<long>(b) * fn1 * fn2 * fn3;
-  auto fv = [&](int kk, int jj, int ii) { return f[at(kk, jj, ii, fn2, fn1)]; };
+  const View f{const_cast<double *>(fine) + static_cast<long>(b) * fn1 * fn2 * fn3, fn2, fn1};
   const int k = (ck - S.ng) * 2 + S.ng, j = (cj - S.ng) * 2 + S.ng, i = (ci - S.ng) * 2 + S.ng;
-  double v;
-  if (comp == 0) {
-    const double a00 = F.dx2f[j] * F.dx3f[k], a01 = F.dx2f[j + 1] * F.dx3f[k];
-    const double a10 = F.dx2f[j] * F.dx3f[k + 1], a11 = F.dx2f[j + 1] * F.dx3f[k + 1];
-    const double tarea = a00 + a01 + a10 + a11;
-    v = (fv(k, j, i) * a00 + fv(k, j + 1, i) * a01 + fv(k + 1, j, i) * a10 +
-         fv(k + 1, j + 1, i) * a11) / tarea;
-  } else if (comp == 1) {
-    const double a00 = F.dx1f[i] * F.dx3f[k], a01 = F.dx1f[i + 1] * F.dx3f[k];
-    const double a10 = F.dx1f[i] * F.dx3f[k + 1], a11 = F.dx1f[i + 1] * F.dx3f[k + 1];
-    const double tarea = a00 + a01 + a10 + a11;
-    v = (fv(k, j, i) * a00 + fv(k, j, i + 1) * a01 + fv(k + 1, j, i) * a10 +
-         fv(k + 1, j, i + 1) * a11) / tarea;
-  } else {
-    const double a00 = F.dx1f[i] * F.dx2f[j], a01 = F.dx1f[i + 1] * F.dx2f[j];
-    const double a10 = F.dx1f[i] * F.dx2f[j + 1], a11 = F.dx1f[i + 1] * F.dx2f[j + 1];
-    const double tarea = a00 + a01 + a10 + a11;
-    v = (fv(k, j, i) * a00 + fv(k, j, i + 1) * a01 + fv(k, j + 1, i) * a10 +
-         fv(k, j + 1, i + 1) * a11) / tarea;
-  }
-  coarse[static_cast<long>(b) * cn1 * cn2 * cn3 + at(ck, cj, ci, cn2, cn1)] = v;
+  coarse[static_cast<long>(b) * cn1 * cn2 * cn3 + at(ck, cj, ci, cn2, cn1)] =
+      refops::restrict_fc_at(comp, f, F, k, j, i);
 }
 
 __global__ void __launch_bounds__(RWG) k_prolongate_cc(RShape S, RRange r, int nb, int nvar,
@@ -153,32 +107,10 @@ __global__ void __launch_bounds__(RWG) k_prolongate_cc(RShape S, RRange r, int n
   const RGrid C(ccoord + b * S.ccl, S.ncc1, S.ncc2, S.ncc3);
   const long fn = static_cast<long>(S.nc1) * S.nc2 * S.nc3;
   const long cn = static_cast<long>(S.ncc1) * S.ncc2 * S.ncc3;
-  const double *c = coarse + (static_cast<long>(b) * nvar + n) * cn;
-  double *f = fine + (static_cast<long>(b) * nvar + n) * fn;
-  auto cv = [&](int kk, int jj, int ii) { return c[at(kk, jj, ii, S.ncc2, S.ncc1)]; };
-  const int fk = (k - S.ng) * 2 + S.ng, fj = (j - S.ng) * 2 + S.ng, fi = (i - S.ng) * 2 + S.ng;
-  const double x3m = C.x3v[k - 1], x3c = C.x3v[k], x3p = C.x3v[k + 1];
-  const double dx3m = x3c - x3m, dx3p = x3p - x3c;
-  const double dx3fm = x3c - F.x3v[fk], dx3fp = F.x3v[fk + 1] - x3c;
-  const double x2m = C.x2v[j - 1], x2c = C.x2v[j], x2p = C.x2v[j + 1];
-  const double dx2m = x2c - x2m, dx2p = x2p - x2c;
-  const double dx2fm = x2c - F.x2v[fj], dx2fp = F.x2v[fj + 1] - x2c;
-  const double x1m = C.x1v[i - 1], x1c = C.x1v[i], x1p = C.x1v[i + 1];
-  const double dx1m = x1c - x1m, dx1p = x1p - x1c;
-  const double dx1fm = x1c - F.x1v[fi], dx1fp = F.x1v[fi + 1] - x1c;
-  const double cc = cv(k, j, i);
-  const double gx1c = minmod_c((cc - cv(k, j, i - 1)) / dx1m, (cv(k, j, i + 1) - cc) / dx1p);
-  const double gx2c = minmod_c((cc - cv(k, j - 1, i)) / dx2m, (cv(k, j + 1, i) - cc) / dx2p);
-  const double gx3c = minmod_c((cc - cv(k - 1, j, i)) / dx3m, (cv(k + 1, j, i) - cc) / dx3p);
-  auto fw = [&](int kk, int jj, int ii, double v) { f[at(kk, jj, ii, S.nc2, S.nc1)] = v; };
-  fw(fk, fj, fi, cc - (gx1c * dx1fm + gx2c * dx2fm + gx3c * dx3fm));
-  fw(fk, fj, fi + 1, cc + (gx1c * dx1fp - gx2c * dx2fm - gx3c * dx3fm));
-  fw(fk, fj + 1, fi, cc - (gx1c * dx1fm - gx2c * dx2fp + gx3c * dx3fm));
-  fw(fk, fj + 1, fi + 1, cc + (gx1c * dx1fp + gx2c * dx2fp - gx3c * dx3fm));
-  fw(fk + 1, fj, fi, cc - (gx1c * dx1fm + gx2c * dx2fm - gx3c * dx3fp));
-  fw(fk + 1, fj, fi + 1, cc + (gx1c * dx1fp - gx2c * dx2fm + gx3c * dx3fp));
-  fw(fk + 1, fj + 1, fi, cc - (gx1c * dx1fm - gx2c * dx2fp - gx3c * dx3fp));
-  fw(fk + 1, fj + 1, fi + 1, cc + (gx1c * dx1fp + gx2c * dx2fp + gx3c * dx3fp));
+  const View c{const_cast<double *>(coarse) + (static_cast<long>(b) * nvar + n) * cn, S.ncc2,
+               S.ncc1};
+  const View f{fine + (static_cast<long>(b) * nvar + n) * fn, S.nc2, S.nc1};
+  refops::prolong_cc_at(c, f, F, C, S.ng, k, j, i);
 }
 
 template <int comp>
@@ -193,52 +125,9 @@ __global__ void __launch_bounds__(RWG) k_prolongate_fc_shared(RShape S, RRange r
   const RGrid C(ccoord + b * S.ccl, S.ncc1, S.ncc2, S.ncc3);
   const int fn1 = S.nc1 + (comp == 0), fn2 = S.nc2 + (comp == 1), fn3 = S.nc3 + (comp == 2);
   const int cn1 = S.ncc1 + (comp == 0), cn2 = S.ncc2 + (comp == 1), cn3 = S.ncc3 + (comp == 2);
-  const double *c = coarse + static_cast<long>(b) * cn1 * cn2 * cn3;
-  double *f = fine + static_cast<long>(b) * fn1 * fn2 * fn3;
-  auto cv = [&](int kk, int jj, int ii) { return c[at(kk, jj, ii, cn2, cn1)]; };
-  auto fw = [&](int kk, int jj, int ii, double v) { f[at(kk, jj, ii, fn2, fn1)] = v; };
-  const int fk = (k - S.ng) * 2 + S.ng, fj = (j - S.ng) * 2 + S.ng, fi = (i - S.ng) * 2 + S.ng;
-  const double cc = cv(k, j, i);
-  if (comp == 0) {
-    const double x3m = C.x3v[k - 1], x3c = C.x3v[k], x3p = C.x3v[k + 1];
-    const double dx3m = x3c - x3m, dx3p = x3p - x3c;
-    const double fx3m = F.x3v[fk], fx3p = F.x3v[fk + 1];
-    const double x2m = C.x2v[j - 1], x2c = C.x2v[j], x2p = C.x2v[j + 1];
-    const double dx2m = x2c - x2m, dx2p = x2p - x2c;
-    const double fx2m = F.x2v[fj], fx2p = F.x2v[fj + 1];
-    const double gx2c = minmod_c((cc - cv(k, j - 1, i)) / dx2m, (cv(k, j + 1, i) - cc) / dx2p);
-    const double gx3c = minmod_c((cc - cv(k - 1, j, i)) / dx3m, (cv(k + 1, j, i) - cc) / dx3p);
-    fw(fk, fj, fi, cc - gx2c * (x2c - fx2m) - gx3c * (x3c - fx3m));
-    fw(fk, fj + 1, fi, cc + gx2c * (fx2p - x2c) - gx3c * (x3c - fx3m));
-    fw(fk + 1, fj, fi, cc - gx2c * (x2c - fx2m) + gx3c * (fx3p - x3c));
-    fw(fk + 1, fj + 1, fi, cc + gx2c * (fx2p - x2c) + gx3c * (fx3p - x3c));
-  } else if (comp == 1) {
-    const double x3m = C.x3v[k - 1], x3c = C.x3v[k], x3p = C.x3v[k + 1];
-    const double dx3m = x3c - x3m, dx3p = x3p - x3c;
-    const double fx3m = F.x3v[fk], fx3p = F.x3v[fk + 1];
-    const double x1m = C.x1v[i - 1], x1c = C.x1v[i], x1p = C.x1v[i + 1];
-    const double dx1m = x1c - x1m, dx1p = x1p - x1c;
-    const double fx1m = F.x1v[fi], fx1p = F.x1v[fi + 1];
-    const double gx1c = minmod_c((cc - cv(k, j, i - 1)) / dx1m, (cv(k, j, i + 1) - cc) / dx1p);
-    const double gx3c = minmod_c((cc - cv(k - 1, j, i)) / dx3m, (cv(k + 1, j, i) - cc) / dx3p);
-    fw(fk, fj, fi, cc - gx1c * (x1c - fx1m) - gx3c * (x3c - fx3m));
-    fw(fk, fj, fi + 1, cc + gx1c * (fx1p - x1c) - gx3c * (x3c - fx3m));
-    fw(fk + 1, fj, fi, cc - gx1c * (x1c - fx1m) + gx3c * (fx3p - x3c));
-    fw(fk + 1, fj, fi + 1, cc + gx1c * (fx1p - x1c) + gx3c * (fx3p - x3c));
-  } else {
-    const double x2m = C.x2v[j - 1], x2c = C.x2v[j], x2p = C.x2v[j + 1];
-    const double dx2m = x2c - x2m, dx2p = x2p - x2c;
-    const double fx2m = F.x2v[fj], fx2p = F.x2v[fj + 1];
-    const double x1m = C.x1v[i - 1], x1c = C.x1v[i], x1p = C.x1v[i + 1];
-    const double dx1m = x1c - x1m, dx1p = x1p - x1c;
-    const double fx1m = F.x1v[fi], fx1p = F.x1v[fi + 1];
-    const double gx1c = minmod_c((cc - cv(k, j, i - 1)) / dx1m, (cv(k, j, i + 1) - cc) / dx1p);
-    const double gx2c = minmod_c((cc - cv(k, j - 1, i)) / dx2m, (cv(k, j + 1, i) - cc) / dx2p);
-    fw(fk, fj, fi, cc - gx1c * (x1c - fx1m) - gx2c * (x2c - fx2m));
-    fw(fk, fj, fi + 1, cc + gx1c * (fx1p - x1c) - gx2c * (x2c - fx2m));
-    fw(fk, fj + 1, fi, cc - gx1c * (x1c - fx1m) + gx2c * (fx2p - x2c));
-    fw(fk, fj + 1, fi + 1, cc + gx1c * (fx1p - x1c) + gx2c * (fx2p - x2c));
-  }
+  const View c{const_cast<double *>(coarse) + static_cast<long>(b) * cn1 * cn2 * cn3, cn2, cn1};
+  const View f{fine + static_cast<long>(b) * fn1 * fn2 * fn3, fn2, fn1};
+  refops::prolong_fc_shared_at(comp, c, f, F, C, S.ng, k, j, i);
 }
 
 __global__ void __launch_bounds__(RWG) k_prolongate_fc_internal(RShape S, RRange r, int nb,
@@ -251,70 +140,10 @@ __global__ void __launch_bounds__(RWG) k_prolongate_fc_internal(RShape S, RRange
   const long n1f = static_cast<long>(S.nc3) * S.nc2 * (S.nc1 + 1);
   const long n2f = static_cast<long>(S.nc3) * (S.nc2 + 1) * S.nc1;
   const long n3f = static_cast<long>(S.nc3 + 1) * S.nc2 * S.nc1;
-  double *x1f = b1 + b * n1f, *x2f = b2 + b * n2f, *x3f = b3 + b * n3f;
-  auto X1 = [&](int kk, int jj, int ii) -> double & { return x1f[at(kk, jj, ii, S.nc2, S.nc1 + 1)]; };
-  auto X2 = [&](int kk, int jj, int ii) -> double & { return x2f[at(kk, jj, ii, S.nc2 + 1, S.nc1)]; };
-  auto X3 = [&](int kk, int jj, int ii) -> double & { return x3f[at(kk, jj, ii, S.nc2, S.nc1)]; };
-  auto A1 = [&](int kk, int jj) { return F.dx2f[jj] * F.dx3f[kk]; };
-  auto A2 = [&](int kk, int ii) { return F.dx1f[ii] * F.dx3f[kk]; };
-  auto A3 = [&](int jj, int ii) { return F.dx1f[ii] * F.dx2f[jj]; };
-  const int fk = (k - S.ng) * 2 + S.ng, fj = (j - S.ng) * 2 + S.ng, fi = (i - S.ng) * 2 + S.ng;
-  double Uxx = 0.0, Vyy = 0.0, Wzz = 0.0;
-  double Uxyz = 0.0, Vxyz = 0.0, Wxyz = 0.0;
-#pragma unroll
-  for (int jj = 0; jj < 2; jj++) {
-    const int js = 2 * jj - 1, fjj = fj + jj, fjp = fj + 2 * jj;
-#pragma unroll
-    for (int ii = 0; ii < 2; ii++) {
-      const int is = 2 * ii - 1, fii = fi + ii, fip = fi + 2 * ii;
-      Uxx += is * (js * (X2(fk, fjp, fii) * A2(fk, fii) + X2(fk + 1, fjp, fii) * A2(fk + 1, fii)) +
-                   (X3(fk + 2, fjj, fii) * A3(fjj, fii) - X3(fk, fjj, fii) * A3(fjj, fii)));
-      Vyy += js * ((X3(fk + 2, fjj, fii) * A3(fjj, fii) - X3(fk, fjj, fii) * A3(fjj, fii)) +
-                   is * (X1(fk, fjj, fip) * A1(fk, fjj) + X1(fk + 1, fjj, fip) * A1(fk + 1, fjj)));
-      Wzz += is * (X1(fk + 1, fjj, fip) * A1(fk + 1, fjj) - X1(fk, fjj, fip) * A1(fk, fjj)) +
-             js * (X2(fk + 1, fjp, fii) * A2(fk + 1, fii) - X2(fk, fjp, fii) * A2(fk, fii));
-      Uxyz += is * js * (X1(fk + 1, fjj, fip) * A1(fk + 1, fjj) - X1(fk, fjj, fip) * A1(fk, fjj));
-      Vxyz += is * js * (X2(fk + 1, fjp, fii) * A2(fk + 1, fii) - X2(fk, fjp, fii) * A2(fk, fii));
-      Wxyz += is * js * (X3(fk + 2, fjj, fii) * A3(fjj, fii) - X3(fk, fjj, fii) * A3(fjj, fii));
-    }
-  }
-  const double s1 = F.dx1f[fi] + F.dx1f[fi + 1], s2 = F.dx2f[fj] + F.dx2f[fj + 1];
-  const double s3 = F.dx3f[fk] + F.dx3f[fk + 1];
-  const double Sdx1 = s1 * s1, Sdx2 = s2 * s2, Sdx3 = s3 * s3;
-  Uxx *= 0.125; Vyy *= 0.125; Wzz *= 0.125;
-  Uxyz *= 0.125 / (Sdx2 + Sdx3);
-  Vxyz *= 0.125 / (Sdx1 + Sdx3);
-  Wxyz *= 0.125 / (Sdx1 + Sdx2);
-#pragma unroll
-  for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-    for (int jj = 0; jj < 2; ++jj) {
-      const double t3 = kk ? 1.0 : -1.0, t2 = jj ? 1.0 : -1.0;
-      X1(fk + kk, fj + jj, fi + 1) =
-          (0.5 * (X1(fk + kk, fj + jj, fi) * A1(fk + kk, fj + jj) +
-                  X1(fk + kk, fj + jj, fi + 2) * A1(fk + kk, fj + jj)) +
-           Uxx + t3 * Sdx3 * Vxyz + t2 * Sdx2 * Wxyz) / A1(fk + kk, fj + jj);
-    }
-#pragma unroll
-  for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-    for (int ii = 0; ii < 2; ++ii) {
-      const double t3 = kk ? 1.0 : -1.0, t1 = ii ? 1.0 : -1.0;
-      X2(fk + kk, fj + 1, fi + ii) =
-          (0.5 * (X2(fk + kk, fj, fi + ii) * A2(fk + kk, fi + ii) +
-                  X2(fk + kk, fj + 2, fi + ii) * A2(fk + kk, fi + ii)) +
-           Vyy + t3 * Sdx3 * Uxyz + t1 * Sdx1 * Wxyz) / A2(fk + kk, fi + ii);
-    }
-#pragma unroll
-  for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-    for (int ii = 0; ii < 2; ++ii) {
-      const double t2 = jj ? 1.0 : -1.0, t1 = ii ? 1.0 : -1.0;
-      X3(fk + 1, fj + jj, fi + ii) =
-          (0.5 * (X3(fk + 2, fj + jj, fi + ii) * A3(fj + jj, fi + ii) +
-                  X3(fk, fj + jj, fi + ii) * A3(fj + jj, fi + ii)) +
-           Wzz + t2 * Sdx2 * Uxyz + t1 * Sdx1 * Vxyz) / A3(fj + jj, fi + ii);
-    }
+  const View X1{b1 + b * n1f, S.nc2, S.nc1 + 1};
+  const View X2{b2 + b * n2f, S.nc2 + 1, S.nc1};
+  const View X3{b3 + b * n3f, S.nc2, S.nc1};
+  refops::prolong_fc_internal_at(X1, X2, X3, F, S.ng, k, j, i);
 }
 
 // The EMF restriction of FaceCenteredBoundaryVariable::LoadFluxBoundaryBufferToCoarser
@@ -330,20 +159,10 @@ __global__ void __launch_bounds__(RWG) k_restrict_emf(RShape S, RRange r, int nb
   const RGrid F(fcoord + b * S.fcl, S.nc1, S.nc2, S.nc3);
   const int fn1 = S.nc1 + (comp != 0), fn2 = S.nc2 + (comp != 1), fn3 = S.nc3 + (comp != 2);
   const int cn1 = S.ncc1 + (comp != 0), cn2 = S.ncc2 + (comp != 1), cn3 = S.ncc3 + (comp != 2);
-  const double *f = fine + static_cast<long>(b) * fn1 * fn2 * fn3;
+  const View f{const_cast<double *>(fine) + static_cast<long>(b) * fn1 * fn2 * fn3, fn2, fn1};
   const int k = (ck - S.ng) * 2 + S.ng, j = (cj - S.ng) * 2 + S.ng, i = (ci - S.ng) * 2 + S.ng;
-  double v;
-  if (comp == 0) {
-    const double l1 = F.dx1f[i], l2 = F.dx1f[i + 1];
-    v = (f[at(k, j, i, fn2, fn1)] * l1 + f[at(k, j, i + 1, fn2, fn1)] * l2) / (l1 + l2);
-  } else if (comp == 1) {
-    const double l1 = F.dx2f[j], l2 = F.dx2f[j + 1];
-    v = (f[at(k, j, i, fn2, fn1)] * l1 + f[at(k, j + 1, i, fn2, fn1)] * l2) / (l1 + l2);
-  } else {
-    const double l1 = F.dx3f[k], l2 = F.dx3f[k + 1];
-    v = (f[at(k, j, i, fn2, fn1)] * l1 + f[at(k + 1, j, i, fn2, fn1)] * l2) / (l1 + l2);
-  }
-  coarse[static_cast<long>(b) * cn1 * cn2 * cn3 + at(ck, cj, ci, cn2, cn1)] = v;
+  coarse[static_cast<long>(b) * cn1 * cn2 * cn3 + at(ck, cj, ci, cn2, cn1)] =
+      refops::restrict_emf_at(comp, f, F, k, j, i);
 }
 
 //----------------------------------------------------------------------------------------

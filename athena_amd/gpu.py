@@ -45,8 +45,9 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_block_add_flux_divergence", "ath_gpu_block_cons2prim",
            "ath_gpu_block_new_dt", "ath_gpu_block_new_dt_noalloc",
            "ath_gpu_mesh_set_fused", "ath_gpu_mesh_get_fused", "ath_gpu_flux_div_plan",
-           "ath_gpu_mesh_set_graph"]
-NKERNELS = 23
+           "ath_gpu_mesh_set_graph", "ath_gpu_smr_mesh_create", "ath_gpu_smr_block_level",
+           "ath_gpu_smr_plan"]
+NKERNELS = 26
 
 
 class AthGpuError(RuntimeError):
@@ -181,6 +182,12 @@ def load():
     L.ath_gpu_block_cons2prim.argtypes = [P, P, P, P, P, P, P, P, I, I, I, I, I, I, P]
     L.ath_gpu_block_new_dt.argtypes = [P, P, P, P, P, P, P, P]
     L.ath_gpu_block_new_dt_noalloc.argtypes = [P, P, P, P, P, P, P, P]
+    L.ath_gpu_smr_mesh_create.argtypes = [ctypes.POINTER(MeshDescC),
+                                          ctypes.POINTER(ctypes.c_longlong), I, I,
+                                          ctypes.POINTER(P)]
+    L.ath_gpu_smr_block_level.argtypes = [P, I, IP]
+    L.ath_gpu_smr_plan.argtypes = [ctypes.POINTER(MeshDescC), ctypes.POINTER(ctypes.c_longlong),
+                                   I, I, I, IP, I, IP, LP]
     _lib = L
     return L
 
@@ -256,15 +263,29 @@ def desc_of(m: MeshDesc) -> MeshDescC:
 
 
 class GpuMesh:
-    """Device mirror of one uniform mesh (all registers resident in HBM)."""
+    """Device mirror of one mesh (all registers resident in HBM): uniform, or statically
+    refined when ``m`` comes from ``athena_amd.mesh.refined``."""
 
     def __init__(self, m: MeshDesc):
         L = load()
         self.m = m
         self._desc = desc_of(m)
         h = ctypes.c_void_p()
-        _check(L.ath_gpu_mesh_create(ctypes.byref(self._desc), ctypes.byref(h)))
+        if m.smr:
+            self._locs = np.ascontiguousarray(m.locs, dtype=np.int64)
+            _check(L.ath_gpu_smr_mesh_create(
+                ctypes.byref(self._desc),
+                self._locs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                len(self._locs), int(m.root_level), ctypes.byref(h)))
+        else:
+            _check(L.ath_gpu_mesh_create(ctypes.byref(self._desc), ctypes.byref(h)))
         self.h = h
+
+    def level(self, gid):
+        """Logical level of MeshBlock gid (0 on a uniform mesh)."""
+        lv = ctypes.c_int()
+        _check(load().ath_gpu_smr_block_level(self.h, gid, ctypes.byref(lv)))
+        return lv.value
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -733,6 +754,22 @@ class Block:
         device scratch array, without allocating."""
         _check(load().ath_gpu_block_new_dt_noalloc(self.h, w, bcc, b1, b2, b3, dt_out,
                                                    self.stream))
+
+
+def smr_plan(m: MeshDesc, gid=0):
+    """Host-only check of a refined mesh (ath_gpu_smr_plan): returns (neighbour list of
+    block gid as (n, 9) int32 rows, step counts of the flux, EMF and boundary programs)."""
+    L = load()
+    desc = desc_of(m)
+    locs = np.ascontiguousarray(m.locs, dtype=np.int64)
+    out = np.zeros(9 * 64, dtype=np.int32)
+    nn = ctypes.c_int()
+    steps = np.zeros(3, dtype=np.int64)
+    _check(L.ath_gpu_smr_plan(ctypes.byref(desc),
+                              locs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(locs),
+                              int(m.root_level), gid, _ip(out), 64, ctypes.byref(nn),
+                              steps.ctypes.data_as(ctypes.POINTER(ctypes.c_long))))
+    return out[:9 * nn.value].reshape(nn.value, 9), steps
 
 
 def flux_div_plan(nx, ng, rows):

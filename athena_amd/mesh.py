@@ -58,6 +58,12 @@ class MeshDesc:
     beta_min: float = -1.0
     gamma_max: float = -1.0
     bc: tuple = ("periodic",) * 6   # ix1_bc, ox1_bc, ix2_bc, ox2_bc, ix3_bc, ox3_bc
+    # Static mesh refinement (``refined()``): the MeshBlock list in the tree's Z order as
+    # rows (level, lx1, lx2, lx3), the logical level of the root grid, and per block the
+    # neighbour levels nblevel[o3+1][o2+1][o1+1] (-1: none).  None: a uniform mesh.
+    locs: object = None
+    root_level: int = 0
+    nblevel: object = None
 
     def __post_init__(self):
         self.nx = tuple(int(v) for v in self.nx)
@@ -90,8 +96,18 @@ class MeshDesc:
 
     @property
     def nblocks(self):
+        if self.locs is not None:
+            return len(self.locs)
         a, b, c = self.nbx
         return a * b * c
+
+    @property
+    def smr(self):
+        return self.locs is not None
+
+    def level_shift(self, gid):
+        """Levels of block gid above the root grid (0 on a uniform mesh)."""
+        return 0 if self.locs is None else int(self.locs[gid][0]) - self.root_level
 
     @property
     def nc(self):
@@ -99,6 +115,8 @@ class MeshDesc:
         return tuple(self.bnx[d] + 2 * self.ng for d in range(3))
 
     def loc(self, gid):
+        if self.locs is not None:
+            return tuple(int(v) for v in self.locs[gid][1:])
         n1, n2, _ = self.nbx
         return (gid % n1, (gid // n1) % n2, gid // (n1 * n2))
 
@@ -150,21 +168,23 @@ class Coords:
 def block_coords(m: MeshDesc, gid: int) -> Coords:
     """Face/centre coordinates of one block (coordinates.cpp:129-146, cartesian.cpp)."""
     lx = m.loc(gid)
-    nbx = m.nbx
+    shift = m.level_shift(gid)
     out = Coords()
     for d in range(3):
         nx, ng = m.bnx[d], m.ng
         nc = nx + 2 * ng
         l = lx[d]
-        bmin = m.xmin[d] if l == 0 else _uniform_gen(_mesh_gen_x(l, nbx[d]), m.xmin[d], m.xmax[d])
-        bmax = (m.xmax[d] if l == nbx[d] - 1
-                else _uniform_gen(_mesh_gen_x(l + 1, nbx[d]), m.xmin[d], m.xmax[d]))
+        nbl = m.nbx[d] << shift          # blocks along d at this level
+        nroot = m.nx[d] << shift         # cells along d at this level
+        bmin = m.xmin[d] if l == 0 else _uniform_gen(_mesh_gen_x(l, nbl), m.xmin[d], m.xmax[d])
+        bmax = (m.xmax[d] if l == nbl - 1
+                else _uniform_gen(_mesh_gen_x(l + 1, nbl), m.xmin[d], m.xmax[d]))
         il, iu = ng, ng + nx - 1
         dx = (bmax - bmin) / (iu - il + 1)
         xf = np.zeros(nc + 1)
         for i in range(il - ng, iu + ng + 2):
             noff = i - il + l * nx
-            xf[i] = _uniform_gen(_mesh_gen_x(noff, m.nx[d]), m.xmin[d], m.xmax[d])
+            xf[i] = _uniform_gen(_mesh_gen_x(noff, nroot), m.xmin[d], m.xmax[d])
         xf[il] = bmin
         xf[iu + 1] = bmax
         dxf = np.full(nc, dx)
@@ -173,6 +193,32 @@ def block_coords(m: MeshDesc, gid: int) -> Coords:
         setattr(out, "x%dv" % (d + 1), xv)
         setattr(out, "dx%df" % (d + 1), dxf)
     return out
+
+
+def refined(m: MeshDesc, regions) -> MeshDesc:
+    """The statically refined version of uniform mesh ``m``: ``m.nx`` is the root grid,
+    ``regions`` the <refinementN> blocks as (xmin[3], xmax[3], level) triples.  The
+    MeshBlock list and the neighbour levels come from the library's tree builder
+    (MeshBlockTree, meshblock_tree.cpp:87-468; a host function, no GPU needed)."""
+    import dataclasses
+    from athena_amd import gpu
+    bcf = [BOUNDARY[f] for f in m.bc]
+    locs, nbr, root_level = gpu.tree_build(m.nbx, bcf, m.xmin, m.xmax, regions)
+    n = len(locs)
+    nbl = np.full((n, 3, 3, 3), -1, dtype=np.int64)
+    for g in range(n):
+        for o3 in range(3):
+            for o2 in range(3):
+                for o1 in range(3):
+                    v = int(nbr[g, o3, o2, o1])
+                    if (o1, o2, o3) == (1, 1, 1):
+                        nbl[g, o3, o2, o1] = locs[g][0]
+                    elif v >= 0:
+                        nbl[g, o3, o2, o1] = locs[v][0]
+                    elif v == -2:
+                        nbl[g, o3, o2, o1] = locs[g][0] + 1
+    return dataclasses.replace(m, locs=np.asarray(locs, dtype=np.int64),
+                               root_level=int(root_level), nblevel=nbl)
 
 
 def new_block_arrays(m: MeshDesc):

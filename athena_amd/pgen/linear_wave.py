@@ -195,6 +195,8 @@ class LinearWave:
         a1 = self.A1(c.x1v[I], c.x2f[J], c.x3f[K])
         a2 = self.A2(c.x1f[I], c.x2v[J], c.x3f[K])
         a3 = self.A3(c.x1f[I], c.x2f[J], c.x3v[K])
+        if m.nblevel is not None:
+            self._refined_edges(m, gid, c, a1, a2, a3, (is_, ie, js, je, ks, ke))
         # shapes (n3+1, n2+1, n1+1); local index (k-ks, j-js, i-is)
         dx1 = c.dx1f[is_:ie + 2][None, None, :]
         dx2 = c.dx2f[js:je + 2][None, :, None]
@@ -230,6 +232,53 @@ class LinearWave:
         e = e + 0.5 * (self.bx0 * self.bx0 + self.by0 * self.by0 + self.bz0 * self.bz0)
         u[(IEN,) + sl] = e
         return a
+
+    def _refined_edges(self, m, gid, c, a1, a2, a3, rng):
+        """On a statically refined mesh, a block edge that touches a finer neighbour
+        takes the mean of the vector potential at the two fine edge centres, so that the
+        coarse faces there equal the restriction of the fine ones (linear_wave.cpp:
+        454-510).  a1..a3 are indexed (k-ks, j-js, i-is) over [ks..ke+1] x [js..je+1] x
+        [is..ie+1]."""
+        is_, ie, js, je, ks, ke = rng
+        nbl = m.nblevel[gid]
+        lev = int(m.locs[gid][0])
+        fin = lambda o3, o2, o1: nbl[o3 + 1, o2 + 1, o1 + 1] > lev  # noqa: E731
+        for k in range(ks, ke + 2):
+            for j in range(js, je + 2):
+                for i in range(is_, ie + 2):
+                    kk, jj, ii = k - ks, j - js, i - is_
+                    lo_j, hi_j, lo_k, hi_k = j == js, j == je + 1, k == ks, k == ke + 1
+                    lo_i, hi_i = i == is_, i == ie + 1
+                    if ((fin(0, -1, 0) and lo_j) or (fin(0, 1, 0) and hi_j)
+                            or (fin(-1, 0, 0) and lo_k) or (fin(1, 0, 0) and hi_k)
+                            or (fin(-1, -1, 0) and lo_j and lo_k)
+                            or (fin(-1, 1, 0) and hi_j and lo_k)
+                            or (fin(1, -1, 0) and lo_j and hi_k)
+                            or (fin(1, 1, 0) and hi_j and hi_k)):
+                        x1l = c.x1f[i] + 0.25 * c.dx1f[i]
+                        x1r = c.x1f[i] + 0.75 * c.dx1f[i]
+                        a1[kk, jj, ii] = 0.5 * (self.A1(x1l, c.x2f[j], c.x3f[k]) +
+                                                self.A1(x1r, c.x2f[j], c.x3f[k]))
+                    if ((fin(0, 0, -1) and lo_i) or (fin(0, 0, 1) and hi_i)
+                            or (fin(-1, 0, 0) and lo_k) or (fin(1, 0, 0) and hi_k)
+                            or (fin(-1, 0, -1) and lo_i and lo_k)
+                            or (fin(-1, 0, 1) and hi_i and lo_k)
+                            or (fin(1, 0, -1) and lo_i and hi_k)
+                            or (fin(1, 0, 1) and hi_i and hi_k)):
+                        x2l = c.x2f[j] + 0.25 * c.dx2f[j]
+                        x2r = c.x2f[j] + 0.75 * c.dx2f[j]
+                        a2[kk, jj, ii] = 0.5 * (self.A2(c.x1f[i], x2l, c.x3f[k]) +
+                                                self.A2(c.x1f[i], x2r, c.x3f[k]))
+                    if ((fin(0, 0, -1) and lo_i) or (fin(0, 0, 1) and hi_i)
+                            or (fin(0, -1, 0) and lo_j) or (fin(0, 1, 0) and hi_j)
+                            or (fin(0, -1, -1) and lo_i and lo_j)
+                            or (fin(0, -1, 1) and hi_i and lo_j)
+                            or (fin(0, 1, -1) and lo_i and hi_j)
+                            or (fin(0, 1, 1) and hi_i and hi_j)):
+                        x3l = c.x3f[k] + 0.25 * c.dx3f[k]
+                        x3r = c.x3f[k] + 0.75 * c.dx3f[k]
+                        a3[kk, jj, ii] = 0.5 * (self.A3(c.x1f[i], c.x2f[j], x3l) +
+                                                self.A3(c.x1f[i], c.x2f[j], x3r))
 
     def errors(self, get_block):
         """Volume-weighted L1 errors (UserWorkAfterLoop :190-370).  ``get_block(gid)``

@@ -63,6 +63,34 @@ enum {
 /* Creates the device mirror of a mesh (replaces the per-block allocations of the
  * Hydro/Field constructors, hydro.cpp:31-60, field.cpp:27-105). */
 int ath_gpu_mesh_create(const ath_mesh_desc *desc, ath_mesh_t *out);
+
+/* Creates the device mirror of a statically refined mesh (<mesh> refinement = static;
+ * Mesh::Mesh with <refinementN> regions, mesh.cpp:306-482).  desc describes the root grid
+ * (nx: root-grid cells, bnx: MeshBlock cells); locs[4 g .. 4 g + 3] = (level, lx1, lx2,
+ * lx3) of MeshBlock g in the tree's order, as ath_gpu_tree_build lists them, root_level
+ * the logical level of the root grid.  Neighbours follow BoundaryBase::
+ * SearchAndSetNeighbors (bvals_base.cpp:298-700).  Every stage of ath_gpu_cycle then adds,
+ * as the multilevel task list does (time_integrator.cpp:922-1185): the restricted fine
+ * fluxes on coarse/fine faces (cc/flux_correction_cc.cpp:77-286), the EMF correction
+ * (fc/flux_correction_fc.cpp:51-1548), and the boundary exchange between levels with
+ * restriction and prolongation (cc/bvals_cc.cpp:287-540, fc/bvals_fc.cpp:344-975,
+ * bvals_refine.cpp:98-673) in place of the same-level exchange.  Periodic faces and
+ * Newtonian MHD only; nghost even (as MeshRefinement requires, mesh_refinement.cpp:42-48),
+ * MeshBlock sizes even and at least 2 nghost; the list must cover the
+ * root grid with adjacent MeshBlocks at most one level apart.  The fused flux kernel and
+ * the autotuner are refused on such meshes. */
+int ath_gpu_smr_mesh_create(const ath_mesh_desc *desc, const long long *locs, int nblocks,
+                            int root_level, ath_mesh_t *out);
+/* Logical level of MeshBlock gid (0 on a uniform mesh). */
+int ath_gpu_smr_block_level(ath_mesh_t mesh, int gid, int *level);
+/* Host only, no GPU: checks a refined mesh's MeshBlock list as ath_gpu_smr_mesh_create
+ * does and builds its step programs (every box checked against its arrays).  Writes
+ * *nnbr and, up to cap entries, block gid's neighbour list in SearchAndSetNeighbors order
+ * as 9 ints each (gid, level, ox1, ox2, ox3, type 1 face / 2 edge / 3 corner, fi1, fi2,
+ * eid for edges or fid for faces), and nsteps[0..2] = the step counts of the flux, EMF and
+ * boundary programs of the whole mesh. */
+int ath_gpu_smr_plan(const ath_mesh_desc *desc, const long long *locs, int nblocks,
+                     int root_level, int gid, int *nbr_out, int cap, int *nnbr, long *nsteps);
 int ath_gpu_mesh_destroy(ath_mesh_t mesh);
 int ath_gpu_nblocks(ath_mesh_t mesh, int *nblocks);
 /* Number of doubles of one block's register `what`. */

@@ -1,0 +1,1581 @@
+//========================================================================================
+// smr_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of Athena++'s VL2 time stepping on a statically refined mesh, the
+// checker for the device SMR path of athena_amd/csrc (ath_gpu_smr_mesh_create).  Only
+// tests/ may load it.  Scope: 3-D, uniform Cartesian coordinates, periodic mesh faces,
+// Newtonian MHD, NGHOST >= 2, MeshBlocks of equal size with adjacent levels differing by
+// at most one (the tree the reference builds for <refinementN> regions).
+//
+// Per stage it restates the multilevel task list of TimeIntegratorTaskList
+// (time_integrator.cpp:922-1185): CalculateFluxes; SEND/RECV_HYDFLX, the restricted fine
+// fluxes replacing the coarse ones on a coarse/fine face (cc/flux_correction_cc.cpp:77-
+// 170, 237-286); ComputeCornerE; SEND/RECV_FLDFLX, the EMF correction
+// (fc/flux_correction_fc.cpp:51-573 pack, 623-678 send, 689-1191 add, 1214-1354 clear,
+// 1355-1548 average, 1610-1700 receive order: same level, clear, finer); IntegrateHydro
+// and IntegrateField; SEND/SETB_HYD and _FLD with the buffers of cc/bvals_cc.cpp:287-520
+// and fc/bvals_fc.cpp:344-975 (same level, to/from coarser, to/from finer, including the
+// multilevel overlapping faces of edge and corner neighbours); PROLONG
+// (bvals_refine.cpp:98-673: RestrictGhostCellsOnSameLevel, ConservedToPrimitive on the
+// coarse buffer, ProlongateCellCenteredValues on primitives, ProlongateSharedField and
+// ProlongateInternalField, then bcc and PrimitiveToConserved on the fine ghosts);
+// CONS2PRIM over the whole block; NEW_DT.  Neighbour lists follow
+// BoundaryBase::SearchAndSetNeighbors (bvals_base.cpp:298-700, order faces, x1x2 edges,
+// x1x3 edges, x2x3 edges, corners; a coarser edge or corner neighbour listed only where it
+// is not already a face or edge neighbour) and CountFineEdges (bvals_fc.cpp:1129-1197).
+// Block and coarse-buffer coordinates follow Mesh::SetBlockSizeAndBoundaries
+// (mesh.cpp:1984-2060) and the Coordinates constructor (coordinates.cpp:30-146) at the
+// block's level.
+//
+// Every block packs its buffers before any block unpacks (the task list sends before it
+// receives, and packs read only active data), so the stage runs block by block in
+// lockstep.  The same-level EMF contributions are added in neighbour order; the
+// reference adds the ones that have arrived at its first receive attempt first, an
+// order that depends on task scheduling and changes only the last bit.
+//
+// Parity pinning (tests/test_smr_oracle.py): on a mesh without refined blocks it agrees
+// with the uniform restatement (athena_oracle.cpp) to round-off; a uniform state stays
+// uniform bit for bit; total mass, momentum and energy are conserved and div(B) stays at
+// round-off through coarse/fine boundaries (they would not without the flux and EMF
+// corrections); the reference's SMR linear-wave errors stay under the thresholds of
+// tst/regression/scripts/tests/mhd/mhd_linwave.py:122-153 and converge.
+//========================================================================================
+#include <array>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "oracle_core.hpp"
+
+extern "C" {   // refine_oracle.cpp
+void ora_restrict_cc(const int *shape, int nvar, const double *fcoord, const double *fine,
+                     double *coarse, const int *r);
+void ora_restrict_fc(const int *shape, int comp, const double *fcoord, const double *fine,
+                     double *coarse, const int *r);
+void ora_prolongate_cc(const int *shape, int nvar, const double *fcoord,
+                       const double *ccoord, const double *coarse, double *fine,
+                       const int *r);
+void ora_prolongate_fc_shared(const int *shape, int comp, const double *fcoord,
+                              const double *ccoord, const double *coarse, double *fine,
+                              const int *r);
+void ora_prolongate_fc_internal(const int *shape, const double *fcoord, double *b1,
+                                double *b2, double *b3, const int *r);
+}
+
+namespace ora {
+// athena_oracle.cpp
+void CalculateFluxes(Mesh &m, Block &b, int order);
+void ComputeCornerE(Mesh &m, Block &b);
+void AddFluxDivergence(Block &b, Real wght, Arr &u_out);
+void CT(Block &b, Real wght);
+void ConservedToPrimitive(Mesh &m, Block &b, Arr &cons, Arr &prim_old, Arr &prim);
+void Cons2PrimNewtonian(Mesh &m, Block &b, Arr &cons, Arr &prim, int il, int iu, int jl,
+                        int ju, int kl, int ku);
+void Prim2Cons(Mesh &m, Block &b, int il, int iu, int jl, int ju, int kl, int ku);
+void NewBlockTimeStep(Mesh &m, Block &b);
+void NewTimeStep(Mesh &m, Real tlim);
+Real MeshGenX(std::int64_t index, std::int64_t nrange);
+Real UniformGen(Real x, Real xmin, Real xmax);
+}  // namespace ora
+
+namespace {
+
+using namespace ora;
+
+enum { FACE = 1, EDGE = 2, CORNER = 3 };
+
+// NeighborBlock (bvals.hpp) as SearchAndSetNeighbors fills it
+struct Nbr {
+  int gid, level;
+  int ox1, ox2, ox3;
+  int type;
+  int fi1, fi2;
+  int fid, eid;
+};
+
+struct SmrBlock {
+  int level;
+  std::int64_t lx[3];
+  std::vector<Nbr> nb;
+  int nblevel[3][3][3];
+  bool edge_flag[12];
+  int nedge_fine[12];
+  Arr ccons, cprim, cbcc, cb1, cb2, cb3;   // coarse buffers (hydro.cpp:48-52, field.cpp:39-44)
+  std::vector<Real> fco, cco;              // coordinate tables, refine_oracle.cpp layout
+  std::vector<Real> cdx1f, cdx2f, cdx3f;   // coarse spacings (bcc on the coarse buffer)
+};
+
+struct SmrMesh : Mesh {
+  int root_level = 0;
+  int nrb[3] = {1, 1, 1};
+  int cng = 2;                             // cnghost (meshblock.cpp:92)
+  int cis = 0, cie = 0, cjs = 0, cje = 0, cks = 0, cke = 0;
+  std::vector<SmrBlock> sb;
+  std::map<std::array<std::int64_t, 4>, int> leaf;   // (level, lx1, lx2, lx3) -> gid
+  std::string err;
+  int skip = 0;   // planted omissions for the tests: 1 flux correction, 2 EMF correction
+};
+
+using Key = std::array<int, 5>;            // receiver, sender, ox1, ox2, ox3 (receiver's)
+using Buffers = std::map<Key, std::vector<Real>>;
+
+// ---------------------------------------------------------------------------- geometry
+int leaf_gid(const SmrMesh &m, int level, std::int64_t l1, std::int64_t l2, std::int64_t l3) {
+  auto it = m.leaf.find({level, l1, l2, l3});
+  return (it == m.leaf.end()) ? -1 : it->second;
+}
+
+// Block bounds and coordinates at the block's level (mesh.cpp:1984-2060,
+// coordinates.cpp:129-146 uniform branch, cartesian.cpp x?v).  coarse: the coarse-buffer
+// grid (coarse_flag; nx/2 cells, faces at every second fine face).
+void level_coords(const SmrMesh &m, const SmrBlock &s, const Block &b, int d, bool coarse,
+                  std::vector<Real> &xf, std::vector<Real> &xv, std::vector<Real> &dxf) {
+  const int shift = s.level - m.root_level;
+  const std::int64_t nrbl = static_cast<std::int64_t>(m.nrb[d]) << shift;
+  const std::int64_t nroot = static_cast<std::int64_t>(m.c.nx[d]) << shift;
+  const std::int64_t l = s.lx[d];
+  const int nxb = (d == 0) ? b.nx1 : ((d == 1) ? b.nx2 : b.nx3);
+  const Real bmin = (l == 0) ? m.c.xmin[d] : UniformGen(MeshGenX(l, nrbl), m.c.xmin[d], m.c.xmax[d]);
+  const Real bmax = (l == nrbl - 1) ? m.c.xmax[d]
+                                    : UniformGen(MeshGenX(l + 1, nrbl), m.c.xmin[d], m.c.xmax[d]);
+  const int ng = b.ng;
+  const int n = coarse ? nxb / 2 : nxb;
+  const int nc = n + 2 * ng, il = ng, iu = ng + n - 1;
+  xf.assign(nc + 1, 0.0); xv.assign(nc, 0.0); dxf.assign(nc, 0.0);
+  const Real dx = (bmax - bmin) / (iu - il + 1);
+  for (int i = il - ng; i <= iu + ng + 1; ++i) {
+    const std::int64_t noff = coarse ? static_cast<std::int64_t>((i - il) * 2 + l * nxb)
+                                     : static_cast<std::int64_t>(i - il + l * nxb);
+    xf[i] = UniformGen(MeshGenX(noff, nroot), m.c.xmin[d], m.c.xmax[d]);
+  }
+  xf[il] = bmin;
+  xf[iu + 1] = bmax;
+  for (int i = il - ng; i <= iu + ng; ++i) dxf[i] = dx;
+  for (int i = il - ng; i <= iu + ng; ++i) xv[i] = 0.5 * (xf[i + 1] + xf[i]);
+}
+
+void set_block_coords(SmrMesh &m, int g) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  std::vector<Real> xf[3], xv[3], dxf[3], cxf[3], cxv[3], cdxf[3];
+  for (int d = 0; d < 3; ++d) {
+    level_coords(m, s, b, d, false, xf[d], xv[d], dxf[d]);
+    level_coords(m, s, b, d, true, cxf[d], cxv[d], cdxf[d]);
+  }
+  b.x1f = xf[0]; b.x2f = xf[1]; b.x3f = xf[2];
+  b.x1v = xv[0]; b.x2v = xv[1]; b.x3v = xv[2];
+  b.dx1f = dxf[0]; b.dx2f = dxf[1]; b.dx3f = dxf[2];
+  b.dx1v = dxf[0]; b.dx2v = dxf[1]; b.dx3v = dxf[2];
+  for (int d = 0; d < 3; ++d) {
+    b.xbmin[d] = xf[d][b.ng];
+    b.xbmax[d] = xf[d][xf[d].size() - 1 - b.ng];
+  }
+  s.fco.clear(); s.cco.clear();
+  for (int d = 0; d < 3; ++d) {
+    s.fco.insert(s.fco.end(), xf[d].begin(), xf[d].end());
+    s.fco.insert(s.fco.end(), xv[d].begin(), xv[d].end());
+    s.fco.insert(s.fco.end(), dxf[d].begin(), dxf[d].end());
+    s.cco.insert(s.cco.end(), cxf[d].begin(), cxf[d].end());
+    s.cco.insert(s.cco.end(), cxv[d].begin(), cxv[d].end());
+    s.cco.insert(s.cco.end(), cdxf[d].begin(), cdxf[d].end());
+  }
+  s.cdx1f = cdxf[0]; s.cdx2f = cdxf[1]; s.cdx3f = cdxf[2];
+}
+
+// MeshBlockTree::FindNeighbor for a periodic mesh (meshblock_tree.cpp:362-468): the leaf at
+// the same level, the coarser leaf that covers the location, or the refined node there.
+struct Found {
+  int kind;               // 0 same or coarser leaf, 1 refined node (finer neighbours)
+  int gid, level;
+  std::int64_t nl[3];     // location of the node at the block's own level
+};
+
+Found find_neighbor(const SmrMesh &m, const SmrBlock &s, int o1, int o2, int o3) {
+  const int o[3] = {o1, o2, o3};
+  Found f{0, -1, s.level, {0, 0, 0}};
+  for (int d = 0; d < 3; ++d) {
+    const std::int64_t nrbl = static_cast<std::int64_t>(m.nrb[d]) << (s.level - m.root_level);
+    std::int64_t l = s.lx[d] + o[d];
+    if (l < 0) l += nrbl;
+    if (l >= nrbl) l -= nrbl;
+    f.nl[d] = l;
+  }
+  int g = leaf_gid(m, s.level, f.nl[0], f.nl[1], f.nl[2]);
+  if (g >= 0) { f.gid = g; return f; }
+  if (s.level > m.root_level) {
+    g = leaf_gid(m, s.level - 1, f.nl[0] >> 1, f.nl[1] >> 1, f.nl[2] >> 1);
+    if (g >= 0) { f.gid = g; f.level = s.level - 1; return f; }
+  }
+  f.kind = 1;
+  f.level = s.level + 1;
+  return f;
+}
+
+int child(const SmrMesh &m, const Found &f, int c1, int c2, int c3) {
+  const int g = leaf_gid(m, f.level, 2 * f.nl[0] + c1, 2 * f.nl[1] + c2, 2 * f.nl[2] + c3);
+  if (g < 0) throw std::runtime_error("neighbour refined by more than one level");
+  return g;
+}
+
+void add_nbr(SmrBlock &s, int gid, int level, int o1, int o2, int o3, int type, int fi1,
+             int fi2) {
+  Nbr n{gid, level, o1, o2, o3, type, fi1, fi2, -1, -1};
+  if (type == FACE) {
+    if (o1 == -1) n.fid = 0;
+    else if (o1 == 1) n.fid = 1;
+    else if (o2 == -1) n.fid = 2;
+    else if (o2 == 1) n.fid = 3;
+    else if (o3 == -1) n.fid = 4;
+    else n.fid = 5;
+  }
+  if (type == EDGE) {   // NeighborBlock::SetNeighbor (bvals_base.cpp:62-66)
+    if (o3 == 0) n.eid = ((o1 + 1) >> 1) | ((o2 + 1) & 2);
+    else if (o2 == 0) n.eid = 4 + (((o1 + 1) >> 1) | ((o3 + 1) & 2));
+    else n.eid = 8 + (((o2 + 1) >> 1) | ((o3 + 1) & 2));
+  }
+  s.nb.push_back(n);
+}
+
+void search_neighbors(SmrMesh &m, int g) {
+  SmrBlock &s = m.sb[g];
+  s.nb.clear();
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) s.nblevel[k][j][i] = -1;
+  s.nblevel[1][1][1] = s.level;
+  const int my1 = (s.lx[0] & 1) ? 1 : -1, my2 = (s.lx[1] & 1) ? 1 : -1;
+  const int my3 = (s.lx[2] & 1) ? 1 : -1;
+  // faces
+  for (int d = 0; d < 3; ++d)
+    for (int n = -1; n <= 1; n += 2) {
+      int o[3] = {0, 0, 0};
+      o[d] = n;
+      Found f = find_neighbor(m, s, o[0], o[1], o[2]);
+      s.nblevel[o[2] + 1][o[1] + 1][o[0] + 1] = f.level;
+      if (f.kind == 1) {
+        const int fface = 1 - (n + 1) / 2;
+        for (int f2 = 0; f2 < 2; ++f2)
+          for (int f1 = 0; f1 < 2; ++f1) {
+            int c[3];
+            if (d == 0) { c[0] = fface; c[1] = f1; c[2] = f2; }
+            else if (d == 1) { c[0] = f1; c[1] = fface; c[2] = f2; }
+            else { c[0] = f1; c[1] = f2; c[2] = fface; }
+            add_nbr(s, child(m, f, c[0], c[1], c[2]), f.level, o[0], o[1], o[2], FACE, f1, f2);
+          }
+      } else {
+        add_nbr(s, f.gid, f.level, o[0], o[1], o[2], FACE, 0, 0);
+      }
+    }
+  // edges: (x1, x2), (x1, x3), (x2, x3); outer loop over the second direction
+  const int ed[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int e = 0; e < 3; ++e) {
+    const int da = ed[e][0], db = ed[e][1];
+    const int dc = 3 - da - db;             // the edge's own direction
+    for (int mm = -1; mm <= 1; mm += 2)
+      for (int n = -1; n <= 1; n += 2) {
+        int o[3] = {0, 0, 0};
+        o[da] = n;
+        o[db] = mm;
+        Found f = find_neighbor(m, s, o[0], o[1], o[2]);
+        s.nblevel[o[2] + 1][o[1] + 1][o[0] + 1] = f.level;
+        if (f.kind == 1) {
+          for (int f1 = 0; f1 < 2; ++f1) {
+            int c[3];
+            c[da] = 1 - (n + 1) / 2;
+            c[db] = 1 - (mm + 1) / 2;
+            c[dc] = f1;
+            add_nbr(s, child(m, f, c[0], c[1], c[2]), f.level, o[0], o[1], o[2], EDGE, f1, 0);
+          }
+        } else {
+          const int mya = (da == 0) ? my1 : my2, myb = (db == 1) ? my2 : my3;
+          if (f.level >= s.level || (mya == n && myb == mm))
+            add_nbr(s, f.gid, f.level, o[0], o[1], o[2], EDGE, 0, 0);
+        }
+      }
+  }
+  // corners
+  for (int l = -1; l <= 1; l += 2)
+    for (int mm = -1; mm <= 1; mm += 2)
+      for (int n = -1; n <= 1; n += 2) {
+        Found f = find_neighbor(m, s, n, mm, l);
+        int gid = f.gid;
+        if (f.kind == 1) gid = child(m, f, 1 - (n + 1) / 2, 1 - (mm + 1) / 2, 1 - (l + 1) / 2);
+        s.nblevel[l + 1][mm + 1][n + 1] = f.level;
+        if (f.level >= s.level || (my1 == n && my2 == mm && my3 == l))
+          add_nbr(s, gid, f.level, n, mm, l, CORNER, 0, 0);
+      }
+  // CountFineEdges (bvals_fc.cpp:1129-1197)
+  int eid = 0;
+  for (int e = 0; e < 3; ++e) {
+    const int da = ed[e][0], db = ed[e][1];
+    for (int ob = -1; ob <= 1; ob += 2)
+      for (int oa = -1; oa <= 1; oa += 2) {
+        int nf = 0, fl = s.level;
+        for (int qb = std::max(ob - 1, -1); qb <= std::min(ob + 1, 1); ++qb)
+          for (int qa = std::max(oa - 1, -1); qa <= std::min(oa + 1, 1); ++qa) {
+            int o[3] = {0, 0, 0};
+            o[da] = qa;
+            o[db] = qb;
+            const int lv = s.nblevel[o[2] + 1][o[1] + 1][o[0] + 1];
+            if (lv > fl) { fl++; nf = 0; }
+            if (lv == fl) nf++;
+          }
+        s.edge_flag[eid] = (fl == s.level);
+        s.nedge_fine[eid++] = nf;
+      }
+  }
+  // the uniform-mesh helpers read nb[][][] >= 0 as "a neighbour exists on that side"
+  Block &b = m.blocks[g];
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) b.nb[k][j][i] = (s.nblevel[k][j][i] >= 0) ? 0 : -1;
+}
+
+// ------------------------------------------------------------------- box pack / unpack
+struct Box { int si, ei, sj, ej, sk, ek; };
+
+void pack(Arr &a, int nl, int nu, const Box &r, std::vector<Real> &buf) {
+  for (int n = nl; n <= nu; ++n)
+    for (int k = r.sk; k <= r.ek; ++k)
+      for (int j = r.sj; j <= r.ej; ++j)
+        for (int i = r.si; i <= r.ei; ++i) buf.push_back(a(n, k, j, i));
+}
+
+void unpack(const std::vector<Real> &buf, std::size_t &p, Arr &a, int nl, int nu,
+            const Box &r) {
+  for (int n = nl; n <= nu; ++n)
+    for (int k = r.sk; k <= r.ek; ++k)
+      for (int j = r.sj; j <= r.ej; ++j)
+        for (int i = r.si; i <= r.ei; ++i) {
+          if (p >= buf.size()) throw std::runtime_error("boundary buffer underrun");
+          a(n, k, j, i) = buf[p++];
+        }
+}
+
+int shape_of(const Block &b, int *sh) {
+  sh[0] = b.nx1; sh[1] = b.nx2; sh[2] = b.nx3; sh[3] = b.ng;
+  return 0;
+}
+
+void restrict_cc(SmrBlock &s, Block &b, Arr &fine, Arr &coarse, int nvar, const Box &r) {
+  int sh[4];
+  shape_of(b, sh);
+  const int rr[6] = {r.si, r.ei, r.sj, r.ej, r.sk, r.ek};
+  ora_restrict_cc(sh, nvar, s.fco.data(), fine.v.data(), coarse.v.data(), rr);
+}
+void restrict_fc(SmrBlock &s, Block &b, int comp, Arr &fine, Arr &coarse, const Box &r) {
+  int sh[4];
+  shape_of(b, sh);
+  const int rr[6] = {r.si, r.ei, r.sj, r.ej, r.sk, r.ek};
+  ora_restrict_fc(sh, comp, s.fco.data(), fine.v.data(), coarse.v.data(), rr);
+}
+
+// fi of the finer neighbour along the transverse directions of a range computation
+// (bvals_cc.cpp:352-372): direction 2 uses fi1 when ox1 != 0, direction 3 uses fi1 when
+// both ox1 and ox2 are nonzero.
+int fi_x2(const Nbr &nb) { return (nb.ox1 != 0) ? nb.fi1 : nb.fi2; }
+int fi_x3(const Nbr &nb) { return (nb.ox1 != 0 && nb.ox2 != 0) ? nb.fi1 : nb.fi2; }
+
+// -------------------------------------------------------- cell-centred buffers (u)
+// LoadBoundaryBufferSameLevel (bvals_cc.cpp:287-301)
+Box cc_same_send(const Block &b, const Nbr &nb) {
+  const int NG = b.ng;
+  Box r;
+  r.si = (nb.ox1 > 0) ? (b.ie - NG + 1) : b.is;  r.ei = (nb.ox1 < 0) ? (b.is + NG - 1) : b.ie;
+  r.sj = (nb.ox2 > 0) ? (b.je - NG + 1) : b.js;  r.ej = (nb.ox2 < 0) ? (b.js + NG - 1) : b.je;
+  r.sk = (nb.ox3 > 0) ? (b.ke - NG + 1) : b.ks;  r.ek = (nb.ox3 < 0) ? (b.ks + NG - 1) : b.ke;
+  return r;
+}
+// LoadBoundaryBufferToCoarser (bvals_cc.cpp:309-330), in coarse indices
+Box cc_coarser_send(const SmrMesh &m, const Block &b, const Nbr &nb) {
+  const int cn = b.ng - 1;
+  Box r;
+  r.si = (nb.ox1 > 0) ? (m.cie - cn) : m.cis;  r.ei = (nb.ox1 < 0) ? (m.cis + cn) : m.cie;
+  r.sj = (nb.ox2 > 0) ? (m.cje - cn) : m.cjs;  r.ej = (nb.ox2 < 0) ? (m.cjs + cn) : m.cje;
+  r.sk = (nb.ox3 > 0) ? (m.cke - cn) : m.cks;  r.ek = (nb.ox3 < 0) ? (m.cks + cn) : m.cke;
+  return r;
+}
+// LoadBoundaryBufferToFiner (bvals_cc.cpp:337-377)
+Box cc_finer_send(const SmrMesh &m, const Block &b, const Nbr &nb) {
+  const int cn = m.cng - 1;
+  Box r;
+  r.si = (nb.ox1 > 0) ? (b.ie - cn) : b.is;  r.ei = (nb.ox1 < 0) ? (b.is + cn) : b.ie;
+  r.sj = (nb.ox2 > 0) ? (b.je - cn) : b.js;  r.ej = (nb.ox2 < 0) ? (b.js + cn) : b.je;
+  r.sk = (nb.ox3 > 0) ? (b.ke - cn) : b.ks;  r.ek = (nb.ox3 < 0) ? (b.ks + cn) : b.ke;
+  if (nb.ox1 == 0) {
+    if (nb.fi1 == 1) r.si += b.nx1 / 2 - m.cng;
+    else r.ei -= b.nx1 / 2 - m.cng;
+  }
+  if (nb.ox2 == 0) {
+    if (fi_x2(nb) == 1) r.sj += b.nx2 / 2 - m.cng;
+    else r.ej -= b.nx2 / 2 - m.cng;
+  }
+  if (nb.ox3 == 0) {
+    if (fi_x3(nb) == 1) r.sk += b.nx3 / 2 - m.cng;
+    else r.ek -= b.nx3 / 2 - m.cng;
+  }
+  return r;
+}
+// SetBoundarySameLevel (bvals_cc.cpp:386-427)
+Box cc_same_recv(const Block &b, const Nbr &nb) {
+  const int NG = b.ng;
+  Box r;
+  if (nb.ox1 == 0) { r.si = b.is; r.ei = b.ie; }
+  else if (nb.ox1 > 0) { r.si = b.ie + 1; r.ei = b.ie + NG; }
+  else { r.si = b.is - NG; r.ei = b.is - 1; }
+  if (nb.ox2 == 0) { r.sj = b.js; r.ej = b.je; }
+  else if (nb.ox2 > 0) { r.sj = b.je + 1; r.ej = b.je + NG; }
+  else { r.sj = b.js - NG; r.ej = b.js - 1; }
+  if (nb.ox3 == 0) { r.sk = b.ks; r.ek = b.ke; }
+  else if (nb.ox3 > 0) { r.sk = b.ke + 1; r.ek = b.ke + NG; }
+  else { r.sk = b.ks - NG; r.ek = b.ks - 1; }
+  return r;
+}
+// SetBoundaryFromCoarser (bvals_cc.cpp:434-472), in coarse indices
+Box cc_coarser_recv(const SmrMesh &m, const SmrBlock &s, const Nbr &nb) {
+  const int cng = m.cng;
+  Box r;
+  if (nb.ox1 == 0) {
+    r.si = m.cis; r.ei = m.cie;
+    if ((s.lx[0] & 1) == 0) r.ei += cng; else r.si -= cng;
+  } else if (nb.ox1 > 0) { r.si = m.cie + 1; r.ei = m.cie + cng; }
+  else { r.si = m.cis - cng; r.ei = m.cis - 1; }
+  if (nb.ox2 == 0) {
+    r.sj = m.cjs; r.ej = m.cje;
+    if ((s.lx[1] & 1) == 0) r.ej += cng; else r.sj -= cng;
+  } else if (nb.ox2 > 0) { r.sj = m.cje + 1; r.ej = m.cje + cng; }
+  else { r.sj = m.cjs - cng; r.ej = m.cjs - 1; }
+  if (nb.ox3 == 0) {
+    r.sk = m.cks; r.ek = m.cke;
+    if ((s.lx[2] & 1) == 0) r.ek += cng; else r.sk -= cng;
+  } else if (nb.ox3 > 0) { r.sk = m.cke + 1; r.ek = m.cke + cng; }
+  else { r.sk = m.cks - cng; r.ek = m.cks - 1; }
+  return r;
+}
+// SetBoundaryFromFiner (bvals_cc.cpp:495-540)
+Box cc_finer_recv(const Block &b, const Nbr &nb) {
+  const int NG = b.ng;
+  Box r;
+  if (nb.ox1 == 0) {
+    r.si = b.is; r.ei = b.ie;
+    if (nb.fi1 == 1) r.si += b.nx1 / 2; else r.ei -= b.nx1 / 2;
+  } else if (nb.ox1 > 0) { r.si = b.ie + 1; r.ei = b.ie + NG; }
+  else { r.si = b.is - NG; r.ei = b.is - 1; }
+  if (nb.ox2 == 0) {
+    r.sj = b.js; r.ej = b.je;
+    if (fi_x2(nb) == 1) r.sj += b.nx2 / 2; else r.ej -= b.nx2 / 2;
+  } else if (nb.ox2 > 0) { r.sj = b.je + 1; r.ej = b.je + NG; }
+  else { r.sj = b.js - NG; r.ej = b.js - 1; }
+  if (nb.ox3 == 0) {
+    r.sk = b.ks; r.ek = b.ke;
+    if (fi_x3(nb) == 1) r.sk += b.nx3 / 2; else r.ek -= b.nx3 / 2;
+  } else if (nb.ox3 > 0) { r.sk = b.ke + 1; r.ek = b.ke + NG; }
+  else { r.sk = b.ks - NG; r.ek = b.ks - 1; }
+  return r;
+}
+
+void send_cc(SmrMesh &m, int g, Buffers &bufs) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  for (const Nbr &nb : s.nb) {
+    std::vector<Real> buf;
+    if (nb.level == s.level) {
+      pack(b.u, 0, NHYDRO - 1, cc_same_send(b, nb), buf);
+    } else if (nb.level < s.level) {
+      const Box r = cc_coarser_send(m, b, nb);
+      restrict_cc(s, b, b.u, s.ccons, NHYDRO, r);
+      pack(s.ccons, 0, NHYDRO - 1, r, buf);
+    } else {
+      pack(b.u, 0, NHYDRO - 1, cc_finer_send(m, b, nb), buf);
+    }
+    bufs[{nb.gid, g, -nb.ox1, -nb.ox2, -nb.ox3}] = buf;
+  }
+}
+
+const std::vector<Real> &recv_buf(SmrMesh &m, Buffers &bufs, int g, const Nbr &nb) {
+  auto it = bufs.find({g, nb.gid, nb.ox1, nb.ox2, nb.ox3});
+  if (it == bufs.end()) throw std::runtime_error("no matching send for a neighbour");
+  (void)m;
+  return it->second;
+}
+
+void set_cc(SmrMesh &m, int g, Buffers &bufs) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  for (const Nbr &nb : s.nb) {
+    const std::vector<Real> &buf = recv_buf(m, bufs, g, nb);
+    std::size_t p = 0;
+    if (nb.level == s.level) unpack(buf, p, b.u, 0, NHYDRO - 1, cc_same_recv(b, nb));
+    else if (nb.level < s.level) unpack(buf, p, s.ccons, 0, NHYDRO - 1, cc_coarser_recv(m, s, nb));
+    else unpack(buf, p, b.u, 0, NHYDRO - 1, cc_finer_recv(b, nb));
+    if (p != buf.size()) throw std::runtime_error("cell-centred buffer size mismatch");
+  }
+}
+
+// ----------------------------------------------------------- face-centred buffers (b)
+// Ranges of the three components; each returns the three boxes in packing order.
+// LoadBoundaryBufferSameLevel (bvals_fc.cpp:344-399), multilevel
+void fc_same_send(const Block &b, const Nbr &nb, Box r[3]) {
+  const int NG = b.ng;
+  const bool nf = nb.type != FACE;
+  int si, ei, sj, ej, sk, ek;
+  // bx1
+  if (nb.ox1 == 0) { si = b.is; ei = b.ie + 1; }
+  else if (nb.ox1 > 0) { si = b.ie - NG + 1; ei = b.ie; }
+  else { si = b.is + 1; ei = b.is + NG; }
+  if (nb.ox2 == 0) { sj = b.js; ej = b.je; }
+  else if (nb.ox2 > 0) { sj = b.je - NG + 1; ej = b.je; }
+  else { sj = b.js; ej = b.js + NG - 1; }
+  if (nb.ox3 == 0) { sk = b.ks; ek = b.ke; }
+  else if (nb.ox3 > 0) { sk = b.ke - NG + 1; ek = b.ke; }
+  else { sk = b.ks; ek = b.ks + NG - 1; }
+  if (nf) { if (nb.ox1 > 0) ei++; else if (nb.ox1 < 0) si--; }
+  r[0] = {si, ei, sj, ej, sk, ek};
+  // bx2
+  if (nb.ox1 == 0) { si = b.is; ei = b.ie; }
+  else if (nb.ox1 > 0) { si = b.ie - NG + 1; ei = b.ie; }
+  else { si = b.is; ei = b.is + NG - 1; }
+  if (nb.ox2 == 0) { sj = b.js; ej = b.je + 1; }
+  else if (nb.ox2 > 0) { sj = b.je - NG + 1; ej = b.je; }
+  else { sj = b.js + 1; ej = b.js + NG; }
+  if (nf) { if (nb.ox2 > 0) ej++; else if (nb.ox2 < 0) sj--; }
+  r[1] = {si, ei, sj, ej, sk, ek};
+  // bx3
+  if (nb.ox2 == 0) { sj = b.js; ej = b.je; }
+  else if (nb.ox2 > 0) { sj = b.je - NG + 1; ej = b.je; }
+  else { sj = b.js; ej = b.js + NG - 1; }
+  if (nb.ox3 == 0) { sk = b.ks; ek = b.ke + 1; }
+  else if (nb.ox3 > 0) { sk = b.ke - NG + 1; ek = b.ke; }
+  else { sk = b.ks + 1; ek = b.ks + NG; }
+  if (nf) { if (nb.ox3 > 0) ek++; else if (nb.ox3 < 0) sk--; }
+  r[2] = {si, ei, sj, ej, sk, ek};
+}
+// LoadBoundaryBufferToCoarser (bvals_fc.cpp:403-472), coarse indices, cng = NGHOST
+void fc_coarser_send(const SmrMesh &m, const Block &b, const Nbr &nb, Box r[3]) {
+  const int cng = b.ng;
+  const bool nf = nb.type != FACE;
+  int si, ei, sj, ej, sk, ek;
+  if (nb.ox1 == 0) { si = m.cis; ei = m.cie + 1; }
+  else if (nb.ox1 > 0) { si = m.cie - cng + 1; ei = m.cie; }
+  else { si = m.cis + 1; ei = m.cis + cng; }
+  if (nb.ox2 == 0) { sj = m.cjs; ej = m.cje; }
+  else if (nb.ox2 > 0) { sj = m.cje - cng + 1; ej = m.cje; }
+  else { sj = m.cjs; ej = m.cjs + cng - 1; }
+  if (nb.ox3 == 0) { sk = m.cks; ek = m.cke; }
+  else if (nb.ox3 > 0) { sk = m.cke - cng + 1; ek = m.cke; }
+  else { sk = m.cks; ek = m.cks + cng - 1; }
+  if (nf) { if (nb.ox1 > 0) ei++; else if (nb.ox1 < 0) si--; }
+  r[0] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox1 == 0) { si = m.cis; ei = m.cie; }
+  else if (nb.ox1 > 0) { si = m.cie - cng + 1; ei = m.cie; }
+  else { si = m.cis; ei = m.cis + cng - 1; }
+  if (nb.ox2 == 0) { sj = m.cjs; ej = m.cje + 1; }
+  else if (nb.ox2 > 0) { sj = m.cje - cng + 1; ej = m.cje; }
+  else { sj = m.cjs + 1; ej = m.cjs + cng; }
+  if (nf) { if (nb.ox2 > 0) ej++; else if (nb.ox2 < 0) sj--; }
+  r[1] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox2 == 0) { sj = m.cjs; ej = m.cje; }
+  else if (nb.ox2 > 0) { sj = m.cje - cng + 1; ej = m.cje; }
+  else { sj = m.cjs; ej = m.cjs + cng - 1; }
+  if (nb.ox3 == 0) { sk = m.cks; ek = m.cke + 1; }
+  else if (nb.ox3 > 0) { sk = m.cke - cng + 1; ek = m.cke; }
+  else { sk = m.cks + 1; ek = m.cks + cng; }
+  if (nf) { if (nb.ox3 > 0) ek++; else if (nb.ox3 < 0) sk--; }
+  r[2] = {si, ei, sj, ej, sk, ek};
+}
+// LoadBoundaryBufferToFiner (bvals_fc.cpp:476-575)
+void fc_finer_send(const SmrMesh &m, const Block &b, const Nbr &nb, Box r[3]) {
+  const int cn = m.cng - 1, cg = m.cng;
+  const int h1 = b.nx1 / 2 - cg, h2 = b.nx2 / 2 - cg, h3 = b.nx3 / 2 - cg;
+  int si, ei, sj, ej, sk, ek;
+  // bx1
+  if (nb.ox1 == 0) {
+    if (nb.fi1 == 1) { si = b.is + h1; ei = b.ie + 1; }
+    else { si = b.is; ei = b.ie + 1 - h1; }
+  } else if (nb.ox1 > 0) { si = b.ie + 1 - cg; ei = b.ie + 1; }
+  else { si = b.is; ei = b.is + cg; }
+  if (nb.ox2 == 0) {
+    sj = b.js; ej = b.je;
+    if (fi_x2(nb) == 1) sj += h2; else ej -= h2;
+  } else if (nb.ox2 > 0) { sj = b.je - cn; ej = b.je; }
+  else { sj = b.js; ej = b.js + cn; }
+  if (nb.ox3 == 0) {
+    sk = b.ks; ek = b.ke;
+    if (fi_x3(nb) == 1) sk += h3; else ek -= h3;
+  } else if (nb.ox3 > 0) { sk = b.ke - cn; ek = b.ke; }
+  else { sk = b.ks; ek = b.ks + cn; }
+  r[0] = {si, ei, sj, ej, sk, ek};
+  // bx2
+  if (nb.ox1 == 0) {
+    if (nb.fi1 == 1) { si = b.is + h1; ei = b.ie; }
+    else { si = b.is; ei = b.ie - h1; }
+  } else if (nb.ox1 > 0) { si = b.ie - cn; ei = b.ie; }
+  else { si = b.is; ei = b.is + cn; }
+  if (nb.ox2 == 0) {
+    sj = b.js; ej = b.je + 1;
+    if (fi_x2(nb) == 1) sj += h2; else ej -= h2;
+  } else if (nb.ox2 > 0) { sj = b.je + 1 - cg; ej = b.je + 1; }
+  else { sj = b.js; ej = b.js + cg; }
+  r[1] = {si, ei, sj, ej, sk, ek};
+  // bx3
+  if (nb.ox2 == 0) {
+    sj = b.js; ej = b.je;
+    if (fi_x2(nb) == 1) sj += h2; else ej -= h2;
+  } else if (nb.ox2 > 0) { sj = b.je - cn; ej = b.je; }
+  else { sj = b.js; ej = b.js + cn; }
+  if (nb.ox3 == 0) {
+    sk = b.ks; ek = b.ke + 1;
+    if (fi_x3(nb) == 1) sk += h3; else ek -= h3;
+  } else if (nb.ox3 > 0) { sk = b.ke + 1 - cg; ek = b.ke + 1; }
+  else { sk = b.ks; ek = b.ks + cg; }
+  r[2] = {si, ei, sj, ej, sk, ek};
+}
+// SetBoundarySameLevel (bvals_fc.cpp:583-686), multilevel
+void fc_same_recv(const Block &b, const Nbr &nb, Box r[3]) {
+  const int NG = b.ng;
+  const bool nf = nb.type != FACE;
+  int si, ei, sj, ej, sk, ek;
+  if (nb.ox1 == 0) { si = b.is; ei = b.ie + 1; }
+  else if (nb.ox1 > 0) { si = b.ie + 2; ei = b.ie + NG + 1; }
+  else { si = b.is - NG; ei = b.is - 1; }
+  if (nb.ox2 == 0) { sj = b.js; ej = b.je; }
+  else if (nb.ox2 > 0) { sj = b.je + 1; ej = b.je + NG; }
+  else { sj = b.js - NG; ej = b.js - 1; }
+  if (nb.ox3 == 0) { sk = b.ks; ek = b.ke; }
+  else if (nb.ox3 > 0) { sk = b.ke + 1; ek = b.ke + NG; }
+  else { sk = b.ks - NG; ek = b.ks - 1; }
+  if (nf) { if (nb.ox1 > 0) si--; else if (nb.ox1 < 0) ei++; }
+  r[0] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox1 == 0) { si = b.is; ei = b.ie; }
+  else if (nb.ox1 > 0) { si = b.ie + 1; ei = b.ie + NG; }
+  else { si = b.is - NG; ei = b.is - 1; }
+  if (nb.ox2 == 0) { sj = b.js; ej = b.je + 1; }
+  else if (nb.ox2 > 0) { sj = b.je + 2; ej = b.je + NG + 1; }
+  else { sj = b.js - NG; ej = b.js - 1; }
+  if (nf) { if (nb.ox2 > 0) sj--; else if (nb.ox2 < 0) ej++; }
+  r[1] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox2 == 0) { sj = b.js; ej = b.je; }
+  else if (nb.ox2 > 0) { sj = b.je + 1; ej = b.je + NG; }
+  else { sj = b.js - NG; ej = b.js - 1; }
+  if (nb.ox3 == 0) { sk = b.ks; ek = b.ke + 1; }
+  else if (nb.ox3 > 0) { sk = b.ke + 2; ek = b.ke + NG + 1; }
+  else { sk = b.ks - NG; ek = b.ks - 1; }
+  if (nf) { if (nb.ox3 > 0) sk--; else if (nb.ox3 < 0) ek++; }
+  r[2] = {si, ei, sj, ej, sk, ek};
+}
+// SetBoundaryFromCoarser (bvals_fc.cpp:690-807), coarse indices
+void fc_coarser_recv(const SmrMesh &m, const SmrBlock &s, const Nbr &nb, Box r[3]) {
+  const int cng = m.cng;
+  const bool e1 = (s.lx[0] & 1) == 0, e2 = (s.lx[1] & 1) == 0, e3 = (s.lx[2] & 1) == 0;
+  int si, ei, sj, ej, sk, ek;
+  if (nb.ox1 == 0) {
+    si = m.cis; ei = m.cie + 1;
+    if (e1) ei += cng; else si -= cng;
+  } else if (nb.ox1 > 0) { si = m.cie + 1; ei = m.cie + 1 + cng; }
+  else { si = m.cis - cng; ei = m.cis; }
+  if (nb.ox2 == 0) {
+    sj = m.cjs; ej = m.cje;
+    if (e2) ej += cng; else sj -= cng;
+  } else if (nb.ox2 > 0) { sj = m.cje + 1; ej = m.cje + cng; }
+  else { sj = m.cjs - cng; ej = m.cjs - 1; }
+  if (nb.ox3 == 0) {
+    sk = m.cks; ek = m.cke;
+    if (e3) ek += cng; else sk -= cng;
+  } else if (nb.ox3 > 0) { sk = m.cke + 1; ek = m.cke + cng; }
+  else { sk = m.cks - cng; ek = m.cks - 1; }
+  r[0] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox1 == 0) {
+    si = m.cis; ei = m.cie;
+    if (e1) ei += cng; else si -= cng;
+  } else if (nb.ox1 > 0) { si = m.cie + 1; ei = m.cie + cng; }
+  else { si = m.cis - cng; ei = m.cis - 1; }
+  if (nb.ox2 == 0) {
+    sj = m.cjs; ej = m.cje + 1;
+    if (e2) ej += cng; else sj -= cng;
+  } else if (nb.ox2 > 0) { sj = m.cje + 1; ej = m.cje + 1 + cng; }
+  else { sj = m.cjs - cng; ej = m.cjs; }
+  r[1] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox2 == 0) {
+    sj = m.cjs; ej = m.cje;
+    if (e2) ej += cng; else sj -= cng;
+  } else if (nb.ox2 > 0) { sj = m.cje + 1; ej = m.cje + cng; }
+  else { sj = m.cjs - cng; ej = m.cjs - 1; }
+  if (nb.ox3 == 0) {
+    sk = m.cks; ek = m.cke + 1;
+    if (e3) ek += cng; else sk -= cng;
+  } else if (nb.ox3 > 0) { sk = m.cke + 1; ek = m.cke + 1 + cng; }
+  else { sk = m.cks - cng; ek = m.cks; }
+  r[2] = {si, ei, sj, ej, sk, ek};
+}
+// SetBoundaryFromFiner (bvals_fc.cpp:813-975)
+void fc_finer_recv(const Block &b, const Nbr &nb, Box r[3]) {
+  const int NG = b.ng;
+  const bool nf = nb.type != FACE;
+  const int h1 = b.nx1 / 2, h2 = b.nx2 / 2, h3 = b.nx3 / 2;
+  int si, ei, sj, ej, sk, ek;
+  if (nb.ox1 == 0) {
+    si = b.is; ei = b.ie + 1;
+    if (nb.fi1 == 1) si += h1; else ei -= h1;
+  } else if (nb.ox1 > 0) { si = b.ie + 2; ei = b.ie + NG + 1; }
+  else { si = b.is - NG; ei = b.is - 1; }
+  if (nf) { if (nb.ox1 > 0) si--; else if (nb.ox1 < 0) ei++; }
+  if (nb.ox2 == 0) {
+    sj = b.js; ej = b.je;
+    if (fi_x2(nb) == 1) sj += h2; else ej -= h2;
+  } else if (nb.ox2 > 0) { sj = b.je + 1; ej = b.je + NG; }
+  else { sj = b.js - NG; ej = b.js - 1; }
+  if (nb.ox3 == 0) {
+    sk = b.ks; ek = b.ke;
+    if (fi_x3(nb) == 1) sk += h3; else ek -= h3;
+  } else if (nb.ox3 > 0) { sk = b.ke + 1; ek = b.ke + NG; }
+  else { sk = b.ks - NG; ek = b.ks - 1; }
+  r[0] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox1 == 0) {
+    si = b.is; ei = b.ie;
+    if (nb.fi1 == 1) si += h1; else ei -= h1;
+  } else if (nb.ox1 > 0) { si = b.ie + 1; ei = b.ie + NG; }
+  else { si = b.is - NG; ei = b.is - 1; }
+  if (nb.ox2 == 0) {
+    sj = b.js; ej = b.je + 1;
+    if (fi_x2(nb) == 1) sj += h2; else ej -= h2;
+  } else if (nb.ox2 > 0) { sj = b.je + 2; ej = b.je + NG + 1; }
+  else { sj = b.js - NG; ej = b.js - 1; }
+  if (nf) { if (nb.ox2 > 0) sj--; else if (nb.ox2 < 0) ej++; }
+  r[1] = {si, ei, sj, ej, sk, ek};
+  if (nb.ox2 == 0) {
+    sj = b.js; ej = b.je;
+    if (fi_x2(nb) == 1) sj += h2; else ej -= h2;
+  } else if (nb.ox2 > 0) { sj = b.je + 1; ej = b.je + NG; }
+  else { sj = b.js - NG; ej = b.js - 1; }
+  if (nb.ox3 == 0) {
+    sk = b.ks; ek = b.ke + 1;
+    if (fi_x3(nb) == 1) sk += h3; else ek -= h3;
+  } else if (nb.ox3 > 0) { sk = b.ke + 2; ek = b.ke + NG + 1; }
+  else { sk = b.ks - NG; ek = b.ks - 1; }
+  if (nf) { if (nb.ox3 > 0) sk--; else if (nb.ox3 < 0) ek++; }
+  r[2] = {si, ei, sj, ej, sk, ek};
+}
+
+void send_fc(SmrMesh &m, int g, Buffers &bufs) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  Arr *fine[3] = {&b.b1f, &b.b2f, &b.b3f};
+  Arr *coarse[3] = {&s.cb1, &s.cb2, &s.cb3};
+  for (const Nbr &nb : s.nb) {
+    std::vector<Real> buf;
+    Box r[3];
+    if (nb.level == s.level) {
+      fc_same_send(b, nb, r);
+      for (int c = 0; c < 3; ++c) pack(*fine[c], 0, 0, r[c], buf);
+    } else if (nb.level < s.level) {
+      fc_coarser_send(m, b, nb, r);
+      for (int c = 0; c < 3; ++c) {
+        restrict_fc(s, b, c, *fine[c], *coarse[c], r[c]);
+        pack(*coarse[c], 0, 0, r[c], buf);
+      }
+    } else {
+      fc_finer_send(m, b, nb, r);
+      for (int c = 0; c < 3; ++c) pack(*fine[c], 0, 0, r[c], buf);
+    }
+    bufs[{nb.gid, g, -nb.ox1, -nb.ox2, -nb.ox3}] = buf;
+  }
+}
+
+void set_fc(SmrMesh &m, int g, Buffers &bufs) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  Arr *fine[3] = {&b.b1f, &b.b2f, &b.b3f};
+  Arr *coarse[3] = {&s.cb1, &s.cb2, &s.cb3};
+  for (const Nbr &nb : s.nb) {
+    const std::vector<Real> &buf = recv_buf(m, bufs, g, nb);
+    std::size_t p = 0;
+    Box r[3];
+    if (nb.level == s.level) {
+      fc_same_recv(b, nb, r);
+      for (int c = 0; c < 3; ++c) unpack(buf, p, *fine[c], 0, 0, r[c]);
+    } else if (nb.level < s.level) {
+      fc_coarser_recv(m, s, nb, r);
+      for (int c = 0; c < 3; ++c) unpack(buf, p, *coarse[c], 0, 0, r[c]);
+    } else {
+      fc_finer_recv(b, nb, r);
+      for (int c = 0; c < 3; ++c) unpack(buf, p, *fine[c], 0, 0, r[c]);
+    }
+    if (p != buf.size()) throw std::runtime_error("face-centred buffer size mismatch");
+  }
+}
+
+// ------------------------------------------------------------------------ prolongation
+// RestrictGhostCellsOnSameLevel (bvals_refine.cpp:255-352)
+void restrict_same_level(SmrMesh &m, int g, const Nbr &nb, int nk, int nj, int ni) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  Box r;
+  if (ni == 0) {
+    r.si = m.cis; r.ei = m.cie;
+    if (nb.ox1 == 1) r.si = m.cie; else if (nb.ox1 == -1) r.ei = m.cis;
+  } else if (ni == 1) { r.si = r.ei = m.cie + 1; }
+  else { r.si = r.ei = m.cis - 1; }
+  if (nj == 0) {
+    r.sj = m.cjs; r.ej = m.cje;
+    if (nb.ox2 == 1) r.sj = m.cje; else if (nb.ox2 == -1) r.ej = m.cjs;
+  } else if (nj == 1) { r.sj = r.ej = m.cje + 1; }
+  else { r.sj = r.ej = m.cjs - 1; }
+  if (nk == 0) {
+    r.sk = m.cks; r.ek = m.cke;
+    if (nb.ox3 == 1) r.sk = m.cke; else if (nb.ox3 == -1) r.ek = m.cks;
+  } else if (nk == 1) { r.sk = r.ek = m.cke + 1; }
+  else { r.sk = r.ek = m.cks - 1; }
+  restrict_cc(s, b, b.u, s.ccons, NHYDRO, r);
+  const int lv = s.level;
+  int rs = r.si, re = r.ei + 1;
+  if (rs == m.cis && s.nblevel[nk + 1][nj + 1][ni] < lv) rs++;
+  if (re == m.cie + 1 && s.nblevel[nk + 1][nj + 1][ni + 2] < lv) re--;
+  restrict_fc(s, b, 0, b.b1f, s.cb1, {rs, re, r.sj, r.ej, r.sk, r.ek});
+  rs = r.sj; re = r.ej + 1;
+  if (rs == m.cjs && s.nblevel[nk + 1][nj][ni + 1] < lv) rs++;
+  if (re == m.cje + 1 && s.nblevel[nk + 1][nj + 2][ni + 1] < lv) re--;
+  restrict_fc(s, b, 1, b.b2f, s.cb2, {r.si, r.ei, rs, re, r.sk, r.ek});
+  rs = r.sk; re = r.ek + 1;
+  if (rs == m.cks && s.nblevel[nk][nj + 1][ni + 1] < lv) rs++;
+  if (re == m.cke + 1 && s.nblevel[nk + 2][nj + 1][ni + 1] < lv) re--;
+  restrict_fc(s, b, 2, b.b3f, s.cb3, {r.si, r.ei, r.sj, r.ej, rs, re});
+}
+
+// ConservedToPrimitive on the coarse buffer (adiabatic_mhd.cpp:41-90 with coarse_b and
+// the coarse coordinates), as ApplyPhysicalBoundariesOnCoarseLevel calls it.
+void coarse_cons2prim(SmrMesh &m, int g, const Box &r) {
+  SmrBlock &s = m.sb[g];
+  const Real gm1 = m.c.gamma - 1.0;
+  for (int k = r.sk; k <= r.ek; ++k)
+    for (int j = r.sj; j <= r.ej; ++j)
+      for (int i = r.si; i <= r.ei; ++i) {
+        s.cbcc(IB1, k, j, i) = 0.5 * s.cb1(k, j, i) + 0.5 * s.cb1(k, j, i + 1);
+        s.cbcc(IB2, k, j, i) = 0.5 * s.cb2(k, j, i) + 0.5 * s.cb2(k, j + 1, i);
+        s.cbcc(IB3, k, j, i) = 0.5 * s.cb3(k, j, i) + 0.5 * s.cb3(k + 1, j, i);
+      }
+  for (int k = r.sk; k <= r.ek; ++k)
+    for (int j = r.sj; j <= r.ej; ++j)
+      for (int i = r.si; i <= r.ei; ++i) {
+        Real &u_d = s.ccons(IDN, k, j, i), &u_e = s.ccons(IEN, k, j, i);
+        const Real u_m1 = s.ccons(IM1, k, j, i), u_m2 = s.ccons(IM2, k, j, i);
+        const Real u_m3 = s.ccons(IM3, k, j, i);
+        u_d = (u_d > m.dfloor) ? u_d : m.dfloor;
+        s.cprim(IDN, k, j, i) = u_d;
+        const Real di = 1.0 / u_d;
+        s.cprim(IVX, k, j, i) = u_m1 * di;
+        s.cprim(IVY, k, j, i) = u_m2 * di;
+        s.cprim(IVZ, k, j, i) = u_m3 * di;
+        const Real pb = 0.5 * (SQR(s.cbcc(IB1, k, j, i)) + SQR(s.cbcc(IB2, k, j, i)) +
+                               SQR(s.cbcc(IB3, k, j, i)));
+        const Real e_k = 0.5 * di * (SQR(u_m1) + SQR(u_m2) + SQR(u_m3));
+        Real w_p = gm1 * (u_e - e_k - pb);
+        u_e = (w_p > m.pfloor) ? u_e : ((m.pfloor / gm1) + e_k + pb);
+        s.cprim(IPR, k, j, i) = (w_p > m.pfloor) ? w_p : m.pfloor;
+      }
+}
+
+// BoundaryValues::ProlongateBoundaries (bvals_refine.cpp:98-246) and ProlongateGhostCells
+// (:540-673) for a periodic mesh (no physical boundary functions on the coarse level).
+void prolongate_boundaries(SmrMesh &m, int g) {
+  Block &b = m.blocks[g];
+  SmrBlock &s = m.sb[g];
+  int sh[4];
+  shape_of(b, sh);
+  for (const Nbr &nb : s.nb) {
+    if (nb.level >= s.level) continue;
+    const int nis = std::max(nb.ox1 - 1, -1), nie = std::min(nb.ox1 + 1, 1);
+    const int njs = std::max(nb.ox2 - 1, -1), nje = std::min(nb.ox2 + 1, 1);
+    const int nks = std::max(nb.ox3 - 1, -1), nke = std::min(nb.ox3 + 1, 1);
+    for (int nk = nks; nk <= nke; ++nk)
+      for (int nj = njs; nj <= nje; ++nj)
+        for (int ni = nis; ni <= nie; ++ni) {
+          const int ntype = std::abs(ni) + std::abs(nj) + std::abs(nk);
+          if (ntype == 0 || s.nblevel[nk + 1][nj + 1][ni + 1] != s.level) continue;
+          restrict_same_level(m, g, nb, nk, nj, ni);
+        }
+    const int cn = m.cng - 1;
+    int si, ei, sj, ej, sk, ek;
+    if (nb.ox1 == 0) {
+      si = m.cis; ei = m.cie;
+      if ((s.lx[0] & 1) == 0) ei += cn; else si -= cn;
+    } else if (nb.ox1 > 0) { si = m.cie + 1; ei = m.cie + cn; }
+    else { si = m.cis - cn; ei = m.cis - 1; }
+    if (nb.ox2 == 0) {
+      sj = m.cjs; ej = m.cje;
+      if ((s.lx[1] & 1) == 0) ej += cn; else sj -= cn;
+    } else if (nb.ox2 > 0) { sj = m.cje + 1; ej = m.cje + cn; }
+    else { sj = m.cjs - cn; ej = m.cjs - 1; }
+    if (nb.ox3 == 0) {
+      sk = m.cks; ek = m.cke;
+      if ((s.lx[2] & 1) == 0) ek += cn; else sk -= cn;
+    } else if (nb.ox3 > 0) { sk = m.cke + 1; ek = m.cke + cn; }
+    else { sk = m.cks - cn; ek = m.cks - 1; }
+    // ApplyPhysicalBoundariesOnCoarseLevel: on a periodic mesh every side has a
+    // neighbour, so the recovery covers the prolongation box grown by one coarse cell
+    coarse_cons2prim(m, g, {si - 1, ei + 1, sj - 1, ej + 1, sk - 1, ek + 1});
+    // ProlongateGhostCells
+    const int rr[6] = {si, ei, sj, ej, sk, ek};
+    ora_prolongate_cc(sh, NHYDRO, s.fco.data(), s.cco.data(), s.cprim.v.data(), b.w.v.data(), rr);
+    const int lv = s.level;
+    int il = si, iu = ei + 1, jl = sj, ju = ej + 1, kl = sk, ku = ek + 1;
+    const int o1 = nb.ox1, o2 = nb.ox2, o3 = nb.ox3;
+    if (o1 >= 0 && s.nblevel[o3 + 1][o2 + 1][o1] >= lv) il++;
+    if (o1 <= 0 && s.nblevel[o3 + 1][o2 + 1][o1 + 2] >= lv) iu--;
+    if (o2 >= 0 && s.nblevel[o3 + 1][o2][o1 + 1] >= lv) jl++;
+    if (o2 <= 0 && s.nblevel[o3 + 1][o2 + 2][o1 + 1] >= lv) ju--;
+    if (o3 >= 0 && s.nblevel[o3][o2 + 1][o1 + 1] >= lv) kl++;
+    if (o3 <= 0 && s.nblevel[o3 + 2][o2 + 1][o1 + 1] >= lv) ku--;
+    const int r1[6] = {il, iu, sj, ej, sk, ek};
+    const int r2[6] = {si, ei, jl, ju, sk, ek};
+    const int r3[6] = {si, ei, sj, ej, kl, ku};
+    ora_prolongate_fc_shared(sh, 0, s.fco.data(), s.cco.data(), s.cb1.v.data(), b.b1f.v.data(), r1);
+    ora_prolongate_fc_shared(sh, 1, s.fco.data(), s.cco.data(), s.cb2.v.data(), b.b2f.v.data(), r2);
+    ora_prolongate_fc_shared(sh, 2, s.fco.data(), s.cco.data(), s.cb3.v.data(), b.b3f.v.data(), r3);
+    ora_prolongate_fc_internal(sh, s.fco.data(), b.b1f.v.data(), b.b2f.v.data(), b.b3f.v.data(), rr);
+    const int fsi = (si - m.cis) * 2 + b.is, fei = (ei - m.cis) * 2 + b.is + 1;
+    const int fsj = (sj - m.cjs) * 2 + b.js, fej = (ej - m.cjs) * 2 + b.js + 1;
+    const int fsk = (sk - m.cks) * 2 + b.ks, fek = (ek - m.cks) * 2 + b.ks + 1;
+    CellCenteredField(b, fsi, fei, fsj, fej, fsk, fek);
+    Prim2Cons(m, b, fsi, fei, fsj, fej, fsk, fek);
+  }
+}
+
+// --------------------------------------------------------- hydro flux correction
+// LoadFluxBoundaryBufferToCoarser (flux_correction_cc.cpp:77-166), 3-D
+void flux_to_coarser(Block &b, const Nbr &nb, std::vector<Real> &buf) {
+  if (nb.fid == 0 || nb.fid == 1) {
+    const int i = b.is + (b.ie - b.is + 1) * nb.fid;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int k = b.ks; k <= b.ke; k += 2)
+        for (int j = b.js; j <= b.je; j += 2) {
+          const Real amm = b.dx2f[j] * b.dx3f[k], amp = b.dx2f[j + 1] * b.dx3f[k];
+          const Real apm = b.dx2f[j] * b.dx3f[k + 1], app = b.dx2f[j + 1] * b.dx3f[k + 1];
+          const Real tarea = amm + amp + apm + app;
+          Arr &f = b.flux[0];
+          buf.push_back((f(n, k, j, i) * amm + f(n, k, j + 1, i) * amp +
+                         f(n, k + 1, j, i) * apm + f(n, k + 1, j + 1, i) * app) / tarea);
+        }
+  } else if (nb.fid == 2 || nb.fid == 3) {
+    const int j = b.js + (b.je - b.js + 1) * (nb.fid & 1);
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int k = b.ks; k <= b.ke; k += 2)
+        for (int i = b.is; i <= b.ie; i += 2) {
+          const Real s00 = b.dx1f[i] * b.dx3f[k], s01 = b.dx1f[i + 1] * b.dx3f[k];
+          const Real s10 = b.dx1f[i] * b.dx3f[k + 1], s11 = b.dx1f[i + 1] * b.dx3f[k + 1];
+          const Real tarea = s00 + s01 + s10 + s11;
+          Arr &f = b.flux[1];
+          buf.push_back((f(n, k, j, i) * s00 + f(n, k, j, i + 1) * s01 +
+                         f(n, k + 1, j, i) * s10 + f(n, k + 1, j, i + 1) * s11) / tarea);
+        }
+  } else {
+    const int k = b.ks + (b.ke - b.ks + 1) * (nb.fid & 1);
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int j = b.js; j <= b.je; j += 2)
+        for (int i = b.is; i <= b.ie; i += 2) {
+          const Real s00 = b.dx1f[i] * b.dx2f[j], s01 = b.dx1f[i + 1] * b.dx2f[j];
+          const Real s10 = b.dx1f[i] * b.dx2f[j + 1], s11 = b.dx1f[i + 1] * b.dx2f[j + 1];
+          const Real tarea = s00 + s01 + s10 + s11;
+          Arr &f = b.flux[2];
+          buf.push_back((f(n, k, j, i) * s00 + f(n, k, j, i + 1) * s01 +
+                         f(n, k, j + 1, i) * s10 + f(n, k, j + 1, i + 1) * s11) / tarea);
+        }
+  }
+}
+
+// SetFluxBoundaryFromFiner (flux_correction_cc.cpp:237-286)
+void flux_from_finer(Block &b, const Nbr &nb, const std::vector<Real> &buf) {
+  std::size_t p = 0;
+  if (nb.fid == 0 || nb.fid == 1) {
+    const int il = b.is + (b.ie - b.is) * nb.fid + nb.fid;
+    int jl = b.js, ju = b.je, kl = b.ks, ku = b.ke;
+    if (nb.fi1 == 0) ju -= b.nx2 / 2; else jl += b.nx2 / 2;
+    if (nb.fi2 == 0) ku -= b.nx3 / 2; else kl += b.nx3 / 2;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int k = kl; k <= ku; ++k)
+        for (int j = jl; j <= ju; ++j) b.flux[0](n, k, j, il) = buf.at(p++);
+  } else if (nb.fid == 2 || nb.fid == 3) {
+    const int jl = b.js + (b.je - b.js) * (nb.fid & 1) + (nb.fid & 1);
+    int il = b.is, iu = b.ie, kl = b.ks, ku = b.ke;
+    if (nb.fi1 == 0) iu -= b.nx1 / 2; else il += b.nx1 / 2;
+    if (nb.fi2 == 0) ku -= b.nx3 / 2; else kl += b.nx3 / 2;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int k = kl; k <= ku; ++k)
+        for (int i = il; i <= iu; ++i) b.flux[1](n, k, jl, i) = buf.at(p++);
+  } else {
+    const int kl = b.ks + (b.ke - b.ks) * (nb.fid & 1) + (nb.fid & 1);
+    int il = b.is, iu = b.ie, jl = b.js, ju = b.je;
+    if (nb.fi1 == 0) iu -= b.nx1 / 2; else il += b.nx1 / 2;
+    if (nb.fi2 == 0) ju -= b.nx2 / 2; else jl += b.nx2 / 2;
+    for (int n = 0; n < NHYDRO; ++n)
+      for (int j = jl; j <= ju; ++j)
+        for (int i = il; i <= iu; ++i) b.flux[2](n, kl, j, i) = buf.at(p++);
+  }
+  if (p != buf.size()) throw std::runtime_error("flux correction buffer size mismatch");
+}
+
+void hydro_flux_correction(SmrMesh &m) {
+  const int nbk = static_cast<int>(m.blocks.size());
+  Buffers bufs;
+  for (int g = 0; g < nbk; ++g)
+    for (const Nbr &nb : m.sb[g].nb) {
+      if (nb.type != FACE) break;
+      if (nb.level < m.sb[g].level) {
+        std::vector<Real> buf;
+        flux_to_coarser(m.blocks[g], nb, buf);
+        bufs[{nb.gid, g, -nb.ox1, -nb.ox2, -nb.ox3}] = buf;
+      }
+    }
+  for (int g = 0; g < nbk; ++g)
+    for (const Nbr &nb : m.sb[g].nb) {
+      if (nb.type != FACE) break;
+      if (nb.level > m.sb[g].level) flux_from_finer(m.blocks[g], nb, recv_buf(m, bufs, g, nb));
+    }
+}
+
+// ---------------------------------------------------------------- EMF correction
+inline Real rest2(Real a, Real b, Real l1, Real l2) { return (a * l1 + b * l2) / (l1 + l2); }
+
+// LoadFluxBoundaryBufferSameLevel (flux_correction_fc.cpp:51-313), 3-D
+void emf_same_pack(Block &b, const Nbr &nb, std::vector<Real> &buf) {
+  if (nb.type == FACE) {
+    if (nb.fid < 2) {
+      const int i = (nb.fid == 0) ? b.is : b.ie + 1;
+      for (int k = b.ks; k <= b.ke + 1; ++k)
+        for (int j = b.js; j <= b.je; ++j) buf.push_back(b.e2(k, j, i));
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je + 1; ++j) buf.push_back(b.e3(k, j, i));
+    } else if (nb.fid < 4) {
+      const int j = (nb.fid == 2) ? b.js : b.je + 1;
+      for (int k = b.ks; k <= b.ke + 1; ++k)
+        for (int i = b.is; i <= b.ie; ++i) buf.push_back(b.e1(k, j, i));
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int i = b.is; i <= b.ie + 1; ++i) buf.push_back(b.e3(k, j, i));
+    } else {
+      const int k = (nb.fid == 4) ? b.ks : b.ke + 1;
+      for (int j = b.js; j <= b.je + 1; ++j)
+        for (int i = b.is; i <= b.ie; ++i) buf.push_back(b.e1(k, j, i));
+      for (int j = b.js; j <= b.je; ++j)
+        for (int i = b.is; i <= b.ie + 1; ++i) buf.push_back(b.e2(k, j, i));
+    }
+  } else if (nb.type == EDGE) {
+    const int lo = nb.eid & 1, hi = nb.eid & 2;
+    if (nb.eid < 4) {
+      const int i = lo ? b.ie + 1 : b.is, j = hi ? b.je + 1 : b.js;
+      for (int k = b.ks; k <= b.ke; ++k) buf.push_back(b.e3(k, j, i));
+    } else if (nb.eid < 8) {
+      const int i = lo ? b.ie + 1 : b.is, k = hi ? b.ke + 1 : b.ks;
+      for (int j = b.js; j <= b.je; ++j) buf.push_back(b.e2(k, j, i));
+    } else {
+      const int j = lo ? b.je + 1 : b.js, k = hi ? b.ke + 1 : b.ks;
+      for (int i = b.is; i <= b.ie; ++i) buf.push_back(b.e1(k, j, i));
+    }
+  }
+}
+
+// LoadFluxBoundaryBufferToCoarser (flux_correction_fc.cpp:315-573), 3-D Cartesian
+void emf_coarser_pack(Block &b, const Nbr &nb, std::vector<Real> &buf) {
+  if (nb.type == FACE) {
+    if (nb.fid < 2) {
+      const int i = (nb.fid == 0) ? b.is : b.ie + 1;
+      for (int k = b.ks; k <= b.ke + 1; k += 2)
+        for (int j = b.js; j <= b.je; j += 2)
+          buf.push_back(rest2(b.e2(k, j, i), b.e2(k, j + 1, i), b.dx2f[j], b.dx2f[j + 1]));
+      for (int k = b.ks; k <= b.ke; k += 2)
+        for (int j = b.js; j <= b.je + 1; j += 2)
+          buf.push_back(rest2(b.e3(k, j, i), b.e3(k + 1, j, i), b.dx3f[k], b.dx3f[k + 1]));
+    } else if (nb.fid < 4) {
+      const int j = (nb.fid == 2) ? b.js : b.je + 1;
+      for (int k = b.ks; k <= b.ke + 1; k += 2)
+        for (int i = b.is; i <= b.ie; i += 2)
+          buf.push_back(rest2(b.e1(k, j, i), b.e1(k, j, i + 1), b.dx1f[i], b.dx1f[i + 1]));
+      for (int k = b.ks; k <= b.ke; k += 2)
+        for (int i = b.is; i <= b.ie + 1; i += 2)
+          buf.push_back(rest2(b.e3(k, j, i), b.e3(k + 1, j, i), b.dx3f[k], b.dx3f[k + 1]));
+    } else {
+      const int k = (nb.fid == 4) ? b.ks : b.ke + 1;
+      for (int j = b.js; j <= b.je + 1; j += 2)
+        for (int i = b.is; i <= b.ie; i += 2)
+          buf.push_back(rest2(b.e1(k, j, i), b.e1(k, j, i + 1), b.dx1f[i], b.dx1f[i + 1]));
+      for (int j = b.js; j <= b.je; j += 2)
+        for (int i = b.is; i <= b.ie + 1; i += 2)
+          buf.push_back(rest2(b.e2(k, j, i), b.e2(k, j + 1, i), b.dx2f[j], b.dx2f[j + 1]));
+    }
+  } else if (nb.type == EDGE) {
+    const int lo = nb.eid & 1, hi = nb.eid & 2;
+    if (nb.eid < 4) {
+      const int i = lo ? b.ie + 1 : b.is, j = hi ? b.je + 1 : b.js;
+      for (int k = b.ks; k <= b.ke; k += 2)
+        buf.push_back(rest2(b.e3(k, j, i), b.e3(k + 1, j, i), b.dx3f[k], b.dx3f[k + 1]));
+    } else if (nb.eid < 8) {
+      const int i = lo ? b.ie + 1 : b.is, k = hi ? b.ke + 1 : b.ks;
+      for (int j = b.js; j <= b.je; j += 2)
+        buf.push_back(rest2(b.e2(k, j, i), b.e2(k, j + 1, i), b.dx2f[j], b.dx2f[j + 1]));
+    } else {
+      const int j = lo ? b.je + 1 : b.js, k = hi ? b.ke + 1 : b.ks;
+      for (int i = b.is; i <= b.ie; i += 2)
+        buf.push_back(rest2(b.e1(k, j, i), b.e1(k, j, i + 1), b.dx1f[i], b.dx1f[i + 1]));
+    }
+  }
+}
+
+// SetFluxBoundarySameLevel (flux_correction_fc.cpp:689-943), 3-D: add the neighbour's
+void emf_same_add(Block &b, const Nbr &nb, const std::vector<Real> &buf) {
+  std::size_t p = 0;
+  if (nb.type == FACE) {
+    if (nb.fid < 2) {
+      const int i = (nb.fid == 0) ? b.is : b.ie + 1;
+      for (int k = b.ks; k <= b.ke + 1; ++k)
+        for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) += buf.at(p++);
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je + 1; ++j) b.e3(k, j, i) += buf.at(p++);
+    } else if (nb.fid < 4) {
+      const int j = (nb.fid == 2) ? b.js : b.je + 1;
+      for (int k = b.ks; k <= b.ke + 1; ++k)
+        for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) += buf.at(p++);
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int i = b.is; i <= b.ie + 1; ++i) b.e3(k, j, i) += buf.at(p++);
+    } else {
+      const int k = (nb.fid == 4) ? b.ks : b.ke + 1;
+      for (int j = b.js; j <= b.je + 1; ++j)
+        for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) += buf.at(p++);
+      for (int j = b.js; j <= b.je; ++j)
+        for (int i = b.is; i <= b.ie + 1; ++i) b.e2(k, j, i) += buf.at(p++);
+    }
+  } else {
+    const int lo = nb.eid & 1, hi = nb.eid & 2;
+    if (nb.eid < 4) {
+      const int i = lo ? b.ie + 1 : b.is, j = hi ? b.je + 1 : b.js;
+      for (int k = b.ks; k <= b.ke; ++k) b.e3(k, j, i) += buf.at(p++);
+    } else if (nb.eid < 8) {
+      const int i = lo ? b.ie + 1 : b.is, k = hi ? b.ke + 1 : b.ks;
+      for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) += buf.at(p++);
+    } else {
+      const int j = lo ? b.je + 1 : b.js, k = hi ? b.ke + 1 : b.ks;
+      for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) += buf.at(p++);
+    }
+  }
+  if (p != buf.size()) throw std::runtime_error("EMF buffer size mismatch");
+}
+
+// SetFluxBoundaryFromFiner (flux_correction_fc.cpp:944-1191), 3-D
+void emf_finer_add(Block &b, const Nbr &nb, const std::vector<Real> &buf) {
+  std::size_t p = 0;
+  const int h1 = b.nx1 / 2, h2 = b.nx2 / 2, h3 = b.nx3 / 2;
+  if (nb.type == FACE) {
+    if (nb.fid < 2) {
+      const int i = (nb.fid == 0) ? b.is : b.ie + 1;
+      int jl = b.js, ju = b.je, kl = b.ks, ku = b.ke;
+      if (nb.fi1 == 0) ju = b.js + h2 - 1; else jl = b.js + h2;
+      if (nb.fi2 == 0) ku = b.ks + h3 - 1; else kl = b.ks + h3;
+      for (int k = kl; k <= ku + 1; ++k)
+        for (int j = jl; j <= ju; ++j) b.e2(k, j, i) += buf.at(p++);
+      for (int k = kl; k <= ku; ++k)
+        for (int j = jl; j <= ju + 1; ++j) b.e3(k, j, i) += buf.at(p++);
+    } else if (nb.fid < 4) {
+      const int j = (nb.fid == 2) ? b.js : b.je + 1;
+      int il = b.is, iu = b.ie, kl = b.ks, ku = b.ke;
+      if (nb.fi1 == 0) iu = b.is + h1 - 1; else il = b.is + h1;
+      if (nb.fi2 == 0) ku = b.ks + h3 - 1; else kl = b.ks + h3;
+      for (int k = kl; k <= ku + 1; ++k)
+        for (int i = il; i <= iu; ++i) b.e1(k, j, i) += buf.at(p++);
+      for (int k = kl; k <= ku; ++k)
+        for (int i = il; i <= iu + 1; ++i) b.e3(k, j, i) += buf.at(p++);
+    } else {
+      const int k = (nb.fid == 4) ? b.ks : b.ke + 1;
+      int il = b.is, iu = b.ie, jl = b.js, ju = b.je;
+      if (nb.fi1 == 0) iu = b.is + h1 - 1; else il = b.is + h1;
+      if (nb.fi2 == 0) ju = b.js + h2 - 1; else jl = b.js + h2;
+      for (int j = jl; j <= ju + 1; ++j)
+        for (int i = il; i <= iu; ++i) b.e1(k, j, i) += buf.at(p++);
+      for (int j = jl; j <= ju; ++j)
+        for (int i = il; i <= iu + 1; ++i) b.e2(k, j, i) += buf.at(p++);
+    }
+  } else {
+    const int lo = nb.eid & 1, hi = nb.eid & 2;
+    if (nb.eid < 4) {
+      const int i = lo ? b.ie + 1 : b.is, j = hi ? b.je + 1 : b.js;
+      int kl = b.ks, ku = b.ke;
+      if (nb.fi1 == 0) ku = b.ks + h3 - 1; else kl = b.ks + h3;
+      for (int k = kl; k <= ku; ++k) b.e3(k, j, i) += buf.at(p++);
+    } else if (nb.eid < 8) {
+      const int i = lo ? b.ie + 1 : b.is, k = hi ? b.ke + 1 : b.ks;
+      int jl = b.js, ju = b.je;
+      if (nb.fi1 == 0) ju = b.js + h2 - 1; else jl = b.js + h2;
+      for (int j = jl; j <= ju; ++j) b.e2(k, j, i) += buf.at(p++);
+    } else {
+      const int j = lo ? b.je + 1 : b.js, k = hi ? b.ke + 1 : b.ks;
+      int il = b.is, iu = b.ie;
+      if (nb.fi1 == 0) iu = b.is + h1 - 1; else il = b.is + h1;
+      for (int i = il; i <= iu; ++i) b.e1(k, j, i) += buf.at(p++);
+    }
+  }
+  if (p != buf.size()) throw std::runtime_error("EMF buffer size mismatch");
+}
+
+// ClearCoarseFluxBoundary (flux_correction_fc.cpp:1214-1352), 3-D
+void emf_clear_coarse(Block &b, SmrBlock &s) {
+  for (int n = 0; n < 6; ++n) {
+    const int d = n / 2, side = n & 1;
+    int o[3] = {0, 0, 0};
+    o[d] = side ? 1 : -1;
+    if (s.nblevel[o[2] + 1][o[1] + 1][o[0] + 1] <= s.level) continue;
+    if (d == 0) {
+      const int i = side ? b.ie + 1 : b.is;
+      for (int k = b.ks + 1; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) = 0.0;
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js + 1; j <= b.je; ++j) b.e3(k, j, i) = 0.0;
+    } else if (d == 1) {
+      const int j = side ? b.je + 1 : b.js;
+      for (int k = b.ks + 1; k <= b.ke; ++k)
+        for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) = 0.0;
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int i = b.is + 1; i <= b.ie; ++i) b.e3(k, j, i) = 0.0;
+    } else {
+      const int k = side ? b.ke + 1 : b.ks;
+      for (int j = b.js + 1; j <= b.je; ++j)
+        for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) = 0.0;
+      for (int j = b.js; j <= b.je; ++j)
+        for (int i = b.is + 1; i <= b.ie; ++i) b.e2(k, j, i) = 0.0;
+    }
+  }
+  for (int n = 0; n < 12; ++n) {
+    if (s.edge_flag[n]) continue;
+    const int lo = n & 1, hi = n & 2;
+    if (n < 4) {
+      const int i = lo ? b.ie + 1 : b.is, j = hi ? b.je + 1 : b.js;
+      for (int k = b.ks; k <= b.ke; ++k) b.e3(k, j, i) = 0.0;
+    } else if (n < 8) {
+      const int i = lo ? b.ie + 1 : b.is, k = hi ? b.ke + 1 : b.ks;
+      for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) = 0.0;
+    } else {
+      const int j = lo ? b.je + 1 : b.js, k = hi ? b.ke + 1 : b.ks;
+      for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) = 0.0;
+    }
+  }
+}
+
+// AverageFluxBoundary (flux_correction_fc.cpp:1355-1548), 3-D, periodic faces
+void emf_average(Block &b, SmrBlock &s) {
+  for (int n = 0; n < 6; ++n) {
+    const int d = n / 2, side = n & 1;
+    int o[3] = {0, 0, 0};
+    o[d] = side ? 1 : -1;
+    const int nl = s.nblevel[o[2] + 1][o[1] + 1][o[0] + 1];
+    if (d == 0) {
+      const int i = side ? b.ie + 1 : b.is;
+      if (nl == s.level) {
+        for (int k = b.ks + 1; k <= b.ke; ++k)
+          for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) *= 0.5;
+        for (int k = b.ks; k <= b.ke; ++k)
+          for (int j = b.js + 1; j <= b.je; ++j) b.e3(k, j, i) *= 0.5;
+      } else if (nl > s.level) {
+        const int k = b.ks + b.nx3 / 2;
+        for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) *= 0.5;
+        const int j = b.js + b.nx2 / 2;
+        for (int k2 = b.ks; k2 <= b.ke; ++k2) b.e3(k2, j, i) *= 0.5;
+      }
+    } else if (d == 1) {
+      const int j = side ? b.je + 1 : b.js;
+      if (nl == s.level) {
+        for (int k = b.ks + 1; k <= b.ke; ++k)
+          for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) *= 0.5;
+        for (int k = b.ks; k <= b.ke; ++k)
+          for (int i = b.is + 1; i <= b.ie; ++i) b.e3(k, j, i) *= 0.5;
+      } else if (nl > s.level) {
+        const int k = b.ks + b.nx3 / 2;
+        for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) *= 0.5;
+        const int i = b.is + b.nx1 / 2;
+        for (int k2 = b.ks; k2 <= b.ke; ++k2) b.e3(k2, j, i) *= 0.5;
+      }
+    } else {
+      const int k = side ? b.ke + 1 : b.ks;
+      if (nl == s.level) {
+        for (int j = b.js + 1; j <= b.je; ++j)
+          for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) *= 0.5;
+        for (int j = b.js; j <= b.je; ++j)
+          for (int i = b.is + 1; i <= b.ie; ++i) b.e2(k, j, i) *= 0.5;
+      } else if (nl > s.level) {
+        const int jf = b.js + b.nx2 / 2;
+        for (int i = b.is; i <= b.ie; ++i) b.e1(k, jf, i) *= 0.5;
+        const int iff = b.is + b.nx1 / 2;
+        for (int j = b.js; j <= b.je; ++j) b.e2(k, j, iff) *= 0.5;
+      }
+    }
+  }
+  for (int n = 0; n < 12; ++n) {
+    if (s.nedge_fine[n] == 1) continue;
+    const Real div = 1.0 / static_cast<Real>(s.nedge_fine[n]);
+    const int lo = n & 1, hi = n & 2;
+    if (n < 4) {
+      const int i = lo ? b.ie + 1 : b.is, j = hi ? b.je + 1 : b.js;
+      for (int k = b.ks; k <= b.ke; ++k) b.e3(k, j, i) *= div;
+    } else if (n < 8) {
+      const int i = lo ? b.ie + 1 : b.is, k = hi ? b.ke + 1 : b.ks;
+      for (int j = b.js; j <= b.je; ++j) b.e2(k, j, i) *= div;
+    } else {
+      const int j = lo ? b.je + 1 : b.js, k = hi ? b.ke + 1 : b.ks;
+      for (int i = b.is; i <= b.ie; ++i) b.e1(k, j, i) *= div;
+    }
+  }
+}
+
+void emf_correction(SmrMesh &m) {
+  const int nbk = static_cast<int>(m.blocks.size());
+  Buffers same, fine;
+  for (int g = 0; g < nbk; ++g) {   // SendFluxCorrection: every block packs first
+    SmrBlock &s = m.sb[g];
+    for (const Nbr &nb : s.nb) {
+      if (nb.type != FACE && nb.type != EDGE) break;
+      std::vector<Real> buf;
+      if (nb.level == s.level) {
+        if (nb.type == FACE || s.edge_flag[nb.eid]) {
+          emf_same_pack(m.blocks[g], nb, buf);
+          same[{nb.gid, g, -nb.ox1, -nb.ox2, -nb.ox3}] = buf;
+        }
+      } else if (nb.level == s.level - 1) {
+        emf_coarser_pack(m.blocks[g], nb, buf);
+        fine[{nb.gid, g, -nb.ox1, -nb.ox2, -nb.ox3}] = buf;
+      }
+    }
+  }
+  for (int g = 0; g < nbk; ++g) {   // ReceiveFluxCorrection
+    SmrBlock &s = m.sb[g];
+    Block &b = m.blocks[g];
+    for (const Nbr &nb : s.nb) {
+      if (nb.type != FACE && nb.type != EDGE) break;
+      if (nb.level != s.level) continue;
+      if (nb.type == FACE || s.edge_flag[nb.eid]) emf_same_add(b, nb, recv_buf(m, same, g, nb));
+    }
+    emf_clear_coarse(b, s);
+    for (const Nbr &nb : s.nb) {
+      if (nb.type != FACE && nb.type != EDGE) break;
+      if (nb.level != s.level + 1) continue;
+      emf_finer_add(b, nb, recv_buf(m, fine, g, nb));
+    }
+    emf_average(b, s);
+  }
+}
+
+// ------------------------------------------------------------------------ stepping
+void boundaries(SmrMesh &m) {
+  const int nbk = static_cast<int>(m.blocks.size());
+  Buffers bc, bf;
+  for (int g = 0; g < nbk; ++g) { send_cc(m, g, bc); send_fc(m, g, bf); }
+  for (int g = 0; g < nbk; ++g) { set_cc(m, g, bc); set_fc(m, g, bf); }
+  for (int g = 0; g < nbk; ++g) prolongate_boundaries(m, g);
+}
+
+void smr_stage(SmrMesh &m, int stage) {
+  const Real beta = (stage == 1) ? 0.5 : 1.0;
+  const Real delta = (stage == 1) ? 1.0 : 0.0;
+  const int nbk = static_cast<int>(m.blocks.size());
+  const int order = (stage == 1) ? 1 : m.c.xorder;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    if (stage == 1) { b.u1.zero(); b.bb1f.zero(); b.bb2f.zero(); b.bb3f.zero(); }
+    CalculateFluxes(m, b, order);
+  }
+  if (!(m.skip & 1)) hydro_flux_correction(m);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) ComputeCornerE(m, m.blocks[g]);
+  if (!(m.skip & 2)) emf_correction(m);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    if (delta != 0.0) {
+      for (int n = 0; n < NHYDRO; ++n)
+        for (int k = b.ks; k <= b.ke; ++k)
+          for (int j = b.js; j <= b.je; ++j)
+            for (int i = b.is; i <= b.ie; ++i) b.u1(n, k, j, i) += delta * b.u(n, k, j, i);
+    }
+    b.u.swap(b.u1);
+    AddFluxDivergence(b, beta * m.dt, b.u);
+    if (delta != 0.0) {
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je; ++j)
+          for (int i = b.is; i <= b.ie + 1; ++i) b.bb1f(k, j, i) += delta * b.b1f(k, j, i);
+      for (int k = b.ks; k <= b.ke; ++k)
+        for (int j = b.js; j <= b.je + 1; ++j)
+          for (int i = b.is; i <= b.ie; ++i) b.bb2f(k, j, i) += delta * b.b2f(k, j, i);
+      for (int k = b.ks; k <= b.ke + 1; ++k)
+        for (int j = b.js; j <= b.je; ++j)
+          for (int i = b.is; i <= b.ie; ++i) b.bb3f(k, j, i) += delta * b.b3f(k, j, i);
+    }
+    b.b1f.swap(b.bb1f); b.b2f.swap(b.bb2f); b.b3f.swap(b.bb3f);
+    CT(b, beta * m.dt);
+  }
+  boundaries(m);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    ConservedToPrimitive(m, b, b.u, b.w, b.w1);
+    b.w.swap(b.w1);
+    if (stage == 2) NewBlockTimeStep(m, b);
+  }
+}
+
+void smr_initialize(SmrMesh &m, Real tlim) {
+  boundaries(m);
+  const int nbk = static_cast<int>(m.blocks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int g = 0; g < nbk; ++g) {
+    Block &b = m.blocks[g];
+    ConservedToPrimitive(m, b, b.u, b.w1, b.w);
+    NewBlockTimeStep(m, b);
+  }
+  NewTimeStep(m, tlim);
+}
+
+}  // namespace
+
+//========================================================================================
+// C ABI (tests/oracle_bind.py).  The handle is an ora::Mesh*, so ora_set / ora_get /
+// ora_state / ora_nblocks / ora_free work on it as on a uniform mesh.
+
+extern "C" {
+
+static std::string g_smr_err;
+
+const char *ora_smr_error() { return g_smr_err.c_str(); }
+
+// cfg: the root mesh (nx = root-level cells, bnx = MeshBlock cells, bc all periodic);
+// locs[4 g .. 4 g + 3] = (level, lx1, lx2, lx3) of MeshBlock g, levels counted from the
+// tree's root (the root grid is root_level).  Returns null with ora_smr_error() set.
+void *ora_smr_new(const ora_cfg *cfg, int nblocks, const long long *locs, int root_level) {
+  try {
+    for (int f = 0; f < 6; ++f)
+      if (cfg->bc[f] != 0) throw std::runtime_error("the SMR restatement is periodic only");
+    if (cfg->relativistic) throw std::runtime_error("the SMR restatement is Newtonian only");
+    if (cfg->ng % 2)   // mesh_refinement.cpp:42-48
+      throw std::runtime_error("nghost must be even with mesh refinement");
+    for (int d = 0; d < 3; ++d)
+      if (cfg->bnx[d] % 2 || cfg->bnx[d] < 2 * cfg->ng || cfg->nx[d] % cfg->bnx[d])
+        throw std::runtime_error("MeshBlock sizes must be even, >= 2 NGHOST and divide the mesh");
+    SmrMesh *m = new SmrMesh;
+    m->c = *cfg;
+    const Real fmin = std::numeric_limits<float>::min();
+    m->dfloor = (cfg->dfloor < 0.0) ? std::sqrt(1024 * fmin) : cfg->dfloor;
+    m->pfloor = (cfg->pfloor < 0.0) ? std::sqrt(1024 * fmin) : cfg->pfloor;
+    m->sigma_max = 0.0; m->beta_min = 0.0; m->gamma_max = 1000.0;
+    m->gm1 = cfg->gamma - 1.0;
+    m->root_level = root_level;
+    for (int d = 0; d < 3; ++d) { m->nrb[d] = cfg->nx[d] / cfg->bnx[d]; m->nbx[d] = m->nrb[d]; }
+    const int ng = cfg->ng;
+    m->cng = (ng + 1) / 2 + 1;
+    m->cis = ng; m->cie = ng + cfg->bnx[0] / 2 - 1;
+    m->cjs = ng; m->cje = ng + cfg->bnx[1] / 2 - 1;
+    m->cks = ng; m->cke = ng + cfg->bnx[2] / 2 - 1;
+    m->blocks.resize(nblocks);
+    m->sb.resize(nblocks);
+    for (int g = 0; g < nblocks; ++g) {
+      SmrBlock &s = m->sb[g];
+      s.level = static_cast<int>(locs[4 * g]);
+      for (int d = 0; d < 3; ++d) s.lx[d] = locs[4 * g + 1 + d];
+      if (s.level < root_level) throw std::runtime_error("a block below the root level");
+      m->leaf[{s.level, s.lx[0], s.lx[1], s.lx[2]}] = g;
+    }
+    for (int g = 0; g < nblocks; ++g) {
+      Block &b = m->blocks[g];
+      SmrBlock &s = m->sb[g];
+      b.gid = g;
+      for (int d = 0; d < 3; ++d) b.lx[d] = static_cast<int>(s.lx[d]);
+      b.nx1 = cfg->bnx[0]; b.nx2 = cfg->bnx[1]; b.nx3 = cfg->bnx[2]; b.ng = ng;
+      b.is = ng; b.ie = ng + b.nx1 - 1; b.js = ng; b.je = ng + b.nx2 - 1;
+      b.ks = ng; b.ke = ng + b.nx3 - 1;
+      b.nc1 = b.nx1 + 2 * ng; b.nc2 = b.nx2 + 2 * ng; b.nc3 = b.nx3 + 2 * ng;
+      const int c1 = b.nc1, c2 = b.nc2, c3 = b.nc3;
+      b.u.alloc(NHYDRO, c3, c2, c1); b.u1.alloc(NHYDRO, c3, c2, c1);
+      b.w.alloc(NHYDRO, c3, c2, c1); b.w1.alloc(NHYDRO, c3, c2, c1);
+      b.bcc.alloc(3, c3, c2, c1); b.cce.alloc(3, c3, c2, c1);
+      b.b1f.alloc(1, c3, c2, c1 + 1); b.b2f.alloc(1, c3, c2 + 1, c1); b.b3f.alloc(1, c3 + 1, c2, c1);
+      b.bb1f.alloc(1, c3, c2, c1 + 1); b.bb2f.alloc(1, c3, c2 + 1, c1); b.bb3f.alloc(1, c3 + 1, c2, c1);
+      b.flux[0].alloc(NHYDRO, c3, c2, c1 + 1);
+      b.flux[1].alloc(NHYDRO, c3, c2 + 1, c1);
+      b.flux[2].alloc(NHYDRO, c3 + 1, c2, c1);
+      b.e2x1.alloc(1, c3, c2, c1 + 1); b.e3x1.alloc(1, c3, c2, c1 + 1); b.wx1.alloc(1, c3, c2, c1 + 1);
+      b.e1x2.alloc(1, c3, c2 + 1, c1); b.e3x2.alloc(1, c3, c2 + 1, c1); b.wx2.alloc(1, c3, c2 + 1, c1);
+      b.e1x3.alloc(1, c3 + 1, c2, c1); b.e2x3.alloc(1, c3 + 1, c2, c1); b.wx3.alloc(1, c3 + 1, c2, c1);
+      b.e1.alloc(1, c3 + 1, c2 + 1, c1); b.e2.alloc(1, c3 + 1, c2, c1 + 1); b.e3.alloc(1, c3, c2 + 1, c1 + 1);
+      const int cc1 = b.nx1 / 2 + 2 * ng, cc2 = b.nx2 / 2 + 2 * ng, cc3 = b.nx3 / 2 + 2 * ng;
+      s.ccons.alloc(NHYDRO, cc3, cc2, cc1); s.cprim.alloc(NHYDRO, cc3, cc2, cc1);
+      s.cbcc.alloc(3, cc3, cc2, cc1);
+      s.cb1.alloc(1, cc3, cc2, cc1 + 1); s.cb2.alloc(1, cc3, cc2 + 1, cc1);
+      s.cb3.alloc(1, cc3 + 1, cc2, cc1);
+      set_block_coords(*m, g);
+    }
+    for (int g = 0; g < nblocks; ++g) search_neighbors(*m, g);
+    return static_cast<Mesh *>(m);
+  } catch (const std::exception &e) {
+    g_smr_err = e.what();
+    return nullptr;
+  }
+}
+
+static SmrMesh *smr(void *h) { return static_cast<SmrMesh *>(static_cast<Mesh *>(h)); }
+
+void ora_smr_free(void *h) { delete smr(h); }
+
+int ora_smr_initialize(void *h, double tlim) {
+  try {
+    smr_initialize(*smr(h), tlim);
+    return 0;
+  } catch (const std::exception &e) {
+    g_smr_err = e.what();
+    return -1;
+  }
+}
+
+// Runs up to ncycles VL2 cycles while time < tlim; returns the cycles done, -1 on error.
+int ora_smr_run(void *h, int ncycles, double tlim) {
+  SmrMesh &m = *smr(h);
+  int n = 0;
+  try {
+    while (n < ncycles && m.time < tlim) {
+      smr_stage(m, 1);
+      smr_stage(m, 2);
+      m.ncycle++;
+      m.time += m.dt;
+      NewTimeStep(m, tlim);
+      ++n;
+    }
+  } catch (const std::exception &e) {
+    g_smr_err = e.what();
+    return -1;
+  }
+  return n;
+}
+
+// Test knob: leave out the flux correction (bit 0) or the EMF correction (bit 1), to show
+// that the conservation and div(B) checks notice either omission.
+void ora_smr_skip(void *h, int mask) { smr(h)->skip = mask; }
+
+// Neighbour list of block g as SearchAndSetNeighbors orders it: 9 ints per entry
+// (gid, level, ox1, ox2, ox3, type 1 face / 2 edge / 3 corner, fi1, fi2, eid or fid).
+int ora_smr_neighbors(void *h, int g, int *out, int cap) {
+  SmrMesh &m = *smr(h);
+  const std::vector<Nbr> &nb = m.sb[g].nb;
+  const int n = static_cast<int>(nb.size());
+  for (int q = 0; q < n && q < cap; ++q) {
+    const Nbr &e = nb[q];
+    const int v[9] = {e.gid, e.level, e.ox1, e.ox2, e.ox3, e.type, e.fi1, e.fi2,
+                      (e.type == EDGE) ? e.eid : e.fid};
+    for (int t = 0; t < 9; ++t) out[9 * q + t] = v[t];
+  }
+  return n;
+}
+
+// Block g's coordinate table (x1f, x1v, dx1f, x2f, ... ; coarse: the coarse buffer's).
+long ora_smr_coords(void *h, int g, int coarse, double *out, long cap) {
+  SmrMesh &m = *smr(h);
+  const std::vector<Real> &c = coarse ? m.sb[g].cco : m.sb[g].fco;
+  const long n = static_cast<long>(c.size());
+  for (long q = 0; q < n && q < cap; ++q) out[q] = c[q];
+  return n;
+}
+
+}  // extern "C"

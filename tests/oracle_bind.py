@@ -77,6 +77,18 @@ def lib():
         L.ora_mlp_forward.argtypes = [D, ctypes.c_int, D, D]
         L.ora_mlp_grad.argtypes = [D, ctypes.c_int, D, D, D, D]
         L.ora_mlp_features.argtypes = [ctypes.c_int, D, D, D]
+        L.ora_smr_new.restype = P
+        L.ora_smr_new.argtypes = [ctypes.POINTER(OraCfg), ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.ora_smr_error.restype = ctypes.c_char_p
+        L.ora_smr_free.argtypes = [P]
+        L.ora_smr_initialize.argtypes = [P, ctypes.c_double]
+        L.ora_smr_run.argtypes = [P, ctypes.c_int, ctypes.c_double]
+        L.ora_smr_neighbors.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.c_int]
+        L.ora_smr_coords.restype = ctypes.c_long
+        L.ora_smr_skip.argtypes = [P, ctypes.c_int]
+        L.ora_smr_coords.argtypes = [P, ctypes.c_int, ctypes.c_int, D, ctypes.c_long]
         _lib = L
     return _lib
 
@@ -168,6 +180,55 @@ class OracleMesh:
         wl = np.ascontiguousarray(wl, dtype=np.float64)
         wr = np.ascontiguousarray(wr, dtype=np.float64)
         lib().ora_block_riemann(self.h, gid, d, k, j, il, iu, _dp(wl), _dp(wr))
+
+
+class OracleSmrMesh(OracleMesh):
+    """CPU restatement of a statically refined mesh (``athena_amd.mesh.refined``):
+    periodic faces, Newtonian MHD."""
+
+    def __init__(self, m: MeshDesc):
+        assert m.smr, "OracleSmrMesh needs a refined MeshDesc"
+        self.m = m
+        self._cfg = cfg_of(m)
+        self._locs = np.ascontiguousarray(m.locs, dtype=np.int64)
+        self.h = lib().ora_smr_new(ctypes.byref(self._cfg), len(self._locs),
+                                   self._locs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                   int(m.root_level))
+        if not self.h:
+            raise ValueError(lib().ora_smr_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_smr_free(self.h)
+            self.h = None
+
+    def initialize(self, tlim=1e300):
+        if lib().ora_smr_initialize(self.h, tlim) != 0:
+            raise RuntimeError(lib().ora_smr_error().decode())
+
+    def run(self, ncycles, tlim=1e300):
+        n = lib().ora_smr_run(self.h, ncycles, tlim)
+        if n < 0:
+            raise RuntimeError(lib().ora_smr_error().decode())
+        return n
+
+    def neighbors(self, gid):
+        """SearchAndSetNeighbors order: rows (gid, level, ox1, ox2, ox3, type, fi1, fi2,
+        eid or fid), type 1 face, 2 edge, 3 corner."""
+        out = np.zeros(9 * 56, dtype=np.int32)
+        n = lib().ora_smr_neighbors(self.h, gid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                    56)
+        return out[:9 * n].reshape(n, 9)
+
+    def skip(self, mask):
+        """Planted omissions (test knob): 1 the flux correction, 2 the EMF correction."""
+        lib().ora_smr_skip(self.h, mask)
+
+    def coords(self, gid, coarse=False):
+        n = lib().ora_smr_coords(self.h, gid, int(coarse), None, 0)
+        a = np.empty(n)
+        lib().ora_smr_coords(self.h, gid, int(coarse), _dp(a), n)
+        return a
 
 
 def riemann(solver, relativistic, gamma, wl, wr, bx):

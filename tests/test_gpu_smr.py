@@ -1,0 +1,157 @@
+"""Static mesh refinement on the GPU (ath_gpu_smr_mesh_create): device runs against the
+CPU restatement (oracle/smr_oracle.cpp) bit for bit, and the reference's own SMR
+known-answer test.
+
+* Parity: the refined linear-wave mesh of tst/regression/scripts/tests/mhd/
+  mhd_linwave.py:42-97 (athinput.linear_wave3d with its five <refinementN> regions and
+  MeshBlocks of nx/4) at nx1 = 32 (72 MeshBlocks of 8^3 on two levels), every register of
+  every block, ghosts included, after initialisation and after cycles, for donor cell,
+  PLM and PPM (nghost 4: mesh refinement needs an even nghost, mesh_refinement.cpp:42-48),
+  HLLD and HLLE, and on a mesh refined two levels deep.
+* Known answer: the same runs at nx1 = 32 and 64 to the reference's tlim for the fast,
+  Alfven, slow and entropy waves: the RMS L1 error at 64 under the script's thresholds
+  (mhd_linwave.py:122-153: 4.5e-8, 4.0e-8, 5.0e-8, 2.75e-8) and the ratio 64/32 under 0.4.
+  On the uniform grid the same errors at 64 (tests/golden/linear_wave_errors.json: 5.97e-8
+  for the fast wave) are above those thresholds: only the refined runs meet them.
+* Invariants: total mass and energy (volume-weighted over both levels) stay conserved to
+  round-off, and both sides of every MeshBlock face hold the same magnetic flux (a coarse
+  face the mean of the four fine faces on it).
+"""
+import numpy as np
+import pytest
+
+from athena_amd.mesh import refined
+from athena_amd.pgen.linear_wave import WAVE_TLIM, LinearWave, linear_wave_mesh
+
+pytestmark = pytest.mark.gpu
+
+# <refinement1..5> of inputs/mhd/athinput.linear_wave3d
+REGIONS = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((0.3, 1.0, 1.0), (0.4, 1.1, 1.1), 1),
+           ((1.0, 1.0, 0.3), (1.1, 1.1, 0.4), 1), ((1.0, 0.3, 1.0), (1.1, 0.4, 1.1), 1),
+           ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+THRESH = {0: 4.5e-8, 1: 4.0e-8, 2: 5.0e-8, 3: 2.75e-8}
+REGS = ("u", "w", "bcc", "b1", "b2", "b3")
+
+
+def smr_wave(nx1, wave_flag=0, regions=REGIONS, **kw):
+    m = refined(linear_wave_mesh(nx1, wave_flag, bnx=(nx1 // 4,) * 3, **kw), regions)
+    lw = LinearWave(m, wave_flag=wave_flag, vflow=1.0 if wave_flag == 3 else 0.0)
+    return m, lw
+
+
+def load_both(m, lw):
+    from athena_amd.gpu import GpuMesh
+    from oracle_bind import OracleSmrMesh
+    blocks = [lw.block(g) for g in range(m.nblocks)]
+    dev = GpuMesh(m)
+    ora = OracleSmrMesh(m)
+    for g, b in enumerate(blocks):
+        for k in ("u", "b1", "b2", "b3"):
+            dev.set(g, k, b[k])
+            ora.set(g, k, b[k])
+    return dev, ora
+
+
+def compare(dev, ora, m, where):
+    for g in range(m.nblocks):
+        for k in REGS:
+            a, b = dev.get(g, k), ora.get(g, k)
+            if not np.array_equal(a, b):
+                bad = np.argwhere(a != b)
+                raise AssertionError("%s: block %d (level %d) %s differs at %d entries, first %s: "
+                                     "%r vs %r" % (where, g, int(m.locs[g][0]), k, len(bad),
+                                                   tuple(bad[0]), a[tuple(bad[0])],
+                                                   b[tuple(bad[0])]))
+
+
+@pytest.mark.parametrize("xorder,rsolver,ng", [(2, "hlld", 2), (1, "hlld", 2), (3, "hlld", 4),
+                                               (2, "hlle", 2)])
+def test_smr_linear_wave_bitwise(xorder, rsolver, ng):
+    m, lw = smr_wave(32, 0, xorder=xorder, rsolver=rsolver, ng=ng)
+    assert m.nblocks == 72
+    dev, ora = load_both(m, lw)
+    dev.initialize(0.5)
+    ora.initialize(0.5)
+    compare(dev, ora, m, "after initialisation")
+    for c in (1, 3):
+        done = dev.run(c, 0.5)
+        assert ora.run(c, 0.5) == done == c
+        t_d, dt_d, _ = dev.state()
+        t_o, dt_o, _ = ora.state()
+        assert (t_d, dt_d) == (t_o, dt_o)
+        compare(dev, ora, m, "after %d more cycles" % c)
+
+
+def test_smr_two_levels_bitwise():
+    """A region refined twice: three levels, the 2:1 rule refining the neighbours of the
+    deepest blocks (the tree builder's doing)."""
+    regions = [((0.6, 0.5, 0.5), (0.9, 0.8, 0.8), 2)]
+    m, lw = smr_wave(32, 1, regions=regions)
+    assert len(set(int(v) for v in m.locs[:, 0])) == 3
+    dev, ora = load_both(m, lw)
+    dev.initialize(1.0)
+    ora.initialize(1.0)
+    compare(dev, ora, m, "after initialisation")
+    dev.run(4, 1.0)
+    ora.run(4, 1.0)
+    compare(dev, ora, m, "after 4 cycles")
+
+
+def test_smr_graph_replay_matches_eager():
+    m, lw = smr_wave(32, 0)
+    from athena_amd.gpu import GpuMesh
+    blocks = [lw.block(g) for g in range(m.nblocks)]
+    runs = []
+    for graph in (False, True):
+        dev = GpuMesh(m)
+        for g, b in enumerate(blocks):
+            for k in ("u", "b1", "b2", "b3"):
+                dev.set(g, k, b[k])
+        dev.set_graph(graph)
+        dev.initialize(0.5)
+        dev.run(6, 0.5)
+        runs.append([dev.get(g, "u") for g in range(m.nblocks)])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+def test_smr_conservation_and_face_consistency():
+    from athena_amd import gpu
+    from smr_common import face_mismatch, totals
+    m, lw = smr_wave(32, 0)
+    lw = LinearWave(m, wave_flag=0, amp=1e-2)   # coarse/fine mismatches above round-off
+    dev = gpu.GpuMesh(m)
+    for g in range(m.nblocks):
+        b = lw.block(g)
+        for k in ("u", "b1", "b2", "b3"):
+            dev.set(g, k, b[k])
+    dev.initialize(0.5)
+    nbrs = {g: gpu.smr_plan(m, g)[0] for g in range(m.nblocks)}
+    t0, f0 = totals(dev, m), face_mismatch(dev, m, nbrs.__getitem__)
+    dev.run(20, 0.5)
+    t1, f1 = totals(dev, m), face_mismatch(dev, m, nbrs.__getitem__)
+    assert f0 < 1e-13 and f1 < 1e-13
+    assert abs(t1[0] - t0[0]) <= 1e-14 * abs(t0[0])
+    assert abs(t1[4] - t0[4]) <= 1e-14 * abs(t0[4])
+
+
+@pytest.mark.parametrize("wave_flag", [0, 1, 2, 3])
+def test_smr_linear_wave_known_answer(wave_flag):
+    from athena_amd.gpu import GpuMesh
+    tlim = WAVE_TLIM[wave_flag]
+    rms = {}
+    for nx1 in (32, 64):
+        m, lw = smr_wave(nx1, wave_flag)
+        dev = GpuMesh(m)
+        for g in range(m.nblocks):
+            b = lw.block(g)
+            for k in ("u", "b1", "b2", "b3"):
+                dev.set(g, k, b[k])
+        dev.initialize(tlim)
+        dev.run(10 ** 6, tlim)
+        assert dev.state()[0] == pytest.approx(tlim, abs=1e-12)
+        rms[nx1] = lw.errors(lambda g: {"u": dev.get(g, "u"), "bcc": dev.get(g, "bcc")})[0]
+    print("wave %d: rms %.6e (32), %.6e (64), ratio %.3f"
+          % (wave_flag, rms[32], rms[64], rms[64] / rms[32]))
+    assert rms[64] < THRESH[wave_flag]
+    assert rms[64] / rms[32] < 0.4
