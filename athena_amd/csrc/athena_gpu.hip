@@ -2071,7 +2071,11 @@ __device__ __forceinline__ double smr_flux_restrict(const MeshK &M, const SmrArg
           f(k, j + 1, i + 1) * s11) / tarea;
 }
 
-__global__ void __launch_bounds__(WG) k_smr(MeshK M, SmrArgs a) {
+// Work-group size of k_smr (one work-group per MeshBlock).  512 and 1024 measured no
+// faster on the two-level linear-wave mesh of 338 MeshBlocks of 32^3.
+constexpr int SMR_WG = 256;
+
+__global__ void __launch_bounds__(SMR_WG) k_smr(MeshK M, SmrArgs a) {
   const int g = blockIdx.x;
   const SmrDims D = smr_dims(M);
   const refops::RGrid Fg(a.fco + g * a.fcl, M.nc1, M.nc2, M.nc3);
@@ -2088,7 +2092,7 @@ __global__ void __launch_bounds__(WG) k_smr(MeshK M, SmrArgs a) {
                      st.op == smrplan::PROLONG_CC || st.op == smrplan::FLUX_RESTRICT;
     const int nb = st.n[0] * st.n[1] * st.n[2];
     const int total = nv5 ? NHYDRO * nb : nb;
-    for (int t = threadIdx.x; t < total; t += WG) {
+    for (int t = threadIdx.x; t < total; t += SMR_WG) {
       const int n = t / nb;
       int q = t - n * nb;
       const int di = q % st.n[0];
@@ -2677,7 +2681,7 @@ SmrArgs smr_args(const ath_mesh *m, int prog, double *u, double *const *b, doubl
 int launch_smr(ath_mesh *m, int prog, const SmrArgs &a) {
   static const int kid[3] = {KID_SMR_FLUX, KID_SMR_EMF, KID_SMR_BND};
   ProfScope ps(m, kid[prog]);
-  hipLaunchKernelGGL(k_smr, dim3(m->M.nblocks), dim3(WG), 0, m->stream, m->M, a);
+  hipLaunchKernelGGL(k_smr, dim3(m->M.nblocks), dim3(SMR_WG), 0, m->stream, m->M, a);
   HIPCHK(hipGetLastError());
   return 0;
 }

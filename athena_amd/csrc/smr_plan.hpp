@@ -471,17 +471,94 @@ inline void fc_finer_recv(const Shape &S, const Nbr &nb, Box r[3]) {
 }
 
 //----------------------------------------------------------------------------------------
-// Program assembly with bounds checks: `ext` are the extents of the destination array
-// (and of the source array, `sext`), in (i, j, k).
+// Program assembly.  A block's steps run in order on one work-group; a barrier goes in
+// only where a step touches what an earlier step since the last barrier wrote, or writes
+// what one read (the own-block arrays below; other blocks' arrays are only read, and only
+// their active data, which no step of the launch writes).
+enum ArrId { A_U, A_B0, A_CU = 4, A_CW, A_CBCC, A_CB0, A_W = 10, A_BCC, A_F0, A_E0 = 15 };
+
+struct Acc {
+  int arr;
+  Box box;
+  bool write;
+};
+
 struct Program {
   std::vector<Step> steps;
-  std::vector<int> offs;   // steps of block g: [offs[g], offs[g+1])
+  std::vector<int> offs;     // steps of block g: [offs[g], offs[g+1])
+  Shape S{};
+  std::vector<Acc> pending;  // own-array accesses since the last barrier
+  void begin_block() { pending.clear(); }
 };
 
 inline void check_box(const Box &b, const int *ext, const char *what) {
   for (int d = 0; d < 3; ++d)
     if (b.lo[d] < 0 || b.hi[d] >= ext[d])
       throw std::runtime_error(std::string("SMR plan: box outside the array (") + what + ")");
+}
+
+inline Box grow(Box b, int lo, int hi) {
+  for (int d = 0; d < 3; ++d) { b.lo[d] += lo; b.hi[d] += hi; }
+  return b;
+}
+// the fine cells and faces under a box of coarse indices (one more face per direction)
+inline Box fine_box(const Shape &S, const Box &c) {
+  Box f;
+  for (int d = 0; d < 3; ++d) {
+    f.lo[d] = (c.lo[d] - S.cis[d]) * 2 + S.is[d];
+    f.hi[d] = (c.hi[d] - S.cis[d]) * 2 + S.is[d] + 2;
+  }
+  return f;
+}
+inline bool overlap(const Box &a, const Box &b) {
+  for (int d = 0; d < 3; ++d)
+    if (a.hi[d] < b.lo[d] || b.hi[d] < a.lo[d]) return false;
+  return true;
+}
+
+// The own-block accesses of one step (conservative boxes).
+inline void own_accesses(const Shape &S, const Step &st, const Box &dst, std::vector<Acc> &a) {
+  const int c = st.comp;
+  a.clear();
+  switch (st.op) {
+    case CC_COPY: case CC_RESTRICT: a.push_back({A_U, dst, true}); break;
+    case CC_TO_COARSE: a.push_back({A_CU, dst, true}); break;
+    case CC_OWN_RESTRICT:
+      a.push_back({A_CU, dst, true});
+      a.push_back({A_U, fine_box(S, dst), false});
+      break;
+    case FC_COPY: case FC_RESTRICT: a.push_back({A_B0 + c, dst, true}); break;
+    case FC_TO_COARSE: a.push_back({A_CB0 + c, dst, true}); break;
+    case FC_OWN_RESTRICT:
+      a.push_back({A_CB0 + c, dst, true});
+      a.push_back({A_B0 + c, fine_box(S, dst), false});
+      break;
+    case C2P_COARSE:
+      a.push_back({A_CU, dst, true});
+      a.push_back({A_CW, dst, true});
+      a.push_back({A_CBCC, dst, true});
+      for (int q = 0; q < 3; ++q) a.push_back({A_CB0 + q, grow(dst, 0, 1), false});
+      break;
+    case PROLONG_CC:
+      a.push_back({A_CW, grow(dst, -1, 1), false});
+      a.push_back({A_W, fine_box(S, dst), true});
+      break;
+    case PROLONG_FC_SHARED:
+      a.push_back({A_CB0 + c, grow(dst, -1, 1), false});
+      a.push_back({A_B0 + c, fine_box(S, dst), true});
+      break;
+    case PROLONG_FC_INTERNAL:
+      for (int q = 0; q < 3; ++q) a.push_back({A_B0 + q, fine_box(S, dst), true});
+      break;
+    case BCC_P2C:
+      a.push_back({A_W, dst, false});
+      for (int q = 0; q < 3; ++q) a.push_back({A_B0 + q, grow(dst, 0, 1), false});
+      a.push_back({A_U, dst, true});
+      a.push_back({A_BCC, dst, true});
+      break;
+    case FLUX_RESTRICT: a.push_back({A_F0 + c, dst, true}); break;
+    default: a.push_back({A_E0 + c, dst, true}); break;   // EMF read-modify-writes
+  }
 }
 
 inline void emit(Program &p, int op, int comp, int src, const Box &dst, const int *s0, int mul,
@@ -494,13 +571,20 @@ inline void emit(Program &p, int op, int comp, int src, const Box &dst, const in
     s.n[d] = dst.size(d);
     s.s0[d] = s0 ? s0[d] : dst.lo[d];
   }
+  std::vector<Acc> acc;
+  own_accesses(p.S, s, dst, acc);
+  bool conflict = false;
+  for (const Acc &x : acc)
+    for (const Acc &y : p.pending)
+      if (x.arr == y.arr && (x.write || y.write) && overlap(x.box, y.box)) conflict = true;
+  if (conflict) {
+    Step b{};
+    b.op = BARRIER;
+    p.steps.push_back(b);
+    p.pending.clear();
+  }
   p.steps.push_back(s);
-}
-inline void barrier(Program &p) {
-  if (!p.steps.empty() && p.steps.back().op == BARRIER) return;
-  Step s{};
-  s.op = BARRIER;
-  p.steps.push_back(s);
+  p.pending.insert(p.pending.end(), acc.begin(), acc.end());
 }
 
 // Array extents (i, j, k) of the registers a program addresses.
@@ -537,6 +621,8 @@ inline int fine_of(const Shape &S, int d, int c) { return (c - S.cis[d]) * 2 + S
 // neighbour order, then ProlongateBoundaries for each coarser neighbour.
 inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, int g,
                              Program &p) {
+  p.S = S;
+  p.begin_block();
   const Ext E = extents(S);
   const BlockInfo &s = bl[g];
   // LoadBoundaryBufferToCoarser restricts into the sender's own coarse buffer
@@ -552,7 +638,6 @@ inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, i
       emit(p, FC_OWN_RESTRICT, c, -1, rf[c], nullptr, 1);
     }
   }
-  barrier(p);
   for (const Nbr &nb : s.nb) {
     const Nbr &bk = back(bl, nb.gid, g, nb.ox1, nb.ox2, nb.ox3);
     Box rf[3], sf[3];
@@ -603,7 +688,6 @@ inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, i
         emit(p, FC_TO_COARSE, c, nb.gid, rf[c], sf[c].lo, 1);
       }
     }
-    barrier(p);
   }
   // ProlongateBoundaries (bvals_refine.cpp:98-246, 540-673), periodic mesh
   const int lv = s.level;
@@ -645,7 +729,6 @@ inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, i
             emit(p, FC_OWN_RESTRICT, c, -1, rb, nullptr, 1);
           }
         }
-    barrier(p);
     const int cn = S.cng - 1;
     Box pr;
     for (int d = 0; d < 3; ++d) {
@@ -660,7 +743,6 @@ inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, i
     for (int d = 0; d < 3; ++d) { cr.lo[d] -= 1; cr.hi[d] += 1; }
     check_box(cr, E.ccell, "coarse recovery");
     emit(p, C2P_COARSE, 0, -1, cr, nullptr, 1);
-    barrier(p);
     check_box(pr, E.ccell, "cell prolongation");
     for (int d = 0; d < 3; ++d)
       if (pr.lo[d] < 1 || pr.hi[d] + 1 >= S.ncc[d] || fine_of(S, d, pr.hi[d]) + 1 >= S.nc[d])
@@ -679,9 +761,7 @@ inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, i
       if (!fb.empty()) check_box(fb, E.cface[c], "face prolongation");
       emit(p, PROLONG_FC_SHARED, c, -1, fb, nullptr, 1);
     }
-    barrier(p);
     emit(p, PROLONG_FC_INTERNAL, 0, -1, pr, nullptr, 1);
-    barrier(p);
     Box fr;
     for (int d = 0; d < 3; ++d) {
       fr.lo[d] = fine_of(S, d, pr.lo[d]);
@@ -689,13 +769,14 @@ inline void boundary_program(const Shape &S, const std::vector<BlockInfo> &bl, i
     }
     check_box(fr, E.cell, "fine recovery box");
     emit(p, BCC_P2C, 0, -1, fr, nullptr, 1);
-    barrier(p);
   }
 }
 
 // The hydro flux correction of block g: restricted fluxes of each finer face neighbour
 // replace its own on that quarter of the face (SetFluxBoundaryFromFiner).
 inline void flux_program(const Shape &S, const std::vector<BlockInfo> &bl, int g, Program &p) {
+  p.S = S;
+  p.begin_block();
   const Ext E = extents(S);
   const BlockInfo &s = bl[g];
   for (const Nbr &nb : s.nb) {
@@ -750,6 +831,8 @@ inline Box emf_edge_box(const Shape &S, int eid, int &comp) {
 // order, ClearCoarseFluxBoundary, the restricted EMFs of finer neighbours, then
 // AverageFluxBoundary.
 inline void emf_program(const Shape &S, const std::vector<BlockInfo> &bl, int g, Program &p) {
+  p.S = S;
+  p.begin_block();
   const Ext E = extents(S);
   const BlockInfo &s = bl[g];
   for (const Nbr &nb : s.nb) {
@@ -775,7 +858,6 @@ inline void emf_program(const Shape &S, const std::vector<BlockInfo> &bl, int g,
       check_src(mine, theirs.lo, 1, E.edge[c], 0, "EMF edge sum source");
       emit(p, EMF_ADD_SAME, c, nb.gid, mine, theirs.lo, 1);
     }
-    barrier(p);
   }
   // ClearCoarseFluxBoundary (flux_correction_fc.cpp:1214-1352)
   for (int f = 0; f < 6; ++f) {
@@ -801,7 +883,6 @@ inline void emf_program(const Shape &S, const std::vector<BlockInfo> &bl, int g,
     const Box b = emf_edge_box(S, e, c);
     emit(p, EMF_CLEAR, c, -1, b, nullptr, 1);
   }
-  barrier(p);
   for (const Nbr &nb : s.nb) {
     if (nb.type != FACE && nb.type != EDGE) break;
     if (nb.level != s.level + 1) continue;
@@ -852,7 +933,6 @@ inline void emf_program(const Shape &S, const std::vector<BlockInfo> &bl, int g,
       check_box(src, E.edge[c], "EMF fine edge source");
       emit(p, EMF_ADD_FINE, c, nb.gid, b, s0, 2);
     }
-    barrier(p);
   }
   // AverageFluxBoundary (flux_correction_fc.cpp:1355-1548), periodic faces
   for (int f = 0; f < 6; ++f) {
@@ -882,7 +962,6 @@ inline void emf_program(const Shape &S, const std::vector<BlockInfo> &bl, int g,
     const Box b = emf_edge_box(S, e, c);
     emit(p, EMF_SCALE, c, -1, b, nullptr, 1, 1.0 / static_cast<double>(s.nedge_fine[e]));
   }
-  barrier(p);
 }
 
 }  // namespace smrplan

@@ -4,8 +4,8 @@ Restates ``src/pgen/linear_wave.cpp`` for adiabatic MHD:
 * ``Mesh::InitUserMeshData`` wave geometry and eigensystem (``:78-184``),
 * ``Eigensystem`` adiabatic-MHD right eigenvectors (``:626-773``),
 * ``MeshBlock::ProblemGenerator`` face fields from the vector potential and cell
-  conserved variables (``:436-580``; uniform mesh, so the SMR averaging branch of the
-  vector potential is never taken),
+  conserved variables (``:436-580``, with the averaged vector potential on edges next to
+  finer MeshBlocks of a statically refined mesh),
 * ``Mesh::UserWorkAfterLoop`` volume-weighted L1 errors (``:190-430``).
 """
 from __future__ import annotations
@@ -242,43 +242,55 @@ class LinearWave:
         is_, ie, js, je, ks, ke = rng
         nbl = m.nblevel[gid]
         lev = int(m.locs[gid][0])
-        fin = lambda o3, o2, o1: nbl[o3 + 1, o2 + 1, o1 + 1] > lev  # noqa: E731
-        for k in range(ks, ke + 2):
-            for j in range(js, je + 2):
-                for i in range(is_, ie + 2):
-                    kk, jj, ii = k - ks, j - js, i - is_
-                    lo_j, hi_j, lo_k, hi_k = j == js, j == je + 1, k == ks, k == ke + 1
-                    lo_i, hi_i = i == is_, i == ie + 1
-                    if ((fin(0, -1, 0) and lo_j) or (fin(0, 1, 0) and hi_j)
-                            or (fin(-1, 0, 0) and lo_k) or (fin(1, 0, 0) and hi_k)
-                            or (fin(-1, -1, 0) and lo_j and lo_k)
-                            or (fin(-1, 1, 0) and hi_j and lo_k)
-                            or (fin(1, -1, 0) and lo_j and hi_k)
-                            or (fin(1, 1, 0) and hi_j and hi_k)):
-                        x1l = c.x1f[i] + 0.25 * c.dx1f[i]
-                        x1r = c.x1f[i] + 0.75 * c.dx1f[i]
-                        a1[kk, jj, ii] = 0.5 * (self.A1(x1l, c.x2f[j], c.x3f[k]) +
-                                                self.A1(x1r, c.x2f[j], c.x3f[k]))
-                    if ((fin(0, 0, -1) and lo_i) or (fin(0, 0, 1) and hi_i)
-                            or (fin(-1, 0, 0) and lo_k) or (fin(1, 0, 0) and hi_k)
-                            or (fin(-1, 0, -1) and lo_i and lo_k)
-                            or (fin(-1, 0, 1) and hi_i and lo_k)
-                            or (fin(1, 0, -1) and lo_i and hi_k)
-                            or (fin(1, 0, 1) and hi_i and hi_k)):
-                        x2l = c.x2f[j] + 0.25 * c.dx2f[j]
-                        x2r = c.x2f[j] + 0.75 * c.dx2f[j]
-                        a2[kk, jj, ii] = 0.5 * (self.A2(c.x1f[i], x2l, c.x3f[k]) +
-                                                self.A2(c.x1f[i], x2r, c.x3f[k]))
-                    if ((fin(0, 0, -1) and lo_i) or (fin(0, 0, 1) and hi_i)
-                            or (fin(0, -1, 0) and lo_j) or (fin(0, 1, 0) and hi_j)
-                            or (fin(0, -1, -1) and lo_i and lo_j)
-                            or (fin(0, -1, 1) and hi_i and lo_j)
-                            or (fin(0, 1, -1) and lo_i and hi_j)
-                            or (fin(0, 1, 1) and hi_i and hi_j)):
-                        x3l = c.x3f[k] + 0.25 * c.dx3f[k]
-                        x3r = c.x3f[k] + 0.75 * c.dx3f[k]
-                        a3[kk, jj, ii] = 0.5 * (self.A3(c.x1f[i], c.x2f[j], x3l) +
-                                                self.A3(c.x1f[i], c.x2f[j], x3r))
+        fin = lambda o3, o2, o1: bool(nbl[o3 + 1, o2 + 1, o1 + 1] > lev)  # noqa: E731
+        K = np.arange(ks, ke + 2)[:, None, None]
+        J = np.arange(js, je + 2)[None, :, None]
+        I = np.arange(is_, ie + 2)[None, None, :]
+        lo_i, hi_i, lo_j, hi_j = I == is_, I == ie + 1, J == js, J == je + 1
+        lo_k, hi_k = K == ks, K == ke + 1
+        shp = a1.shape
+        m1 = np.zeros(shp, dtype=bool)
+        for flag, cond in ((fin(0, -1, 0), lo_j), (fin(0, 1, 0), hi_j), (fin(-1, 0, 0), lo_k),
+                           (fin(1, 0, 0), hi_k), (fin(-1, -1, 0), lo_j & lo_k),
+                           (fin(-1, 1, 0), hi_j & lo_k), (fin(1, -1, 0), lo_j & hi_k),
+                           (fin(1, 1, 0), hi_j & hi_k)):
+            if flag:
+                m1 |= np.broadcast_to(cond, shp)
+        m2 = np.zeros(shp, dtype=bool)
+        for flag, cond in ((fin(0, 0, -1), lo_i), (fin(0, 0, 1), hi_i), (fin(-1, 0, 0), lo_k),
+                           (fin(1, 0, 0), hi_k), (fin(-1, 0, -1), lo_i & lo_k),
+                           (fin(-1, 0, 1), hi_i & lo_k), (fin(1, 0, -1), lo_i & hi_k),
+                           (fin(1, 0, 1), hi_i & hi_k)):
+            if flag:
+                m2 |= np.broadcast_to(cond, shp)
+        m3 = np.zeros(shp, dtype=bool)
+        for flag, cond in ((fin(0, 0, -1), lo_i), (fin(0, 0, 1), hi_i), (fin(0, -1, 0), lo_j),
+                           (fin(0, 1, 0), hi_j), (fin(0, -1, -1), lo_i & lo_j),
+                           (fin(0, -1, 1), hi_i & lo_j), (fin(0, 1, -1), lo_i & hi_j),
+                           (fin(0, 1, 1), hi_i & hi_j)):
+            if flag:
+                m3 |= np.broadcast_to(cond, shp)
+        kk, jj, ii = np.nonzero(m1)
+        if len(kk):
+            i, j, k = ii + is_, jj + js, kk + ks
+            x1l = c.x1f[i] + 0.25 * c.dx1f[i]
+            x1r = c.x1f[i] + 0.75 * c.dx1f[i]
+            a1[kk, jj, ii] = 0.5 * (self.A1(x1l, c.x2f[j], c.x3f[k]) +
+                                    self.A1(x1r, c.x2f[j], c.x3f[k]))
+        kk, jj, ii = np.nonzero(m2)
+        if len(kk):
+            i, j, k = ii + is_, jj + js, kk + ks
+            x2l = c.x2f[j] + 0.25 * c.dx2f[j]
+            x2r = c.x2f[j] + 0.75 * c.dx2f[j]
+            a2[kk, jj, ii] = 0.5 * (self.A2(c.x1f[i], x2l, c.x3f[k]) +
+                                    self.A2(c.x1f[i], x2r, c.x3f[k]))
+        kk, jj, ii = np.nonzero(m3)
+        if len(kk):
+            i, j, k = ii + is_, jj + js, kk + ks
+            x3l = c.x3f[k] + 0.25 * c.dx3f[k]
+            x3r = c.x3f[k] + 0.75 * c.dx3f[k]
+            a3[kk, jj, ii] = 0.5 * (self.A3(c.x1f[i], c.x2f[j], x3l) +
+                                    self.A3(c.x1f[i], c.x2f[j], x3r))
 
     def errors(self, get_block):
         """Volume-weighted L1 errors (UserWorkAfterLoop :190-370).  ``get_block(gid)``

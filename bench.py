@@ -486,6 +486,83 @@ def bench_sr_shear(args):
     print(json.dumps(out), flush=True)
 
 
+SMR_METRIC = ("zone-cycles/s on 1 MI355X, 3D MHD on a statically refined mesh "
+              "(VL2+PLM+HLLD+CT FP64, two levels, flux and EMF correction)")
+# <refinement1..5> of inputs/mhd/athinput.linear_wave3d (the reference's SMR regression mesh)
+SMR_REGIONS = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((0.3, 1.0, 1.0), (0.4, 1.1, 1.1), 1),
+               ((1.0, 1.0, 0.3), (1.1, 1.1, 0.4), 1), ((1.0, 0.3, 1.0), (1.1, 0.4, 1.1), 1),
+               ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+
+
+def bench_smr(args):
+    """The reference's SMR linear-wave mesh (athinput.linear_wave3d with its five refined
+    regions) on a root grid of n x n/2 x n/2 cells in MeshBlocks of bn^3: zone-cycles over
+    the active cells of every level, and the share of the cycle spent in the refinement
+    programs (flux correction, EMF correction, boundary exchange with prolongation)."""
+    from athena_amd.gpu import GpuMesh
+    from athena_amd.mesh import refined
+    from athena_amd.pgen.linear_wave import LinearWave, linear_wave_mesh
+    t0 = time.perf_counter()
+    m = refined(linear_wave_mesh(args.n, 0, bnx=(args.bn,) * 3), SMR_REGIONS)
+    lw = LinearWave(m, wave_flag=0)
+    d = GpuMesh(m)
+    for g in range(m.nblocks):
+        b = lw.block(g)
+        for k in ("u", "b1", "b2", "b3"):
+            d.set(g, k, b[k])
+    d.initialize()
+    levels = np.bincount(np.asarray(m.locs)[:, 0] - m.root_level).tolist()
+    cells = m.nblocks * args.bn ** 3
+    print("setup %.1f s: %d MeshBlocks of %d^3 (per level from the root: %s), %d cells"
+          % (time.perf_counter() - t0, m.nblocks, args.bn, levels, cells), file=sys.stderr,
+          flush=True)
+    d.set_graph(not args.no_graph)
+    d.run(args.warmup)
+    ev = HipEvents()
+    stream = d.stream()
+    e0, e1 = ev.new(), ev.new()
+    d.synchronize()
+    ev.record(e0, stream)
+    d.run(args.steps)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1)
+    zcs = args.steps * cells / (ms * 1e-3)
+    print("timed %d cycles: %.3f ms/cycle, %.3e zone-cycles/s" % (args.steps, ms / args.steps,
+                                                                   zcs), file=sys.stderr,
+          flush=True)
+    d.kernel_timing(True)
+    d.run(args.steps)
+    stats = d.kernel_stats()
+    d.kernel_timing(False)
+    tot = sum(v[0] for v in stats.values())
+    for k, (t, c) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        print("  %-20s %8.3f ms total %5d launches %8.4f ms avg %5.1f%%"
+              % (k, t, c, t / c, 100 * t / tot), file=sys.stderr)
+    smr_ms = sum(v[0] for k, v in stats.items() if k.startswith("smr_"))
+    dom = max(stats, key=lambda k: stats[k][0])
+    dom_ms, dom_n = stats[dom]
+    abytes = kernel_algorithmic_bytes(dom, m)
+    roof = {"kernel": dom, "bound": "hbm",
+            "achieved": abytes / (dom_ms / dom_n * 1e-3) / 1e9 if abytes else None,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": abytes / (dom_ms / dom_n * 1e-3) / HBM_PEAK if abytes else None,
+            "traffic": None, "avg_launch_ms": dom_ms / dom_n,
+            "algorithmic_bytes_per_launch": abytes}
+    out = {"metric": SMR_METRIC, "value": zcs, "unit": "zone-cycles/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "MHD fast linear wave on athinput.linear_wave3d's refined "
+                                  "mesh, root %dx%dx%d, MeshBlocks %d^3, five level-1 "
+                                  "regions" % (args.n, args.n // 2, args.n // 2, args.bn),
+                      "meshblocks": m.nblocks, "meshblocks_per_level": levels,
+                      "cells": cells, "hip_graph_cycle": not args.no_graph},
+           "kernels_ms_per_cycle": {k: v[0] / args.steps for k, v in stats.items()},
+           "smr_program_share": smr_ms / tot,
+           "roofline": roof, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline_sr(n=128, ncycles=2, min_seconds=10.0):
     """The oracle on a bounded sample of the SR shear layer: 128^3 as 64 MeshBlocks of
     32^3, PPM + HLLE, reference primitive recovery."""
@@ -533,11 +610,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="turbulence",
-                    choices=["turbulence", "sr_c2p", "ensemble", "sr_shear", "copy"],
+                    choices=["turbulence", "sr_c2p", "ensemble", "sr_shear", "smr", "copy"],
                     help="turbulence: the zone-cycles/s throughput case (default); "
                          "sr_c2p: batched SR primitive recovery with the learned guess; "
                          "ensemble: 48 blast waves of uneven sizes; sr_shear: the SR shear "
-                         "layer with the learned guess (use --steps 50); copy: three "
+                         "layer with the learned guess (use --steps 50); smr: the "
+                         "reference's statically refined linear-wave mesh; copy: three "
                          "512 MiB device copies, the calibration of the HBM counters")
     ap.add_argument("--members", type=int, default=48, help="ensemble members")
     ap.add_argument("--n", type=int, default=256, help="mesh cells per direction")
@@ -568,6 +646,8 @@ def main():
         return bench_sr_shear(args)
     if args.workload == "copy":
         return copy_calibration()
+    if args.workload == "smr":
+        return bench_smr(args)
 
     from athena_amd.gpu import GpuMesh
     from athena_amd.pgen.turbulence import Turbulence, turbulence_mesh
