@@ -1929,15 +1929,15 @@ __global__ void k_sr_c2p_batch(EosK eos, int n, double *cons, double *prim,
 }
 
 //========================================================================================
-// Static mesh refinement: one work-group per MeshBlock runs that block's step program
-// (built on the host by smr_plan.hpp) box by box, with a work-group barrier at every
-// BARRIER step.  A block writes only its own registers; it reads other blocks' active
-// data (boundary exchange, flux correction) or a snapshot of their EMFs (EMF correction),
-// which no work-group of the same launch writes.
+// Static mesh refinement: the step programs built on the host (smr_plan.hpp) run phase
+// by phase, one launch per phase over the steps of every MeshBlock in it, a work-group
+// per chunk of 256 elements of one step.  A step writes only its own block's registers;
+// it reads other blocks' active data (boundary exchange, flux correction) or a snapshot of
+// their EMFs (EMF correction), which nothing in the same launch writes.
 
 struct SmrArgs {
   const smrplan::Step *steps;
-  const int *offs;
+  const int *chunk_step, *chunk_first;
   double *u, *w, *bcc;              // the updated state; w takes prolongated primitives
   double *b[3];
   double *cu, *cw, *cbcc, *cb[3];   // coarse buffers: conserved, primitive, bcc, faces
@@ -2071,28 +2071,22 @@ __device__ __forceinline__ double smr_flux_restrict(const MeshK &M, const SmrArg
           f(k, j + 1, i + 1) * s11) / tarea;
 }
 
-// Work-group size of k_smr (one work-group per MeshBlock).  512 and 1024 measured no
-// faster on the two-level linear-wave mesh of 338 MeshBlocks of 32^3.
-constexpr int SMR_WG = 256;
-
-__global__ void __launch_bounds__(SMR_WG) k_smr(MeshK M, SmrArgs a) {
-  const int g = blockIdx.x;
+__global__ void __launch_bounds__(smrplan::CHUNK_THREADS) k_smr(MeshK M, SmrArgs a, int chunk0) {
+  const int ch = chunk0 + blockIdx.x;
+  const smrplan::Step st = a.steps[a.chunk_step[ch]];
+  const int g = st.blk;
   const SmrDims D = smr_dims(M);
   const refops::RGrid Fg(a.fco + g * a.fcl, M.nc1, M.nc2, M.nc3);
   const refops::RGrid Cg(a.cco + g * a.ccl, D.c1, D.c2, D.c3);
-  const int s_end = a.offs[g + 1];
-  for (int s = a.offs[g]; s < s_end; ++s) {
-    const smrplan::Step st = a.steps[s];
-    if (st.op == smrplan::BARRIER) {
-      __syncthreads();
-      continue;
-    }
+  {
     const bool nv5 = st.op == smrplan::CC_COPY || st.op == smrplan::CC_RESTRICT ||
                      st.op == smrplan::CC_TO_COARSE || st.op == smrplan::CC_OWN_RESTRICT ||
                      st.op == smrplan::PROLONG_CC || st.op == smrplan::FLUX_RESTRICT;
     const int nb = st.n[0] * st.n[1] * st.n[2];
     const int total = nv5 ? NHYDRO * nb : nb;
-    for (int t = threadIdx.x; t < total; t += SMR_WG) {
+    const int t_end = min(total, a.chunk_first[ch] + smrplan::CHUNK);
+    for (int t = a.chunk_first[ch] + static_cast<int>(threadIdx.x); t < t_end;
+         t += smrplan::CHUNK_THREADS) {
       const int n = t / nb;
       int q = t - n * nb;
       const int di = q % st.n[0];
@@ -2230,8 +2224,12 @@ struct ath_mesh {
   // static mesh refinement (ath_gpu_smr_mesh_create)
   bool smr = false;
   std::vector<smrplan::BlockInfo> smr_blocks;
-  smrplan::Step *smr_steps[3] = {nullptr, nullptr, nullptr};   // flux, EMF, boundary
-  int *smr_offs[3] = {nullptr, nullptr, nullptr};
+  // the flux, EMF and boundary programs: steps, per-chunk (step, first element), and the
+  // chunk offset of each phase (one launch per phase)
+  smrplan::Step *smr_steps[3] = {nullptr, nullptr, nullptr};
+  int *smr_chunk_step[3] = {nullptr, nullptr, nullptr};
+  int *smr_chunk_first[3] = {nullptr, nullptr, nullptr};
+  std::vector<int> smr_phases[3];
   double *cu = nullptr, *cw = nullptr, *cbcc = nullptr, *cb[3] = {nullptr, nullptr, nullptr};
   double *fco = nullptr, *cco = nullptr, *esnap[3] = {nullptr, nullptr, nullptr};
   long fcl = 0, ccl = 0;
@@ -2661,7 +2659,8 @@ void set_regs(ath_mesh *m, const Regs &r) {
 SmrArgs smr_args(const ath_mesh *m, int prog, double *u, double *const *b, double *w) {
   SmrArgs a{};
   a.steps = m->smr_steps[prog];
-  a.offs = m->smr_offs[prog];
+  a.chunk_step = m->smr_chunk_step[prog];
+  a.chunk_first = m->smr_chunk_first[prog];
   a.u = u;
   a.w = w;
   a.bcc = m->bcc;
@@ -2681,8 +2680,14 @@ SmrArgs smr_args(const ath_mesh *m, int prog, double *u, double *const *b, doubl
 int launch_smr(ath_mesh *m, int prog, const SmrArgs &a) {
   static const int kid[3] = {KID_SMR_FLUX, KID_SMR_EMF, KID_SMR_BND};
   ProfScope ps(m, kid[prog]);
-  hipLaunchKernelGGL(k_smr, dim3(m->M.nblocks), dim3(SMR_WG), 0, m->stream, m->M, a);
-  HIPCHK(hipGetLastError());
+  const std::vector<int> &ph = m->smr_phases[prog];
+  for (std::size_t k = 0; k + 1 < ph.size(); ++k) {
+    const int nch = ph[k + 1] - ph[k];
+    if (nch <= 0) continue;
+    hipLaunchKernelGGL(k_smr, dim3(nch), dim3(smrplan::CHUNK_THREADS), 0, m->stream, m->M, a,
+                       ph[k]);
+    HIPCHK(hipGetLastError());
+  }
   return 0;
 }
 
@@ -3142,19 +3147,35 @@ int smr_setup(ath_mesh *m, const std::vector<smrplan::BlockInfo> &sb, int root_l
   if (!rc) rc |= m->alloc(&m->fco, static_cast<long>(fco.size()));
   if (!rc) rc |= m->alloc(&m->cco, static_cast<long>(cco.size()));
   for (int q = 0; q < 3 && !rc; ++q) {
-    void *ps = nullptr, *po = nullptr;
-    const size_t nsb = sizeof(smrplan::Step) * std::max<size_t>(prog[q].steps.size(), 1);
-    const size_t nob = sizeof(int) * prog[q].offs.size();
-    if (hipMalloc(&ps, nsb) != hipSuccess) return fail("hipMalloc failed for SMR programs");
+    const smrplan::Phased ph = smrplan::to_phases(prog[q], nb);
+    const size_t nst = std::max<size_t>(ph.steps.size(), 1);
+    const size_t nch = std::max<size_t>(ph.chunk_step.size(), 1);
+    void *ps = nullptr, *pc = nullptr, *pf = nullptr;
+    if (hipMalloc(&ps, sizeof(smrplan::Step) * nst) != hipSuccess ||
+        hipMalloc(&pc, sizeof(int) * nch) != hipSuccess ||
+        hipMalloc(&pf, sizeof(int) * nch) != hipSuccess) {
+      if (ps) (void)hipFree(ps);
+      if (pc) (void)hipFree(pc);
+      if (pf) (void)hipFree(pf);
+      return fail("hipMalloc failed for SMR programs");
+    }
     m->allocs.push_back(ps);
-    if (hipMalloc(&po, nob) != hipSuccess) return fail("hipMalloc failed for SMR programs");
-    m->allocs.push_back(po);
+    m->allocs.push_back(pc);
+    m->allocs.push_back(pf);
     m->smr_steps[q] = static_cast<smrplan::Step *>(ps);
-    m->smr_offs[q] = static_cast<int *>(po);
-    if (!prog[q].steps.empty())
-      HIPCHK(hipMemcpyAsync(ps, prog[q].steps.data(), sizeof(smrplan::Step) * prog[q].steps.size(),
+    m->smr_chunk_step[q] = static_cast<int *>(pc);
+    m->smr_chunk_first[q] = static_cast<int *>(pf);
+    m->smr_phases[q] = ph.phase_chunks;
+    if (!ph.steps.empty()) {
+      HIPCHK(hipMemcpyAsync(ps, ph.steps.data(), sizeof(smrplan::Step) * ph.steps.size(),
                             hipMemcpyHostToDevice, m->stream));
-    HIPCHK(hipMemcpyAsync(po, prog[q].offs.data(), nob, hipMemcpyHostToDevice, m->stream));
+      HIPCHK(hipMemcpyAsync(pc, ph.chunk_step.data(), sizeof(int) * ph.chunk_step.size(),
+                            hipMemcpyHostToDevice, m->stream));
+      HIPCHK(hipMemcpyAsync(pf, ph.chunk_first.data(), sizeof(int) * ph.chunk_first.size(),
+                            hipMemcpyHostToDevice, m->stream));
+    }
+    // the uploads read the host vectors: finish them before ph goes out of scope
+    HIPCHK(hipStreamSynchronize(m->stream));
   }
   if (rc) return 1;
   HIPCHK(hipMemcpyAsync(m->fco, fco.data(), sizeof(double) * fco.size(), hipMemcpyHostToDevice,
@@ -3450,15 +3471,21 @@ int ath_gpu_smr_plan(const ath_mesh_desc *desc, const long long *locs, int nbloc
   const smrplan::Shape S = smrplan::make_shape(d.bnx, d.ng);
   smrplan::Program prog[3];
   try {
+    for (int q = 0; q < 3; ++q) prog[q].offs.push_back(0);
     for (int g = 0; g < nblocks; ++g) {
       smrplan::flux_program(S, sb, g, prog[0]);
       smrplan::emf_program(S, sb, g, prog[1]);
       smrplan::boundary_program(S, sb, g, prog[2]);
+      for (int q = 0; q < 3; ++q) prog[q].offs.push_back(static_cast<int>(prog[q].steps.size()));
     }
   } catch (const std::exception &e) {
     return fail(e.what());
   }
-  for (int q = 0; q < 3; ++q) nsteps[q] = static_cast<long>(prog[q].steps.size());
+  for (int q = 0; q < 3; ++q) {
+    const smrplan::Phased ph = smrplan::to_phases(prog[q], nblocks);
+    nsteps[q] = static_cast<long>(ph.steps.size());
+    nsteps[3 + q] = static_cast<long>(ph.phase_chunks.size()) - 1;
+  }
   const std::vector<smrplan::Nbr> &nb = sb[gid].nb;
   *nnbr = static_cast<int>(nb.size());
   for (int q = 0; q < *nnbr && q < cap && nbr_out; ++q) {

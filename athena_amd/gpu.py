@@ -758,13 +758,14 @@ class Block:
 
 def smr_plan(m: MeshDesc, gid=0):
     """Host-only check of a refined mesh (ath_gpu_smr_plan): returns (neighbour list of
-    block gid as (n, 9) int32 rows, step counts of the flux, EMF and boundary programs)."""
+    block gid as (n, 9) int32 rows, [step counts of the flux, EMF and boundary programs,
+    then their numbers of phases])."""
     L = load()
     desc = desc_of(m)
     locs = np.ascontiguousarray(m.locs, dtype=np.int64)
     out = np.zeros(9 * 64, dtype=np.int32)
     nn = ctypes.c_int()
-    steps = np.zeros(3, dtype=np.int64)
+    steps = np.zeros(6, dtype=np.int64)
     _check(L.ath_gpu_smr_plan(ctypes.byref(desc),
                               locs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(locs),
                               int(m.root_level), gid, _ip(out), 64, ctypes.byref(nn),

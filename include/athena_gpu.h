@@ -87,8 +87,9 @@ int ath_gpu_smr_block_level(ath_mesh_t mesh, int gid, int *level);
  * does and builds its step programs (every box checked against its arrays).  Writes
  * *nnbr and, up to cap entries, block gid's neighbour list in SearchAndSetNeighbors order
  * as 9 ints each (gid, level, ox1, ox2, ox3, type 1 face / 2 edge / 3 corner, fi1, fi2,
- * eid for edges or fid for faces), and nsteps[0..2] = the step counts of the flux, EMF and
- * boundary programs of the whole mesh. */
+ * eid for edges or fid for faces), nsteps[0..2] = the step counts of the flux, EMF and
+ * boundary programs of the whole mesh and nsteps[3..5] their numbers of phases (launches
+ * per stage). */
 int ath_gpu_smr_plan(const ath_mesh_desc *desc, const long long *locs, int nblocks,
                      int root_level, int gid, int *nbr_out, int cap, int *nnbr, long *nsteps);
 int ath_gpu_mesh_destroy(ath_mesh_t mesh);
