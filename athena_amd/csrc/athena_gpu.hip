@@ -290,6 +290,11 @@ struct FaceOut {
   double *flx, *ey, *ez, *wct;
 };
 
+template <int DIR, int ORDER, int RS>
+__device__ __forceinline__ void face_solve(const MeshK &M, int g, int k, int j, int i,
+                                           double *wl, double *wr, const FaceOut &o,
+                                           double dt, bool emit, double *fx);
+
 // One interface along DIR at face (k, j, i) of block g: reconstruction, the Riemann
 // solver, and the face EMFs and CT weight of that face written to o (when emit); the five
 // fluxes are returned in fx[] in conserved order (IDN, IM1, IM2, IM3, IEN).
@@ -340,6 +345,21 @@ __device__ __forceinline__ void face_flux(const MeshK &M, int g, int k, int j, i
       wr[v] = qmR;
     }
   }
+  face_solve<DIR, ORDER, RS>(M, g, k, j, i, wl, wr, o, dt, emit, fx);
+}
+
+// The rest of one interface once its states are reconstructed: the primitive floors
+// (PPM), the Riemann solver, the fluxes returned in fx[] in conserved order, and the face
+// EMFs and CT weight written to o (when emit).  wl and wr are in the rotated order of DIR.
+template <int DIR, int ORDER, int RS>
+__device__ __forceinline__ void face_solve(const MeshK &M, int g, int k, int j, int i,
+                                           double *wl, double *wr, const FaceOut &o,
+                                           double dt, bool emit, double *fx) {
+  constexpr int ivx = IVX + DIR;
+  constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  const int fc = (DIR == 0) ? i : ((DIR == 1) ? j : k);
+  const double *ct = ctab(M, g, DIR);
   if (ORDER == 3) {   // ApplyPrimitiveFloors on both reconstructed states
     wl[IDN] = (wl[IDN] > M.eos.dfloor) ? wl[IDN] : M.eos.dfloor;
     wl[IPR] = (wl[IPR] > M.eos.pfloor) ? wl[IPR] : M.eos.pfloor;
@@ -436,6 +456,181 @@ __global__ void __launch_bounds__(WG) k_flux3(MeshK M, const double *__restrict_
                                                const Clock *__restrict__ clk, TileK T) {
   flux3_body<ORDER, RS>(M, w, bcc, o1, o2, o3, clk->dt, xcd_logical_wg(M.xcd_remap & XCD_FLUX),
                         gridDim.x, T);
+}
+
+// Reconstruction of cell (k, j, i) along DIR: its low-face value qm[v] and high-face value
+// qp[v] for the seven face-state variables in the rotated order of DIR.  The interface at
+// index fc along DIR takes wl = qp of cell fc-1 and wr = qm of cell fc, the same
+// expressions face_flux evaluates (PLM: pf and mf of plm.cpp:149-155; PPM: ppm_cell).
+template <int DIR, int ORDER>
+__device__ __forceinline__ void recon_cell(const MeshK &M, int g, int k, int j, int i,
+                                           const double *__restrict__ wb,
+                                           const double *__restrict__ bb, double *qm,
+                                           double *qp) {
+  const long s = (DIR == 0) ? 1 : ((DIR == 1) ? M.nc1 : static_cast<long>(M.nc1) * M.nc2);
+  const long c = cidx(M, k, j, i);
+  constexpr int ivx = IVX + DIR;
+  constexpr int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  constexpr int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  constexpr int cby = (DIR == 0) ? IB2 : ((DIR == 1) ? IB3 : IB1);
+  constexpr int cbz = (DIR == 0) ? IB3 : ((DIR == 1) ? IB1 : IB2);
+  const int fc = (DIR == 0) ? i : ((DIR == 1) ? j : k);
+  const double *ct = ctab(M, g, DIR);
+  const int ncd = (DIR == 0) ? M.nc1 : ((DIR == 1) ? M.nc2 : M.nc3);
+#pragma unroll
+  for (int v = 0; v < NWAVE; ++v) {
+    const double *src;
+    if (v == IDN) src = wb;
+    else if (v == IVX) src = wb + ivx * M.ncell;
+    else if (v == IVY) src = wb + ivy * M.ncell;
+    else if (v == IVZ) src = wb + ivz * M.ncell;
+    else if (v == IPR) src = wb + IPR * M.ncell;
+    else if (v == IBY) src = bb + cby * M.ncell;
+    else src = bb + cbz * M.ncell;
+    if (ORDER == 1) {
+      qm[v] = qp[v] = src[c];
+    } else if (ORDER == 2) {
+      const double q0 = src[c];
+      const double d = vl_slope(src[c - s], q0, src[c + s]);
+      qp[v] = q0 + ct[ncd + fc] * d;
+      qm[v] = q0 - ct[2 * ncd + fc] * d;
+    } else {
+      double q[5];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) q[m] = src[c + (m - 2) * s];
+      ppm_cell(q, qm[v], qp[v]);
+    }
+  }
+}
+
+// Work split of the z-march flux kernel: each MeshBlock's flux box (ni x nj columns of
+// nk planes) in nseg segments of nearly equal height; positions run over (block, segment,
+// column), column = (j - js + 1) * ni + (i - is + 1).  A wavefront owns FZM_OWN of them
+// and its lane 0 repeats the previous wavefront's last position.
+constexpr int FZM_OWN = 63;
+
+struct FzmGeom {
+  int ni, nj, nk, nseg, hmax;   // box, segments per block, tallest segment
+  long ncol, npos, nwave;
+};
+
+__host__ __device__ __forceinline__ FzmGeom fzm_geom(const MeshK &M, int seg) {
+  FzmGeom G;
+  G.ni = M.nx1 + 2;
+  G.nj = M.nx2 + 2;
+  G.nk = M.nx3 + 2;
+  G.nseg = (G.nk + seg - 1) / seg;
+  G.hmax = (G.nk + G.nseg - 1) / G.nseg;
+  G.ncol = static_cast<long>(G.ni) * G.nj;
+  G.npos = G.ncol * G.nseg * M.nblocks;
+  G.nwave = (G.npos + FZM_OWN - 1) / FZM_OWN;
+  return G;
+}
+
+// Position of lane `lane` of wavefront `wv`: block g, column (j, i) and planes k0..k1;
+// false for lanes past the last position (they compute a clamped copy and store nothing).
+__host__ __device__ __forceinline__ bool fzm_pos(const MeshK &M, const FzmGeom &G, long wv,
+                                                 int lane, int &g, int &j, int &i, int &k0,
+                                                 int &k1) {
+  long p = wv * FZM_OWN + lane - 1;
+  const bool in = p < G.npos;
+  p = (p < 0) ? 0 : (in ? p : G.npos - 1);
+  const long bs = p / G.ncol;
+  const int col = static_cast<int>(p - bs * G.ncol);
+  g = static_cast<int>(bs / G.nseg);
+  const int sg = static_cast<int>(bs - static_cast<long>(g) * G.nseg);
+  j = M.js - 1 + col / G.ni;
+  i = M.is - 1 + col % G.ni;
+  k0 = M.ks - 1 + (sg * G.nk) / G.nseg;
+  k1 = M.ks - 1 + ((sg + 1) * G.nk) / G.nseg - 1;
+  return in;
+}
+
+// Kernel 1, z-march form (tile option of the flux kind).  Each lane owns a column (j, i)
+// of the flux box of one MeshBlock and walks a segment of its planes, computing at each
+// plane the same faces as k_flux3 for cell (k, j, i).  Lanes run i fastest over
+// consecutive columns, so the lane before holds cell (k, j, i-1) whenever the x1 face
+// (k, j, i) exists (i >= is): every lane reconstructs its own cell along x1 and passes its
+// high-face states one lane up, and the x1 face takes its left states from there.  Along
+// x3 the lane keeps the high-face states of cell (k-1, j, i) from the previous plane.  So
+// each cell is reconstructed once along x1 and x3 instead of twice; along x2 both cells of
+// the face are reconstructed by the lane, as in k_flux3.  Every state is the value
+// face_flux computes for that face, so the fluxes, EMFs and weights are the same bits.
+// Bounded to 3 wavefronts per SIMD (at most 168 VGPRs, k_flux3's occupancy): unbounded the
+// compiler keeps more values across the three solves and drops to 2.
+template <int ORDER, int RS>
+__global__ void __launch_bounds__(WG, 3) k_flux_zm(MeshK M, const double *__restrict__ w,
+                                                 const double *__restrict__ bcc, FaceOut o1,
+                                                 FaceOut o2, FaceOut o3,
+                                                 const Clock *__restrict__ clk, int seg) {
+  const FzmGeom G = fzm_geom(M, seg);
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const long wv = static_cast<long>(blockIdx.x) * (WG / 64) + (threadIdx.x >> 6);
+  if (wv >= G.nwave) return;   // whole wavefronts only
+  int g0, j0, i0, k0, k1;
+  const bool in = fzm_pos(M, G, wv, lane, g0, j0, i0, k0, k1);
+  const bool own = in && lane > 0;
+  const double dt = clk->dt;
+
+  double p3[NWAVE] = {};   // high-face x3 states of cell (k-1, j, i)
+  if (k0 >= M.ks) {
+    double m3[NWAVE];
+    recon_cell<2, ORDER>(M, g0, k0 - 1, j0, i0, w + static_cast<long>(g0) * NHYDRO * M.ncell,
+                         bcc + static_cast<long>(g0) * 3 * M.ncell, m3, p3);
+  }
+#pragma unroll 1
+  for (int h = 0; h < G.hmax; ++h) {
+    const bool live = own && k0 + h <= k1;
+    int k = (k0 + h <= k1) ? k0 + h : k1;
+    int g = g0, j = j0, i = i0;
+    // opaque indices: the stencil addresses are formed afresh at each plane instead of
+    // being held across the loop as per-lane pointers, which would take dozens of
+    // registers and cost occupancy
+    asm volatile("" : "+v"(k), "+v"(g), "+v"(j), "+v"(i));
+    const double *wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+    const double *bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+    double wl[NWAVE], wr[NWAVE], fx[NHYDRO];
+    // x1: this lane's cell, left states from the lane before
+    recon_cell<0, ORDER>(M, g, k, j, i, wb, bb, wr, wl);
+#pragma unroll
+    for (int v = 0; v < NWAVE; ++v) wl[v] = __shfl_up(wl[v], 1, 64);
+    if (live && i >= M.is) {
+      face_solve<0, ORDER, RS>(M, g, k, j, i, wl, wr, o1, dt, true, fx);
+      const long fi = f1idx(M, k, j, i);
+      double *fb = o1.flx + static_cast<long>(g) * NHYDRO * M.nf1;
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) fb[n * M.nf1 + fi] = fx[n];
+    }
+    // x2: both cells (indices made opaque again, so that the x1 addresses are not kept
+    // alive through the x1 solve for reuse here)
+    asm volatile("" : "+v"(k), "+v"(g), "+v"(j), "+v"(i));
+    wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+    bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+    if (live && j >= M.js) {
+      double t[NWAVE];
+      recon_cell<1, ORDER>(M, g, k, j - 1, i, wb, bb, t, wl);
+      recon_cell<1, ORDER>(M, g, k, j, i, wb, bb, wr, t);
+      face_solve<1, ORDER, RS>(M, g, k, j, i, wl, wr, o2, dt, true, fx);
+      const long fi = f2idx(M, k, j, i);
+      double *fb = o2.flx + static_cast<long>(g) * NHYDRO * M.nf2;
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) fb[n * M.nf2 + fi] = fx[n];
+    }
+    // x3: low states of this cell, high states of the cell below carried in p3
+    asm volatile("" : "+v"(k), "+v"(g), "+v"(j), "+v"(i));
+    wb = w + static_cast<long>(g) * NHYDRO * M.ncell;
+    bb = bcc + static_cast<long>(g) * 3 * M.ncell;
+    recon_cell<2, ORDER>(M, g, k, j, i, wb, bb, wr, wl);
+    if (live && k >= M.ks) {
+      face_solve<2, ORDER, RS>(M, g, k, j, i, p3, wr, o3, dt, true, fx);
+      const long fi = f3idx(M, k, j, i);
+      double *fb = o3.flx + static_cast<long>(g) * NHYDRO * M.nf3;
+#pragma unroll
+      for (int n = 0; n < NHYDRO; ++n) fb[n * M.nf3 + fi] = fx[n];
+    }
+#pragma unroll
+    for (int v = 0; v < NWAVE; ++v) p3[v] = wl[v];
+  }
 }
 
 //========================================================================================
@@ -2345,12 +2540,15 @@ long kind_chunks(const MeshK &M, int kind, TileK T) {
 }
 
 // Tile shapes the autotuner chooses from (index 0: linear chunks); indices 6 and 7 are
-// the corner kernel's z-march over segments of 8 and 16 planes.
+// the z-march over segments of 8 and 16 planes: the corner kernel's (k_corner_march) and,
+// on single meshes, the flux kernel's (k_flux_zm).
 constexpr int NTILES = 8;
 constexpr TileK TILES[NTILES] = {{0, 0, 0}, {64, 4, 1}, {32, 8, 1}, {16, 16, 1}, {16, 4, 4},
                                  {8, 8, 4}, {-1, 0, 8}, {-1, 0, 16}};
 
 bool tile_ok(int kind, int t) { return TILES[t].tx >= 0 || kind == KK_CORNER; }
+// A single mesh also runs the flux kernel's z-march (k_flux_zm); ensembles do not.
+bool mesh_tile_ok(int kind, int t) { return tile_ok(kind, t) || kind == KK_FLUX; }
 
 // Dynamic LDS bytes of a corner launch with tile T.
 size_t corner_lds(const MeshK &M, TileK T) {
@@ -2426,12 +2624,21 @@ int launch_flux_args(ath_mesh *m, int order, const double *w, const double *bcc,
                      const FaceOut *fo) {
   const MeshK &M = m->M;
   const int rs = solver_code(m->d);
-  const dim3 grid = kind_grid(m, KK_FLUX);
   const TileK T = m->tile[KK_FLUX];
+  const bool zm = T.tx < 0;   // z-march over segments of T.tz planes
+  const dim3 grid = zm ? dim3(static_cast<unsigned>((fzm_geom(M, T.tz).nwave + WG / 64 - 1) /
+                                                    (WG / 64)))
+                       : kind_grid(m, KK_FLUX);
   ProfScope ps(m, KID_FLUX3 + (order - 1));
 #define LF(O, R)                                                                          \
-  hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], fo[1], \
-                     fo[2], m->clk, T)
+  do {                                                                                    \
+    if (zm)                                                                               \
+      hipLaunchKernelGGL((k_flux_zm<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], \
+                         fo[1], fo[2], m->clk, T.tz);                                     \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], \
+                         fo[1], fo[2], m->clk, T);                                        \
+  } while (0)
 #define LF_O(R)                         \
   do {                                  \
     if (order == 1) LF(1, R);           \
@@ -3864,6 +4071,27 @@ int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n) {
   return 0;
 }
 
+int ath_gpu_flux_march_position(const int *shape, int seg, long wave, int lane, int *pos,
+                                long *nwave) {
+  if (!shape || !pos || !nwave) return fail("null argument");
+  if (shape[0] < 1 || shape[1] < 1 || shape[2] < 1 || shape[3] < 1 || shape[4] < 1 ||
+      seg < 1)
+    return fail("bad argument");
+  MeshK M{};
+  M.nx1 = shape[0]; M.nx2 = shape[1]; M.nx3 = shape[2];
+  M.ng = shape[3];
+  M.nblocks = shape[4];
+  M.is = M.js = M.ks = M.ng;
+  M.ie = M.ng + M.nx1 - 1; M.je = M.ng + M.nx2 - 1; M.ke = M.ng + M.nx3 - 1;
+  const FzmGeom G = fzm_geom(M, seg);
+  *nwave = G.nwave;
+  if (wave < 0 || wave >= G.nwave || lane < 0 || lane > 63) return fail("position out of range");
+  int g, j, i, k0, k1;
+  pos[0] = fzm_pos(M, G, wave, lane, g, j, i, k0, k1) ? 1 : 0;
+  pos[1] = g; pos[2] = j; pos[3] = i; pos[4] = k0; pos[5] = k1; pos[6] = G.hmax;
+  return 0;
+}
+
 int ath_gpu_ghost_position(int nx, int ng, int array, int t, int *pos) {
   if (!pos) return fail("null argument");
   if (nx < 1 || ng < 1 || array < 0 || array > 3) return fail("bad argument");
@@ -3883,7 +4111,8 @@ int ath_gpu_mesh_set_tiles(ath_mesh_t m, const int *tile_idx) {
   for (int k = 0; k < KK_N; ++k)
     if (tile_idx[k] < 0 || tile_idx[k] >= NTILES) return fail("tile index out of range");
   for (int k = 0; k < KK_N; ++k)
-    if (!tile_ok(k, tile_idx[k])) return fail("the z-march tile applies to the corner kernel only");
+    if (!mesh_tile_ok(k, tile_idx[k]))
+      return fail("the z-march tiles apply to the flux and corner kernels only");
   HIPCHK(hipStreamSynchronize(m->stream));
   for (int k = 0; k < KK_N; ++k) m->tile[k] = TILES[tile_idx[k]];
   return 0;

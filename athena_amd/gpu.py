@@ -46,7 +46,7 @@ EXPORTS = ["ath_gpu_mesh_create", "ath_gpu_mesh_destroy", "ath_gpu_nblocks",
            "ath_gpu_block_new_dt", "ath_gpu_block_new_dt_noalloc",
            "ath_gpu_mesh_set_fused", "ath_gpu_mesh_get_fused", "ath_gpu_flux_div_plan",
            "ath_gpu_mesh_set_graph", "ath_gpu_smr_mesh_create", "ath_gpu_smr_block_level",
-           "ath_gpu_smr_plan"]
+           "ath_gpu_smr_plan", "ath_gpu_flux_march_position"]
 NKERNELS = 26
 
 
@@ -160,6 +160,8 @@ def load():
     L.ath_gpu_mesh_autotune.argtypes = [P, I, D]
     L.ath_gpu_kernel_box.argtypes = [I, I, I, IP, IP]
     L.ath_gpu_ghost_position.argtypes = [I, I, I, I, IP]
+    L.ath_gpu_flux_march_position.argtypes = [IP, I, ctypes.c_long, I, IP,
+                                              ctypes.POINTER(ctypes.c_long)]
     L.ath_gpu_restrict_cc.argtypes = [IP, I, I, P, P, P, IP, P]
     L.ath_gpu_restrict_fc.argtypes = [IP, I, I, P, P, P, IP, P]
     L.ath_gpu_prolongate_cc.argtypes = [IP, I, I, P, P, P, P, IP, P]
@@ -660,6 +662,18 @@ def ghost_position(nx, ng, array, t):
     pos = np.zeros(9, dtype=np.int32)
     _check(load().ath_gpu_ghost_position(int(nx), int(ng), int(array), int(t), _ip(pos)))
     return pos
+
+
+def flux_march_position(shape, seg, wave, lane):
+    """Lane ``lane`` of wavefront ``wave`` of the flux kernel's z-march on MeshBlocks of
+    shape (nx1, nx2, nx3, ng, nblocks) with segments of about ``seg`` planes:
+    ((in, block, j, i, k0, k1, tallest segment), wavefronts per launch); no device call."""
+    pos = np.zeros(7, dtype=np.int32)
+    nw = ctypes.c_long(0)
+    sh = np.asarray(shape, dtype=np.int32)
+    _check(load().ath_gpu_flux_march_position(_ip(sh), int(seg), int(wave), int(lane),
+                                              _ip(pos), ctypes.byref(nw)))
+    return pos, nw.value
 
 
 class Block:

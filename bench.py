@@ -629,6 +629,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every kernel of a cycle eagerly instead of replaying the "
                          "cycle as one captured HIP graph (turbulence)")
+    ap.add_argument("--flux-tile", type=int, default=-1,
+                    help="flux kernel tile index after tuning (turbulence; 6 and 7: the "
+                         "z-march over segments of 8 and 16 planes; -1: keep)")
     ap.add_argument("--fused-rows", type=int, default=0,
                     help="rows per band of the optional fused flux + divergence kernel "
                          "(turbulence, sr_shear; 0: separate flux and update kernels, the "
@@ -668,6 +671,8 @@ def main():
           flush=True)
     if not args.no_autotune:
         d.autotune(reps=3)
+    if args.flux_tile >= 0:
+        d.set_tiles([args.flux_tile] + list(d.tiles())[1:])
     tiles = [TILE_NAMES[t] for t in d.tiles()]
     print("%s tiles (flux, corner, update, cons2prim): %s"
           % ("default" if args.no_autotune else "autotuned", tiles), file=sys.stderr,

@@ -335,13 +335,23 @@ int ath_gpu_ensemble_stream(ath_ensemble_t ens, void **stream);
 
 /* Tile shape (index as for ensembles) of each kernel kind of a single mesh, and the
  * same autotuner for a single mesh (its state is left as it was; periodic meshes only,
- * like ensembles). */
+ * like ensembles).  A single mesh also accepts indices 6 and 7 for the flux kind: the
+ * flux z-march (each cell reconstructed once along x1 and x3, same bits), which the
+ * autotuner does not try. */
 int ath_gpu_mesh_set_tiles(ath_mesh_t mesh, const int *tile_idx);
 int ath_gpu_mesh_get_tiles(ath_mesh_t mesh, int *tile_idx);
 int ath_gpu_mesh_autotune(ath_mesh_t mesh, int reps, double *best_ms);
 /* Position box (first index lo[3], extent n[3]) of kernel kind 0..3 in a MeshBlock of
  * nx^3 active cells and ng ghosts, for host-side bounds checks; no device call. */
 int ath_gpu_kernel_box(int kind, int nx, int ng, int *lo, int *n);
+/* Lane `lane` of wavefront `wave` of the flux kernel's z-march (flux tile options 6, 7)
+ * on nblocks MeshBlocks of shape = {nx1, nx2, nx3, ng, nblocks} with segments of about
+ * seg planes: pos = {owned or repeated (1) / past the end (0), block, j, i, first plane,
+ * last plane, planes of the tallest segment}; *nwave = wavefronts per launch.  Lane 0
+ * repeats the previous wavefront's last position.  For host-side checks; no device call;
+ * no counterpart in the reference (a launch-geometry query). */
+int ath_gpu_flux_march_position(const int *shape, int seg, long wave, int lane, int *pos,
+                                long *nwave);
 /* Ghost position number t of the same-level exchange kernel in a MeshBlock of nx^3
  * cells and ng ghosts, for array 0 (cell-centred) or 1..3 (x1f, x2f, x3f face field):
  * pos = {i, j, k, source i, source j, source k, neighbour offsets p1, p2, p3}, the

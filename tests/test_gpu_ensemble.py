@@ -188,7 +188,7 @@ def test_single_mesh_tiles_and_autotune(solo):
         assert list(d3.tiles()) == tiles
         d3.run(4)
         assert_same(snapshot(d3), snaps[3][2])
-    for bad in ([6, 0, 0, 0], [0, 0, 7, 0]):   # the z-march is for the corner kernel only
+    for bad in ([0, 0, 7, 0], [0, 0, 0, 6]):   # the z-march is for fluxes and corners only
         with pytest.raises(AthGpuError):
             fresh([(m, ic)])[0].set_tiles(bad)
 
