@@ -556,10 +556,10 @@ __host__ __device__ __forceinline__ bool fzm_pos(const MeshK &M, const FzmGeom &
 // each cell is reconstructed once along x1 and x3 instead of twice; along x2 both cells of
 // the face are reconstructed by the lane, as in k_flux3.  Every state is the value
 // face_flux computes for that face, so the fluxes, EMFs and weights are the same bits.
-// Bounded to 3 wavefronts per SIMD (at most 168 VGPRs, k_flux3's occupancy): unbounded the
-// compiler keeps more values across the three solves and drops to 2.
+// Not bounded to k_flux3's 3 wavefronts per SIMD: at 168 VGPRs the spills cost more
+// (PLM launch 3.40 against 2.84 ms at 256^3) than the third wavefront gains.
 template <int ORDER, int RS>
-__global__ void __launch_bounds__(WG, 3) k_flux_zm(MeshK M, const double *__restrict__ w,
+__global__ void __launch_bounds__(WG) k_flux_zm(MeshK M, const double *__restrict__ w,
                                                  const double *__restrict__ bcc, FaceOut o1,
                                                  FaceOut o2, FaceOut o3,
                                                  const Clock *__restrict__ clk, int seg) {
