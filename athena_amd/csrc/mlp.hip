@@ -6,7 +6,7 @@
 // the batch, Adam), and inferred in BF16 on MFMA (v_mfma_f32_32x32x8_bf16) inside the
 // primitive-recovery kernel, where it seeds an FP64 Newton iteration on the reference's
 // fixed-point residual; the reference Newman-Hamlin iteration stays the fallback and
-// the judge of the result.  No counterpart in the reference.
+// its residual test decides convergence.  No counterpart in the reference.
 #include <hip/hip_runtime.h>
 
 #include <cmath>

@@ -1,6 +1,6 @@
 """GPU parity: the device library against the CPU oracle on identical inputs.
 
-Bar: relative L1 <= 1e-12 per variable after the run; the
+Bar: relative L1 <= 1e-12 per variable after the run (the stated FP64 tolerance); the
 Newtonian path is expected to be bit-identical (same operation order, no FMA
 contraction, IEEE division/sqrt), which is reported and, for the single-block linear
 wave, asserted.  Also: the reference's own linear-wave errors reproduced on the GPU,

@@ -45,7 +45,7 @@ def _parity(m, blocks, ncycles):
     o.run(ncycles)
     d.run(ncycles)
     # u_z and B_z of the shock tubes vanish analytically; with HLLD their transverse
-    # round-off (see ZERO_TOL) is judged on the scale of the register's other components
+    # round-off (see ZERO_TOL) is measured on the scale of the register's other components
     res = compare(m, lambda g, k: d.get(g, k), lambda g, k: o.get(g, k), floor=1e-12)
     for k, (r, eq) in sorted(res.items()):
         print("  %-5s relL1=%.3e bitwise=%s" % (k, r, eq), flush=True)
