@@ -4171,6 +4171,39 @@ int ath_gpu_mesh_autotune(ath_mesh_t m, int reps, double *best_ms) {
       m->tile[k] = TILES[bt];
       if (best_ms) best_ms[k] = best / (reps > 0 ? reps : 3);
     }
+    // the flux z-march exists for single meshes only: time the chosen flux tile and the
+    // z-march tiles with the mesh's own flux launch at its reconstruction order (the
+    // launches write only the flux, EMF and weight registers)
+    if (!rc) {
+      HIPCHK(hipStreamSynchronize(e->stream));
+      const int nrep = reps > 0 ? reps : 3;
+      auto time_flux = [&](TileK T, float *ms) {
+        m->tile[KK_FLUX] = T;
+        if (launch_flux(m, m->d.xorder)) return 1;   // warm-up
+        if (hipEventRecord(a, m->stream) != hipSuccess) return fail("hipEventRecord");
+        for (int r = 0; r < nrep; ++r)
+          if (launch_flux(m, m->d.xorder)) return 1;
+        if (hipEventRecord(b, m->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
+          return fail("event");
+        (void)hipEventElapsedTime(ms, a, b);
+        return 0;
+      };
+      const TileK chosen = m->tile[KK_FLUX];
+      TileK best = chosen;
+      float tbest = 0.0f;
+      rc = time_flux(chosen, &tbest);
+      for (int t = 0; t < NTILES && !rc; ++t) {
+        if (tile_ok(KK_FLUX, t) || !mesh_tile_ok(KK_FLUX, t)) continue;   // z-march tiles
+        float tz = 0.0f;
+        rc = time_flux(TILES[t], &tz);
+        if (!rc && tz < tbest) {
+          tbest = tz;
+          best = TILES[t];
+        }
+      }
+      m->tile[KK_FLUX] = best;
+      if (!rc && best_ms && (best.tx < 0)) best_ms[KK_FLUX] = tbest / nrep;
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (!rc) {

@@ -108,3 +108,26 @@ def test_ensembles_refuse_the_flux_march():
     e = Ensemble([d])
     with pytest.raises(AthGpuError, match="corner"):
         e.set_tiles([6, 6, 0, 0], [0, 0, 0, 0])
+
+
+def test_single_mesh_autotune_times_the_flux_march_and_keeps_the_bits():
+    # PPM, where the z-march can win; the tuned mesh's state is unchanged by tuning and
+    # its run is bit for bit the default run whatever flux tile was kept
+    m, ic = _turb(32, 16, xorder=3, ng=3)
+    print("case autotune with the flux march, turbulence 32^3/16^3 PPM HLLD", flush=True)
+    ref = snapshot(run(m, ic, 5))
+    d = GpuMesh(m)
+    d.load([ic.block(g) for g in range(m.nblocks)])
+    d.initialize()
+    before = snapshot(d)
+    d.autotune(reps=2)
+    print("  chosen tiles %s" % list(d.tiles()), flush=True)
+    after = snapshot(d)
+    assert after[1] == before[1]
+    for key in before[0]:
+        assert np.array_equal(after[0][key], before[0][key]), key
+    d.run(5)
+    s, c = snapshot(d)
+    assert c == ref[1]
+    for key in s:
+        assert np.array_equal(s[key], ref[0][key]), key
