@@ -336,8 +336,9 @@ int ath_gpu_ensemble_stream(ath_ensemble_t ens, void **stream);
 /* Tile shape (index as for ensembles) of each kernel kind of a single mesh, and the
  * same autotuner for a single mesh (its state is left as it was; periodic meshes only,
  * like ensembles).  A single mesh also accepts indices 6 and 7 for the flux kind: the
- * flux z-march (each cell reconstructed once along x1 and x3, same bits); the single-mesh
- * autotuner times it against its chosen flux tile at the mesh's reconstruction order. */
+ * flux z-march for its PLM and PPM launches (each cell reconstructed once along x1 and x3,
+ * same bits; donor-cell launches keep linear chunks); the single-mesh autotuner times it
+ * against its chosen flux tile at the mesh's reconstruction order. */
 int ath_gpu_mesh_set_tiles(ath_mesh_t mesh, const int *tile_idx);
 int ath_gpu_mesh_get_tiles(ath_mesh_t mesh, int *tile_idx);
 int ath_gpu_mesh_autotune(ath_mesh_t mesh, int reps, double *best_ms);

@@ -3,9 +3,10 @@ lane walks a segment of about 8 or 16 planes of one column, reconstructs its cel
 along x1, passing the high-face states one lane up, and once along x3, carrying them to
 the next plane).  Every face state is the value k_flux3 computes for that face, so whole
 runs are bit for bit the default runs, which the parity tests tie to the CPU oracle; one
-case is also compared with the oracle directly.  Cases cover donor cell, PLM and PPM,
-every Riemann solver, NGHOST 2-4, MeshBlocks of unequal sides and the smallest ones,
-outflow faces, and segments taller than the box."""
+case is also compared with the oracle directly.  Cases cover PLM and PPM (donor-cell
+launches keep linear chunks under these tiles), every Riemann solver, NGHOST 2-4,
+MeshBlocks of unequal sides and the smallest ones, outflow faces, and segments taller
+than the box."""
 import numpy as np
 import pytest
 
