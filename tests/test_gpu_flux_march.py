@@ -5,8 +5,8 @@ the next plane).  Every face state is the value k_flux3 computes for that face, 
 runs are bit for bit the default runs, which the parity tests tie to the CPU oracle; one
 case is also compared with the oracle directly.  Cases cover PLM and PPM (donor-cell
 launches keep linear chunks under these tiles), every Riemann solver, NGHOST 2-4,
-MeshBlocks of unequal sides and the smallest ones, outflow faces, and segments taller
-than the box."""
+MeshBlocks of unequal sides and the smallest ones, outflow faces, a statically refined
+mesh, and segments taller than the box."""
 import numpy as np
 import pytest
 
@@ -39,6 +39,16 @@ def _sr_hlld():
     return m, ShearLayer(m)
 
 
+def _smr():
+    # the reference's refined linear-wave mesh (two levels, 72 MeshBlocks of 8^3)
+    from athena_amd.mesh import refined
+    regions = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((0.3, 1.0, 1.0), (0.4, 1.1, 1.1), 1),
+               ((1.0, 1.0, 0.3), (1.1, 1.1, 0.4), 1), ((1.0, 0.3, 1.0), (1.1, 0.4, 1.1), 1),
+               ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+    m = refined(linear_wave_mesh(32, 0, bnx=(8, 8, 8)), regions)
+    return m, LinearWave(m, wave_flag=0)
+
+
 def _sr_ppm():
     m = shear_mesh(32, 16, xorder=3, ng=3)
     return m, ShearLayer(m)
@@ -55,6 +65,7 @@ CASES = {
     "turb_ppm_hlld_outflow": lambda: _turb(16, 8, xorder=3, ng=3, bc=OUTFLOW),
     "sr_shear_ppm_hlle": _sr_ppm,
     "sr_shear_plm_hlld": _sr_hlld,
+    "smr_linwave_plm_hlld": _smr,
 }
 
 
