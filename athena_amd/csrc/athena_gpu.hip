@@ -2633,25 +2633,25 @@ int launch_flux_args(ath_mesh *m, int order, const double *w, const double *bcc,
                        : kind_grid(m, KK_FLUX);
   ProfScope ps(m, KID_FLUX3 + (order - 1));
 #define LF(O, R)                                                                          \
-  do {                                                                                    \
-    if (zm)                                                                               \
-      hipLaunchKernelGGL((k_flux_zm<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], \
-                         fo[1], fo[2], m->clk, T.tz);                                     \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], \
-                         fo[1], fo[2], m->clk, T);                                        \
-  } while (0)
-#define LF_O(R)                         \
-  do {                                  \
-    if (order == 1) LF(1, R);           \
-    else if (order == 2) LF(2, R);      \
-    else LF(3, R);                      \
+  hipLaunchKernelGGL((k_flux3<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0], fo[1], \
+                     fo[2], m->clk, T)
+#define LZ(O, R)                                                                          \
+  hipLaunchKernelGGL((k_flux_zm<O, R>), grid, dim3(WG), 0, m->stream, M, w, bcc, fo[0],   \
+                     fo[1], fo[2], m->clk, T.tz)
+#define LF_O(R)                                    \
+  do {                                             \
+    if (order == 1) LF(1, R);                      \
+    else if (order == 2 && zm) LZ(2, R);           \
+    else if (order == 2) LF(2, R);                 \
+    else if (zm) LZ(3, R);                         \
+    else LF(3, R);                                 \
   } while (0)
   if (rs == 0) LF_O(0);
   else if (rs == 1) LF_O(1);
   else if (rs == 2) LF_O(2);
   else LF_O(3);
 #undef LF_O
+#undef LZ
 #undef LF
   HIPCHK(hipGetLastError());
   return 0;
