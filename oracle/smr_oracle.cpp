@@ -34,11 +34,12 @@
 // order that depends on task scheduling and changes only the last bit.
 //
 // Parity pinning (tests/test_smr_oracle.py): on a mesh without refined blocks it agrees
-// with the uniform restatement (athena_oracle.cpp) to round-off; a uniform state stays
-// uniform bit for bit; total mass, momentum and energy are conserved and div(B) stays at
-// round-off through coarse/fine boundaries (they would not without the flux and EMF
-// corrections); the reference's SMR linear-wave errors stay under the thresholds of
-// tst/regression/scripts/tests/mhd/mhd_linwave.py:122-153 and converge.
+// with the uniform restatement (athena_oracle.cpp) to round-off in every active cell and
+// face; volume-weighted mass and energy stay conserved across coarse/fine boundaries and
+// fail to when the flux correction is left out; the two sides of every MeshBlock face
+// hold the same magnetic flux and fail to when the EMF correction is left out; the
+// reference's SMR fast-wave errors stay under the thresholds of
+// tst/regression/scripts/tests/mhd/mhd_linwave.py:122-135 and converge.
 //========================================================================================
 #include <array>
 #include <cstdint>
