@@ -34,14 +34,14 @@ def test_python_mirror_struct_matches_header_order():
     assert names == fields, (names, fields)
 
 
-def build_caller(outdir):
-    """Compile and link tests/abi_caller.cpp with g++ against the header and the library
+def build_caller(outdir, name="abi_caller"):
+    """Compile and link tests/<name>.cpp with g++ against the header and the library
     (host code only: hipMalloc and friends come from the HIP runtime library)."""
-    exe = os.path.join(outdir, "abi_caller")
+    exe = os.path.join(outdir, name)
     libdir = os.path.dirname(gpu.LIBPATH)
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
            "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "tests", "abi_caller.cpp"), "-o", exe,
+           os.path.join(ROOT, "tests", name + ".cpp"), "-o", exe,
            "-L" + libdir, "-lathena_gpu", "-Wl,-rpath," + libdir,
            "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True, capture_output=True, timeout=300)
@@ -51,6 +51,8 @@ def build_caller(outdir):
 def test_cpp_caller_compiles_and_links(tmp_path):
     # the header is valid C++ for an outside caller and every entry it uses links
     exe = build_caller(str(tmp_path))
+    assert os.path.exists(exe)
+    exe = build_caller(str(tmp_path), "abi_caller_smr")
     assert os.path.exists(exe)
 
 

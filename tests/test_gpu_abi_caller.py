@@ -12,9 +12,9 @@ import numpy as np
 import pytest
 
 from athena_amd.gpu import LIBPATH
-from athena_amd.mesh import BOUNDARY, RSOLVER, block_coords
+from athena_amd.mesh import BOUNDARY, RSOLVER, block_coords, refined
 from athena_amd.pgen.linear_wave import LinearWave, linear_wave_mesh
-from oracle_bind import OracleMesh
+from oracle_bind import OracleMesh, OracleSmrMesh
 from test_abi import build_caller
 
 pytestmark = pytest.mark.gpu
@@ -79,3 +79,46 @@ def test_cpp_caller_matches_oracle(tmp_path):
         assert np.array_equal(got, o.get(0, k)), ("block", k)
     assert take((1,))[0] == new_dt
     assert pos == raw.size
+
+
+def test_cpp_caller_on_a_refined_mesh_matches_oracle(tmp_path):
+    # tests/abi_caller_smr.cpp follows the refined-mesh binding of the integration notes:
+    # the tree's block list (Mesh::loclist order) into ath_gpu_smr_mesh_create, registers
+    # per gid, Mesh::Initialize, ath_gpu_cycle; the reference's refined linear-wave mesh
+    exe = build_caller(str(tmp_path), "abi_caller_smr")
+    regions = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((0.3, 1.0, 1.0), (0.4, 1.1, 1.1), 1),
+               ((1.0, 1.0, 0.3), (1.1, 1.1, 0.4), 1), ((1.0, 0.3, 1.0), (1.1, 0.4, 1.1), 1),
+               ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+    m = refined(linear_wave_mesh(32, 0, bnx=(8, 8, 8)), regions)
+    lw = LinearWave(m, wave_flag=0, amp=1e-6)
+    blocks = [lw.block(g) for g in range(m.nblocks)]
+    ncycles = 3
+    inp, outp = str(tmp_path / "in_smr.bin"), str(tmp_path / "out_smr.bin")
+    with open(inp, "wb") as f:
+        f.write(struct.pack("=12i", *m.nx, *m.bnx, m.ng, m.xorder, RSOLVER[m.rsolver],
+                            ncycles, m.nblocks, m.root_level))
+        f.write(struct.pack("=8d", *m.xmin, *m.xmax, m.gamma, m.cfl))
+        f.write(np.ascontiguousarray(m.locs, dtype=np.int64).tobytes())
+        for b in blocks:
+            for k in ("u", "b1", "b2", "b3"):
+                f.write(np.ascontiguousarray(b[k], dtype=np.float64).tobytes())
+    print("case abi_caller_smr: %d MeshBlocks on two levels, %d cycles"
+          % (m.nblocks, ncycles), flush=True)
+    r = subprocess.run([exe, inp, outp], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(outp, dtype=np.float64)
+    o = OracleSmrMesh(m)
+    o.load(blocks)
+    o.initialize()
+    assert o.run(ncycles) == ncycles
+    pos = 0
+    for g in range(m.nblocks):
+        for k in ("u", "w", "bcc", "b1", "b2", "b3"):
+            ref = o.get(g, k)
+            got = raw[pos:pos + ref.size].reshape(ref.shape)
+            pos += ref.size
+            assert np.array_equal(got, ref), ("smr", g, k)
+    t_o, dt_o, _ = o.state()
+    assert (raw[pos], raw[pos + 1]) == (t_o, dt_o)
+    assert pos + 2 == raw.size
