@@ -14,8 +14,11 @@ scalars, radiation or user data:
 * per MeshBlock, in the same order: u, b.x1f, b.x2f, b.x3f as AthenaArray data.
 
 MeshBlocks appear in the reference's global-id order, the Z-order traversal of the
-block tree (meshblock_tree.cpp:87-110, 336-352); the device library numbers blocks
-row-major, so writing and reading map between the two orders."""
+block tree (meshblock_tree.cpp:87-110, 336-352); the device library numbers the blocks
+of a uniform mesh row-major, so writing and reading map between the two orders.  A
+statically refined mesh (``MeshDesc.locs``) already numbers its blocks in the tree's
+order, with each block's level in its LogicalLocation, and its parameters say
+``refinement = static``, as the reference writes them."""
 from __future__ import annotations
 
 import struct
@@ -55,6 +58,17 @@ def ztree_order(nbx):
     return sorted(locs, key=key)
 
 
+def block_order(m: MeshDesc):
+    """((level, l1, l2, l3) of every MeshBlock in the reference's global-id order, the
+    device gid of each, the root level)."""
+    if m.smr:
+        locs = [tuple(int(v) for v in r) for r in m.locs]
+        return locs, list(range(m.nblocks)), int(m.root_level)
+    lev = root_level(m.nbx)
+    order = ztree_order(m.nbx)
+    return [(lev,) + tuple(loc) for loc in order], [m.gid(*loc) for loc in order], lev
+
+
 def default_params(m: MeshDesc, tlim=1.0e300, problem_id="gpu"):
     """Input blocks describing m, in the reference's athinput vocabulary."""
     mesh = {}
@@ -64,6 +78,8 @@ def default_params(m: MeshDesc, tlim=1.0e300, problem_id="gpu"):
         mesh["x%dmax" % d] = repr(float(m.xmax[d - 1]))
         mesh["ix%d_bc" % d] = m.bc[2 * d - 2]
         mesh["ox%d_bc" % d] = m.bc[2 * d - 1]
+    if m.smr:
+        mesh["refinement"] = "static"
     mb = {"nx%d" % d: str(m.bnx[d - 1]) for d in (1, 2, 3)}
     hydro = {"gamma": repr(float(m.gamma))}
     dfl, pfl = m.floors
@@ -92,8 +108,7 @@ def param_dump(blocks):
 def write(path, m: MeshDesc, get, time, dt, ncycle, params=None):
     """Writes the registers of mesh m (get(gid, what) -> array, row-major gids)."""
     text = param_dump(params if params is not None else default_params(m))
-    lev = root_level(m.nbx)
-    order = ztree_order(m.nbx)
+    order, gids, lev = block_order(m)
     shapes = [m.shape(k) for k in ("u", "b1", "b2", "b3")]
     datasize = 8 * sum(int(np.prod(s)) for s in shapes)
     with open(path, "wb") as f:
@@ -103,10 +118,9 @@ def write(path, m: MeshDesc, get, time, dt, ncycle, params=None):
         # written field by field as restart.cpp does: no padding between them
         f.write(struct.pack("<dd", time, dt) + struct.pack("<i", ncycle)
                 + struct.pack("<Q", datasize))
-        for loc in order:
-            f.write(LOCCOST.pack(loc[0], loc[1], loc[2], lev, 1.0))
-        for loc in order:
-            g = m.gid(*loc)
+        for level, l1, l2, l3 in order:
+            f.write(LOCCOST.pack(l1, l2, l3, level, 1.0))
+        for g in gids:
             for k in ("u", "b1", "b2", "b3"):
                 f.write(np.ascontiguousarray(get(g, k), dtype="<f8").tobytes())
 
@@ -129,22 +143,27 @@ def read(path, m: MeshDesc):
     p += 8
     if nbtotal != m.nblocks or tuple(region[9:12]) != tuple(m.nx):
         raise ValueError("restart file does not describe this mesh")
-    locs = []
+    order, gids, _ = block_order(m)
+    gid_of = dict(zip(order, gids))
+    locs, keys = [], []
     for _ in range(nbtotal):
         l1, l2, l3, level, cost = LOCCOST.unpack_from(raw, p)
-        locs.append((l1, l2, l3))
+        if (level, l1, l2, l3) not in gid_of:
+            raise ValueError("restart file lists a MeshBlock this mesh does not have")
+        keys.append((level, l1, l2, l3))
+        locs.append((level, l1, l2, l3) if m.smr else (l1, l2, l3))
         p += LOCCOST.size
     shapes = {k: m.shape(k) for k in ("u", "b1", "b2", "b3")}
     if datasize != 8 * sum(int(np.prod(s)) for s in shapes.values()):
         raise ValueError("restart block size does not match the mesh")
     blocks = {}
-    for loc in locs:
+    for key in keys:
         b = {}
         for k in ("u", "b1", "b2", "b3"):
             n = int(np.prod(shapes[k]))
             b[k] = np.frombuffer(raw, dtype="<f8", count=n, offset=p).reshape(shapes[k]).copy()
             p += 8 * n
-        blocks[m.gid(*loc)] = b
+        blocks[gid_of[key]] = b
     head = {"nbtotal": nbtotal, "root_level": lev, "region": region, "time": time,
             "dt": dt, "ncycle": ncycle, "datasize": datasize, "locations": locs}
     return head, blocks, params

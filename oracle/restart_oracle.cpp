@@ -22,11 +22,11 @@
 //     b.x3f, each an AthenaArray of (nx + 2 NGHOST) cells per direction (one more face
 //     along its own direction).
 // The tree order is restated here on its own: the root grid is the part of a complete
-// octree of depth root_level that covers nrbx1 x nrbx2 x nrbx3 blocks, its leaves
-// visited depth first with the children of a node in the order ox1 fastest, then ox2,
-// then ox3 (meshblock_tree.cpp:87-110, 336-356).  A file whose blocks are not in that
-// order is read with the data of one block under the location of another, exactly as
-// the reference would read it.
+// octree of depth root_level that covers nrbx1 x nrbx2 x nrbx3 blocks, the listed finer
+// locations are added along their paths, and the leaves are visited depth first with the
+// children of a node in the order ox1 fastest, then ox2, then ox3 (meshblock_tree.cpp:
+// 87-160, 336-356).  A file whose blocks are not in that order is read with the data of
+// one block under the location of another, exactly as the reference would read it.
 //========================================================================================
 #include <cstdint>
 #include <cstdio>
@@ -102,20 +102,67 @@ int get_int(const Params &p, const char *b, const char *n, int dflt) {
   return (iv == ib->second.end()) ? dflt : std::stoi(iv->second);
 }
 
-// Leaves of the root grid in the tree's order (the block list of an unrefined mesh).
-void root_leaves(int level, std::int64_t l1, std::int64_t l2, std::int64_t l3, int root,
-                 const int *nrb, std::vector<LogicalLocation> &out) {
-  const std::int64_t span = std::int64_t(1) << (root - level);   // root cells per node
-  if (l1 * span >= nrb[0] || l2 * span >= nrb[1] || l3 * span >= nrb[2]) return;
-  if (level == root) {
-    out.push_back({l1, l2, l3, level});
-    return;
+// The block tree as the restart Mesh constructor rebuilds it (mesh.cpp:868-879):
+// CreateRootGrid (meshblock_tree.cpp:87-110) makes the part of a complete octree of depth
+// root_level that covers the root grid; AddMeshBlockWithoutRefine (:138-160) then walks
+// from the root to each listed location, creating only the nodes on that path (a leaf it
+// passes through gets its child array, with just that child, and stops being a leaf);
+// GetMeshBlockList (:336-352) lists the leaves depth first, children n = i + 2j + 4k in
+// increasing n.
+struct TreeNode {
+  LogicalLocation loc;
+  int child[8];
+  bool split;                  // the node has a child array (it is not a leaf)
+};
+
+struct Tree {
+  std::vector<TreeNode> nodes;
+  int add(const LogicalLocation &l) {
+    TreeNode t;
+    t.loc = l;
+    for (int &c : t.child) c = -1;
+    t.split = false;
+    nodes.push_back(t);
+    return static_cast<int>(nodes.size()) - 1;
   }
-  for (int o3 = 0; o3 < 2; ++o3)
-    for (int o2 = 0; o2 < 2; ++o2)
-      for (int o1 = 0; o1 < 2; ++o1)
-        root_leaves(level + 1, 2 * l1 + o1, 2 * l2 + o2, 2 * l3 + o3, root, nrb, out);
-}
+  void create_root_grid(int q, int root, const int *nrb) {
+    const LogicalLocation l = nodes[q].loc;
+    if (l.level == root) return;
+    nodes[q].split = true;
+    const std::int64_t levfac = std::int64_t(1) << (root - l.level - 1);
+    for (int n = 0; n < 8; ++n) {
+      const int i = n & 1, j = (n >> 1) & 1, k = (n >> 2) & 1;
+      if ((l.lx3 * 2 + k) * levfac < nrb[2] && (l.lx2 * 2 + j) * levfac < nrb[1] &&
+          (l.lx1 * 2 + i) * levfac < nrb[0]) {
+        const int c = add({2 * l.lx1 + i, 2 * l.lx2 + j, 2 * l.lx3 + k, l.level + 1});
+        nodes[q].child[n] = c;
+        create_root_grid(c, root, nrb);
+      }
+    }
+  }
+  void add_without_refine(int q, const LogicalLocation &r) {
+    const LogicalLocation l = nodes[q].loc;
+    if (l.level == r.level) return;
+    nodes[q].split = true;
+    const int sh = r.level - l.level - 1;
+    const int mx = ((r.lx1 >> sh) & 1) == 1, my = ((r.lx2 >> sh) & 1) == 1;
+    const int mz = ((r.lx3 >> sh) & 1) == 1;
+    const int n = mx + (my << 1) + (mz << 2);
+    if (nodes[q].child[n] < 0) {
+      const int c = add({2 * l.lx1 + mx, 2 * l.lx2 + my, 2 * l.lx3 + mz, l.level + 1});
+      nodes[q].child[n] = c;
+    }
+    add_without_refine(nodes[q].child[n], r);
+  }
+  void list(int q, std::vector<LogicalLocation> &out) const {
+    if (!nodes[q].split) {
+      out.push_back(nodes[q].loc);
+      return;
+    }
+    for (int n = 0; n < 8; ++n)
+      if (nodes[q].child[n] >= 0) list(nodes[q].child[n], out);
+  }
+};
 
 }  // namespace
 
@@ -172,21 +219,24 @@ int ora_restart_read(const char *path, int ng, int *ints, double *reals, long lo
   }
   const std::size_t headeroffset = pos + listsize * nbtotal;
 
-  // tree reconstruction: the listed locations must be exactly the root grid's leaves
+  // tree reconstruction and the list replaced by the tree's own order
+  Tree bt;
+  bt.add({0, 0, 0, 0});
+  bt.create_root_grid(0, root_level, nrb);
+  for (int i = 0; i < nbtotal; ++i) {
+    if (loclist[i].level < root_level || loclist[i].level > root_level + 30)
+      return fail("a listed location has an impossible level");
+    bt.add_without_refine(0, loclist[i]);
+  }
   std::vector<LogicalLocation> tree;
-  root_leaves(0, 0, 0, 0, root_level, nrb, tree);
+  bt.list(0, tree);
   if (static_cast<int>(tree.size()) != nbtotal)
     return fail("tree reconstruction failed: the total numbers of the blocks do not match");
   bool same_order = true;
-  for (int i = 0; i < nbtotal; ++i) {
-    bool found = false;
-    for (int q = 0; q < nbtotal && !found; ++q)
-      found = tree[q].lx1 == loclist[i].lx1 && tree[q].lx2 == loclist[i].lx2 &&
-              tree[q].lx3 == loclist[i].lx3 && tree[q].level == loclist[i].level;
-    if (!found) return fail("a listed location is not a block of the root grid");
+  for (int i = 0; i < nbtotal; ++i)
     same_order = same_order && tree[i].lx1 == loclist[i].lx1 &&
-                 tree[i].lx2 == loclist[i].lx2 && tree[i].lx3 == loclist[i].lx3;
-  }
+                 tree[i].lx2 == loclist[i].lx2 && tree[i].lx3 == loclist[i].lx3 &&
+                 tree[i].level == loclist[i].level;
 
   const long nc[3] = {bnx[0] + 2L * ng, bnx[1] + 2L * ng, bnx[2] + 2L * ng};
   const long nu = 5 * nc[2] * nc[1] * nc[0];

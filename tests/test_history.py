@@ -1,6 +1,7 @@
 """History output: the format of outputs/history.cpp:355-452 (CPU) and the device sums
-against the same volume-weighted sums over the active cells formed on the host (GPU),
-plus their conservation over cycles on a periodic mesh."""
+against the same volume-weighted sums over the active cells formed on the host (GPU), on
+a uniform and on a statically refined mesh, plus their conservation over cycles on a
+periodic mesh."""
 import numpy as np
 import pytest
 
@@ -19,14 +20,31 @@ def test_header_and_line_format(tmp_path):
     assert len(NAMES) == 13
 
 
+def _refined_wave():
+    from athena_amd.mesh import refined
+    from athena_amd.pgen.linear_wave import LinearWave, linear_wave_mesh
+    regions = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((0.3, 1.0, 1.0), (0.4, 1.1, 1.1), 1),
+               ((1.0, 1.0, 0.3), (1.1, 1.1, 0.4), 1), ((1.0, 0.3, 1.0), (1.1, 0.4, 1.1), 1),
+               ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+    m = refined(linear_wave_mesh(32, 0, bnx=(8, 8, 8)), regions)
+    return m, LinearWave(m, wave_flag=0, amp=1e-2)
+
+
+def _turbulence():
+    from athena_amd.pgen.turbulence import Turbulence, turbulence_mesh
+    m = turbulence_mesh(32, 16)
+    return m, Turbulence(m)
+
+
 @pytest.mark.gpu
-def test_device_history_sums():
+@pytest.mark.parametrize("case", ["turbulence", "refined"])
+def test_device_history_sums(case):
+    # on the refined mesh each block's cell volumes are those of its level, so the sums
+    # weight fine cells by 1/8 of a coarse one, as the reference's history does
     from athena_amd.gpu import GpuMesh
     from athena_amd.mesh import block_coords
-    from athena_amd.pgen.turbulence import Turbulence, turbulence_mesh
-    print("case history sums 32^3/16^3 turbulence", flush=True)
-    m = turbulence_mesh(32, 16)
-    ic = Turbulence(m)
+    print("case history sums", case, flush=True)
+    m, ic = _turbulence() if case == "turbulence" else _refined_wave()
     d = GpuMesh(m)
     d.load([ic.block(g) for g in range(m.nblocks)])
     d.initialize()

@@ -198,3 +198,113 @@ def test_checkpoint_resume_is_bitwise(tmp_path):
     for g in range(m.nblocks):
         for k in ("u", "w", "bcc", "b1", "b2", "b3"):
             assert np.array_equal(a.get(g, k), o.get(g, k)), ("oracle", g, k)
+
+
+REGIONS = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((0.3, 1.0, 1.0), (0.4, 1.1, 1.1), 1),
+           ((1.0, 1.0, 0.3), (1.1, 1.1, 0.4), 1), ((1.0, 0.3, 1.0), (1.1, 0.4, 1.1), 1),
+           ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+
+
+def _refined_mesh(regions=REGIONS):
+    from athena_amd.mesh import refined
+    from athena_amd.pgen.linear_wave import linear_wave_mesh
+    return refined(linear_wave_mesh(32, 0, bnx=(8, 8, 8)), regions)
+
+
+def read_refined_as_reference(path, m):
+    """As read_as_reference, for a refined mesh: blocks keyed by the gid of the location
+    (level included) the reference assigns each one's data to."""
+    per = _per_block(m)
+    ints, reals, locs, data = oracle_bind.restart_read(path, m.ng, m.nblocks,
+                                                       per * m.nblocks)
+    gid_of = {tuple(int(v) for v in r): g for g, r in enumerate(m.locs)}
+    blocks = {}
+    for i, loc in enumerate(locs[:ints[0]]):
+        blocks[gid_of[tuple(int(v) for v in loc)]] = _split(m, data[per * i:per * (i + 1)])
+    return ints, reals, [tuple(int(v) for v in r) for r in locs[:ints[0]]], blocks
+
+
+@pytest.mark.parametrize("regions", [REGIONS, [((0.6, 0.5, 0.5), (0.9, 0.8, 0.8), 2)]],
+                         ids=["two-levels", "three-levels"])
+def test_refined_mesh_file_read_by_the_reference_reader(tmp_path, regions):
+    # the reference rebuilds the tree from the listed locations (CreateRootGrid, then
+    # AddMeshBlockWithoutRefine along each path) and assigns block data in the order of
+    # its own leaf list: the written list must be that order, levels included
+    m = _refined_mesh(regions)
+    data = _sample(m, 2)
+    path = str(tmp_path / "smr.rst")
+    restart.write(path, m, lambda g, k: data[(g, k)], 0.125, 2e-3, 9)
+    assert "refinement = static" in open(path, "rb").read().split(b"<par_end>")[0].decode()
+    ints, reals, locs, blocks = read_refined_as_reference(path, m)
+    assert ints[:3] == [m.nblocks, m.root_level, 9]
+    assert ints[9] == 1
+    assert locs == [tuple(int(v) for v in r) for r in m.locs]
+    assert tuple(reals[:2]) == (0.125, 2e-3)
+    for g in range(m.nblocks):
+        for k in ("u", "b1", "b2", "b3"):
+            assert np.array_equal(blocks[g][k], data[(g, k)]), (g, k)
+    head, back, _ = restart.read(path, m)
+    assert head["root_level"] == m.root_level and len(head["locations"]) == m.nblocks
+    for g in range(m.nblocks):
+        for k in ("u", "b1", "b2", "b3"):
+            assert np.array_equal(back[g][k], data[(g, k)]), (g, k)
+
+
+def test_refined_reader_catches_a_block_list_out_of_tree_order(tmp_path, monkeypatch):
+    m = _refined_mesh()
+    data = _sample(m, 3)
+    order, gids, lev = restart.block_order(m)
+    # swap two fine blocks of one parent: still a valid tree, a different list order
+    q = next(i for i in range(len(order) - 1) if order[i][0] > lev and order[i + 1][0] > lev)
+    order2, gids2 = list(order), list(gids)
+    order2[q], order2[q + 1] = order2[q + 1], order2[q]
+    gids2[q], gids2[q + 1] = gids2[q + 1], gids2[q]
+    path = str(tmp_path / "swap.rst")
+    with monkeypatch.context() as mp:
+        mp.setattr(restart, "block_order", lambda mm: (order2, gids2, lev))
+        restart.write(path, m, lambda g, k: data[(g, k)], 0.0, 1e-3, 0)
+    ints, _, locs, blocks = read_refined_as_reference(path, m)
+    assert ints[9] == 0
+    # the reference puts each swapped block's data under the other's location
+    a, b = gids[q], gids[q + 1]
+    assert np.array_equal(blocks[a]["u"], data[(b, "u")])
+    assert np.array_equal(blocks[b]["u"], data[(a, "u")])
+
+
+@pytest.mark.gpu
+def test_refined_checkpoint_resume_is_bitwise(tmp_path):
+    from athena_amd.gpu import GpuMesh
+    from athena_amd.pgen.linear_wave import LinearWave
+    print("case restart resume on the refined linear-wave mesh", flush=True)
+    m = _refined_mesh()
+    lw = LinearWave(m, wave_flag=0)
+    blocks = [lw.block(g) for g in range(m.nblocks)]
+    a = GpuMesh(m)
+    a.load(blocks)
+    a.initialize()
+    a.run(3)
+    t, dt, nc = a.state()
+    path = str(tmp_path / "c.rst")
+    restart.write(path, m, a.get, t, dt, nc)
+    a.run(3)
+    head, rb, _ = restart.read(path, m)
+    b = GpuMesh(m)
+    b.load([rb[g] for g in range(m.nblocks)])
+    b.initialize()
+    b.set_clock(head["time"], head["dt"], head["ncycle"])
+    b.run(3)
+    assert a.state() == b.state()
+    for g in range(m.nblocks):
+        for k in ("u", "w", "bcc", "b1", "b2", "b3"):
+            assert np.array_equal(a.get(g, k), b.get(g, k)), (g, k)
+    # read the way the reference reads it and continued by the multilevel restatement
+    ints, reals, _, rblocks = read_refined_as_reference(path, m)
+    o = oracle_bind.OracleSmrMesh(m)
+    o.load([rblocks[g] for g in range(m.nblocks)])
+    o.initialize()
+    o.set_clock(reals[0], reals[1], ints[2])
+    o.run(3)
+    assert o.state()[:2] == a.state()[:2]
+    for g in range(m.nblocks):
+        for k in ("u", "w", "bcc", "b1", "b2", "b3"):
+            assert np.array_equal(a.get(g, k), o.get(g, k)), ("oracle", g, k)
