@@ -97,6 +97,23 @@ def test_smr_two_levels_bitwise():
     compare(dev, ora, m, "after 4 cycles")
 
 
+def test_smr_noncubic_blocks_bitwise():
+    """MeshBlocks of 8 x 4 x 4 cells (the linear-wave mesh's 32 x 16 x 16 root grid in
+    4 x 4 x 4 blocks) with a refined region: unequal coarse-buffer sides, more blocks per
+    direction, the regions' Z-ordered tree of non-cubic leaves."""
+    regions = [((0.3, 0.3, 0.3), (0.4, 0.4, 0.4), 1), ((1.9, 0.3, 0.3), (2.0, 1.2, 1.2), 1)]
+    m = refined(linear_wave_mesh(32, 0, bnx=(8, 4, 4)), regions)
+    lw = LinearWave(m, wave_flag=0)
+    dev, ora = load_both(m, lw)
+    dev.initialize(0.5)
+    ora.initialize(0.5)
+    compare(dev, ora, m, "after initialisation")
+    dev.run(3, 0.5)
+    ora.run(3, 0.5)
+    assert dev.state()[:2] == ora.state()[:2]
+    compare(dev, ora, m, "after 3 cycles")
+
+
 def test_smr_graph_replay_matches_eager():
     m, lw = smr_wave(32, 0)
     from athena_amd.gpu import GpuMesh
