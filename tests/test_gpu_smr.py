@@ -114,6 +114,22 @@ def test_smr_noncubic_blocks_bitwise():
     compare(dev, ora, m, "after 3 cycles")
 
 
+@pytest.mark.parametrize("regions", [[((0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 1)],
+                                     [((2.9, 0.0, 0.6), (3.0, 1.5, 0.7), 1)]],
+                         ids=["corner-wrapping", "slab-across-x1"])
+def test_smr_refined_across_periodic_faces_bitwise(regions):
+    """Refined regions at the domain's corner and across its x1 faces: coarse/fine
+    neighbours found through the periodic wrap in every exchange and correction."""
+    m, lw = smr_wave(32, 0, regions=regions)
+    dev, ora = load_both(m, lw)
+    dev.initialize(0.5)
+    ora.initialize(0.5)
+    dev.run(3, 0.5)
+    ora.run(3, 0.5)
+    assert dev.state()[:2] == ora.state()[:2]
+    compare(dev, ora, m, "after 3 cycles")
+
+
 def test_smr_graph_replay_matches_eager():
     m, lw = smr_wave(32, 0)
     from athena_amd.gpu import GpuMesh
