@@ -81,6 +81,11 @@ def test_graph_is_recaptured_when_the_configuration_changes():
     eager.run(2)
     graph.run(2)
     same(snap(graph), snap(eager), "after set_tiles")
+    for d in (eager, graph):
+        d.set_tiles([6, 6, 0, 0])             # the flux z-march: another kernel
+    eager.run(2)
+    graph.run(2)
+    same(snap(graph), snap(eager), "after the flux z-march tile")
     graph.kernel_timing(True)                 # timed cycles run eagerly
     eager.run(1)
     graph.run(1)
