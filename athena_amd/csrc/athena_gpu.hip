@@ -903,7 +903,8 @@ __device__ __forceinline__ void corner_body(const MeshK &M, const CornerArgs &ca
 
 // Z-marching corner EMFs (tile option with tx < 0, tz = planes per segment): a
 // work-group owns a TI x TJ tile of (i, j) edge positions (TI = the box width up to 256,
-// TJ = 256 / TI rows) and walks up a segment of planes.  The cell-centred E of the
+// TJ = 256 / (TI+1) - 1 rows but at least one, so that for TI < 255 its 256 threads
+// stage the (TI+1) x (TJ+1) cells in one pass) and walks up a segment of planes.  The cell-centred E of the
 // (TI+1) x (TJ+1) cells of a plane that the tile's edges touch is evaluated once into
 // LDS (planes k-1 and k, double-buffered), and every thread carries the four face values
 // of plane k-1 that its edges need from the previous step, so each array element is
@@ -917,7 +918,8 @@ __host__ __device__ __forceinline__ MarchGeom march_geom(const MeshK &M, int ks)
   kind_box(M, KK_CORNER, lo, n);
   MarchGeom G;
   G.ti = (n[0] < WG) ? n[0] : WG;
-  G.tj = WG / G.ti;
+  G.tj = WG / (G.ti + 1) - 1;   // stage (ti+1) x (tj+1) cells in one pass
+  if (G.tj < 1) G.tj = 1;
   if (G.tj > n[1]) G.tj = n[1];
   G.nti = (n[0] + G.ti - 1) / G.ti;
   G.ntj = (n[1] + G.tj - 1) / G.tj;
