@@ -65,12 +65,12 @@ class HipEvents:
 # linear), fetched KiB x 2.0 + written KiB x 1.0 (the factors a 512 MiB device copy
 # gives): profiles/vl2_hlld_plm_turbulence256_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    "flux_xyz_plm": 11398313984,
-    "flux_xyz_dc": 8297477120,
-    "update_u_b_ct": 7938470912,
-    "corner_emf": 3726039040,
-    "cons2prim": 3309647872,
-    "ghost_exchange": 2298912768,
+    "flux_xyz_plm": 11396524816,
+    "flux_xyz_dc": 8299554176,
+    "update_u_b_ct": 7940971036,
+    "corner_emf": 3773546268,
+    "cons2prim": 3309664999,
+    "ghost_exchange": 2299010489,
 }
 TRAFFIC_TILES = ["linear", "zmarch8", "linear", "linear"]
 # FP64 FLOP per launch (64 x (ADD + MUL + 2 FMA) of the SQ_INSTS_VALU_*_F64 counters) on
@@ -79,7 +79,7 @@ MEASURED_FP64_FLOP = {
     "flux_xyz_plm": 47786607216,
     "flux_xyz_dc": 31422754608,
     "update_u_b_ct": 4173332480,
-    "corner_emf": 2138603520,
+    "corner_emf": 2422603776,
     "cons2prim": 3371907300,
 }
 FP64_PEAK = 78.6e12
